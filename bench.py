@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Benchmark of the plane-wave inner loop on one MI355X.
+
+A step is one SCF step of the workload named in BASELINE (512 states on the
+192^3 mesh, 198,204 Gamma plane waves, cubic a = 32 bohr, 256 atoms, ecut
+80 Ry): rhoofr + vpsi, with every input already resident in HBM.  The
+wavefunctions are the seeded synthetic start (complex Gaussian
+~exp(-|G|^2/8), Loewdin-orthonormalised) and V(r) a seeded smooth potential.
+
+Prints one JSON line (see the measurement contract in DESIGN.md).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "CP step ms at 512 states/192³: mixed vs FP64 GPU, |ΔE|<1e-6 Ha/atom"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(ctx_geom, kid: str, np_pairs: int) -> float:
+    """Algorithmic HBM bytes of one launch of kernel `kid` processing np_pairs
+    state pairs (FP64 complex = 16 B, real = 8 B)."""
+    ngw, S, Zw, n1, n2, n3 = ctx_geom
+    c, r = 16.0, 8.0
+    nnn = n1 * n2 * n3
+    col = n3 * Zw * n2          # column workspace per pair (complex)
+    stick = S * n3              # stick workspace per pair (complex)
+    if kid == "zinv":
+        return np_pairs * c * (2 * ngw + stick)
+    if kid == "yinv" or kid == "yfwd":
+        return np_pairs * c * (stick + col)
+    if kid == "xrho":
+        return np_pairs * c * col + 2 * r * nnn
+    if kid == "xvpsi":
+        return np_pairs * c * 2 * col + r * nnn
+    if kid == "zfwd":
+        return np_pairs * c * (stick + 6 * ngw)
+    raise KeyError(kid)
+
+
+def step_model_bytes(ctx_geom, nstate: int) -> float:
+    """Per-step algorithmic bytes of rhoofr + vpsi: per state pair
+    vpsi = c(4 S N + 4 N^2 Zw + 6 NGW) + r N^3, rhoofr = c(2 S N + 2 N^2 Zw + 2 NGW),
+    plus rho written once (r N^3)."""
+    ngw, S, Zw, n1, n2, n3 = ctx_geom
+    c, r = 16.0, 8.0
+    npairs = (nstate + 1) // 2
+    vp = c * (4 * S * n3 + 4 * n1 * n2 * Zw + 6 * ngw) + r * n1 * n2 * n3
+    rh = c * (2 * S * n3 + 2 * n1 * n2 * Zw + 2 * ngw)
+    return npairs * (vp + rh) + 2 * r * n1 * n2 * n3
+
+
+def cpu_baseline(sysm, c0, v, max_pairs: int = 1):
+    """Oracle (single-thread C restatement of CPMD's rhoofr + vpsi) on a bounded
+    sample of the same step: `max_pairs` state pairs, scaled to all pairs."""
+    import oracle
+    nstate = c0.shape[1]
+    npairs = (nstate + 1) // 2
+    ns = 2 * max_pairs
+    L = oracle.Layout(sysm.cell, sysm.gset)
+    cs = np.asfortranarray(c0[:, :ns])
+    f = sysm.f[:ns]
+    t0 = time.perf_counter()
+    oracle.rhoofr(L, cs, f)
+    oracle.vpsi(L, cs, np.zeros_like(cs), f, v)
+    dt = time.perf_counter() - t0
+    return {
+        "value": dt * 1e3 * npairs / max_pairs,
+        "unit": "ms/step",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle rhoofr+vpsi on {max_pairs} of {npairs} state pairs ({dt:.1f} s), scaled x{npairs // max_pairs}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="cfg4")
+    ap.add_argument("--nstate", type=int, default=0)
+    ap.add_argument("--batch-rho", type=int, default=16)
+    ap.add_argument("--batch-vpsi", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: the one-GPU build runs the whole workload on a "
+              "single MI355X; multi-GPU sharding is described in DESIGN.md, not launched.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from cpmd_amd import ops
+    from cpmd_amd import system as S
+
+    cfg = S.CONFIGS[args.config]
+    nstate = args.nstate or cfg["nstate"]
+    log(f"[bench] building {args.config}: a={cfg['a']} nat={cfg['nat']} ecut={cfg['ecut']} nstate={nstate}")
+    sysm = S.make_system(cfg["a"], cfg["nat"], cfg["ecut"], nstate=nstate)
+    c0 = S.initial_coefficients(sysm)
+    v = S.real_space_potential(sysm)
+    ngw = sysm.gset.ngw
+    n1, n2, n3 = sysm.cell.nr
+    sticks, cols, _ = ops.layout_stats(sysm.cell, sysm.gset)
+    geom = (ngw, sticks, cols, n1, n2, n3)
+    log(f"[bench] mesh {sysm.cell.nr} ngw {ngw} sticks {sticks} columns {cols}")
+
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    ctx.set_batch(args.batch_rho, args.batch_vpsi)
+    d_c0 = ctx.upload(c0)
+    d_c2 = ctx.alloc_wfn(nstate)
+    d_rho = ops.DeviceArray(8 * ctx.nnr1)
+    d_v = ops.DeviceArray(8 * ctx.nnr1).from_host(v)
+    f = sysm.f
+
+    energies = {}
+
+    def step():
+        ek, cg, cr = ctx.rhoofr_dev(d_c0, f, nstate, d_rho)
+        energies.update(ekin=ek, csumg=cg, csumr=cr)
+        ctx.zero_wfn(d_c2, nstate)
+        ctx.vpsi_dev(d_c0, d_c2, f, d_v, nstate)
+
+    for i in range(args.warmup):
+        step()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    ctx.synchronize()
+
+    ctx.timer_reset()
+    ctx.timer_enable(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ctx.timer_enable(False)
+    ms = (t1 - t0) * 1e3 / args.steps
+    log(f"[bench] {args.steps} steps: {ms:.3f} ms/step (ekin {energies['ekin']:.10f}, "
+        f"csumr-csumg {energies['csumr'] - energies['csumg']:.2e})")
+
+    # per-kernel HIP-event times over the timed region
+    npairs = (nstate + 1) // 2
+    kernels = {}
+    for kid, P in (("zinv", None), ("yinv", None), ("xrho", args.batch_rho), ("xvpsi", args.batch_vpsi),
+                   ("yfwd", args.batch_vpsi), ("zfwd", args.batch_vpsi), ("misc", None)):
+        tot, cnt = ctx.timer_get(kid)
+        if cnt:
+            kernels[kid] = {"total_ms_per_step": tot / args.steps, "launches": cnt, "avg_ms": tot / cnt}
+    # dominant kernel by time per step
+    dom = max((k for k in kernels if k != "misc"), key=lambda k: kernels[k]["total_ms_per_step"])
+    # pairs per launch of the dominant kernel (full batches dominate; average over the step)
+    if dom in ("xrho",):
+        launches_per_step = -(-npairs // min(args.batch_rho, npairs))
+        pairs_per_launch = npairs / launches_per_step
+    elif dom in ("xvpsi", "yfwd", "zfwd"):
+        launches_per_step = -(-npairs // min(args.batch_vpsi, npairs))
+        pairs_per_launch = npairs / launches_per_step
+    else:   # zinv / yinv run in both rhoofr and vpsi batches
+        lr = -(-npairs // min(args.batch_rho, npairs))
+        lv = -(-npairs // min(args.batch_vpsi, npairs))
+        pairs_per_launch = 2 * npairs / (lr + lv)
+    # bytes of the average launch: linear in pairs except the once-per-launch real-space term
+    b_launch = kernel_bytes(geom, dom, 1) * pairs_per_launch
+    if dom in ("xrho", "xvpsi"):
+        b_launch = kernel_bytes(geom, dom, 0) + (kernel_bytes(geom, dom, 1) - kernel_bytes(geom, dom, 0)) * pairs_per_launch
+    achieved = b_launch / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
+    step_bytes = step_model_bytes(geom, nstate)
+    step_gbs = step_bytes / (ms * 1e-3) / 1e9
+
+    traffic = None
+    tf = ROOT / "profiles" / "traffic.json"
+    if tf.exists():
+        try:
+            traffic = json.loads(tf.read_text()).get(dom)
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline (oracle, 1 pair sample) ...")
+        cpu = cpu_baseline(sysm, c0, v)
+        log(f"[bench] cpu baseline {cpu['value']:.1f} ms/step")
+
+    out = {
+        "metric": METRIC,
+        "value": round(ms, 4),
+        "unit": "ms/step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Gaussian wavefunctions, Loewdin-orthonormalised; seeded smooth V(r))",
+        "config": {
+            "workload": f"{args.config}: {nstate} states, {n1}x{n2}x{n3} mesh, {ngw} Gamma plane waves, "
+                        f"a={cfg['a']} bohr, ecut {cfg['ecut']} Ry",
+            "step": "rhoofr + vpsi (FP64 GPU path)",
+            "precision_path": "fp64",
+            "batch_pairs": {"rhoofr": args.batch_rho, "vpsi": args.batch_vpsi},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_launch": b_launch,
+            "avg_launch_ms": kernels[dom]["avg_ms"],
+        },
+        "step_roofline": {"model_bytes": step_bytes, "achieved_GBs": round(step_gbs, 1),
+                          "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,532 @@
+// Context creation, stick layout, wavefunction permutes and the small C-ABI
+// helpers of the micpmd library.
+//
+// Stick layout (the library's own; CPMD's x-ray layout of
+// src/fftprp_utils.mod.F90:209-285 plays the same role):
+//  * a stick is a line along z at fixed (m1, m2) Miller indices;
+//  * an owner stick holds half-sphere G vectors (CPMD's half sphere,
+//    src/loadpa_utils.mod.F90:283-300: m1>0 | m1=0,m2>0 | m1=m2=0,m3>=0);
+//    owner 0 is (0,0), the others are ordered by (m1, m2);
+//  * the mirror of owner o>0 is (-m1,-m2); it receives conj(c) at -m3;
+//  * slots: owner o -> slot o, mirror of owner o -> slot n_own + o - 1;
+//  * wavefunction rows on the device are permuted into stick order: the G
+//    vectors of owner o form the run [own_start[o], own_start[o+1]), sorted by
+//    m3, so G=0 is row 0 of the device layout;
+//  * a column is the set of sticks with the same m1 (x frequency); the y pass
+//    works per column.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "fft_device.h"
+#include "mc_internal.h"
+
+namespace mc {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+
+void DevBuf::ensure(size_t nbytes) {
+  if (nbytes <= bytes) return;
+  release();
+  if (nbytes == 0) return;
+  MC_HIP(hipMalloc(&p, nbytes));
+  bytes = nbytes;
+}
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+FftPlan make_plan(int n) {
+  FftPlan pl{};
+  pl.n = n;
+  pl.nf = 0;
+  MC_CHECK(n >= 2 && n <= 2048, MC_ERR_UNSUPPORTED,
+           "transform length " + std::to_string(n) + " outside [2, 2048]");
+  int m = n;
+  // radix-4 first, then one radix-2, then 3s and 5s
+  while (m % 4 == 0) { pl.rad[pl.nf++] = 4; m /= 4; }
+  while (m % 2 == 0) { pl.rad[pl.nf++] = 2; m /= 2; }
+  while (m % 3 == 0) { pl.rad[pl.nf++] = 3; m /= 3; }
+  while (m % 5 == 0) { pl.rad[pl.nf++] = 5; m /= 5; }
+  MC_CHECK(m == 1 && pl.nf <= kMaxStages, MC_ERR_UNSUPPORTED,
+           "transform length " + std::to_string(n) + " is not of the form 2^a 3^b 5^c");
+  return pl;
+}
+
+static void upload_twiddles(DevBuf& buf, int n) {
+  std::vector<double> tw(2 * (size_t)n);
+  for (int t = 0; t < n; ++t) {
+    // exp(-2 pi i t/n) evaluated in long double and rounded once to double
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)n;
+    tw[2 * t] = (double)cosl(a);
+    tw[2 * t + 1] = (double)sinl(a);
+  }
+  buf.ensure(sizeof(double) * 2 * n);
+  MC_HIP(hipMemcpy(buf.p, tw.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+}
+
+static inline int wrap(int m, int n) { int r = m % n; return r < 0 ? r + n : r; }
+
+struct HostLayout {
+  int n_own = 0, n_slots = 0, ncols = 0, max_col_len = 0;
+  std::vector<int> perm, ginfo, own_start, col_start, col_slot, col_gy, col_gx;
+  std::vector<double> hg;
+};
+
+// Builds the stick layout from CPMD's inyh/hg; host only.
+static HostLayout build_layout(const mc_layout* lay) {
+  MC_CHECK(lay != nullptr, MC_ERR_ARG, "null layout");
+  const int n1 = lay->nr1, n2 = lay->nr2, n3 = lay->nr3;
+  MC_CHECK(n1 > 1 && n2 > 1 && n3 > 1, MC_ERR_ARG, "mesh dimensions must be > 1");
+  MC_CHECK(n3 < 4096, MC_ERR_UNSUPPORTED, "nr3 must be < 4096");
+  MC_CHECK(lay->kr1 >= n1 && lay->kr2 >= n2 && lay->kr3 >= n3, MC_ERR_ARG,
+           "leading dimensions kr must be >= nr");
+  MC_CHECK(lay->ngw > 0 && lay->inyh && lay->hg, MC_ERR_ARG, "empty G set");
+  const int ngw = lay->ngw;
+  const int nh1 = n1 / 2 + 1, nh2 = n2 / 2 + 1, nh3 = n3 / 2 + 1;
+  HostLayout L;
+  // owner sticks keyed by (m1, m2)
+  std::map<std::pair<int, int>, int> own_of;
+  std::vector<std::tuple<int, int, int>> mill(ngw);
+  for (int ig = 0; ig < ngw; ++ig) {
+    int m1 = lay->inyh[3 * ig] - nh1, m2 = lay->inyh[3 * ig + 1] - nh2, m3 = lay->inyh[3 * ig + 2] - nh3;
+    MC_CHECK(2 * std::abs(m1) < n1 && 2 * std::abs(m2) < n2 && 2 * std::abs(m3) < n3, MC_ERR_ARG,
+             "G vector " + std::to_string(ig + 1) + " outside the mesh (aliasing)");
+    mill[ig] = {m1, m2, m3};
+    own_of[{m1, m2}] = 0;
+  }
+  MC_CHECK(own_of.count({0, 0}) || !lay->geq0, MC_ERR_ARG, "geq0 set but no (0,0) stick");
+  // CPMD half sphere: a stick and its mirror never both own G vectors.
+  for (auto& kv : own_of) {
+    auto k = kv.first;
+    if (k.first == 0 && k.second == 0) continue;
+    MC_CHECK(!own_of.count({-k.first, -k.second}), MC_ERR_UNSUPPORTED,
+             "G set is not a CPMD half sphere: sticks (" + std::to_string(k.first) + "," +
+                 std::to_string(k.second) + ") and its mirror both hold G vectors");
+  }
+  // order owners: (0,0) first, then by (m1, m2)
+  std::vector<std::pair<int, int>> owners;
+  if (own_of.count({0, 0})) owners.push_back({0, 0});
+  for (auto& kv : own_of)
+    if (!(kv.first.first == 0 && kv.first.second == 0)) owners.push_back(kv.first);
+  std::sort(owners.begin() + (own_of.count({0, 0}) ? 1 : 0), owners.end());
+  const bool has_center = own_of.count({0, 0}) > 0;
+  MC_CHECK(has_center, MC_ERR_UNSUPPORTED, "G set without the (0,0) stick is not supported");
+  for (size_t o = 0; o < owners.size(); ++o) own_of[owners[o]] = (int)o;
+  L.n_own = (int)owners.size();
+  MC_CHECK(L.n_own < (1 << 19), MC_ERR_UNSUPPORTED, "too many sticks");
+  L.n_slots = 2 * L.n_own - 1;
+  // stick-ordered G: sort by (owner, m3)
+  std::vector<int> idx(ngw);
+  for (int i = 0; i < ngw; ++i) idx[i] = i;
+  std::vector<int> owner_of_g(ngw);
+  for (int ig = 0; ig < ngw; ++ig) owner_of_g[ig] = own_of[{std::get<0>(mill[ig]), std::get<1>(mill[ig])}];
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+    if (owner_of_g[a] != owner_of_g[b]) return owner_of_g[a] < owner_of_g[b];
+    return std::get<2>(mill[a]) < std::get<2>(mill[b]);
+  });
+  L.perm = idx;
+  L.ginfo.resize(ngw);
+  L.hg.resize(ngw);
+  L.own_start.assign(L.n_own + 1, 0);
+  for (int j = 0; j < ngw; ++j) {
+    const int ig = idx[j];
+    const int o = owner_of_g[ig];
+    L.ginfo[j] = (o << 12) | wrap(std::get<2>(mill[ig]), n3);
+    L.hg[j] = lay->hg[ig];
+    L.own_start[o + 1]++;
+  }
+  for (int o = 0; o < L.n_own; ++o) L.own_start[o + 1] += L.own_start[o];
+  if (lay->geq0) {
+    MC_CHECK(std::get<0>(mill[idx[0]]) == 0 && std::get<1>(mill[idx[0]]) == 0 &&
+                 std::get<2>(mill[idx[0]]) == 0,
+             MC_ERR_ARG, "geq0 set but G=0 missing");
+    MC_CHECK(idx[0] == 0, MC_ERR_ARG, "geq0 set but G=0 is not the first G vector");
+  }
+  // columns over all slots
+  std::map<int, std::vector<std::pair<int, int>>> cols;  // m1 -> (m2, slot)
+  for (int o = 0; o < L.n_own; ++o) {
+    auto k = owners[o];
+    cols[k.first].push_back({k.second, o});
+    if (o > 0) cols[-k.first].push_back({-k.second, L.n_own + o - 1});
+  }
+  L.ncols = (int)cols.size();
+  L.col_start.push_back(0);
+  for (auto& kv : cols) {
+    auto v = kv.second;
+    std::sort(v.begin(), v.end());
+    for (auto& e : v) {
+      L.col_slot.push_back(e.second);
+      L.col_gy.push_back(wrap(e.first, n2));
+    }
+    L.col_start.push_back((int)L.col_slot.size());
+    L.col_gx.push_back(wrap(kv.first, n1));
+    L.max_col_len = std::max(L.max_col_len, (int)v.size());
+  }
+  return L;
+}
+
+template <class T>
+static void upload_vec(DevBuf& b, const std::vector<T>& v) {
+  b.ensure(sizeof(T) * std::max<size_t>(v.size(), 1));
+  if (!v.empty()) MC_HIP(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+}
+
+// ---------------------------------------------------------------------------
+// permutes between CPMD row order (staging, pitch ngw) and the device layout
+// ---------------------------------------------------------------------------
+__global__ void k_permute_in(const double2* __restrict__ stage, double2* __restrict__ c,
+                             const int* __restrict__ perm, int ngw, int ldg, int nstate) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (i >= nstate || j >= ldg) return;
+  c[(size_t)i * ldg + j] = (j < ngw) ? stage[(size_t)i * ngw + perm[j]] : make_double2(0.0, 0.0);
+}
+__global__ void k_permute_out(const double2* __restrict__ c, double2* __restrict__ stage,
+                              const int* __restrict__ perm, int ngw, int ldg, int nstate) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (i >= nstate || j >= ngw) return;
+  stage[(size_t)i * ngw + perm[j]] = c[(size_t)i * ldg + j];
+}
+
+__global__ void k_zero(double* __restrict__ x, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] = 0.0;
+}
+
+void wfn_upload(mc_ctx* ctx, const double* host, int ldc, int nstate, double2* dev) {
+  MC_CHECK(ldc >= ctx->ngw && nstate >= 0, MC_ERR_ARG, "wfn_upload: ldc < ngw or nstate < 0");
+  if (nstate == 0) return;
+  const size_t sb = sizeof(double2) * (size_t)ctx->ngw * nstate;
+  ctx->w_c0.ensure(sb);
+  MC_HIP(hipMemcpy2DAsync(ctx->w_c0.p, sizeof(double2) * ctx->ngw, host, sizeof(double2) * (size_t)ldc,
+                          sizeof(double2) * ctx->ngw, nstate, hipMemcpyHostToDevice, ctx->stream));
+  dim3 grid((ctx->ldg + 255) / 256, nstate);
+  hipLaunchKernelGGL(k_permute_in, grid, dim3(256), 0, ctx->stream, ctx->w_c0.as<double2>(), dev,
+                     ctx->d_perm.as<int>(), ctx->ngw, ctx->ldg, nstate);
+  MC_HIP(hipGetLastError());
+  // the staging buffer is reused by the next upload
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void wfn_download(mc_ctx* ctx, const double2* dev, int nstate, double* host, int ldc) {
+  MC_CHECK(ldc >= ctx->ngw && nstate >= 0, MC_ERR_ARG, "wfn_download: ldc < ngw or nstate < 0");
+  if (nstate == 0) return;
+  const size_t sb = sizeof(double2) * (size_t)ctx->ngw * nstate;
+  ctx->w_c2.ensure(sb);
+  dim3 grid((ctx->ngw + 255) / 256, nstate);
+  hipLaunchKernelGGL(k_permute_out, grid, dim3(256), 0, ctx->stream, dev, ctx->w_c2.as<double2>(),
+                     ctx->d_perm.as<int>(), ctx->ngw, ctx->ldg, nstate);
+  MC_HIP(hipGetLastError());
+  MC_HIP(hipMemcpy2DAsync(host, sizeof(double2) * (size_t)ldc, ctx->w_c2.p, sizeof(double2) * ctx->ngw,
+                          sizeof(double2) * ctx->ngw, nstate, hipMemcpyDeviceToHost, ctx->stream));
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// ---------------------------------------------------------------------------
+// kernel timer
+// ---------------------------------------------------------------------------
+void timer_begin(mc_ctx* ctx, int kid) {
+  if (!ctx->timer.enabled) return;
+  hipEvent_t a, b;
+  MC_HIP(hipEventCreate(&a));
+  MC_HIP(hipEventCreate(&b));
+  MC_HIP(hipEventRecord(a, ctx->stream));
+  ctx->timer.ev.push_back(a);
+  ctx->timer.ev.push_back(b);
+  ctx->timer.kid.push_back(kid);
+}
+void timer_end(mc_ctx* ctx, int kid) {
+  (void)kid;
+  if (!ctx->timer.enabled) return;
+  MC_HIP(hipEventRecord(ctx->timer.ev.back(), ctx->stream));
+}
+void timer_flush(mc_ctx* ctx) {
+  auto& t = ctx->timer;
+  if (t.ev.empty()) return;
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  for (size_t k = 0; k < t.kid.size(); ++k) {
+    float ms = 0.f;
+    MC_HIP(hipEventElapsedTime(&ms, t.ev[2 * k], t.ev[2 * k + 1]));
+    t.total_ms[t.kid[k]] += ms;
+    t.count[t.kid[k]] += 1;
+    (void)hipEventDestroy(t.ev[2 * k]);
+    (void)hipEventDestroy(t.ev[2 * k + 1]);
+  }
+  t.ev.clear();
+  t.kid.clear();
+}
+
+}  // namespace mc
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+#define MC_API_BEGIN \
+  try {              \
+    mc::clear_error();
+#define MC_API_END                                   \
+  }                                                  \
+  catch (const mc::Error& e) {                       \
+    mc::set_error(e.msg);                            \
+    return e.code;                                   \
+  }                                                  \
+  catch (const std::exception& e) {                  \
+    mc::set_error(std::string("internal: ") + e.what()); \
+    return MC_ERR_INTERNAL;                          \
+  }                                                  \
+  return MC_OK;
+
+extern "C" {
+
+const char* mc_last_error(void) { return mc::g_last_error.c_str(); }
+int mc_version(void) { return 100; }
+
+int mc_device_count(int* count) {
+  MC_API_BEGIN
+  MC_CHECK(count, MC_ERR_ARG, "null count");
+  int n = 0;
+  MC_HIP(hipGetDeviceCount(&n));
+  *count = n;
+  MC_API_END
+}
+
+int mc_layout_stats(const mc_layout* lay, int* n_sticks, int* n_columns, int* n_owner) {
+  MC_API_BEGIN
+  mc::HostLayout L = mc::build_layout(lay);
+  if (n_sticks) *n_sticks = L.n_slots;
+  if (n_columns) *n_columns = L.ncols;
+  if (n_owner) *n_owner = L.n_own;
+  MC_API_END
+}
+
+int mc_ctx_create(const mc_layout* lay, mc_ctx** out) {
+  MC_API_BEGIN
+  MC_CHECK(out, MC_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  mc::HostLayout L = mc::build_layout(lay);
+  auto* ctx = new mc_ctx();
+  try {
+    MC_HIP(hipGetDevice(&ctx->device));
+    ctx->n1 = lay->nr1; ctx->n2 = lay->nr2; ctx->n3 = lay->nr3;
+    ctx->kr1 = lay->kr1; ctx->kr2 = lay->kr2; ctx->kr3 = lay->kr3;
+    ctx->ngw = lay->ngw;
+    ctx->ldg = (lay->ngw + 63) / 64 * 64;
+    ctx->tpiba2 = lay->tpiba2;
+    ctx->omega = lay->omega;
+    ctx->geq0 = lay->geq0 ? 1 : 0;
+    ctx->plan1 = mc::make_plan(ctx->n1);
+    ctx->plan2 = mc::make_plan(ctx->n2);
+    ctx->plan3 = mc::make_plan(ctx->n3);
+    ctx->n_own = L.n_own;
+    ctx->n_slots = L.n_slots;
+    ctx->ncols = L.ncols;
+    ctx->max_col_len = L.max_col_len;
+    ctx->perm = L.perm;
+    ctx->h_own_start = L.own_start;
+    ctx->h_col_start = L.col_start;
+    MC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->own_stream = true;
+    mc::upload_vec(ctx->d_perm, L.perm);
+    mc::upload_vec(ctx->d_ginfo, L.ginfo);
+    mc::upload_vec(ctx->d_hg, L.hg);
+    mc::upload_vec(ctx->d_own_start, L.own_start);
+    mc::upload_vec(ctx->d_col_start, L.col_start);
+    mc::upload_vec(ctx->d_col_slot, L.col_slot);
+    mc::upload_vec(ctx->d_col_gy, L.col_gy);
+    mc::upload_vec(ctx->d_col_gx, L.col_gx);
+    mc::upload_twiddles(ctx->d_tw1, ctx->n1);
+    mc::upload_twiddles(ctx->d_tw2, ctx->n2);
+    mc::upload_twiddles(ctx->d_tw3, ctx->n3);
+  } catch (...) {
+    if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    throw;
+  }
+  *out = ctx;
+  MC_API_END
+}
+
+int mc_ctx_destroy(mc_ctx* ctx) {
+  MC_API_BEGIN
+  if (!ctx) return MC_OK;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto e : ctx->timer.ev) (void)hipEventDestroy(e);
+  if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  MC_API_END
+}
+
+int mc_ctx_set_stream(mc_ctx* ctx, void* hip_stream) {
+  MC_API_BEGIN
+  MC_CHECK(ctx, MC_ERR_ARG, "null context");
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  if (hip_stream) {
+    if (ctx->own_stream) MC_HIP(hipStreamDestroy(ctx->stream));
+    ctx->stream = (hipStream_t)hip_stream;
+    ctx->own_stream = false;
+  } else if (!ctx->own_stream) {
+    MC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->own_stream = true;
+  }
+  MC_API_END
+}
+
+int mc_ctx_synchronize(mc_ctx* ctx) {
+  MC_API_BEGIN
+  MC_CHECK(ctx, MC_ERR_ARG, "null context");
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  MC_API_END
+}
+
+int mc_ctx_set_batch(mc_ctx* ctx, int pairs_rhoofr, int pairs_vpsi) {
+  MC_API_BEGIN
+  MC_CHECK(ctx, MC_ERR_ARG, "null context");
+  MC_CHECK(pairs_rhoofr >= 1 && pairs_vpsi >= 1 && pairs_rhoofr <= 4096 && pairs_vpsi <= 4096,
+           MC_ERR_ARG, "batch sizes must be in [1, 4096]");
+  ctx->batch_pairs_rho = pairs_rhoofr;
+  ctx->batch_pairs_vpsi = pairs_vpsi;
+  MC_API_END
+}
+
+size_t mc_wfn_ld(const mc_ctx* ctx) { return ctx ? (size_t)ctx->ldg : 0; }
+
+int mc_dev_alloc(size_t bytes, void** ptr) {
+  MC_API_BEGIN
+  MC_CHECK(ptr, MC_ERR_ARG, "null pointer");
+  *ptr = nullptr;
+  if (bytes) MC_HIP(hipMalloc(ptr, bytes));
+  MC_API_END
+}
+int mc_dev_free(void* ptr) {
+  MC_API_BEGIN
+  if (ptr) MC_HIP(hipFree(ptr));
+  MC_API_END
+}
+int mc_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  MC_API_BEGIN
+  if (bytes) MC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  MC_API_END
+}
+int mc_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  MC_API_BEGIN
+  if (bytes) MC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  MC_API_END
+}
+int mc_memset_d(void* dst, int value, size_t bytes) {
+  MC_API_BEGIN
+  if (bytes) MC_HIP(hipMemset(dst, value, bytes));
+  MC_API_END
+}
+
+int mc_wfn_zero(mc_ctx* ctx, void* c_dev, int nstate) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c_dev && nstate >= 0, MC_ERR_ARG, "bad argument");
+  const size_t n = (size_t)ctx->ldg * nstate * 2;
+  if (n) {
+    hipLaunchKernelGGL(mc::k_zero, dim3(1024), dim3(256), 0, ctx->stream, (double*)c_dev, n);
+    MC_HIP(hipGetLastError());
+  }
+  MC_API_END
+}
+
+int mc_wfn_upload(mc_ctx* ctx, const double* c_host, int ldc, int nstate, void* c_dev) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c_host && c_dev, MC_ERR_ARG, "null argument");
+  mc::wfn_upload(ctx, c_host, ldc, nstate, (double2*)c_dev);
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  MC_API_END
+}
+
+int mc_wfn_download(mc_ctx* ctx, const void* c_dev, int nstate, double* c_host, int ldc) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c_host && c_dev, MC_ERR_ARG, "null argument");
+  mc::wfn_download(ctx, (const double2*)c_dev, nstate, c_host, ldc);
+  MC_API_END
+}
+
+int mc_rhoofr_dev(mc_ctx* ctx, const void* c0_dev, const double* f, int nstate, void* rhoe_dev,
+                  double* ekin, double* csumg, double* csumr) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c0_dev && f && rhoe_dev, MC_ERR_ARG, "null argument");
+  MC_CHECK(nstate >= 0, MC_ERR_ARG, "nstate < 0");
+  mc::rhoofr_dev(ctx, (const double2*)c0_dev, f, nstate, (double*)rhoe_dev, ekin, csumg, csumr);
+  MC_API_END
+}
+
+int mc_rhoofr(mc_ctx* ctx, const double* c0, int ldc, const double* f, int nstate, double* rhoe,
+              double* ekin, double* csumg, double* csumr) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c0 && f && rhoe, MC_ERR_ARG, "null argument");
+  MC_CHECK(nstate >= 0, MC_ERR_ARG, "nstate < 0");
+  const size_t nnr1 = (size_t)ctx->kr1 * ctx->kr2 * ctx->kr3;
+  mc::DevBuf cdev;
+  cdev.ensure(sizeof(double2) * (size_t)ctx->ldg * std::max(nstate, 1));
+  mc::wfn_upload(ctx, c0, ldc, nstate, cdev.as<double2>());
+  ctx->w_v.ensure(sizeof(double) * nnr1);
+  mc::rhoofr_dev(ctx, cdev.as<double2>(), f, nstate, ctx->w_v.as<double>(), ekin, csumg, csumr);
+  MC_HIP(hipMemcpyAsync(rhoe, ctx->w_v.p, sizeof(double) * nnr1, hipMemcpyDeviceToHost, ctx->stream));
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  MC_API_END
+}
+
+int mc_vpsi_dev(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const double* f, const void* vpot_dev,
+                int nstate) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c0_dev && c2_dev && f && vpot_dev, MC_ERR_ARG, "null argument");
+  MC_CHECK(nstate >= 0, MC_ERR_ARG, "nstate < 0");
+  mc::vpsi_dev(ctx, (const double2*)c0_dev, (double2*)c2_dev, f, (const double*)vpot_dev, nstate);
+  MC_API_END
+}
+
+int mc_vpsi(mc_ctx* ctx, const double* c0, int ldc, double* c2, const double* f, const double* vpot,
+            int nstate) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c0 && c2 && f && vpot, MC_ERR_ARG, "null argument");
+  MC_CHECK(nstate >= 0, MC_ERR_ARG, "nstate < 0");
+  const size_t nnr1 = (size_t)ctx->kr1 * ctx->kr2 * ctx->kr3;
+  mc::DevBuf c0d, c2d;
+  c0d.ensure(sizeof(double2) * (size_t)ctx->ldg * std::max(nstate, 1));
+  c2d.ensure(sizeof(double2) * (size_t)ctx->ldg * std::max(nstate, 1));
+  mc::wfn_upload(ctx, c0, ldc, nstate, c0d.as<double2>());
+  mc::wfn_upload(ctx, c2, ldc, nstate, c2d.as<double2>());
+  ctx->w_v.ensure(sizeof(double) * nnr1);
+  MC_HIP(hipMemcpyAsync(ctx->w_v.p, vpot, sizeof(double) * nnr1, hipMemcpyHostToDevice, ctx->stream));
+  mc::vpsi_dev(ctx, c0d.as<double2>(), c2d.as<double2>(), f, ctx->w_v.as<double>(), nstate);
+  mc::wfn_download(ctx, c2d.as<double2>(), nstate, c2, ldc);
+  MC_API_END
+}
+
+int mc_timer_enable(mc_ctx* ctx, int on) {
+  MC_API_BEGIN
+  MC_CHECK(ctx, MC_ERR_ARG, "null context");
+  mc::timer_flush(ctx);
+  ctx->timer.enabled = on != 0;
+  MC_API_END
+}
+int mc_timer_reset(mc_ctx* ctx) {
+  MC_API_BEGIN
+  MC_CHECK(ctx, MC_ERR_ARG, "null context");
+  mc::timer_flush(ctx);
+  for (int k = 0; k < MC_KID_COUNT; ++k) { ctx->timer.total_ms[k] = 0; ctx->timer.count[k] = 0; }
+  MC_API_END
+}
+int mc_timer_get(mc_ctx* ctx, int kid, double* total_ms, long long* count) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && kid >= 0 && kid < MC_KID_COUNT, MC_ERR_ARG, "bad kernel id");
+  mc::timer_flush(ctx);
+  if (total_ms) *total_ms = ctx->timer.total_ms[kid];
+  if (count) *count = ctx->timer.count[kid];
+  MC_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+// Internal declarations of the micpmd library (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/micpmd.h"
+
+namespace mc {
+
+// ---------------------------------------------------------------------------
+// error handling: every ABI entry point returns a status and leaves a message
+// in a thread-local buffer (mc_last_error()).
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+void clear_error();
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define MC_HIP(call)                                                           \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      throw ::mc::Error{MC_ERR_HIP, std::string(#call) + ": " +                \
+                                        hipGetErrorString(e_) + " (" +         \
+                                        __FILE__ + ":" +                       \
+                                        std::to_string(__LINE__) + ")"};       \
+  } while (0)
+
+#define MC_CHECK(cond, code, msg)                                              \
+  do {                                                                         \
+    if (!(cond)) throw ::mc::Error{(code), std::string(msg)};                  \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// 1D transform plan: n = prod(rad[0..nf)), executed as mixed-radix Stockham
+// passes in LDS.  Twiddles exp(-2*pi*i*t/n), t in [0,n), live in device memory.
+// ---------------------------------------------------------------------------
+constexpr int kMaxStages = 12;
+struct FftPlan {
+  int n;
+  int nf;
+  int rad[kMaxStages];
+};
+
+FftPlan make_plan(int n);  // throws on unsupported lengths
+
+// Device buffer owned by the context; grows on demand.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t nbytes);
+  void release();
+  ~DevBuf() { release(); }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// Per-kernel timing (HIP events around launches), enabled on request.
+struct KernelTimer {
+  bool enabled = false;
+  std::vector<hipEvent_t> ev;  // pairs (start, stop)
+  std::vector<int> kid;        // kernel id of each pair
+  double total_ms[MC_KID_COUNT] = {0};
+  long long count[MC_KID_COUNT] = {0};
+};
+
+}  // namespace mc
+
+// The context: static layout of the plane-wave set on one GPU plus workspaces.
+struct mc_ctx {
+  // mesh and cell
+  int n1 = 0, n2 = 0, n3 = 0;     // transform lengths
+  int kr1 = 0, kr2 = 0, kr3 = 0;  // padded real-space leading dimensions
+  int ngw = 0;                    // wavefunction plane waves (half sphere)
+  int ldg = 0;                    // device leading dimension of wavefunctions (complex)
+  double tpiba2 = 0.0, omega = 0.0;
+  int geq0 = 1;
+  int device = 0;
+  hipStream_t stream = nullptr;   // borrowed or owned
+  bool own_stream = false;
+
+  // stick layout (see context.hip for the definitions)
+  int n_own = 0;        // owner sticks (hold half-sphere G's); owner 0 is (0,0)
+  int n_slots = 0;      // sticks = 2*n_own - 1
+  int ncols = 0;        // distinct x-frequency columns among sticks
+  std::vector<int> perm;          // stick-order position j -> CPMD index ig
+  std::vector<int> h_own_start;   // n_own+1 run offsets into stick-ordered G
+  std::vector<int> h_col_start;   // ncols+1 offsets into col_slot/col_gy
+  int max_col_len = 0;
+
+  mc::FftPlan plan1, plan2, plan3;
+
+  // device tables
+  mc::DevBuf d_perm;       // int[ngw]     stick-order -> CPMD order
+  mc::DevBuf d_ginfo;      // int[ngw]     (owner << 12) | z index
+  mc::DevBuf d_hg;         // double[ngw]  |G|^2 (tpiba2 units), stick order
+  mc::DevBuf d_own_start;  // int[n_own+1]
+  mc::DevBuf d_col_start;  // int[ncols+1]
+  mc::DevBuf d_col_slot;   // int[n_slots] slot ids grouped by column
+  mc::DevBuf d_col_gy;     // int[n_slots] y frequency index (0..n2-1)
+  mc::DevBuf d_col_gx;     // int[ncols]   x frequency index (0..n1-1)
+  mc::DevBuf d_tw1, d_tw2, d_tw3;  // twiddles, double2[n]
+
+  // workspaces
+  mc::DevBuf w_a;          // sticks:  [pair][slot][z]       complex
+  mc::DevBuf w_b;          // columns: [pair][z][col][y]     complex
+  mc::DevBuf w_c0, w_c2;   // staging of host wavefunctions (device layout)
+  mc::DevBuf w_v;          // staging of host potential / density
+  mc::DevBuf w_red;        // reduction partials
+  mc::DevBuf w_f;          // occupations on device
+
+  int batch_pairs_vpsi = 8;
+  int batch_pairs_rho = 16;
+
+  mc::KernelTimer timer;
+};
+
+namespace mc {
+// launch helpers used by the drivers (timed when ctx->timer.enabled)
+void timer_begin(mc_ctx* ctx, int kid);
+void timer_end(mc_ctx* ctx, int kid);
+void timer_flush(mc_ctx* ctx);
+
+// driver entry points implemented in wfft.hip
+void rhoofr_dev(mc_ctx* ctx, const double2* c0, const double* f_host, int nstate,
+                double* rho, double* ekin, double* csumg, double* csumr);
+void vpsi_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host,
+              const double* vpot, int nstate);
+}  // namespace mc

@@ -1,0 +1,528 @@
+// rhoofr and vpsi on gfx950: the stick-sparse 3D transforms of Gamma state
+// pairs, fused with the wavefunction pack, the |psi|^2 accumulation, the
+// V(r) multiply and the Gamma decode + kinetic term.
+//
+// Per batch of P state pairs the data path through HBM is
+//   zinv : C0 runs (stick order)  -> sticks  A[pair][slot][z]       (pack + z^-1)
+//   yinv : A                       -> columns B[pair][z][col][y]     (y^-1)
+//   xrho : B (all P pairs)         -> rho(r)  (x^-1, |psi|^2, rho read/written once per batch)
+//   xvpsi: B (all P pairs), V(r)   -> B       (x^-1, *V, x^+1; V read once per batch)
+//   yfwd : B                       -> A       (y^+1)
+//   zfwd : A, C0, C2               -> C2      (z^+1, 1/(n1 n2 n3), decode, kinetic, c2 +=)
+// Reference: vpsi src/vpsi_utils.mod.F90:376-675 (one pair per iteration through
+// invfftn/fwfftn, src/fftmain_utils.mod.F90:71-157) and rhoofr
+// src/rhoofr_utils.mod.F90:305-409.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "fft_device.h"
+#include "mc_internal.h"
+
+namespace mc {
+
+struct WParams {
+  int n1, n2, n3, kr1, kr2, kr3;
+  int ngw, ldg, n_own, n_slots, ncols, geq0;
+  double tpiba2, omega;
+  FftPlan p1, p2, p3;
+  const int* ginfo;
+  const double* hg;
+  const int* own_start;
+  const int* col_start;
+  const int* col_slot;
+  const int* col_gy;
+  const int* col_gx;
+  const double2* tw1;
+  const double2* tw2;
+  const double2* tw3;
+};
+
+constexpr int kMaxElem = 16;  // real-space elements per thread kept in registers (x pass)
+
+__device__ __forceinline__ double warp_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// zinv: Gamma pack of a state pair (c1 + i c2 at +G, conj(c1) + i conj(c2) at
+// -G, G=0 once; src/state_utils.mod.F90:171-189) into `lo` owner sticks and
+// their mirrors, inverse z transform, write to A.  Optionally accumulates the
+// per-state norms sum_G w_G |c|^2 (dotp, src/dotp_utils.mod.F90:58-101) and
+// sum_G hg |c|^2 (kin_energy, src/kin_energy_utils.mod.F90:95-103).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_zinv(WParams w, const double2* __restrict__ c0, int nstate,
+                                                   int pair0, double2* __restrict__ A, int lo,
+                                                   double* __restrict__ red) {
+  extern __shared__ double2 smem[];
+  const int n3 = w.n3;
+  double2* tw = smem;
+  double2* buf = smem + n3;
+  const int pb = blockIdx.y;
+  const int is1 = 2 * (pair0 + pb), is2 = is1 + 1;
+  const bool has2 = is2 < nstate;
+  const int o0 = blockIdx.x * lo;
+  const int o1 = min(o0 + lo, w.n_own);
+  const int nl = 2 * (o1 - o0);
+  for (int t = threadIdx.x; t < n3; t += kThreads) tw[t] = w.tw3[t];
+  for (int e = threadIdx.x; e < nl * n3; e += kThreads) buf[e] = make_double2(0.0, 0.0);
+  __syncthreads();
+  const int j0 = w.own_start[o0], j1 = w.own_start[o1];
+  const double2* ca = c0 + (size_t)is1 * w.ldg;
+  const double2* cb = c0 + (size_t)is2 * w.ldg;
+  double dot1 = 0.0, kin1 = 0.0, dot2 = 0.0, kin2 = 0.0;
+  for (int j = j0 + threadIdx.x; j < j1; j += kThreads) {
+    const int gi = w.ginfo[j];
+    const int o = gi >> 12, l = gi & 4095;
+    const int t = o - o0;
+    const double2 a = ca[j];
+    const double2 b = has2 ? cb[j] : make_double2(0.0, 0.0);
+    buf[2 * t * n3 + l] = make_double2(a.x - b.y, a.y + b.x);
+    const bool g0 = (j == 0) && w.geq0;
+    if (!g0) {
+      const int ml = (o == 0) ? 2 * t : 2 * t + 1;
+      buf[ml * n3 + (l == 0 ? 0 : n3 - l)] = make_double2(a.x + b.y, b.x - a.y);
+    }
+    if (red) {
+      const double na = a.x * a.x + a.y * a.y, nb = b.x * b.x + b.y * b.y;
+      const double h = w.hg[j];
+      dot1 += g0 ? a.x * a.x : 2.0 * na;
+      dot2 += g0 ? b.x * b.x : 2.0 * nb;
+      kin1 += h * na;
+      kin2 += h * nb;
+    }
+  }
+  __syncthreads();
+  fft_lines<1>(buf, nl, n3, w.p3, tw);
+  for (int e = threadIdx.x; e < nl * n3; e += kThreads) {
+    const int line = e / n3, z = e - line * n3;
+    const int o = o0 + (line >> 1);
+    const bool mir = line & 1;
+    if (mir && o == 0) continue;
+    const int slot = mir ? w.n_own + o - 1 : o;
+    A[((size_t)pb * w.n_slots + slot) * n3 + z] = buf[e];
+  }
+  if (red) {
+    __shared__ double part[4][kThreads / 64];
+    double v[4] = {warp_sum(dot1), warp_sum(kin1), warp_sum(dot2), warp_sum(kin2)};
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    if (ln == 0)
+      for (int k = 0; k < 4; ++k) part[k][wv] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      double s = 0.0;
+      for (int q = 0; q < kThreads / 64; ++q) s += part[threadIdx.x][q];
+      red[((size_t)pb * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y pass over one x-frequency column and a tile of z planes.
+// DIR=+1: A (sticks of the column) -> B; DIR=-1: B -> A.
+// ---------------------------------------------------------------------------
+template <int DIR>
+__global__ __launch_bounds__(kThreads) void k_ypass(WParams w, double2* __restrict__ A,
+                                                    double2* __restrict__ B, int tz) {
+  extern __shared__ double2 smem[];
+  const int n2 = w.n2, n3 = w.n3;
+  double2* tw = smem;
+  double2* buf = smem + n2;
+  const int c = blockIdx.x;
+  const int z0 = blockIdx.y * tz;
+  const int pb = blockIdx.z;
+  const int nz = min(tz, n3 - z0);
+  const int k0 = w.col_start[c], ncol = w.col_start[c + 1] - k0;
+  for (int t = threadIdx.x; t < n2; t += kThreads) tw[t] = w.tw2[t];
+  const size_t abase = (size_t)pb * w.n_slots * n3;
+  if (DIR > 0) {
+    for (int e = threadIdx.x; e < nz * n2; e += kThreads) buf[e] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < ncol * nz; e += kThreads) {
+      const int k = e / nz, z = e - k * nz;
+      const int slot = w.col_slot[k0 + k], gy = w.col_gy[k0 + k];
+      buf[z * n2 + gy] = A[abase + (size_t)slot * n3 + z0 + z];
+    }
+  } else {
+    for (int e = threadIdx.x; e < nz * n2; e += kThreads) {
+      const int z = e / n2, y = e - z * n2;
+      buf[e] = B[(((size_t)pb * n3 + z0 + z) * w.ncols + c) * n2 + y];
+    }
+  }
+  __syncthreads();
+  fft_lines<DIR>(buf, nz, n2, w.p2, tw);
+  if (DIR > 0) {
+    for (int e = threadIdx.x; e < nz * n2; e += kThreads) {
+      const int z = e / n2, y = e - z * n2;
+      B[(((size_t)pb * n3 + z0 + z) * w.ncols + c) * n2 + y] = buf[e];
+    }
+  } else {
+    for (int e = threadIdx.x; e < ncol * nz; e += kThreads) {
+      const int k = e / nz, z = e - k * nz;
+      const int slot = w.col_slot[k0 + k], gy = w.col_gy[k0 + k];
+      A[abase + (size_t)slot * n3 + z0 + z] = buf[z * n2 + gy];
+    }
+  }
+}
+
+// gather the columns of (z, y-tile) of pair pb into ny zero-padded x lines
+__device__ __forceinline__ void load_xlines(const WParams& w, const double2* __restrict__ B, int pb,
+                                            int z, int y0, int ny, double2* buf) {
+  const int n1 = w.n1;
+  for (int e = threadIdx.x; e < ny * n1; e += kThreads) buf[e] = make_double2(0.0, 0.0);
+  __syncthreads();
+  const size_t bbase = ((size_t)pb * w.n3 + z) * w.ncols;
+  for (int e = threadIdx.x; e < w.ncols * ny; e += kThreads) {
+    const int c = e / ny, y = e - c * ny;
+    buf[y * n1 + w.col_gx[c]] = B[(bbase + c) * w.n2 + y0 + y];
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// xrho: inverse x pass of every pair of the batch for one (z, y-tile) and
+// rho(r) += f1/omega Re(psi)^2 + f2/omega Im(psi)^2 (build_density_sum,
+// src/density_utils.mod.F90:61-80); pairs with f1 = f2 = 0 are skipped.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_xrho(WParams w, const double2* __restrict__ B, int npairs,
+                                                   int pair0, int nstate, const double* __restrict__ f,
+                                                   double* __restrict__ rho, int ty) {
+  extern __shared__ double2 smem[];
+  const int n1 = w.n1;
+  double2* tw = smem;
+  double2* buf = smem + n1;
+  const int y0 = blockIdx.x * ty, z = blockIdx.y;
+  const int ny = min(ty, w.n2 - y0);
+  const int ne = ny * n1;
+  for (int t = threadIdx.x; t < n1; t += kThreads) tw[t] = w.tw1[t];
+  double acc[kMaxElem];
+#pragma unroll
+  for (int k = 0; k < kMaxElem; ++k) acc[k] = 0.0;
+  const double rom = 1.0 / w.omega;
+  for (int pb = 0; pb < npairs; ++pb) {
+    const int is1 = 2 * (pair0 + pb), is2 = is1 + 1;
+    const double fa = f[is1];
+    const double fb = is2 < nstate ? f[is2] : 0.0;
+    if (fa == 0.0 && fb == 0.0) continue;
+    const double c3 = fa * rom, c4 = fb * rom;
+    load_xlines(w, B, pb, z, y0, ny, buf);
+    fft_lines<1>(buf, ny, n1, w.p1, tw);
+#pragma unroll
+    for (int k = 0; k < kMaxElem; ++k) {
+      const int e = threadIdx.x + k * kThreads;
+      if (e < ne) {
+        const double2 v = buf[e];
+        acc[k] = acc[k] + c3 * v.x * v.x + c4 * v.y * v.y;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < kMaxElem; ++k) {
+    const int e = threadIdx.x + k * kThreads;
+    if (e < ne) {
+      const int y = e / n1, x = e - y * n1;
+      const size_t idx = (size_t)x + (size_t)w.kr1 * ((size_t)(y0 + y) + (size_t)w.kr2 * z);
+      rho[idx] += acc[k];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// xvpsi: inverse x pass, psi(r) *= V(r), forward x pass, for every pair of the
+// batch (vpsi:485-493 between invfftn and fwfftn); V is read once per batch.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_xvpsi(WParams w, double2* __restrict__ B, int npairs,
+                                                    const double* __restrict__ vpot, int ty) {
+  extern __shared__ double2 smem[];
+  const int n1 = w.n1;
+  double2* tw = smem;
+  double2* buf = smem + n1;
+  const int y0 = blockIdx.x * ty, z = blockIdx.y;
+  const int ny = min(ty, w.n2 - y0);
+  const int ne = ny * n1;
+  for (int t = threadIdx.x; t < n1; t += kThreads) tw[t] = w.tw1[t];
+  double vr[kMaxElem];
+#pragma unroll
+  for (int k = 0; k < kMaxElem; ++k) {
+    const int e = threadIdx.x + k * kThreads;
+    vr[k] = 0.0;
+    if (e < ne) {
+      const int y = e / n1, x = e - y * n1;
+      vr[k] = vpot[(size_t)x + (size_t)w.kr1 * ((size_t)(y0 + y) + (size_t)w.kr2 * z)];
+    }
+  }
+  for (int pb = 0; pb < npairs; ++pb) {
+    load_xlines(w, B, pb, z, y0, ny, buf);
+    fft_lines<1>(buf, ny, n1, w.p1, tw);
+#pragma unroll
+    for (int k = 0; k < kMaxElem; ++k) {
+      const int e = threadIdx.x + k * kThreads;
+      if (e < ne) {
+        const double2 v = buf[e];
+        buf[e] = make_double2(vr[k] * v.x, vr[k] * v.y);
+      }
+    }
+    __syncthreads();
+    fft_lines<-1>(buf, ny, n1, w.p1, tw);
+    const size_t bbase = ((size_t)pb * w.n3 + z) * w.ncols;
+    for (int e = threadIdx.x; e < w.ncols * ny; e += kThreads) {
+      const int c = e / ny, y = e - c * ny;
+      B[(bbase + c) * w.n2 + y0 + y] = buf[y * n1 + w.col_gx[c]];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// zfwd: forward z pass of owner sticks + mirrors, scale 1/(n1 n2 n3), Gamma
+// decode fp = psi(G) + psi(-G), fm = psi(G) - psi(-G) and
+//   c2(:,is1) += -fi   (tpiba2 hg c0(:,is1) + (Re fp, Im fm))
+//   c2(:,is2) += -fip1 (tpiba2 hg c0(:,is2) + (Im fp, -Re fm))
+// with fi = f/2 (f = 0 -> 1)  (vpsi_utils.mod.F90:627-672, add_wfn :717).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_zfwd(WParams w, const double2* __restrict__ A,
+                                                   const double2* __restrict__ c0, double2* __restrict__ c2,
+                                                   const double* __restrict__ f, int nstate, int pair0,
+                                                   int lo, double scale) {
+  extern __shared__ double2 smem[];
+  const int n3 = w.n3;
+  double2* tw = smem;
+  double2* buf = smem + n3;
+  const int pb = blockIdx.y;
+  const int is1 = 2 * (pair0 + pb), is2 = is1 + 1;
+  const bool has2 = is2 < nstate;
+  const int o0 = blockIdx.x * lo;
+  const int o1 = min(o0 + lo, w.n_own);
+  const int nl = 2 * (o1 - o0);
+  for (int t = threadIdx.x; t < n3; t += kThreads) tw[t] = w.tw3[t];
+  for (int e = threadIdx.x; e < nl * n3; e += kThreads) {
+    const int line = e / n3, z = e - line * n3;
+    const int o = o0 + (line >> 1);
+    const bool mir = line & 1;
+    double2 v = make_double2(0.0, 0.0);
+    if (!(mir && o == 0)) {
+      const int slot = mir ? w.n_own + o - 1 : o;
+      v = A[((size_t)pb * w.n_slots + slot) * n3 + z];
+    }
+    buf[e] = v;
+  }
+  __syncthreads();
+  fft_lines<-1>(buf, nl, n3, w.p3, tw);
+  double fi = f[is1] * 0.5;
+  if (fi == 0.0) fi = 1.0;
+  double fip1 = has2 ? f[is2] * 0.5 : 0.0;
+  if (fip1 == 0.0) fip1 = 1.0;
+  const int j0 = w.own_start[o0], j1 = w.own_start[o1];
+  const double2* ca = c0 + (size_t)is1 * w.ldg;
+  const double2* cb = c0 + (size_t)is2 * w.ldg;
+  double2* da = c2 + (size_t)is1 * w.ldg;
+  double2* db = c2 + (size_t)is2 * w.ldg;
+  for (int j = j0 + threadIdx.x; j < j1; j += kThreads) {
+    const int gi = w.ginfo[j];
+    const int o = gi >> 12, l = gi & 4095;
+    const int t = o - o0;
+    const double2 a = buf[2 * t * n3 + l];
+    const int ml = (o == 0) ? 2 * t : 2 * t + 1;
+    const double2 b = buf[ml * n3 + (l == 0 ? 0 : n3 - l)];
+    const double2 psin = make_double2(a.x * scale, a.y * scale);
+    const double2 psii = make_double2(b.x * scale, b.y * scale);
+    const double2 fp = cadd(psin, psii);
+    const double2 fm = csub(psin, psii);
+    const double g2 = w.tpiba2 * w.hg[j];
+    const double2 x1 = ca[j];
+    double2 r1 = da[j];
+    r1.x += -fi * (g2 * x1.x + fp.x);
+    r1.y += -fi * (g2 * x1.y + fm.y);
+    da[j] = r1;
+    if (has2) {
+      const double2 x2 = cb[j];
+      double2 r2 = db[j];
+      r2.x += -fip1 * (g2 * x2.x + fp.y);
+      r2.y += -fip1 * (g2 * x2.y - fm.x);
+      db[j] = r2;
+    }
+  }
+}
+
+__global__ void k_fill(double* __restrict__ x, size_t n, double v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+// sum of x[0..n) into out[blockIdx.x]; fixed order for reproducibility
+__global__ __launch_bounds__(kThreads) void k_sum_partial(const double* __restrict__ x, size_t n,
+                                                          double* __restrict__ out) {
+  double s = 0.0;
+  for (size_t i = blockIdx.x * (size_t)kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads)
+    s += x[i];
+  s = warp_sum(s);
+  __shared__ double part[kThreads / 64];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < kThreads / 64; ++q) t += part[q];
+    out[blockIdx.x] = t;
+  }
+}
+
+// per-pair sums of the zinv partials: red[(pair*gx + b)*4 + k] -> out[pair*4 + k]
+__global__ void k_reduce_pairs(const double* __restrict__ red, int gx, double* __restrict__ out) {
+  const int pair = blockIdx.x;
+  if (threadIdx.x < 4) {
+    double s = 0.0;
+    for (int b = 0; b < gx; ++b) s += red[((size_t)pair * gx + b) * 4 + threadIdx.x];
+    out[(size_t)pair * 4 + threadIdx.x] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------
+static WParams make_params(mc_ctx* ctx) {
+  WParams w;
+  w.n1 = ctx->n1; w.n2 = ctx->n2; w.n3 = ctx->n3;
+  w.kr1 = ctx->kr1; w.kr2 = ctx->kr2; w.kr3 = ctx->kr3;
+  w.ngw = ctx->ngw; w.ldg = ctx->ldg; w.n_own = ctx->n_own; w.n_slots = ctx->n_slots;
+  w.ncols = ctx->ncols; w.geq0 = ctx->geq0;
+  w.tpiba2 = ctx->tpiba2; w.omega = ctx->omega;
+  w.p1 = ctx->plan1; w.p2 = ctx->plan2; w.p3 = ctx->plan3;
+  w.ginfo = ctx->d_ginfo.as<int>();
+  w.hg = ctx->d_hg.as<double>();
+  w.own_start = ctx->d_own_start.as<int>();
+  w.col_start = ctx->d_col_start.as<int>();
+  w.col_slot = ctx->d_col_slot.as<int>();
+  w.col_gy = ctx->d_col_gy.as<int>();
+  w.col_gx = ctx->d_col_gx.as<int>();
+  w.tw1 = ctx->d_tw1.as<double2>();
+  w.tw2 = ctx->d_tw2.as<double2>();
+  w.tw3 = ctx->d_tw3.as<double2>();
+  return w;
+}
+
+static constexpr int kLdsBudget = 65536;
+
+// lines per workgroup for a transform of length n (register, LDS and 16-line caps)
+static int lines_for(const FftPlan& pl) {
+  int l = max_lines(pl);
+  l = std::min(l, kLdsBudget / (int)(sizeof(double2) * pl.n) - 1);
+  l = std::min(l, 16);
+  MC_CHECK(l >= 1, MC_ERR_UNSUPPORTED, "transform length too large for one workgroup");
+  return l;
+}
+
+struct Shapes {
+  int lo;   // owner sticks per zinv/zfwd workgroup
+  int tz;   // z planes per y-pass workgroup
+  int ty;   // x lines per x-pass workgroup
+  int gx;   // zinv/zfwd workgroups per pair
+};
+
+static Shapes shapes(mc_ctx* ctx) {
+  Shapes s;
+  MC_CHECK(lines_for(ctx->plan3) >= 2, MC_ERR_UNSUPPORTED, "nr3 too large for the z pass");
+  s.lo = lines_for(ctx->plan3) / 2;
+  s.tz = lines_for(ctx->plan2);
+  s.ty = std::min(lines_for(ctx->plan1), (kMaxElem * kThreads) / ctx->n1);
+  MC_CHECK(s.ty >= 1, MC_ERR_UNSUPPORTED, "nr1 too large for the x pass");
+  s.gx = (ctx->n_own + s.lo - 1) / s.lo;
+  return s;
+}
+
+static void ensure_workspace(mc_ctx* ctx, int P) {
+  ctx->w_a.ensure(sizeof(double2) * (size_t)P * ctx->n_slots * ctx->n3);
+  ctx->w_b.ensure(sizeof(double2) * (size_t)P * ctx->n3 * ctx->ncols * ctx->n2);
+}
+
+static void upload_f(mc_ctx* ctx, const double* f_host, int nstate) {
+  ctx->w_f.ensure(sizeof(double) * std::max(nstate, 1));
+  if (nstate)
+    MC_HIP(hipMemcpyAsync(ctx->w_f.p, f_host, sizeof(double) * nstate, hipMemcpyHostToDevice, ctx->stream));
+}
+
+#define LAUNCH(kid, kern, grid, block, shm, ...)                                  \
+  do {                                                                          \
+    timer_begin(ctx, kid);                                                      \
+    hipLaunchKernelGGL(kern, grid, block, shm, ctx->stream, __VA_ARGS__);       \
+    MC_HIP(hipGetLastError());                                                  \
+    timer_end(ctx, kid);                                                        \
+  } while (0)
+
+void rhoofr_dev(mc_ctx* ctx, const double2* c0, const double* f_host, int nstate, double* rho,
+                double* ekin, double* csumg, double* csumr) {
+  const size_t nnr1 = (size_t)ctx->kr1 * ctx->kr2 * ctx->kr3;
+  const WParams w = make_params(ctx);
+  const Shapes s = shapes(ctx);
+  const int npairs = (nstate + 1) / 2;
+  const int P = std::max(1, std::min(ctx->batch_pairs_rho, npairs));
+  ensure_workspace(ctx, P);
+  upload_f(ctx, f_host, nstate);
+  const int nblk_sum = 1024;
+  ctx->w_red.ensure(sizeof(double) * ((size_t)std::max(npairs, 1) * s.gx * 4 + (size_t)npairs * 4 + nblk_sum));
+  double* red = ctx->w_red.as<double>();
+  double* pair_sums = red + (size_t)std::max(npairs, 1) * s.gx * 4;
+  double* rsum_part = pair_sums + (size_t)npairs * 4;
+  LAUNCH(MC_KID_MISC, k_fill, dim3(1024), dim3(256), 0, rho, nnr1, 0.0);
+  const size_t shz = sizeof(double2) * (size_t)ctx->n3 * (1 + 2 * s.lo);
+  const size_t shy = sizeof(double2) * (size_t)ctx->n2 * (1 + s.tz);
+  const size_t shx = sizeof(double2) * (size_t)ctx->n1 * (1 + s.ty);
+  for (int pair0 = 0; pair0 < npairs; pair0 += P) {
+    const int np = std::min(P, npairs - pair0);
+    LAUNCH(MC_KID_ZINV, k_zinv, dim3(s.gx, np), dim3(kThreads), shz, w, c0, nstate, pair0,
+           ctx->w_a.as<double2>(), s.lo, red + (size_t)pair0 * s.gx * 4);
+    LAUNCH(MC_KID_YINV, k_ypass<1>, dim3(ctx->ncols, (ctx->n3 + s.tz - 1) / s.tz, np), dim3(kThreads), shy, w,
+           ctx->w_a.as<double2>(), ctx->w_b.as<double2>(), s.tz);
+    LAUNCH(MC_KID_XRHO, k_xrho, dim3((ctx->n2 + s.ty - 1) / s.ty, ctx->n3), dim3(kThreads), shx, w,
+           ctx->w_b.as<double2>(), np, pair0, nstate, ctx->w_f.as<double>(), rho, s.ty);
+  }
+  if (npairs > 0)
+    LAUNCH(MC_KID_MISC, k_reduce_pairs, dim3(npairs), dim3(64), 0, red, s.gx, pair_sums);
+  LAUNCH(MC_KID_MISC, k_sum_partial, dim3(nblk_sum), dim3(kThreads), 0, rho, nnr1, rsum_part);
+  std::vector<double> hs((size_t)npairs * 4 + nblk_sum);
+  MC_HIP(hipMemcpyAsync(hs.data(), pair_sums, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, ctx->stream));
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  double sg = 0.0, sk = 0.0, sr = 0.0;
+  for (int p = 0; p < npairs; ++p) {
+    const int is1 = 2 * p, is2 = is1 + 1;
+    if (f_host[is1] != 0.0) { sg += f_host[is1] * hs[4 * p + 0]; sk += f_host[is1] * hs[4 * p + 1]; }
+    if (is2 < nstate && f_host[is2] != 0.0) { sg += f_host[is2] * hs[4 * p + 2]; sk += f_host[is2] * hs[4 * p + 3]; }
+  }
+  for (int b = 0; b < nblk_sum; ++b) sr += hs[(size_t)npairs * 4 + b];
+  if (ekin) *ekin = sk * ctx->tpiba2;
+  if (csumg) *csumg = sg;
+  if (csumr) *csumr = sr * ctx->omega / ((double)ctx->n1 * ctx->n2 * ctx->n3);
+}
+
+void vpsi_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, const double* vpot,
+              int nstate) {
+  const WParams w = make_params(ctx);
+  const Shapes s = shapes(ctx);
+  const int npairs = (nstate + 1) / 2;
+  if (npairs == 0) return;
+  const int P = std::max(1, std::min(ctx->batch_pairs_vpsi, npairs));
+  ensure_workspace(ctx, P);
+  upload_f(ctx, f_host, nstate);
+  const size_t shz = sizeof(double2) * (size_t)ctx->n3 * (1 + 2 * s.lo);
+  const size_t shy = sizeof(double2) * (size_t)ctx->n2 * (1 + s.tz);
+  const size_t shx = sizeof(double2) * (size_t)ctx->n1 * (1 + s.ty);
+  const double scale = 1.0 / ((double)ctx->n1 * ctx->n2 * ctx->n3);
+  for (int pair0 = 0; pair0 < npairs; pair0 += P) {
+    const int np = std::min(P, npairs - pair0);
+    LAUNCH(MC_KID_ZINV, k_zinv, dim3(s.gx, np), dim3(kThreads), shz, w, c0, nstate, pair0,
+           ctx->w_a.as<double2>(), s.lo, (double*)nullptr);
+    LAUNCH(MC_KID_YINV, k_ypass<1>, dim3(ctx->ncols, (ctx->n3 + s.tz - 1) / s.tz, np), dim3(kThreads), shy, w,
+           ctx->w_a.as<double2>(), ctx->w_b.as<double2>(), s.tz);
+    LAUNCH(MC_KID_XVPSI, k_xvpsi, dim3((ctx->n2 + s.ty - 1) / s.ty, ctx->n3), dim3(kThreads), shx, w,
+           ctx->w_b.as<double2>(), np, vpot, s.ty);
+    LAUNCH(MC_KID_YFWD, k_ypass<-1>, dim3(ctx->ncols, (ctx->n3 + s.tz - 1) / s.tz, np), dim3(kThreads), shy, w,
+           ctx->w_a.as<double2>(), ctx->w_b.as<double2>(), s.tz);
+    LAUNCH(MC_KID_ZFWD, k_zfwd, dim3(s.gx, np), dim3(kThreads), shz, w, ctx->w_a.as<double2>(), c0, c2,
+           ctx->w_f.as<double>(), nstate, pair0, s.lo, scale);
+  }
+}
+
+}  // namespace mc

@@ -1,0 +1,143 @@
+/*
+ * micpmd -- MI355X-native plane-wave inner loop for CPMD, C ABI.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes and
+ * returns an int status (MC_OK == 0).  On failure mc_last_error() returns a
+ * thread-local message; the Fortran shim turns a non-zero status into
+ * CPMD's stopgm(procedureN, msg, __LINE__, __FILE__) (the convention of
+ * src/cuda_utils.mod.F90:703-710).
+ *
+ * Conventions (identical to CPMD's serial Gamma-point data):
+ *   - complex arrays are interleaved (re,im) float64, i.e. COMPLEX(real_8),
+ *     column-major with a leading dimension in complex elements;
+ *   - wavefunction rows are CPMD's G order (cppt::inyh / cppt::hg), the first
+ *     row being G=0 when geq0;
+ *   - real-space arrays have CPMD's layout v(ix + kr1*(iy + kr2*iz)),
+ *     nnr1 = kr1*kr2*kr3 (fpar%kr1s.. from leadim, src/loadpa_utils.mod.F90:508-525).
+ *
+ * Host-pointer entry points ("drop-in") copy their inputs to the device,
+ * compute, and copy results back before returning.  *_dev entry points take
+ * device pointers to wavefunctions in the library's device layout (rows
+ * permuted into stick order, leading dimension mc_wfn_ld()); they run
+ * asynchronously on the context's stream except where a host result
+ * (energies, matrices) is returned.
+ */
+#ifndef MICPMD_H
+#define MICPMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MC_OK = 0,
+  MC_ERR_ARG = 1,          /* invalid argument / shape                      */
+  MC_ERR_HIP = 2,          /* HIP runtime error                             */
+  MC_ERR_NOTPOSDEF = 3,    /* Cholesky: leading minor not positive definite */
+  MC_ERR_NOCONV = 4,       /* iteration did not converge                    */
+  MC_ERR_UNSUPPORTED = 5,  /* transform length or option not supported      */
+  MC_ERR_INTERNAL = 6
+};
+
+/* kernel ids for the optional per-kernel timer (mc_timer_*) */
+enum {
+  MC_KID_ZINV = 0,   /* pack + inverse z pass over sticks          */
+  MC_KID_YINV = 1,   /* inverse y pass over x-frequency columns    */
+  MC_KID_XRHO = 2,   /* inverse x pass + |psi|^2 accumulation      */
+  MC_KID_XVPSI = 3,  /* inverse x pass * V(r) * forward x pass     */
+  MC_KID_YFWD = 4,   /* forward y pass                             */
+  MC_KID_ZFWD = 5,   /* forward z pass + Gamma decode + kinetic    */
+  MC_KID_MISC = 6,   /* reductions, permutes, small kernels        */
+  MC_KID_GEMM = 7,   /* FP64 MFMA contractions                     */
+  MC_KID_COUNT = 8
+};
+
+typedef struct mc_ctx mc_ctx;
+
+/* Layout of the plane-wave set, as CPMD holds it after rggen/loadpa/fftprp
+ * (serial run).  Replaces the module state read by vpsi/rhoofr through USE
+ * association: spar%nr1s.., fpar%kr1s.., ncpw%ngw, cppt::inyh, cppt::hg,
+ * parm%tpiba2, parm%omega, geq0 (src/fftprp_utils.mod.F90:116-285,
+ * src/rggen_utils.mod.F90:120-130). */
+typedef struct {
+  int nr1, nr2, nr3;    /* FFT mesh                                        */
+  int kr1, kr2, kr3;    /* leading dimensions of real-space arrays         */
+  int ngw;              /* number of wavefunction G vectors (half sphere)  */
+  const int* inyh;      /* (3, ngw) 1-based centred indices nr/2+1+m       */
+  const double* hg;     /* (ngw) |G|^2 in (2 pi/alat)^2                    */
+  double tpiba2;        /* (2 pi/alat)^2                                   */
+  double omega;         /* cell volume, bohr^3                             */
+  int geq0;             /* 1 if G=0 is row 1                               */
+} mc_layout;
+
+const char* mc_last_error(void);
+int mc_version(void);
+int mc_device_count(int* count);
+
+/* Host-only: geometry of the stick layout the library would build
+ * (owner sticks, sticks, x-frequency columns).  Needs no GPU. */
+int mc_layout_stats(const mc_layout* lay, int* n_sticks, int* n_columns, int* n_owner);
+
+/* Context: uploads index maps and twiddles (CPMD: fftprp_default_init +
+ * setfftn upload, src/fftprp_utils.mod.F90:116-430, src/fftnew_utils.mod.F90:53-175). */
+int mc_ctx_create(const mc_layout* lay, mc_ctx** out);
+int mc_ctx_destroy(mc_ctx* ctx);
+/* Use a caller-owned hipStream_t (e.g. torch's current stream); NULL = own stream. */
+int mc_ctx_set_stream(mc_ctx* ctx, void* hip_stream);
+int mc_ctx_synchronize(mc_ctx* ctx);
+/* Pairs of states transformed per launch in rhoofr / vpsi (workspace size). */
+int mc_ctx_set_batch(mc_ctx* ctx, int pairs_rhoofr, int pairs_vpsi);
+size_t mc_wfn_ld(const mc_ctx* ctx);
+
+/* Device memory helpers (hipMalloc / hipFree / hipMemcpy). */
+int mc_dev_alloc(size_t bytes, void** ptr);
+int mc_dev_free(void* ptr);
+int mc_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int mc_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int mc_memset_d(void* dst, int value, size_t bytes);
+
+/* Zero nstate device wavefunctions (stream-ordered; CPMD: zeroing(c2),
+ * src/forces_driver.mod.F90:175). */
+int mc_wfn_zero(mc_ctx* ctx, void* c_dev, int nstate);
+
+/* Wavefunctions host (CPMD order, leading dim ldc) <-> device layout. */
+int mc_wfn_upload(mc_ctx* ctx, const double* c_host, int ldc, int nstate, void* c_dev);
+int mc_wfn_download(mc_ctx* ctx, const void* c_dev, int nstate, double* c_host, int ldc);
+
+/* rhoofr: rhoe(nnr1) = sum_i f_i/omega |psi_i(r)|^2 over Gamma state pairs,
+ * ekin = tpiba2 sum_i f_i sum_G hg |c_iG|^2, csumg = sum_i f_i <c_i|c_i>,
+ * csumr = omega/(nr1 nr2 nr3) sum_r rhoe.
+ * Replaces rhoofr(c0,rhoe,psi,nstate), src/rhoofr_utils.mod.F90:122-644
+ * (closed-shell, no LSD/LSE/USPP branches), with kin_energy
+ * src/kin_energy_utils.mod.F90:28-110 and build_density_sum
+ * src/density_utils.mod.F90:61-80.  Pairs with f1 = f2 = 0 are skipped
+ * (rhoofr:313-316).  The charge-sum check (rhoofr:625-635) is left to the
+ * caller, which receives csumg and csumr. */
+int mc_rhoofr(mc_ctx* ctx, const double* c0, int ldc, const double* f, int nstate,
+              double* rhoe, double* ekin, double* csumg, double* csumr);
+int mc_rhoofr_dev(mc_ctx* ctx, const void* c0_dev, const double* f, int nstate,
+                  void* rhoe_dev, double* ekin, double* csumg, double* csumr);
+
+/* vpsi: c2(:,i) += -(f_i/2)(tpiba2 hg c0(:,i) + 2 P_G[V psi_i]) with f=0 -> 1
+ * (Gamma, closed shell, no LSE/CDFT/TDG/MGGA branches).
+ * Replaces vpsi(c0,c2,f,vpot,psi,nstate,ikind,ispin,redist_c2),
+ * src/vpsi_utils.mod.F90:120-732 (pack :400-435, inverse FFT :443, V :485-493,
+ * forward FFT :552, decode :647-672, add_wfn :717). */
+int mc_vpsi(mc_ctx* ctx, const double* c0, int ldc, double* c2, const double* f,
+            const double* vpot, int nstate);
+int mc_vpsi_dev(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const double* f,
+                const void* vpot_dev, int nstate);
+
+/* Optional per-kernel timing with HIP events on the context stream. */
+int mc_timer_enable(mc_ctx* ctx, int on);
+int mc_timer_reset(mc_ctx* ctx);
+/* Synchronizes and returns total milliseconds and launch count of kernel kid. */
+int mc_timer_get(mc_ctx* ctx, int kid, double* total_ms, long long* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MICPMD_H */

@@ -1,0 +1,85 @@
+"""Layout and G-vector set against CPMD's own printed values.
+
+Known answers (mesh, NGW, NHG, x-ray sticks, z extent) are those CPMD 4.3
+prints / builds for these cells (serial run, default FFT table); see the
+geometry table of the survey of the reference.  The stick geometry is also
+pinned by brute force (counting distinct (m1,m2) projections of the full
+sphere)."""
+import numpy as np
+import pytest
+
+from cpmd_amd import system as S
+
+KNOWN = {
+    # name: (mesh, ngw, nhg or None, sticks, columns)
+    "cfg1": (54, 3625, 29447, 457, 25),
+    "cfg2": (128, 68338, 546832, 3205, 63),
+    "cfg4": (192, 198204, 1583966, 6525, 91),
+    "cfg5": (240, 336419, 2690913, 9281, 109),
+}
+
+
+@pytest.mark.parametrize("name", ["cfg1", "cfg2", "cfg4", "cfg5"])
+def test_mesh_and_counts(name):
+    c = S.CONFIGS[name]
+    cell = S.cubic_cell(c["a"], c["ecut"])
+    mesh, ngw, nhg, sticks, cols = KNOWN[name]
+    assert cell.nr == (mesh, mesh, mesh)
+    assert cell.kr == (mesh + 1,) * 3
+    gs = S.gvectors(cell, with_density=name in ("cfg1", "cfg2"))
+    assert gs.ngw == ngw
+    if name in ("cfg1", "cfg2"):
+        assert gs.nhg == nhg
+    # brute-force stick geometry of the full wavefunction sphere
+    m = gs.miller[:gs.ngw]
+    full = np.vstack([m, -m])
+    proj = {(int(a), int(b)) for a, b in full[:, :2]}
+    assert len(proj) == sticks
+    assert len({p[0] for p in proj}) == cols
+
+
+def test_fftchk_table():
+    assert S.fftchk(49) == 54          # 50 is not in CPMD's default table
+    assert S.fftchk(183) == 192
+    assert S.fftchk(218) == 240
+    assert S.fftchk(128) == 128
+    assert S.fftchk(17, 1) == 18
+    assert S.fftchk(100, -1) == 96
+    assert S.leadim(54) == 55 and S.leadim(55) == 55
+
+
+def test_gset_properties():
+    cell = S.cubic_cell(12.0, 40.0)
+    gs = S.gvectors(cell, with_density=True)
+    m = gs.miller
+    # G = 0 first, half sphere, sorted by |G|^2, wavefunction set first
+    assert tuple(m[0]) == (0, 0, 0)
+    half = (m[:, 0] > 0) | ((m[:, 0] == 0) & (m[:, 1] > 0)) | ((m[:, 0] == 0) & (m[:, 1] == 0) & (m[:, 2] >= 0))
+    assert half.all()
+    assert np.all(np.diff(gs.hg) >= -1e-12)
+    assert np.all(gs.hg[:gs.ngw] < cell.gcutw) and np.all(gs.hg[gs.ngw:] >= cell.gcutw)
+    # no duplicates and no +-G pair both present
+    keys = {tuple(r) for r in m}
+    assert len(keys) == len(m)
+    assert not any((-a, -b, -c) in keys for (a, b, c) in keys if (a, b, c) != (0, 0, 0))
+    # exact |G|^2 of the cubic lattice is integer
+    assert np.allclose(gs.hg, (m.astype(float) ** 2).sum(1))
+    # shell tie-break: within the first shell the latest generated vector sorts first
+    assert [tuple(r) for r in m[1:4]] == [(1, 0, 0), (0, 1, 0), (0, 0, 1)]
+
+
+def test_initial_coefficients_orthonormal():
+    sysm = S.make_system(12.0, 8, 40.0, nstate=16)
+    c = S.initial_coefficients(sysm)
+    s = S.dotp_gamma(c, c)
+    assert np.abs(s - np.eye(16)).max() < 1e-12
+    assert np.all(c[0].imag == 0)
+
+
+def test_atom_placement():
+    tau = S.place_atoms(12.0, 8, seed=3)
+    d = tau[:, None, :] - tau[None, :, :]
+    d -= 12.0 * np.round(d / 12.0)
+    r = np.sqrt((d ** 2).sum(-1)) + np.eye(8) * 100
+    assert r.min() >= 3.0
+    assert np.all((tau >= 0) & (tau < 12.0))
