@@ -245,12 +245,20 @@ CONFIGS = {
 }
 
 
+def _real_rows(a: np.ndarray) -> np.ndarray:
+    """(ngw, n) complex -> (n, 2*ngw) float64 rows (re, im interleaved), no copy
+    when `a` is Fortran-ordered."""
+    return np.ascontiguousarray(np.asfortranarray(a, dtype=np.complex128).T).view(np.float64)
+
+
 def dotp_gamma(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     """Gamma-point real inner product of half-sphere coefficient columns:
     Re(a0)*Re(b0) + 2*sum_{G!=0} Re(conj(a)*b)  (src/dotp_utils.mod.F90:58-101;
     G=0 is the first row)."""
-    r = 2.0 * (a.real.T @ b.real + a.imag.T @ b.imag)
-    r -= np.outer(a[0].real, b[0].real)
+    ra = _real_rows(a)
+    rb = ra if b is a else _real_rows(b)
+    r = 2.0 * (ra @ rb.T)
+    r -= np.outer(ra[:, 0], rb[:, 0])
     return r
 
 
@@ -266,12 +274,15 @@ def initial_coefficients(sys: SyntheticSystem, seed: int = 4321, nstate: int | N
     amp = np.exp(-g2 / 8.0)
     c = (rng.standard_normal((ngw, n)) + 1j * rng.standard_normal((ngw, n))) * amp[:, None]
     c[0, :] = c[0, :].real
+    c = np.asfortranarray(c)
     if orthonormalize:
         s = dotp_gamma(c, c)
         w, z = np.linalg.eigh(s)
-        c = c @ (z @ np.diag(1.0 / np.sqrt(w)) @ z.T)
+        a = z @ np.diag(1.0 / np.sqrt(w)) @ z.T
+        r = a.T @ _real_rows(c)                    # rows of the real view: C <- C A
+        c = np.asfortranarray(r.view(np.complex128).T)
         c[0, :] = c[0, :].real
-    return np.asfortranarray(c)
+    return c
 
 
 def gth_local_potential_g(sys: SyntheticSystem, hg: np.ndarray) -> np.ndarray:
