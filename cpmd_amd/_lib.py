@@ -58,6 +58,24 @@ SIGNATURES = {
     "mc_rhoofr_dev": (C.c_int, [_vp, _vp, _dp, C.c_int, _vp, _dp, _dp, _dp]),
     "mc_vpsi": (C.c_int, [_vp, _dp, C.c_int, _dp, _dp, _dp, C.c_int]),
     "mc_vpsi_dev": (C.c_int, [_vp, _vp, _vp, _dp, _vp, C.c_int]),
+    "mc_ovlap": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, C.c_int]),
+    "mc_ovlap_dev": (C.c_int, [_vp, C.c_int, _dp, _vp, _vp]),
+    "mc_rgs": (C.c_int, [_vp, _dp, C.c_int, C.c_int, _dp]),
+    "mc_rgs_dev": (C.c_int, [_vp, _vp, C.c_int, _dp]),
+    "mc_lowdin": (C.c_int, [_vp, _dp, C.c_int, C.c_int]),
+    "mc_lowdin_dev": (C.c_int, [_vp, _vp, C.c_int, _ip]),
+    "mc_rotate": (C.c_int, [_vp, C.c_double, _dp, C.c_double, _dp, C.c_int, _dp, C.c_int]),
+    "mc_rotate_dev": (C.c_int, [_vp, C.c_double, _vp, C.c_double, _vp, _dp, C.c_int]),
+    "mc_lagrange": (C.c_int, [_vp, _dp, _dp, C.c_int, _dp, C.c_int]),
+    "mc_lagrange_dev": (C.c_int, [_vp, _vp, _vp, _dp, C.c_int]),
+    "mc_rortog": (C.c_int, [_vp, _dp, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_double, C.c_int, _ip]),
+    "mc_rortog_dev": (C.c_int, [_vp, _vp, _vp, _dp, C.c_int, C.c_int, C.c_double, C.c_int, _ip]),
+    "mc_set_projectors": (C.c_int, [_vp, C.c_int, _ip, C.c_int, _ip, _ip, _dp, _dp, C.c_int]),
+    "mc_set_eigr": (C.c_int, [_vp, _dp, C.c_int, C.c_int]),
+    "mc_rnlsm1": (C.c_int, [_vp, _dp, C.c_int, C.c_int, _dp]),
+    "mc_rnlsm1_dev": (C.c_int, [_vp, _vp, C.c_int, _dp]),
+    "mc_fnonloc": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp]),
+    "mc_fnonloc_dev": (C.c_int, [_vp, _vp, _dp, C.c_int, _dp]),
     "mc_timer_enable": (C.c_int, [_vp, C.c_int]),
     "mc_timer_reset": (C.c_int, [_vp]),
     "mc_timer_get": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_longlong)]),

@@ -271,21 +271,6 @@ void timer_flush(mc_ctx* ctx) {
 // ===========================================================================
 // C ABI
 // ===========================================================================
-#define MC_API_BEGIN \
-  try {              \
-    mc::clear_error();
-#define MC_API_END                                   \
-  }                                                  \
-  catch (const mc::Error& e) {                       \
-    mc::set_error(e.msg);                            \
-    return e.code;                                   \
-  }                                                  \
-  catch (const std::exception& e) {                  \
-    mc::set_error(std::string("internal: ") + e.what()); \
-    return MC_ERR_INTERNAL;                          \
-  }                                                  \
-  return MC_OK;
-
 extern "C" {
 
 const char* mc_last_error(void) { return mc::g_last_error.c_str(); }

@@ -34,6 +34,21 @@ struct Error {
                                         std::to_string(__LINE__) + ")"};       \
   } while (0)
 
+#define MC_API_BEGIN \
+  try {              \
+    mc::clear_error();
+#define MC_API_END                                   \
+  }                                                  \
+  catch (const mc::Error& e) {                       \
+    mc::set_error(e.msg);                            \
+    return e.code;                                   \
+  }                                                  \
+  catch (const std::exception& e) {                  \
+    mc::set_error(std::string("internal: ") + e.what()); \
+    return MC_ERR_INTERNAL;                          \
+  }                                                  \
+  return MC_OK;
+
 #define MC_CHECK(cond, code, msg)                                              \
   do {                                                                         \
     if (!(cond)) throw ::mc::Error{(code), std::string(msg)};                  \
@@ -70,6 +85,36 @@ struct KernelTimer {
   std::vector<int> kid;        // kernel id of each pair
   double total_ms[MC_KID_COUNT] = {0};
   long long count[MC_KID_COUNT] = {0};
+};
+
+// Operand of rnlsm1 built on the fly: column ia of the real view of
+// (-i)^l twnl(G) eigr(G, ia), G=0 halved (src/rnlsm1_utils.mod.F90:84-150).
+struct ProjA {
+  const double2* eigr;   // (ldg, nat) device layout
+  const double* twnl;    // (ldg) of this projector
+  int ldg;
+  int lmod4;             // l mod 4 of (-i)^l
+  int geq0;
+};
+
+// Epilogue of fnonloc (src/fnonloc_utils.mod.F90:272-300):
+// c2(g,i) += twnl(g) ctm_i auxc(g,i), ctm_i = -f_i (-i)^l, f_i < 1e-5 -> 1.
+struct FnlEpi {
+  const double* twnl;   // (ldg) of this projector; nullptr for a plain GEMM
+  const double* f;      // (n) occupations on the device
+  int lmod4;
+};
+
+// Nonlocal pseudopotential data of the current geometry.
+struct Projectors {
+  bool ready = false;
+  int nsp = 0, nhxs = 0, nat = 0, eigr_nat = -1, fnl_nstate = -1;
+  std::vector<int> na, ngh, lang;
+  std::vector<double> h_h;
+  DevBuf d_twnl;   // (ldg, nhxs, nsp)
+  DevBuf d_h;      // (nhxs, nhxs, nsp)
+  DevBuf d_eigr;   // (ldg, nat) complex
+  DevBuf d_fnl;    // (nat, nhxs, nstate)
 };
 
 }  // namespace mc
@@ -117,6 +162,11 @@ struct mc_ctx {
   mc::DevBuf w_red;        // reduction partials
   mc::DevBuf w_f;          // occupations on device
 
+  mc::DevBuf w_gemm;       // split-K partials
+  mc::DevBuf w_small[8];    // n x n matrices
+  mc::DevBuf w_red2, w_info, w_wfn, w_f2, w_dd;
+  mc::Projectors proj;
+
   int batch_pairs_vpsi = 8;
   int batch_pairs_rho = 16;
 
@@ -128,6 +178,30 @@ namespace mc {
 void timer_begin(mc_ctx* ctx, int kid);
 void timer_end(mc_ctx* ctx, int kid);
 void timer_flush(mc_ctx* ctx);
+
+__global__ void k_zero(double* __restrict__ x, size_t n);
+
+// contractions (gemm.hip)
+void gemm_tn(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, int K, int m, int n, double alpha,
+             bool g0corr, bool symmetric, double* out, int ldo, const ProjA* proj);
+void gemm_nn(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n, double alpha,
+             double beta, double* D, int ldd, bool upper_g, const FnlEpi* epi);
+
+// orthonormalisation and projectors (ortho.hip)
+void ovlap_dev(mc_ctx* ctx, int n, double* a_dev, int lda, const double2* c1, const double2* c2);
+void rgs_dev(mc_ctx* ctx, double2* c, int n, double* smat_host);
+void lowdin_dev(mc_ctx* ctx, double2* c, int n, int* iters);
+void rotate_dev(mc_ctx* ctx, double a, const double2* c1, double b, double2* c2, const double* gam_dev, int n);
+void lagrange_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, int n);
+void rortog_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n, double epsog,
+                int maxit, int* iters);
+void set_projectors(mc_ctx* ctx, int nsp, const int* na, int nhxs, const int* ngh, const int* lang,
+                    const double* hmat, const double* twnl, int ldtw);
+void set_eigr(mc_ctx* ctx, const double* eigr, int lde, int nat);
+void rnlsm1_dev(mc_ctx* ctx, const double2* c0, int n, double* fnl_host);
+void fnonloc_dev(mc_ctx* ctx, double2* c2, const double* f_host, int n, const double* fnl_host);
+void wfn_upload(mc_ctx* ctx, const double* host, int ldc, int nstate, double2* dev);
+void wfn_download(mc_ctx* ctx, const double2* dev, int nstate, double* host, int ldc);
 
 // driver entry points implemented in wfft.hip
 void rhoofr_dev(mc_ctx* ctx, const double2* c0, const double* f_host, int nstate,

@@ -1,0 +1,513 @@
+// Orthonormalisation, overlap, rotation, Lagrange projection, the CP
+// constraint and the nonlocal projector contractions on gfx950.
+//
+//   ovlap     src/ovlap_utils.mod.F90:57-190       a = 2 Re(C1^H C2) - c1_0 c2_0
+//   rgs       src/rgs_utils.mod.F90:31-162         S = C^T C, S = U^T U, C <- C U^-1, zclean
+//   lowdin    src/lowdin_utils.mod.F90:27-192      C <- C S^-1/2, G=0 imaginary part cleaned
+//   rotate    src/rotate_utils.mod.F90:63-100      c2 = a c1 gam + b c2
+//   lagrange  src/forces_driver.mod.F90:344-370    gam = ovlap(c2,c0) f_i f_j, c2 -= c0 gam, zclean
+//   rortog    src/rortog_utils.mod.F90:315-498     CP constraint fixed point, cm += c0 gam
+//   rnlsm1    src/rnlsm1_utils.mod.F90:42-193      fnl = 2 Re <(-i)^l twnl eigr | c>
+//   fnonloc   src/fnonloc_utils.mod.F90:209-303    c2 += -f (-i)^l twnl sum_ia eigr dd
+//
+// Dense n x n factorisations stay in FP64 on the GPU:
+//  * Cholesky (upper, LAPACK dpotrf semantics) is a left-looking column sweep
+//    in one workgroup; the triangular inverse (dtrtri) solves the columns of
+//    U X = I in parallel, one wavefront per column;
+//  * Loewdin's S^-1/2 is obtained with the coupled Newton-Schulz iteration
+//    (Higham, "Functions of Matrices", 2008, eq. 6.35) on MFMA, which converges
+//    to the same unique SPD inverse square root that CPMD assembles from the
+//    eigenpairs of dspev.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "mc_internal.h"
+
+namespace mc {
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+
+// Cholesky S = U^T U of the upper triangle of S (n x n, column-major lds), in
+// place; lower triangle set to zero.  info = 0 or the order of the first
+// non-positive leading minor (dpotrf).
+__global__ __launch_bounds__(1024) void k_potrf_upper(double* __restrict__ S, int n, int lds, int* info) {
+  __shared__ double col[4096];   // column j of U (rows < j); n <= 4096
+  __shared__ double diag;
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  for (int j = 0; j < n; ++j) {
+    // U(0:j, j) is final; stage it
+    for (int k = threadIdx.x; k < j; k += blockDim.x) col[k] = S[k + (size_t)lds * j];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      double s = 0.0;
+      for (int k = threadIdx.x; k < j; k += 64) s += col[k] * col[k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+      if (threadIdx.x == 0) {
+        const double d = S[j + (size_t)lds * j] - s;
+        if (!(d > 0.0)) { bad = j + 1; diag = 1.0; }
+        else diag = sqrt(d);
+        S[j + (size_t)lds * j] = diag;
+      }
+    }
+    __syncthreads();
+    if (bad) break;
+    const double dj = diag;
+    // row j of U: U(j, i) = (S(j,i) - sum_k U(k,j) U(k,i)) / U(j,j), i > j
+    for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) {
+      const double* ci = S + (size_t)lds * i;
+      double s = 0.0;
+      for (int k = 0; k < j; ++k) s += col[k] * ci[k];
+      S[j + (size_t)lds * i] = (ci[j] - s) / dj;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *info = bad;
+  __syncthreads();
+  for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    if (i > j) S[i + (size_t)lds * j] = 0.0;
+  }
+}
+
+// X = U^{-1} (upper): one 64-thread workgroup per column j solves U x = e_j
+// by back substitution with x staged in LDS (n doubles).
+__global__ __launch_bounds__(64) void k_trtri_upper(const double* __restrict__ U, int n, int ldu,
+                                                    double* __restrict__ X, int ldx) {
+  extern __shared__ double xs[];
+  const int j = blockIdx.x;
+  const int l = threadIdx.x;
+  for (int i = l; i < n; i += 64) xs[i] = 0.0;
+  __syncthreads();
+  if (l == 0) xs[j] = 1.0 / U[j + (size_t)ldu * j];
+  __syncthreads();
+  for (int i = j - 1; i >= 0; --i) {
+    double s = 0.0;
+    for (int k = i + 1 + l; k <= j; k += 64) s += U[i + (size_t)ldu * k] * xs[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (l == 0) xs[i] = -s / U[i + (size_t)ldu * i];
+    __syncthreads();
+  }
+  double* x = X + (size_t)ldx * j;
+  for (int i = l; i < n; i += 64) x[i] = xs[i];
+}
+
+// a(i,j) *= f(i) f(j) with f = (occupation != 0) (gscal, forces_driver.mod.F90:502-518)
+__global__ void k_gscal(double* __restrict__ a, int n, int lda, const double* __restrict__ focc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= n) return;
+  const double fi = focc[i] != 0.0 ? 1.0 : 0.0, fj = focc[j] != 0.0 ? 1.0 : 0.0;
+  a[i + (size_t)lda * j] *= fi * fj;
+}
+
+// Im c(G=0, i) = 0 (zclean, src/utils.mod.F90:116-134); G=0 is row 0 of the device layout
+__global__ void k_zclean(double2* __restrict__ c, int ldg, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[(size_t)ldg * i].y = 0.0;
+}
+
+// out = alpha * a + beta * I (+ gamma * b), n x n
+__global__ void k_axpby_eye(const double* __restrict__ a, double alpha, double beta, const double* __restrict__ b,
+                            double gamma, double* __restrict__ out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= n) return;
+  const size_t e = i + (size_t)n * j;
+  double v = alpha * a[e] + (i == j ? beta : 0.0);
+  if (b) v += gamma * b[e];
+  out[e] = v;
+}
+
+// max |a - b| over n*n into partial[blockIdx.x] (then reduced on the host)
+__global__ void k_maxdiff(const double* __restrict__ a, const double* __restrict__ b, size_t nn,
+                          double* __restrict__ part) {
+  double m = 0.0;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn; e += (size_t)gridDim.x * blockDim.x) {
+    const double d = fabs(a[e] - (b ? b[e] : 0.0));
+    m = d > m ? d : m;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_down(m, off, 64);
+    m = o > m ? o : m;
+  }
+  __shared__ double w[4];
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) r = w[q] > r ? w[q] : r;
+    part[blockIdx.x] = r;
+  }
+}
+
+// symma: a(i,j) = a(j,i) = (a(i,j) + a(j,i))/2 for i < j (src/utils.mod.F90:138-153)
+__global__ void k_symma(double* __restrict__ a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= j || j >= n) return;
+  const double v = 0.5 * (a[i + (size_t)n * j] + a[j + (size_t)n * i]);
+  a[i + (size_t)n * j] = v;
+  a[j + (size_t)n * i] = v;
+}
+
+// dd(ia, i) = sum_jv h(iv, jv) fnl(isa0+ia, jv, i)   (fnonloc Goedecker branch :219-256)
+__global__ void k_fnl_dd(const double* __restrict__ fnl, int nat, int nhxs, int isa0, int na, int ngh,
+                         const double* __restrict__ h, int iv, int nstate, double* __restrict__ dd) {
+  const int ia = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (ia >= na || i >= nstate) return;
+  double s = 0.0;
+  for (int jv = 0; jv < ngh; ++jv) {
+    const double hv = h[iv + nhxs * jv];
+    if (hv != 0.0) s += fnl[(isa0 + ia) + (size_t)nat * (jv + (size_t)nhxs * i)] * hv;
+  }
+  dd[ia + (size_t)na * i] = s;
+}
+
+__global__ void k_permute_rows_in(const double2* __restrict__ src, int lds, double2* __restrict__ dst, int ldd,
+                                  const int* __restrict__ perm, int ngw, int ncol) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (j >= ldd || c >= ncol) return;
+  dst[(size_t)ldd * c + j] = j < ngw ? src[(size_t)lds * c + perm[j]] : make_double2(0.0, 0.0);
+}
+__global__ void k_permute_real_in(const double* __restrict__ src, int lds, double* __restrict__ dst, int ldd,
+                                  const int* __restrict__ perm, int ngw, int ncol) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (j >= ldd || c >= ncol) return;
+  dst[(size_t)ldd * c + j] = j < ngw ? src[(size_t)lds * c + perm[j]] : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+static inline int wfn_rows(const mc_ctx* ctx) { return 2 * ctx->ngw; }   // K of the real view
+static inline int wfn_ld(const mc_ctx* ctx) { return 2 * ctx->ldg; }     // real leading dim
+
+static double* small(mc_ctx* ctx, int which, int n) {
+  DevBuf& b = ctx->w_small[which];
+  b.ensure(sizeof(double) * (size_t)n * n);
+  return b.as<double>();
+}
+
+static double max_abs_diff(mc_ctx* ctx, const double* a, const double* b, size_t nn) {
+  const int nb = 64;
+  ctx->w_red2.ensure(sizeof(double) * nb);
+  hipLaunchKernelGGL(k_maxdiff, dim3(nb), dim3(256), 0, ctx->stream, a, b, nn, ctx->w_red2.as<double>());
+  MC_HIP(hipGetLastError());
+  double h[nb];
+  MC_HIP(hipMemcpyAsync(h, ctx->w_red2.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  double m = 0.0;
+  for (int q = 0; q < nb; ++q) m = std::max(m, h[q]);
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// drivers (device-resident wavefunctions)
+// ---------------------------------------------------------------------------
+void ovlap_dev(mc_ctx* ctx, int n, double* a_dev, int lda, const double2* c1, const double2* c2) {
+  gemm_tn(ctx, (const double*)c1, wfn_ld(ctx), (const double*)c2, wfn_ld(ctx), wfn_rows(ctx), n, n, 2.0,
+          ctx->geq0 != 0, c1 == c2, a_dev, lda, nullptr);
+}
+
+// C <- C X with X (n x n) on the device; through the workspace copy of C.
+static void apply_right(mc_ctx* ctx, double2* c, const double* X, int n, bool upper) {
+  const size_t bytes = sizeof(double2) * (size_t)ctx->ldg * n;
+  ctx->w_wfn.ensure(bytes);
+  MC_HIP(hipMemcpyAsync(ctx->w_wfn.p, c, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  gemm_nn(ctx, ctx->w_wfn.as<double>(), wfn_ld(ctx), X, n, wfn_ld(ctx), n, n, 1.0, 0.0, (double*)c, wfn_ld(ctx),
+          upper, nullptr);
+}
+
+static void zclean(mc_ctx* ctx, double2* c, int n) {
+  if (!ctx->geq0 || n <= 0) return;
+  hipLaunchKernelGGL(k_zclean, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, c, ctx->ldg, n);
+  MC_HIP(hipGetLastError());
+}
+
+void rgs_dev(mc_ctx* ctx, double2* c, int n, double* smat_host) {
+  if (n <= 0) return;
+  MC_CHECK(n <= 4096, MC_ERR_UNSUPPORTED, "rgs: nstate > 4096");
+  double* S = small(ctx, 0, n);
+  double* X = small(ctx, 1, n);
+  ovlap_dev(ctx, n, S, n, c, c);
+  ctx->w_info.ensure(sizeof(int));
+  hipLaunchKernelGGL(k_potrf_upper, dim3(1), dim3(1024), 0, ctx->stream, S, n, n, ctx->w_info.as<int>());
+  MC_HIP(hipGetLastError());
+  int info = 0;
+  MC_HIP(hipMemcpyAsync(&info, ctx->w_info.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  MC_CHECK(info == 0, MC_ERR_NOTPOSDEF,
+           "UINV| the leading minor of order " + std::to_string(info) + " is not positive definite");
+  hipLaunchKernelGGL(k_trtri_upper, dim3(n), dim3(64), sizeof(double) * n, ctx->stream, S, n, n, X, n);
+  MC_HIP(hipGetLastError());
+  apply_right(ctx, c, X, n, true);
+  zclean(ctx, c, n);
+  if (smat_host) {
+    MC_HIP(hipMemcpyAsync(smat_host, X, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
+    MC_HIP(hipStreamSynchronize(ctx->stream));
+  }
+}
+
+// Z = S^{-1/2} by the coupled Newton-Schulz iteration on S/c, c = ||S||_F:
+// T = (3I - Z Y)/2, Y <- Y T, Z <- T Z; Y -> (S/c)^{1/2}, Z -> (S/c)^{-1/2}.
+static void inv_sqrt(mc_ctx* ctx, const double* S, int n, double* Zout, int* iters) {
+  double* Y = small(ctx, 2, n);
+  double* Z = small(ctx, 3, n);
+  double* T = small(ctx, 4, n);
+  double* W = small(ctx, 5, n);
+  std::vector<double> hs((size_t)n * n);
+  MC_HIP(hipMemcpyAsync(hs.data(), S, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  double fro = 0.0;
+  for (double v : hs) fro += v * v;
+  fro = std::sqrt(fro);
+  MC_CHECK(fro > 0.0 && std::isfinite(fro), MC_ERR_ARG, "lowdin: zero or non-finite overlap matrix");
+  // scale so that ||I - S/c||_2 < 1 when S is close to I: use c = max(1, ||S||_2 bound)
+  double cmax = 0.0;
+  for (int j = 0; j < n; ++j) {
+    double r = 0.0;
+    for (int i = 0; i < n; ++i) r += std::fabs(hs[i + (size_t)n * j]);
+    cmax = std::max(cmax, r);
+  }
+  const double c = cmax > 1.0 ? cmax : 1.0;   // ||S||_2 <= ||S||_1 (symmetric)
+  dim3 g2((n + 255) / 256, n);
+  hipLaunchKernelGGL(k_axpby_eye, g2, dim3(256), 0, ctx->stream, S, 1.0 / c, 0.0, (const double*)nullptr, 0.0, Y, n);
+  hipLaunchKernelGGL(k_axpby_eye, g2, dim3(256), 0, ctx->stream, S, 0.0, 1.0, (const double*)nullptr, 0.0, Z, n);
+  MC_HIP(hipGetLastError());
+  int it = 0;
+  const int maxit = 100;
+  bool done = false;
+  for (it = 1; it <= maxit; ++it) {
+    gemm_nn(ctx, Z, n, Y, n, n, n, n, 1.0, 0.0, W, n, false, nullptr);                 // W = Z Y
+    hipLaunchKernelGGL(k_axpby_eye, g2, dim3(256), 0, ctx->stream, W, -0.5, 1.5, (const double*)nullptr, 0.0, T, n);
+    MC_HIP(hipGetLastError());
+    gemm_nn(ctx, Y, n, T, n, n, n, n, 1.0, 0.0, W, n, false, nullptr);                 // W = Y T
+    MC_HIP(hipMemcpyAsync(Y, W, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->stream));
+    gemm_nn(ctx, T, n, Z, n, n, n, n, 1.0, 0.0, W, n, false, nullptr);                 // W = T Z
+    MC_HIP(hipMemcpyAsync(Z, W, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->stream));
+    if (done) break;
+    // ||T - I||: converged once the correction is at rounding level; one more sweep then
+    hipLaunchKernelGGL(k_axpby_eye, g2, dim3(256), 0, ctx->stream, T, 1.0, -1.0, (const double*)nullptr, 0.0, W, n);
+    const double dev = max_abs_diff(ctx, W, nullptr, (size_t)n * n);
+    MC_CHECK(std::isfinite(dev), MC_ERR_NOCONV, "lowdin: Newton-Schulz iteration diverged");
+    if (dev < 1e-15 * n) done = true;
+  }
+  MC_CHECK(done, MC_ERR_NOCONV, "lowdin: Newton-Schulz iteration did not converge in 100 steps");
+  hipLaunchKernelGGL(k_axpby_eye, g2, dim3(256), 0, ctx->stream, Z, 1.0 / std::sqrt(c), 0.0, (const double*)nullptr,
+                     0.0, Zout, n);
+  MC_HIP(hipGetLastError());
+  if (iters) *iters = it;
+}
+
+void lowdin_dev(mc_ctx* ctx, double2* c, int n, int* iters) {
+  if (n <= 0) return;
+  double* S = small(ctx, 0, n);
+  double* A = small(ctx, 1, n);
+  ovlap_dev(ctx, n, S, n, c, c);
+  inv_sqrt(ctx, S, n, A, iters);
+  apply_right(ctx, c, A, n, false);
+  zclean(ctx, c, n);
+}
+
+void rotate_dev(mc_ctx* ctx, double a, const double2* c1, double b, double2* c2, const double* gam_dev, int n) {
+  MC_CHECK(c1 != c2, MC_ERR_ARG, "rotate: c1 and c2 must not alias");
+  gemm_nn(ctx, (const double*)c1, wfn_ld(ctx), gam_dev, n, wfn_ld(ctx), n, n, a, b, (double*)c2, wfn_ld(ctx), false,
+          nullptr);
+}
+
+void lagrange_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, int n) {
+  if (n <= 0) return;
+  double* G = small(ctx, 0, n);
+  ovlap_dev(ctx, n, G, n, c2, c0);
+  ctx->w_f2.ensure(sizeof(double) * n);
+  MC_HIP(hipMemcpyAsync(ctx->w_f2.p, f_host, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_gscal, dim3((n + 255) / 256, n), dim3(256), 0, ctx->stream, G, n, n, ctx->w_f2.as<double>());
+  MC_HIP(hipGetLastError());
+  rotate_dev(ctx, -1.0, c0, 1.0, c2, G, n);
+  zclean(ctx, c2, n);
+}
+
+// CP orthonormality constraint, rortho (src/rortog_utils.mod.F90:315-498).
+void rortog_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n, double epsog,
+                int maxit, int* iters) {
+  if (n <= 0) return;
+  double* rho = small(ctx, 0, n);
+  double* sig = small(ctx, 1, n);
+  double* del = small(ctx, 2, n);
+  double* gam = small(ctx, 3, n);
+  double* gamn = small(ctx, 4, n);
+  double* scr = small(ctx, 5, n);
+  double* scr2 = small(ctx, 6, n);
+  dim3 g2((n + 255) / 256, n);
+  // rho = 1 - symma(ovlap(c0, cm)); sig = 1 - ovlap(cm, cm); del = -(ovlap(c0,c0) - 1)/2
+  ovlap_dev(ctx, n, rho, n, c0, cm);
+  hipLaunchKernelGGL(k_symma, g2, dim3(256), 0, ctx->stream, rho, n);
+  hipLaunchKernelGGL(k_axpby_eye, g2, dim3(256), 0, ctx->stream, rho, -1.0, 1.0, (const double*)nullptr, 0.0, rho, n);
+  ovlap_dev(ctx, n, sig, n, cm, cm);
+  hipLaunchKernelGGL(k_axpby_eye, g2, dim3(256), 0, ctx->stream, sig, -1.0, 1.0, (const double*)nullptr, 0.0, sig, n);
+  if (tnon) {
+    ovlap_dev(ctx, n, del, n, c0, c0);
+    hipLaunchKernelGGL(k_axpby_eye, g2, dim3(256), 0, ctx->stream, del, -0.5, 0.5, (const double*)nullptr, 0.0, del, n);
+  }
+  MC_HIP(hipGetLastError());
+  // gam = sig/2; sig <- rho^T rho / 2 + sig / 2
+  hipLaunchKernelGGL(k_axpby_eye, g2, dim3(256), 0, ctx->stream, sig, 0.5, 0.0, (const double*)nullptr, 0.0, gam, n);
+  gemm_tn(ctx, rho, n, rho, n, n, n, n, 0.5, false, false, scr, n, nullptr);
+  hipLaunchKernelGGL(k_axpby_eye, g2, dim3(256), 0, ctx->stream, scr, 1.0, 0.0, sig, 0.5, sig, n);
+  MC_HIP(hipGetLastError());
+  int it = 0;
+  double difgam = 0.0;
+  while (true) {
+    ++it;
+    // scr = rho - gam; gamn = sig - scr^T scr / 2 [+ gam^T (del gam)]
+    hipLaunchKernelGGL(k_axpby_eye, g2, dim3(256), 0, ctx->stream, rho, 1.0, 0.0, gam, -1.0, scr, n);
+    gemm_tn(ctx, scr, n, scr, n, n, n, n, -0.5, false, false, gamn, n, nullptr);
+    hipLaunchKernelGGL(k_axpby_eye, g2, dim3(256), 0, ctx->stream, gamn, 1.0, 0.0, sig, 1.0, gamn, n);
+    if (tnon) {
+      gemm_nn(ctx, del, n, gam, n, n, n, n, 1.0, 0.0, scr2, n, false, nullptr);       // scr2 = del gam
+      gemm_tn(ctx, gam, n, scr2, n, n, n, n, 1.0, false, false, scr, n, nullptr);    // scr = gam^T scr2
+      hipLaunchKernelGGL(k_axpby_eye, g2, dim3(256), 0, ctx->stream, gamn, 1.0, 0.0, scr, 1.0, gamn, n);
+    }
+    MC_HIP(hipGetLastError());
+    difgam = max_abs_diff(ctx, gamn, gam, (size_t)n * n);
+    MC_HIP(hipMemcpyAsync(gam, gamn, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->stream));
+    if (!(difgam > epsog * 0.5 && it <= maxit)) break;
+  }
+  MC_CHECK(it <= maxit, MC_ERR_NOCONV,
+           "RORTHO: DIFGAM=" + std::to_string(difgam) + " ITER=" + std::to_string(it) + " > " + std::to_string(maxit));
+  hipLaunchKernelGGL(k_symma, g2, dim3(256), 0, ctx->stream, gam, n);
+  MC_HIP(hipGetLastError());
+  rotate_dev(ctx, 1.0, c0, 1.0, cm, gam, n);
+  if (gam_host) {
+    MC_HIP(hipMemcpyAsync(gam_host, gam, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
+    MC_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  if (iters) *iters = it;
+}
+
+// ---------------------------------------------------------------------------
+// nonlocal projectors
+// ---------------------------------------------------------------------------
+void set_projectors(mc_ctx* ctx, int nsp, const int* na, int nhxs, const int* ngh, const int* lang,
+                    const double* hmat, const double* twnl, int ldtw) {
+  MC_CHECK(nsp >= 0 && nhxs >= 0, MC_ERR_ARG, "set_projectors: bad sizes");
+  auto& P = ctx->proj;
+  P.nsp = nsp;
+  P.nhxs = nhxs;
+  P.na.assign(na, na + nsp);
+  P.ngh.assign(ngh, ngh + nsp);
+  P.lang.assign(lang, lang + (size_t)nhxs * nsp);
+  P.nat = 0;
+  for (int is = 0; is < nsp; ++is) {
+    MC_CHECK(na[is] >= 0 && ngh[is] >= 0 && ngh[is] <= nhxs, MC_ERR_ARG, "set_projectors: bad na/ngh");
+    P.nat += na[is];
+  }
+  const size_t nh = (size_t)nhxs * nhxs * nsp;
+  P.d_h.ensure(sizeof(double) * std::max<size_t>(nh, 1));
+  if (nh) MC_HIP(hipMemcpy(P.d_h.p, hmat, sizeof(double) * nh, hipMemcpyHostToDevice));
+  P.h_h.assign(hmat, hmat + nh);
+  // twnl (ngw, nhxs, nsp) in CPMD order -> device rows (ldg) in stick order
+  const int ncol = nhxs * nsp;
+  P.d_twnl.ensure(sizeof(double) * (size_t)ctx->ldg * std::max(ncol, 1));
+  if (ncol) {
+    DevBuf stage;
+    stage.ensure(sizeof(double) * (size_t)ldtw * ncol);
+    MC_HIP(hipMemcpy(stage.p, twnl, sizeof(double) * (size_t)ldtw * ncol, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_permute_real_in, dim3((ctx->ldg + 255) / 256, ncol), dim3(256), 0, ctx->stream,
+                       stage.as<double>(), ldtw, P.d_twnl.as<double>(), ctx->ldg, ctx->d_perm.as<int>(), ctx->ngw,
+                       ncol);
+    MC_HIP(hipGetLastError());
+    MC_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  P.ready = true;
+}
+
+void set_eigr(mc_ctx* ctx, const double* eigr, int lde, int nat) {
+  MC_CHECK(lde >= ctx->ngw && nat >= 0, MC_ERR_ARG, "set_eigr: lde < ngw");
+  auto& P = ctx->proj;
+  P.d_eigr.ensure(sizeof(double2) * (size_t)ctx->ldg * std::max(nat, 1));
+  P.eigr_nat = nat;
+  if (!nat) return;
+  DevBuf stage;
+  stage.ensure(sizeof(double2) * (size_t)lde * nat);
+  MC_HIP(hipMemcpy(stage.p, eigr, sizeof(double2) * (size_t)lde * nat, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_permute_rows_in, dim3((ctx->ldg + 255) / 256, nat), dim3(256), 0, ctx->stream,
+                     stage.as<double2>(), lde, P.d_eigr.as<double2>(), ctx->ldg, ctx->d_perm.as<int>(), ctx->ngw, nat);
+  MC_HIP(hipGetLastError());
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void rnlsm1_dev(mc_ctx* ctx, const double2* c0, int n, double* fnl_host) {
+  auto& P = ctx->proj;
+  MC_CHECK(P.ready, MC_ERR_ARG, "rnlsm1: projectors not set (mc_set_projectors)");
+  MC_CHECK(P.eigr_nat == P.nat, MC_ERR_ARG, "rnlsm1: structure factors not set for all atoms (mc_set_eigr)");
+  const size_t nf = (size_t)P.nat * P.nhxs * std::max(n, 1);
+  P.d_fnl.ensure(sizeof(double) * nf);
+  P.fnl_nstate = n;
+  hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, ctx->stream, P.d_fnl.as<double>(), nf);
+  MC_HIP(hipGetLastError());
+  int isa0 = 0;
+  for (int is = 0; is < P.nsp; ++is) {
+    for (int iv = 0; iv < P.ngh[is]; ++iv) {
+      if (P.na[is] == 0 || n == 0) continue;
+      ProjA pa;
+      pa.eigr = P.d_eigr.as<double2>() + (size_t)isa0 * ctx->ldg;
+      pa.twnl = P.d_twnl.as<double>() + (size_t)(is * P.nhxs + iv) * ctx->ldg;
+      pa.ldg = ctx->ldg;
+      pa.lmod4 = ((P.lang[iv + (size_t)P.nhxs * is] % 4) + 4) % 4;
+      pa.geq0 = ctx->geq0;
+      gemm_tn(ctx, nullptr, 0, (const double*)c0, wfn_ld(ctx), wfn_rows(ctx), P.na[is], n, 2.0, false, false,
+              P.d_fnl.as<double>() + isa0 + (size_t)P.nat * iv, P.nat * P.nhxs, &pa);
+    }
+    isa0 += P.na[is];
+  }
+  if (fnl_host) {
+    MC_HIP(hipMemcpyAsync(fnl_host, P.d_fnl.p, sizeof(double) * (size_t)P.nat * P.nhxs * n, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    MC_HIP(hipStreamSynchronize(ctx->stream));
+  }
+}
+
+void fnonloc_dev(mc_ctx* ctx, double2* c2, const double* f_host, int n, const double* fnl_host) {
+  auto& P = ctx->proj;
+  MC_CHECK(P.ready, MC_ERR_ARG, "fnonloc: projectors not set (mc_set_projectors)");
+  MC_CHECK(P.eigr_nat == P.nat, MC_ERR_ARG, "fnonloc: structure factors not set (mc_set_eigr)");
+  if (n <= 0) return;
+  if (fnl_host) {
+    P.d_fnl.ensure(sizeof(double) * (size_t)P.nat * P.nhxs * n);
+    MC_HIP(hipMemcpyAsync(P.d_fnl.p, fnl_host, sizeof(double) * (size_t)P.nat * P.nhxs * n, hipMemcpyHostToDevice,
+                          ctx->stream));
+    P.fnl_nstate = n;
+  }
+  MC_CHECK(P.fnl_nstate == n, MC_ERR_ARG, "fnonloc: no fnl for this number of states (call rnlsm1 first)");
+  ctx->w_f2.ensure(sizeof(double) * n);
+  MC_HIP(hipMemcpyAsync(ctx->w_f2.p, f_host, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  int namax = 0;
+  for (int is = 0; is < P.nsp; ++is) namax = std::max(namax, P.na[is]);
+  ctx->w_dd.ensure(sizeof(double) * (size_t)std::max(namax, 1) * n);
+  int isa0 = 0;
+  for (int is = 0; is < P.nsp; ++is) {
+    const int na = P.na[is];
+    for (int iv = 0; iv < P.ngh[is] && na > 0; ++iv) {
+      hipLaunchKernelGGL(k_fnl_dd, dim3((na + 63) / 64, n), dim3(64), 0, ctx->stream, P.d_fnl.as<double>(), P.nat,
+                         P.nhxs, isa0, na, P.ngh[is], P.d_h.as<double>() + (size_t)P.nhxs * P.nhxs * is, iv, n,
+                         ctx->w_dd.as<double>());
+      MC_HIP(hipGetLastError());
+      FnlEpi e;
+      e.twnl = P.d_twnl.as<double>() + (size_t)(is * P.nhxs + iv) * ctx->ldg;
+      e.f = ctx->w_f2.as<double>();
+      e.lmod4 = ((P.lang[iv + (size_t)P.nhxs * is] % 4) + 4) % 4;
+      gemm_nn(ctx, (const double*)(P.d_eigr.as<double2>() + (size_t)isa0 * ctx->ldg), wfn_ld(ctx),
+              ctx->w_dd.as<double>(), na, wfn_rows(ctx), na, n, 1.0, 1.0, (double*)c2, wfn_ld(ctx), false, &e);
+    }
+    isa0 += na;
+  }
+}
+
+}  // namespace mc

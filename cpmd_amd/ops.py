@@ -173,6 +173,125 @@ class Context:
         f = np.ascontiguousarray(f, dtype=np.float64)
         check(lib().mc_vpsi_dev(self.h, c0.ptr, c2.ptr, _p(f), vpot.ptr, nstate))
 
+    # ------------------------------------------------------------ contractions
+    def ovlap(self, c1: np.ndarray, c2: np.ndarray) -> np.ndarray:
+        c1 = np.asfortranarray(c1, dtype=np.complex128)
+        c2 = c1 if c2 is c1 else np.asfortranarray(c2, dtype=np.complex128)
+        n = c1.shape[1]
+        a = np.zeros((n, n), order="F")
+        check(lib().mc_ovlap(self.h, n, _p(a), _p(c1), _p(c2), c1.shape[0]))
+        return a
+
+    def ovlap_dev(self, c1: DeviceArray, c2: DeviceArray, nstate: int) -> np.ndarray:
+        a = np.zeros((nstate, nstate), order="F")
+        check(lib().mc_ovlap_dev(self.h, nstate, _p(a), c1.ptr, c2.ptr))
+        return a
+
+    def rgs(self, cp: np.ndarray):
+        """Returns (orthonormalised copy of cp, U^-1)."""
+        out = np.array(cp, dtype=np.complex128, order="F", copy=True)
+        n = out.shape[1]
+        smat = np.zeros((n, n), order="F")
+        check(lib().mc_rgs(self.h, _p(out), out.shape[0], n, _p(smat)))
+        return out, smat
+
+    def rgs_dev(self, c: DeviceArray, nstate: int, want_smat: bool = False):
+        smat = np.zeros((nstate, nstate), order="F") if want_smat else None
+        check(lib().mc_rgs_dev(self.h, c.ptr, nstate, _p(smat) if want_smat else None))
+        return smat
+
+    def lowdin(self, cp: np.ndarray) -> np.ndarray:
+        out = np.array(cp, dtype=np.complex128, order="F", copy=True)
+        check(lib().mc_lowdin(self.h, _p(out), out.shape[0], out.shape[1]))
+        return out
+
+    def lowdin_dev(self, c: DeviceArray, nstate: int) -> int:
+        it = C.c_int()
+        check(lib().mc_lowdin_dev(self.h, c.ptr, nstate, C.byref(it)))
+        return it.value
+
+    def rotate(self, a: float, c1: np.ndarray, b: float, c2: np.ndarray, gam: np.ndarray) -> np.ndarray:
+        c1 = np.asfortranarray(c1, dtype=np.complex128)
+        out = np.array(c2, dtype=np.complex128, order="F", copy=True)
+        gam = np.asfortranarray(gam, dtype=np.float64)
+        check(lib().mc_rotate(self.h, a, _p(c1), b, _p(out), c1.shape[0], _p(gam), c1.shape[1]))
+        return out
+
+    def rotate_dev(self, a: float, c1: DeviceArray, b: float, c2: DeviceArray, gam: np.ndarray, nstate: int):
+        gam = np.asfortranarray(gam, dtype=np.float64)
+        check(lib().mc_rotate_dev(self.h, a, c1.ptr, b, c2.ptr, _p(gam), nstate))
+
+    def lagrange(self, c0: np.ndarray, c2: np.ndarray, f: np.ndarray) -> np.ndarray:
+        c0 = np.asfortranarray(c0, dtype=np.complex128)
+        out = np.array(c2, dtype=np.complex128, order="F", copy=True)
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        check(lib().mc_lagrange(self.h, _p(c0), _p(out), c0.shape[0], _p(f), c0.shape[1]))
+        return out
+
+    def lagrange_dev(self, c0: DeviceArray, c2: DeviceArray, f: np.ndarray, nstate: int):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        check(lib().mc_lagrange_dev(self.h, c0.ptr, c2.ptr, _p(f), nstate))
+
+    def rortog(self, c0: np.ndarray, cm: np.ndarray, tnon: bool = False, epsog: float = 1e-8, maxit: int = 30):
+        c0 = np.asfortranarray(c0, dtype=np.complex128)
+        out = np.array(cm, dtype=np.complex128, order="F", copy=True)
+        n = c0.shape[1]
+        gam = np.zeros((n, n), order="F")
+        it = C.c_int()
+        check(lib().mc_rortog(self.h, _p(c0), _p(out), c0.shape[0], _p(gam), int(tnon), n, epsog, maxit,
+                              C.byref(it)))
+        return out, gam, it.value
+
+    def rortog_dev(self, c0: DeviceArray, cm: DeviceArray, nstate: int, tnon: bool = False, epsog: float = 1e-8,
+                   maxit: int = 30):
+        gam = np.zeros((nstate, nstate), order="F")
+        it = C.c_int()
+        check(lib().mc_rortog_dev(self.h, c0.ptr, cm.ptr, _p(gam), int(tnon), nstate, epsog, maxit, C.byref(it)))
+        return gam, it.value
+
+    # ------------------------------------------------------------ projectors
+    def set_projectors(self, na, nhxs, ngh, lang, hmat, twnl):
+        self._na = np.ascontiguousarray(na, dtype=np.int32)
+        self._ngh = np.ascontiguousarray(ngh, dtype=np.int32)
+        self._lang = np.asfortranarray(np.asarray(lang, dtype=np.int32))
+        self._hmat = np.asfortranarray(np.asarray(hmat, dtype=np.float64))
+        tw = np.asfortranarray(np.asarray(twnl, dtype=np.float64))
+        self.nat = int(self._na.sum())
+        self.nhxs = int(nhxs)
+        check(lib().mc_set_projectors(self.h, len(self._na), self._na.ctypes.data_as(C.POINTER(C.c_int)), nhxs,
+                                      self._ngh.ctypes.data_as(C.POINTER(C.c_int)),
+                                      self._lang.ctypes.data_as(C.POINTER(C.c_int)), _p(self._hmat), _p(tw),
+                                      tw.shape[0]))
+
+    def set_eigr(self, eigr: np.ndarray):
+        e = np.asfortranarray(eigr, dtype=np.complex128)
+        check(lib().mc_set_eigr(self.h, _p(e), e.shape[0], e.shape[1]))
+
+    def rnlsm1(self, c0: np.ndarray) -> np.ndarray:
+        c0 = np.asfortranarray(c0, dtype=np.complex128)
+        n = c0.shape[1]
+        fnl = np.zeros((self.nat, self.nhxs, n), order="F")
+        check(lib().mc_rnlsm1(self.h, _p(c0), c0.shape[0], n, _p(fnl)))
+        return fnl
+
+    def rnlsm1_dev(self, c0: DeviceArray, nstate: int, want_fnl: bool = False):
+        fnl = np.zeros((self.nat, self.nhxs, nstate), order="F") if want_fnl else None
+        check(lib().mc_rnlsm1_dev(self.h, c0.ptr, nstate, _p(fnl) if want_fnl else None))
+        return fnl
+
+    def fnonloc(self, c2: np.ndarray, f: np.ndarray, fnl: np.ndarray | None = None) -> np.ndarray:
+        out = np.array(c2, dtype=np.complex128, order="F", copy=True)
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        fp = _p(np.asfortranarray(fnl, dtype=np.float64)) if fnl is not None else None
+        keep = fnl
+        check(lib().mc_fnonloc(self.h, _p(out), out.shape[0], _p(f), out.shape[1], fp))
+        del keep
+        return out
+
+    def fnonloc_dev(self, c2: DeviceArray, f: np.ndarray, nstate: int):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        check(lib().mc_fnonloc_dev(self.h, c2.ptr, _p(f), nstate, None))
+
     # ------------------------------------------------------------ timing
     def timer_enable(self, on: bool = True):
         check(lib().mc_timer_enable(self.h, 1 if on else 0))

@@ -322,6 +322,15 @@ def structure_factors(sys: SyntheticSystem, miller: np.ndarray) -> np.ndarray:
     return np.asfortranarray(np.exp(1j * phase))
 
 
+def projector_set(sys: SyntheticSystem, miller: np.ndarray, hg: np.ndarray):
+    """The one-species, one-s-projector Goedecker set of the synthetic systems as
+    the projector arguments of the library: (na, nhxs, ngh, lang, hmat, twnl, eigr)."""
+    twnl = gth_projector_g(sys, hg)[:, None, None]            # (ngw, nhxs=1, nsp=1)
+    eigr = structure_factors(sys, miller)
+    return (np.array([sys.nat], np.int32), 1, np.array([1], np.int32), np.zeros((1, 1), np.int32),
+            np.full((1, 1, 1), sys.pp.h11), np.asfortranarray(twnl), eigr)
+
+
 def real_space_potential(sys: SyntheticSystem, seed: int = 99, amplitude: float = 1.0) -> np.ndarray:
     """A smooth, seeded real potential V(r) on the padded mesh, layout
     v[ix + kr1*(iy + kr2*iz)] (CPMD's vpot(nnr1)).  Padding entries are zero."""

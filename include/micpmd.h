@@ -130,6 +130,89 @@ int mc_vpsi(mc_ctx* ctx, const double* c0, int ldc, double* c2, const double* f,
 int mc_vpsi_dev(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const double* f,
                 const void* vpot_dev, int nstate);
 
+/* ------------------------------------------------------------------------
+ * Dense contractions (FP64 MFMA).  Wavefunctions are viewed as real
+ * (2*ngw x nstate) matrices; matrices are column-major nstate x nstate on the
+ * host.
+ * ------------------------------------------------------------------------ */
+
+/* ovlap: a = 2 Re(C1^H C2) - Re c1(G=0) Re c2(G=0)^T.
+ * Replaces ovlap(nstate,a,c1,c2), src/ovlap_utils.mod.F90:57-190 (dsyrk+dmatc
+ * when c1 == c2, dgemm otherwise, dger for G=0). */
+int mc_ovlap(mc_ctx* ctx, int nstate, double* a, const double* c1, const double* c2, int ldc);
+int mc_ovlap_dev(mc_ctx* ctx, int nstate, double* a, const void* c1_dev, const void* c2_dev);
+
+/* rgs: Gram-Schmidt by Cholesky: S = ovlap(C,C) = U^T U, C <- C U^-1,
+ * Im c(G=0) = 0; smat (optional, may be NULL) receives U^-1 (upper).
+ * Replaces rgs(cp,nstate,smat), src/rgs_utils.mod.F90:31-93 and uinv :95-162;
+ * MC_ERR_NOTPOSDEF mirrors the dpotrf failure stop of uinv :138-158. */
+int mc_rgs(mc_ctx* ctx, double* cp, int ldc, int nstate, double* smat);
+int mc_rgs_dev(mc_ctx* ctx, void* c_dev, int nstate, double* smat);
+
+/* lowdin: C <- C S^-1/2 with S = ovlap(C,C), Im c(G=0) = 0.
+ * Replaces lowdin(cp,cn,nstate), src/lowdin_utils.mod.F90:27-192 (closed shell).
+ * iters (may be NULL) returns the Newton-Schulz iteration count. */
+int mc_lowdin(mc_ctx* ctx, double* cp, int ldc, int nstate);
+int mc_lowdin_dev(mc_ctx* ctx, void* c_dev, int nstate, int* iters);
+
+/* rotate: c2 = a c1 gam + b c2 (gam nstate x nstate, host).
+ * Replaces rotate(a,c1,b,c2,gam,nstate,2*ngw,tlsd=.false.,..),
+ * src/rotate_utils.mod.F90:63-100.  c1 and c2 must not alias. */
+int mc_rotate(mc_ctx* ctx, double a, const double* c1, double b, double* c2, int ldc, const double* gam, int nstate);
+int mc_rotate_dev(mc_ctx* ctx, double a, const void* c1_dev, double b, void* c2_dev, const double* gam, int nstate);
+
+/* Lagrange projection of the electronic gradient (iproj = 2):
+ * gam = ovlap(c2,c0), gam(i,j) *= fsc(i) fsc(j) (fsc = f != 0), c2 -= c0 gam,
+ * Im c2(G=0) = 0.  Replaces the closed-shell branch of addup2 in
+ * src/forces_driver.mod.F90:284-370 (gscal :502-518). */
+int mc_lagrange(mc_ctx* ctx, const double* c0, double* c2, int ldc, const double* f, int nstate);
+int mc_lagrange_dev(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const double* f, int nstate);
+
+/* rortog: Car-Parrinello orthonormality constraint (rortho fixed point,
+ * max|dgam| <= epsog/2 within maxit iterations), cm += c0 gam.
+ * gam (may be NULL) receives the constraint matrix.  MC_ERR_NOCONV mirrors
+ * the stop of src/rortog_utils.mod.F90:464-471.
+ * Replaces rortog(c0,cm,gam,tnon,nstate,prteig) -> rortho,
+ * src/rortog_utils.mod.F90:44-87, 315-498 (non-distributed, prteig=.false.). */
+int mc_rortog(mc_ctx* ctx, const double* c0, double* cm, int ldc, double* gam, int tnon, int nstate,
+              double epsog, int maxit, int* iters);
+int mc_rortog_dev(mc_ctx* ctx, const void* c0_dev, void* cm_dev, double* gam, int tnon, int nstate,
+                  double epsog, int maxit, int* iters);
+
+/* ------------------------------------------------------------------------
+ * Nonlocal pseudopotential projectors
+ * ------------------------------------------------------------------------ */
+
+/* Projector set of the current cell (CPMD: nlps_com%ngh, nghtol, sgpp2%hlsg /
+ * wsg, cppt::twnl; src/nlps.mod.F90, src/sgpp.mod.F90, src/cppt.mod.F90).
+ *   na(nsp)            atoms per species (isa ordering as ions0%na)
+ *   ngh(nsp)           projectors per species, <= nhxs
+ *   lang(nhxs,nsp)     angular momentum l of projector iv (nghtol)
+ *   hmat(nhxs,nhxs,nsp) coupling h(iv,jv): Goedecker hlsg(ki,kj,l) for equal
+ *                      (l, m), zero otherwise; wsg(iv) on the diagonal for
+ *                      BHS/TM potentials (fnonloc.mod.F90:209-357)
+ *   twnl(ldtw,nhxs,nsp) form factors in CPMD G order. */
+int mc_set_projectors(mc_ctx* ctx, int nsp, const int* na, int nhxs, const int* ngh, const int* lang,
+                      const double* hmat, const double* twnl, int ldtw);
+/* Structure factors eigr(G, isa) = exp(-i G.R), complex (lde, nat), CPMD G
+ * order (phfac, called after every ionic move, src/mdmain_utils.mod.F90:740). */
+int mc_set_eigr(mc_ctx* ctx, const double* eigr, int lde, int nat);
+
+/* rnlsm1: fnl(isa, iv, i) = 2 Re sum_G conj((-i)^l twnl eigr) c (G=0 halved);
+ * fnl is (nat, nhxs, nstate) float64 (sfac::fnl with imagp = 1).  The
+ * library keeps a device copy for mc_fnonloc.
+ * Replaces rnlsm1(c0,nstate,ikind), src/rnlsm1_utils.mod.F90:42-193. */
+int mc_rnlsm1(mc_ctx* ctx, const double* c0, int ldc, int nstate, double* fnl);
+int mc_rnlsm1_dev(mc_ctx* ctx, const void* c0_dev, int nstate, double* fnl);
+
+/* fnonloc: c2(G,i) += -f_i (-i)^l twnl(G) sum_ia eigr(G,ia) dd(ia,i),
+ * dd(ia,i) = sum_jv h(iv,jv) fnl(isa,jv,i), f_i < 1e-5 -> 1.  fnl: host
+ * array (nat,nhxs,nstate) or NULL for the device copy of the last rnlsm1.
+ * Replaces fnonloc(c2,f,nstate,ikind,ispin,redist_c2),
+ * src/fnonloc_utils.mod.F90:55-397 (Gamma). */
+int mc_fnonloc(mc_ctx* ctx, double* c2, int ldc, const double* f, int nstate, const double* fnl);
+int mc_fnonloc_dev(mc_ctx* ctx, void* c2_dev, const double* f, int nstate, const double* fnl);
+
 /* Optional per-kernel timing with HIP events on the context stream. */
 int mc_timer_enable(mc_ctx* ctx, int on);
 int mc_timer_reset(mc_ctx* ctx);

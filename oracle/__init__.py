@@ -94,3 +94,145 @@ def vpsi(L: Layout, c0: np.ndarray, c2: np.ndarray, f: np.ndarray, vpot: np.ndar
     lib().or_vpsi(_p(L.dims, _ip), _p(L.inyh, _ip), _p(L.hg), L.tpiba2, L.omega,
                   _p(c0), c0.shape[0], _p(out), _p(f), _p(v), c0.shape[1])
     return out
+
+
+# --------------------------------------------------------------------------
+# dense contractions
+# --------------------------------------------------------------------------
+def _setup_dense(L):
+    if getattr(L, "_dense", False):
+        return
+    L._dense = True
+    _ci = C.c_int
+    L.or_ovlap.argtypes = [_ci, _ci, _dp, _dp, _ci, _ci, _dp]
+    L.or_potrf_upper.argtypes = [_dp, _ci]
+    L.or_potrf_upper.restype = _ci
+    L.or_trtri_upper.argtypes = [_dp, _ci, _dp]
+    L.or_rgs.argtypes = [_ci, _ci, _dp, _ci, _ci, _dp]
+    L.or_rgs.restype = _ci
+    L.or_jacobi_eigen.argtypes = [_dp, _ci, _dp, _dp]
+    L.or_lowdin.argtypes = [_ci, _ci, _dp, _ci, _ci, _dp]
+    L.or_rotate.argtypes = [_ci, C.c_double, _dp, C.c_double, _dp, _ci, _dp, _ci]
+    L.or_lagrange.argtypes = [_ci, _ci, _dp, _dp, _ci, _dp, _ci]
+    L.or_rortho.argtypes = [_ci, _ci, _dp, _dp, _ci, _dp, _ci, _ci, C.c_double, _ci]
+    L.or_rortho.restype = _ci
+    L.or_rnlsm1.argtypes = [_ci, _ci, _ci, _ip, _ci, _ip, _ip, _dp, _ci, _dp, _ci, _dp, _ci, _ci, _dp]
+    L.or_fnonloc.argtypes = [_ci, _ci, _ip, _ci, _ip, _ip, _dp, _dp, _ci, _dp, _ci, _dp, _dp, _dp, _ci, _ci]
+
+
+def _L():
+    L = lib()
+    _setup_dense(L)
+    return L
+
+
+def _F(a, dtype=np.complex128):
+    return np.array(a, dtype=dtype, order="F", copy=True)
+
+
+def ovlap(c1, c2, geq0=True):
+    c1 = np.asfortranarray(c1, dtype=np.complex128)
+    c2 = np.asfortranarray(c2, dtype=np.complex128)
+    n = c1.shape[1]
+    a = np.zeros((n, n), order="F")
+    _L().or_ovlap(c1.shape[0], int(geq0), _p(c1), _p(c2), c1.shape[0], n, _p(a))
+    return a
+
+
+def potrf_upper(s):
+    u = _F(s, np.float64)
+    info = _L().or_potrf_upper(_p(u), u.shape[0])
+    return u, info
+
+
+def trtri_upper(u):
+    u = _F(u, np.float64)
+    x = np.zeros_like(u, order="F")
+    _L().or_trtri_upper(_p(u), u.shape[0], _p(x))
+    return x
+
+
+def rgs(c, geq0=True):
+    c = _F(c)
+    n = c.shape[1]
+    smat = np.zeros((n, n), order="F")
+    info = _L().or_rgs(c.shape[0], int(geq0), _p(c), c.shape[0], n, _p(smat))
+    if info:
+        raise np.linalg.LinAlgError(f"oracle rgs: leading minor {info} not positive definite")
+    return c, smat
+
+
+def jacobi_eigen(a):
+    a = _F(a, np.float64)
+    n = a.shape[0]
+    w = np.zeros(n)
+    z = np.zeros((n, n), order="F")
+    _L().or_jacobi_eigen(_p(a), n, _p(w), _p(z))
+    return w, z
+
+
+def lowdin(c, geq0=True):
+    c = _F(c)
+    n = c.shape[1]
+    amat = np.zeros((n, n), order="F")
+    _L().or_lowdin(c.shape[0], int(geq0), _p(c), c.shape[0], n, _p(amat))
+    return c, amat
+
+
+def rotate(a, c1, b, c2, gam):
+    c1 = np.asfortranarray(c1, dtype=np.complex128)
+    out = _F(c2)
+    gam = np.asfortranarray(gam, dtype=np.float64)
+    _L().or_rotate(c1.shape[0], a, _p(c1), b, _p(out), c1.shape[0], _p(gam), c1.shape[1])
+    return out
+
+
+def lagrange(c0, c2, f, geq0=True):
+    c0 = np.asfortranarray(c0, dtype=np.complex128)
+    out = _F(c2)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    _L().or_lagrange(c0.shape[0], int(geq0), _p(c0), _p(out), c0.shape[0], _p(f), c0.shape[1])
+    return out
+
+
+def rortho(c0, cm, tnon=False, epsog=1e-8, maxit=30, geq0=True):
+    c0 = np.asfortranarray(c0, dtype=np.complex128)
+    out = _F(cm)
+    n = c0.shape[1]
+    gam = np.zeros((n, n), order="F")
+    it = _L().or_rortho(c0.shape[0], int(geq0), _p(c0), _p(out), c0.shape[0], _p(gam), int(tnon), n, epsog, maxit)
+    return out, gam, it
+
+
+class Projectors:
+    """Arguments of the projector routines (CPMD's nlps/sgpp/cppt data)."""
+
+    def __init__(self, na, nhxs, ngh, lang, hmat, twnl, eigr):
+        self.na = np.ascontiguousarray(na, dtype=np.int32)
+        self.nsp = len(self.na)
+        self.nhxs = int(nhxs)
+        self.ngh = np.ascontiguousarray(ngh, dtype=np.int32)
+        self.lang = np.asfortranarray(np.asarray(lang, dtype=np.int32).reshape(self.nhxs, self.nsp, order="F"))
+        self.hmat = np.asfortranarray(np.asarray(hmat, dtype=np.float64).reshape(self.nhxs, self.nhxs, self.nsp, order="F"))
+        self.twnl = np.asfortranarray(np.asarray(twnl, dtype=np.float64))      # (ngw, nhxs, nsp)
+        self.eigr = np.asfortranarray(np.asarray(eigr, dtype=np.complex128))   # (ngw, nat)
+        self.nat = int(self.na.sum())
+
+
+def rnlsm1(P: Projectors, c0, geq0=True):
+    c0 = np.asfortranarray(c0, dtype=np.complex128)
+    n = c0.shape[1]
+    fnl = np.zeros((P.nat, P.nhxs, n), order="F")
+    _L().or_rnlsm1(c0.shape[0], int(geq0), P.nsp, _p(P.na, _ip), P.nhxs, _p(P.ngh, _ip), _p(P.lang, _ip),
+                   _p(P.twnl), P.twnl.shape[0], _p(P.eigr), P.eigr.shape[0], _p(c0), c0.shape[0], n, _p(fnl))
+    return fnl
+
+
+def fnonloc(P: Projectors, c2, f, fnl):
+    out = _F(c2)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    fnl = np.asfortranarray(fnl, dtype=np.float64)
+    _L().or_fnonloc(out.shape[0], P.nsp, _p(P.na, _ip), P.nhxs, _p(P.ngh, _ip), _p(P.lang, _ip), _p(P.hmat),
+                    _p(P.twnl), P.twnl.shape[0], _p(P.eigr), P.eigr.shape[0], _p(fnl), _p(f), _p(out), out.shape[0],
+                    out.shape[1])
+    return out

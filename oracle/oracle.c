@@ -238,3 +238,305 @@ void or_vpsi(const int* dims, const int* inyh, const double* hg, double tpiba2, 
   }
   free(psi);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Dense contractions (plain loops; column-major, CPMD G order, G=0 row 0)    */
+/* ------------------------------------------------------------------------- */
+
+/* ovlap, src/ovlap_utils.mod.F90:57-190: a = 2 Re(c1^H c2) - Re c1(0) Re c2(0) */
+void or_ovlap(int ngw, int geq0, const double* c1d, const double* c2d, int ldc, int n, double* a) {
+  const cplx* c1 = (const cplx*)c1d;
+  const cplx* c2 = (const cplx*)c2d;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      double s = 0.0;
+      const cplx* x = c1 + (size_t)i * ldc;
+      const cplx* y = c2 + (size_t)j * ldc;
+      for (int g = 0; g < ngw; ++g) s += creal(x[g]) * creal(y[g]) + cimag(x[g]) * cimag(y[g]);
+      s *= 2.0;
+      if (geq0) s -= creal(x[0]) * creal(y[0]);
+      a[i + (size_t)n * j] = s;
+    }
+}
+
+/* upper Cholesky S = U^T U in place (LAPACK dpotf2 'U' order); returns info */
+int or_potrf_upper(double* s, int n) {
+  for (int j = 0; j < n; ++j) {
+    double d = s[j + (size_t)n * j];
+    for (int k = 0; k < j; ++k) d -= s[k + (size_t)n * j] * s[k + (size_t)n * j];
+    if (!(d > 0.0)) return j + 1;
+    d = sqrt(d);
+    s[j + (size_t)n * j] = d;
+    for (int i = j + 1; i < n; ++i) {
+      double t = s[j + (size_t)n * i];
+      for (int k = 0; k < j; ++k) t -= s[k + (size_t)n * j] * s[k + (size_t)n * i];
+      s[j + (size_t)n * i] = t / d;
+    }
+  }
+  for (int j = 0; j < n; ++j)
+    for (int i = j + 1; i < n; ++i) s[i + (size_t)n * j] = 0.0;
+  return 0;
+}
+
+/* inverse of an upper triangular matrix (dtrti2 'U','N' semantics) */
+void or_trtri_upper(const double* u, int n, double* x) {
+  memset(x, 0, sizeof(double) * n * n);
+  for (int j = 0; j < n; ++j) {
+    x[j + (size_t)n * j] = 1.0 / u[j + (size_t)n * j];
+    for (int i = j - 1; i >= 0; --i) {
+      double s = 0.0;
+      for (int k = i + 1; k <= j; ++k) s += u[i + (size_t)n * k] * x[k + (size_t)n * j];
+      x[i + (size_t)n * j] = -s / u[i + (size_t)n * i];
+    }
+  }
+}
+
+/* c (real view 2*ngw x n, ld 2*ldc) <- c * m (m n x n) */
+static void right_mul(int ngw, double* cd, int ldc, const double* m, int n) {
+  double* row = malloc(sizeof(double) * n);
+  for (int r = 0; r < 2 * ngw; ++r) {
+    for (int j = 0; j < n; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < n; ++k) s += cd[r + (size_t)2 * ldc * k] * m[k + (size_t)n * j];
+      row[j] = s;
+    }
+    for (int j = 0; j < n; ++j) cd[r + (size_t)2 * ldc * j] = row[j];
+  }
+  free(row);
+}
+
+/* rgs, src/rgs_utils.mod.F90:31-93 with uinv :95-162; smat receives U^-1 */
+int or_rgs(int ngw, int geq0, double* cp, int ldc, int n, double* smat) {
+  double* s = malloc(sizeof(double) * n * n);
+  or_ovlap(ngw, geq0, cp, cp, ldc, n, s);
+  int info = or_potrf_upper(s, n);
+  if (info) { free(s); return info; }
+  or_trtri_upper(s, n, smat);
+  right_mul(ngw, cp, ldc, smat, n);
+  if (geq0)
+    for (int i = 0; i < n; ++i) cp[(size_t)2 * ldc * i + 1] = 0.0;
+  free(s);
+  return 0;
+}
+
+/* symmetric eigen-decomposition by cyclic Jacobi rotations: a (n x n) is
+ * destroyed, w eigenvalues, z eigenvectors (columns) */
+void or_jacobi_eigen(double* a, int n, double* w, double* z) {
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) z[i + (size_t)n * j] = (i == j);
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < j; ++i) off += a[i + (size_t)n * j] * a[i + (size_t)n * j];
+    if (off < 1e-30) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double apq = a[p + (size_t)n * q];
+        if (fabs(apq) < 1e-300) continue;
+        double app = a[p + (size_t)n * p], aqq = a[q + (size_t)n * q];
+        double theta = (aqq - app) / (2.0 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          double akp = a[k + (size_t)n * p], akq = a[k + (size_t)n * q];
+          a[k + (size_t)n * p] = c * akp - s * akq;
+          a[k + (size_t)n * q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          double apk = a[p + (size_t)n * k], aqk = a[q + (size_t)n * k];
+          a[p + (size_t)n * k] = c * apk - s * aqk;
+          a[q + (size_t)n * k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          double zkp = z[k + (size_t)n * p], zkq = z[k + (size_t)n * q];
+          z[k + (size_t)n * p] = c * zkp - s * zkq;
+          z[k + (size_t)n * q] = s * zkp + c * zkq;
+        }
+      }
+  }
+  for (int i = 0; i < n; ++i) w[i] = a[i + (size_t)n * i];
+}
+
+/* lowdin, src/lowdin_utils.mod.F90:27-192 (closed shell):
+ * A(i,j) = sum_k z(i,k) w_k^-1/2 z(j,k), C <- C A, Im c(G=0) = 0 */
+void or_lowdin(int ngw, int geq0, double* cp, int ldc, int n, double* amat) {
+  double* s = malloc(sizeof(double) * n * n);
+  double* z = malloc(sizeof(double) * n * n);
+  double* w = malloc(sizeof(double) * n);
+  or_ovlap(ngw, geq0, cp, cp, ldc, n, s);
+  or_jacobi_eigen(s, n, w, z);
+  for (int k = 0; k < n; ++k) w[k] = 1.0 / sqrt(w[k]);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      double t = 0.0;
+      for (int k = 0; k < n; ++k) t += w[k] * z[j + (size_t)n * k] * z[i + (size_t)n * k];
+      amat[i + (size_t)n * j] = t;
+    }
+  right_mul(ngw, cp, ldc, amat, n);
+  if (geq0)
+    for (int i = 0; i < n; ++i) cp[(size_t)2 * ldc * i + 1] = 0.0;
+  free(s); free(z); free(w);
+}
+
+/* rotate, src/rotate_utils.mod.F90:63-100: c2 = a c1 gam + b c2 */
+void or_rotate(int ngw, double a, const double* c1, double b, double* c2, int ldc, const double* gam, int n) {
+  double* row = malloc(sizeof(double) * n);
+  for (int r = 0; r < 2 * ngw; ++r) {
+    for (int j = 0; j < n; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < n; ++k) s += c1[r + (size_t)2 * ldc * k] * gam[k + (size_t)n * j];
+      row[j] = a * s + b * c2[r + (size_t)2 * ldc * j];
+    }
+    for (int j = 0; j < n; ++j) c2[r + (size_t)2 * ldc * j] = row[j];
+  }
+  free(row);
+}
+
+/* Lagrange projection, src/forces_driver.mod.F90:344-370 (iproj = 2) */
+void or_lagrange(int ngw, int geq0, const double* c0, double* c2, int ldc, const double* f, int n) {
+  double* gam = malloc(sizeof(double) * n * n);
+  or_ovlap(ngw, geq0, c2, c0, ldc, n, gam);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) gam[i + (size_t)n * j] *= (f[i] != 0.0 ? 1.0 : 0.0) * (f[j] != 0.0 ? 1.0 : 0.0);
+  or_rotate(ngw, -1.0, c0, 1.0, c2, ldc, gam, n);
+  if (geq0)
+    for (int i = 0; i < n; ++i) c2[(size_t)2 * ldc * i + 1] = 0.0;
+  free(gam);
+}
+
+/* rortho, src/rortog_utils.mod.F90:315-498; returns iterations, -1 if > maxit */
+int or_rortho(int ngw, int geq0, const double* c0, double* cm, int ldc, double* gam, int tnon, int n, double epsog,
+              int maxit) {
+  size_t nn = (size_t)n * n;
+  double *rho = malloc(sizeof(double) * nn), *sig = malloc(sizeof(double) * nn), *del = malloc(sizeof(double) * nn);
+  double *gamn = malloc(sizeof(double) * nn), *scr = malloc(sizeof(double) * nn), *scr2 = malloc(sizeof(double) * nn);
+#define M(a, i, j) a[(i) + (size_t)n * (j)]
+  or_ovlap(ngw, geq0, c0, cm, ldc, n, rho);
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) { double v = 0.5 * (M(rho, i, j) + M(rho, j, i)); M(rho, i, j) = v; M(rho, j, i) = v; }
+  for (size_t e = 0; e < nn; ++e) rho[e] = -rho[e];
+  for (int i = 0; i < n; ++i) M(rho, i, i) += 1.0;
+  or_ovlap(ngw, geq0, cm, cm, ldc, n, sig);
+  for (size_t e = 0; e < nn; ++e) sig[e] = -sig[e];
+  for (int i = 0; i < n; ++i) M(sig, i, i) += 1.0;
+  if (tnon) {
+    or_ovlap(ngw, geq0, c0, c0, ldc, n, del);
+    for (int i = 0; i < n; ++i) M(del, i, i) -= 1.0;
+    for (size_t e = 0; e < nn; ++e) del[e] *= -0.5;
+  }
+  for (size_t e = 0; e < nn; ++e) gam[e] = 0.5 * sig[e];
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < n; ++k) s += M(rho, k, i) * M(rho, k, j);
+      scr[i + (size_t)n * j] = 0.5 * s + 0.5 * M(sig, i, j);
+    }
+  memcpy(sig, scr, sizeof(double) * nn);
+  int iter = 0;
+  double difgam;
+  do {
+    ++iter;
+    memcpy(gamn, sig, sizeof(double) * nn);
+    for (size_t e = 0; e < nn; ++e) scr[e] = rho[e] - gam[e];
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < n; ++k) s += M(scr, k, i) * M(scr, k, j);
+        M(gamn, i, j) += -0.5 * s;
+      }
+    if (tnon) {
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          double s = 0.0;
+          for (int k = 0; k < n; ++k) s += M(del, i, k) * M(gam, k, j);
+          M(scr2, i, j) = s;
+        }
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          double s = 0.0;
+          for (int k = 0; k < n; ++k) s += M(gam, k, i) * M(scr2, k, j);
+          M(gamn, i, j) += s;
+        }
+    }
+    difgam = 0.0;
+    for (size_t e = 0; e < nn; ++e) { double d = fabs(gamn[e] - gam[e]); if (d > difgam) difgam = d; }
+    memcpy(gam, gamn, sizeof(double) * nn);
+  } while (difgam > epsog * 0.5 && iter <= maxit);
+  int ret = iter > maxit ? -1 : iter;
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) { double v = 0.5 * (M(gam, i, j) + M(gam, j, i)); M(gam, i, j) = v; M(gam, j, i) = v; }
+  or_rotate(ngw, 1.0, c0, 1.0, cm, ldc, gam, n);
+#undef M
+  free(rho); free(sig); free(del); free(gamn); free(scr); free(scr2);
+  return ret;
+}
+
+/* (-i)^l exactly, as CPMD's integer power (0,-1)**l */
+static cplx mi_pow(int l) {
+  switch (((l % 4) + 4) % 4) {
+    case 0: return 1.0;
+    case 1: return -I;
+    case 2: return -1.0;
+    default: return I;
+  }
+}
+
+/* rnlsm1, src/rnlsm1_utils.mod.F90:42-193 (Gamma, imagp = 1):
+ * fnl(isa,iv,i) = 2 sum_G Re(conj(scr) c), scr = (-i)^l twnl eigr, G=0 halved */
+void or_rnlsm1(int ngw, int geq0, int nsp, const int* na, int nhxs, const int* ngh, const int* lang,
+               const double* twnl, int ldtw, const double* eigrd, int lde, const double* c0d, int ldc, int n,
+               double* fnl) {
+  const cplx* eigr = (const cplx*)eigrd;
+  const cplx* c0 = (const cplx*)c0d;
+  int nat = 0;
+  for (int is = 0; is < nsp; ++is) nat += na[is];
+  memset(fnl, 0, sizeof(double) * nat * nhxs * n);
+  int isa0 = 0;
+  for (int is = 0; is < nsp; ++is) {
+    for (int iv = 0; iv < ngh[is]; ++iv) {
+      cplx ci = mi_pow(lang[iv + nhxs * is]);
+      for (int ia = 0; ia < na[is]; ++ia)
+        for (int i = 0; i < n; ++i) {
+          double s = 0.0;
+          for (int g = 0; g < ngw; ++g) {
+            cplx scr = twnl[g + (size_t)ldtw * (iv + nhxs * is)] * ci * eigr[g + (size_t)lde * (isa0 + ia)];
+            if (geq0 && g == 0) scr *= 0.5;
+            s += creal(scr) * creal(c0[g + (size_t)ldc * i]) + cimag(scr) * cimag(c0[g + (size_t)ldc * i]);
+          }
+          fnl[(isa0 + ia) + (size_t)nat * (iv + (size_t)nhxs * i)] = 2.0 * s;
+        }
+    }
+    isa0 += na[is];
+  }
+}
+
+/* fnonloc, src/fnonloc_utils.mod.F90:209-303 (Gamma, generic h(iv,jv)) */
+void or_fnonloc(int ngw, int nsp, const int* na, int nhxs, const int* ngh, const int* lang, const double* hmat,
+                const double* twnl, int ldtw, const double* eigrd, int lde, const double* fnl, const double* f,
+                double* c2d, int ldc, int n) {
+  const cplx* eigr = (const cplx*)eigrd;
+  cplx* c2 = (cplx*)c2d;
+  int nat = 0;
+  for (int is = 0; is < nsp; ++is) nat += na[is];
+  int isa0 = 0;
+  for (int is = 0; is < nsp; ++is) {
+    for (int iv = 0; iv < ngh[is]; ++iv) {
+      for (int i = 0; i < n; ++i) {
+        double ffi = f[i] < 1e-5 ? 1.0 : f[i];
+        cplx ctm = -ffi * mi_pow(lang[iv + nhxs * is]);
+        for (int g = 0; g < ngw; ++g) {
+          cplx aux = 0;
+          for (int ia = 0; ia < na[is]; ++ia) {
+            double dd = 0.0;
+            for (int jv = 0; jv < ngh[is]; ++jv)
+              dd += fnl[(isa0 + ia) + (size_t)nat * (jv + (size_t)nhxs * i)] *
+                    hmat[iv + nhxs * (jv + (size_t)nhxs * is)];
+            aux += eigr[g + (size_t)lde * (isa0 + ia)] * dd;
+          }
+          c2[g + (size_t)ldc * i] += twnl[g + (size_t)ldtw * (iv + nhxs * is)] * ctm * aux;
+        }
+      }
+    }
+    isa0 += na[is];
+  }
+}

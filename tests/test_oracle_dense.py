@@ -1,0 +1,192 @@
+"""Pins of the oracle's dense routines by the mathematics and by library
+routines (numpy/scipy LAPACK), not by restating the oracle's loops:
+
+* ovlap equals the real-space integral (1/N^3) sum_r psi_i(r) psi_j(r) of the
+  Gamma states (direct G sums, no FFT) -- pins the factor 2 and the G=0 term;
+* Cholesky / triangular inverse / eigenpairs against scipy / numpy LAPACK;
+* rgs and lowdin produce orthonormal sets equal to C U^-1 and C S^-1/2;
+* the Lagrange projection annihilates gradients in the span of C0 with a
+  symmetric coefficient matrix (and keeps the unoccupied rows);
+* rortho returns a symmetric gam and an orthonormal set (the CP constraint);
+* rnlsm1 equals the full-sphere projection sum_G conj(beta(G)) c(G), for an s
+  and a p projector;
+* fnonloc is -1/2 the gradient of E_NL = sum_i f_i fnl^T h fnl in the Gamma
+  metric (central finite differences).
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import oracle
+from cpmd_amd import system as S
+from util import small_system, psi_direct
+
+
+def test_ovlap_is_real_space_integral():
+    cell, gs, c = small_system("ortho2", 4, seed=11)
+    rng = np.random.default_rng(0)
+    c = c + 0.3 * (rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape)) * np.exp(-gs.hg[:gs.ngw, None])
+    c[0] = c[0].real
+    a = oracle.ovlap(c, c)
+    psis = [psi_direct(cell, gs.miller[:gs.ngw], c[:, i]).real for i in range(4)]
+    nn = np.prod(cell.nr)
+    ref = np.array([[np.sum(psis[i] * psis[j]) / nn for j in range(4)] for i in range(4)])
+    assert np.abs(a - ref).max() < 1e-12
+    _, _, c2 = small_system("ortho2", 4, seed=12)
+    b = oracle.ovlap(c, c2)
+    psis2 = [psi_direct(cell, gs.miller[:gs.ngw], c2[:, i]).real for i in range(4)]
+    ref2 = np.array([[np.sum(psis[i] * psis2[j]) / nn for j in range(4)] for i in range(4)])
+    assert np.abs(b - ref2).max() < 1e-12
+
+
+def _spd(n, seed=0, cond=10.0):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return q @ np.diag(np.linspace(1.0, cond, n)) @ q.T
+
+
+@pytest.mark.parametrize("n", [1, 5, 33])
+def test_cholesky_trtri_eigen_vs_lapack(n):
+    s = _spd(n, n)
+    u, info = oracle.potrf_upper(s)
+    assert info == 0
+    assert np.abs(u - sla.cholesky(s, lower=False)).max() < 1e-12
+    x = oracle.trtri_upper(u)
+    assert np.abs(x - sla.solve_triangular(u, np.eye(n))).max() < 1e-12
+    w, z = oracle.jacobi_eigen(s)
+    order = np.argsort(w)
+    assert np.abs(w[order] - np.linalg.eigvalsh(s)).max() < 1e-12
+    assert np.abs(z @ np.diag(w) @ z.T - s).max() < 1e-12
+    bad = s.copy()
+    bad[2 % n, 2 % n] = -1.0
+    assert oracle.potrf_upper(bad)[1] == (2 % n) + 1
+
+
+def _perturbed(seed=3, n=6, eps=0.05):
+    cell, gs, c = small_system("cubic", n, seed=seed)
+    rng = np.random.default_rng(seed)
+    d = (rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape)) * np.exp(-gs.hg[:gs.ngw, None] / 4)
+    d[0] = d[0].real
+    return cell, gs, np.asfortranarray(c + eps * d)
+
+
+def test_rgs_and_lowdin():
+    cell, gs, c = _perturbed()
+    s = S.dotp_gamma(c, c)
+    u = sla.cholesky(s, lower=False)
+    cr, smat = oracle.rgs(c)
+    assert np.abs(smat - np.linalg.inv(u)).max() < 1e-12
+    ref = c @ np.linalg.inv(u)
+    ref[0] = ref[0].real
+    assert np.abs(cr - ref).max() < 1e-12
+    assert np.abs(S.dotp_gamma(cr, cr) - np.eye(c.shape[1])).max() < 1e-13
+    cl, amat = oracle.lowdin(c)
+    w, z = np.linalg.eigh(s)
+    ainv = z @ np.diag(w ** -0.5) @ z.T
+    assert np.abs(amat - ainv).max() < 1e-12
+    ref = c @ ainv
+    ref[0] = ref[0].real
+    assert np.abs(cl - ref).max() < 1e-12
+    assert np.abs(S.dotp_gamma(cl, cl) - np.eye(c.shape[1])).max() < 1e-13
+
+
+def test_rotate_and_lagrange():
+    cell, gs, c0 = small_system("cubic", 5, seed=1)
+    rng = np.random.default_rng(2)
+    c2 = np.asfortranarray(rng.standard_normal(c0.shape) + 1j * rng.standard_normal(c0.shape))
+    gam = rng.standard_normal((5, 5))
+    r = oracle.rotate(0.7, c0, -1.3, c2, gam)
+    assert np.abs(r - (0.7 * c0 @ gam - 1.3 * c2)).max() < 1e-13
+    # a gradient c0 M with M symmetric lies in the span and is projected away
+    m = rng.standard_normal((5, 5))
+    m = m + m.T
+    f = np.array([2.0, 2.0, 0.0, 2.0, 1.0])
+    g = np.asfortranarray(c0 @ m)
+    out = oracle.lagrange(c0, g, f)
+    occ = f != 0
+    # occupied-occupied block removed; rows/cols touching the empty state kept
+    mm = m * np.outer(occ, occ)
+    ref = g - c0 @ mm
+    ref[0] = ref[0].real
+    assert np.abs(out - ref).max() < 1e-12
+    assert np.abs(out[:, occ] - c0[:, ~occ] @ m[np.ix_(~occ, occ)]).max() < 1e-12
+
+
+def test_rortho_constraint():
+    """With c0 orthonormal and A = <c0|cm> symmetric (the CP case: the step lies
+    along a velocity that satisfies the velocity constraint), the fixed point
+    makes cm + c0 gam orthonormal; in general it solves
+    gam^2 + sym(A) gam + gam sym(A) = I - <cm|cm>."""
+    cell, gs, c0 = small_system("cubic", 6, seed=4)
+    rng = np.random.default_rng(5)
+    d = (rng.standard_normal(c0.shape) + 1j * rng.standard_normal(c0.shape)) * np.exp(-gs.hg[:gs.ngw, None] / 4)
+    d[0] = d[0].real
+    d = d - c0 @ S.dotp_gamma(c0, d)                     # orthogonal to c0
+    m = rng.standard_normal((6, 6))
+    m = m + m.T
+    cm = np.asfortranarray(c0 + 0.02 * (d + c0 @ m))
+    out, gam, it = oracle.rortho(c0, cm, epsog=1e-12, maxit=50)
+    assert it > 1
+    assert np.abs(gam - gam.T).max() == 0.0
+    assert np.abs(S.dotp_gamma(out, out) - np.eye(6)).max() < 1e-11
+    assert np.abs(out - (cm + c0 @ gam)).max() < 1e-13
+    # general cm: the quadratic the iteration solves
+    cm2 = np.asfortranarray(c0 + 0.02 * d + 0.01 * (c0 @ rng.standard_normal((6, 6))))
+    _, gam2, _ = oracle.rortho(c0, cm2, epsog=1e-12, maxit=50)
+    a = S.dotp_gamma(c0, cm2)
+    asym = 0.5 * (a + a.T)
+    res = gam2 @ gam2 + asym @ gam2 + gam2 @ asym - (np.eye(6) - S.dotp_gamma(cm2, cm2))
+    assert np.abs(res).max() < 1e-11
+    # non-convergence is reported
+    assert oracle.rortho(c0, cm2, epsog=1e-30, maxit=3)[2] == -1
+
+
+def _proj_case(lval):
+    cell, gs, c = small_system("ortho", 3, seed=8)
+    ngw = gs.ngw
+    g = gs.miller[:ngw] @ cell.bvec * cell.tpiba       # bohr^-1
+    g2 = (g ** 2).sum(1)
+    if lval == 0:
+        tw = np.exp(-0.3 * g2)
+    else:
+        tw = g[:, 0] * np.exp(-0.3 * g2)                 # odd in G: a p_x form factor
+    tau = np.array([[1.0, 2.0, 0.5], [3.0, 0.2, 4.0]])
+    frac = tau @ np.linalg.inv(cell.avec)
+    eigr = np.exp(-2j * math.pi * (gs.miller[:ngw] @ frac.T))
+    P = oracle.Projectors(na=[2], nhxs=1, ngh=[1], lang=[lval], hmat=[1.7], twnl=tw[:, None, None], eigr=eigr)
+    return cell, gs, c, P, tw, eigr
+
+
+@pytest.mark.parametrize("lval", [0, 1])
+def test_rnlsm1_full_sphere(lval):
+    cell, gs, c, P, tw, eigr = _proj_case(lval)
+    fnl = oracle.rnlsm1(P, c)
+    # full sphere: beta(-G) = conj(beta(G)) for both parities of twnl
+    m = gs.miller[:gs.ngw]
+    for ia in range(2):
+        beta = ((-1j) ** lval) * tw * eigr[:, ia]
+        for i in range(3):
+            s = np.sum(np.conj(beta) * c[:, i]) + np.sum(np.conj(np.conj(beta[1:])) * np.conj(c[1:, i]))
+            assert abs(s.imag) < 1e-12
+            assert fnl[ia, 0, i] == pytest.approx(s.real, abs=1e-12)
+
+
+@pytest.mark.parametrize("lval", [0, 1])
+def test_fnonloc_is_gradient(lval):
+    cell, gs, c, P, tw, eigr = _proj_case(lval)
+    f = np.array([2.0, 1.0, 2.0])
+
+    def energy(cc):
+        fnl = oracle.rnlsm1(P, cc)
+        return sum(f[i] * 1.7 * np.sum(fnl[:, 0, i] ** 2) for i in range(3))
+
+    fnl = oracle.rnlsm1(P, c)
+    c2 = oracle.fnonloc(P, np.zeros_like(c), f, fnl)
+    rng = np.random.default_rng(9)
+    d = rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape)
+    d[0] = d[0].real
+    h = 1e-5
+    de = (energy(c + h * d) - energy(c - h * d)) / (2 * h)
+    assert np.trace(S.dotp_gamma(c2, d)) == pytest.approx(-0.5 * de, rel=1e-7)
