@@ -1,11 +1,17 @@
 #!/usr/bin/env python3
 """Benchmark of the plane-wave inner loop on one MI355X.
 
-A step is one SCF step of the workload named in BASELINE (512 states on the
-192^3 mesh, 198,204 Gamma plane waves, cubic a = 32 bohr, 256 atoms, ecut
-80 Ry): rhoofr + vpsi, with every input already resident in HBM.  The
-wavefunctions are the seeded synthetic start (complex Gaussian
-~exp(-|G|^2/8), Loewdin-orthonormalised) and V(r) a seeded smooth potential.
+A step is one wavefunction-optimisation step of the 512-state workload on the
+192^3 mesh (198,204 Gamma plane waves, cubic a = 32 bohr, 256 atoms, ecut
+80 Ry) with every input resident in HBM:
+
+  rnlsm1 -> rhoofr -> vpsi -> fnonloc -> Lagrange projection
+         -> steepest-descent update c0 += dt c2 -> Loewdin orthonormalisation
+
+(CPMD: forces_driver rnlsm / rscpot(rhoofr) / vpsi / fnonloc / addup2, then
+updwf + ortho).  The local potential V(r) is a fixed seeded input (vofrho is
+not part of the measured path).  Wavefunctions: seeded complex Gaussian
+~exp(-|G|^2/8), Loewdin-orthonormalised; one s projector per atom (GTH).
 
 Prints one JSON line (see the measurement contract in DESIGN.md).
 """
@@ -96,6 +102,7 @@ def main():
     ap.add_argument("--nstate", type=int, default=0)
     ap.add_argument("--batch-rho", type=int, default=16)
     ap.add_argument("--batch-vpsi", type=int, default=8)
+    ap.add_argument("--dt", type=float, default=0.02)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -123,19 +130,31 @@ def main():
 
     ctx = ops.Context(sysm.cell, sysm.gset)
     ctx.set_batch(args.batch_rho, args.batch_vpsi)
+    na, nhxs, ngh, lang, hmat, twnl, eigr = S.projector_set(sysm, sysm.gset.miller[:ngw], sysm.gset.hg[:ngw])
+    ctx.set_projectors(na, nhxs, ngh, lang, hmat, twnl)
+    ctx.set_eigr(eigr)
+    del eigr, twnl
     d_c0 = ctx.upload(c0)
     d_c2 = ctx.alloc_wfn(nstate)
     d_rho = ops.DeviceArray(8 * ctx.nnr1)
     d_v = ops.DeviceArray(8 * ctx.nnr1).from_host(v)
     f = sysm.f
+    nat = int(np.sum(na))
+    dt = args.dt
 
     energies = {}
+    lowdin_iters = []
 
     def step():
+        ctx.rnlsm1_dev(d_c0, nstate)
         ek, cg, cr = ctx.rhoofr_dev(d_c0, f, nstate, d_rho)
         energies.update(ekin=ek, csumg=cg, csumr=cr)
         ctx.zero_wfn(d_c2, nstate)
         ctx.vpsi_dev(d_c0, d_c2, f, d_v, nstate)
+        ctx.fnonloc_dev(d_c2, f, nstate)
+        ctx.lagrange_dev(d_c0, d_c2, f, nstate)
+        ctx.axpy_wfn(dt, d_c2, d_c0, nstate)
+        lowdin_iters.append(ctx.lowdin_dev(d_c0, nstate))
 
     for i in range(args.warmup):
         step()
@@ -159,12 +178,12 @@ def main():
     npairs = (nstate + 1) // 2
     kernels = {}
     for kid, P in (("zinv", None), ("yinv", None), ("xrho", args.batch_rho), ("xvpsi", args.batch_vpsi),
-                   ("yfwd", args.batch_vpsi), ("zfwd", args.batch_vpsi), ("misc", None)):
+                   ("yfwd", args.batch_vpsi), ("zfwd", args.batch_vpsi), ("misc", None), ("gemm", None)):
         tot, cnt = ctx.timer_get(kid)
         if cnt:
             kernels[kid] = {"total_ms_per_step": tot / args.steps, "launches": cnt, "avg_ms": tot / cnt}
     # dominant kernel by time per step
-    dom = max((k for k in kernels if k != "misc"), key=lambda k: kernels[k]["total_ms_per_step"])
+    dom = max((k for k in kernels if k not in ("misc", "gemm")), key=lambda k: kernels[k]["total_ms_per_step"])
     # pairs per launch of the dominant kernel (full batches dominate; average over the step)
     if dom in ("xrho",):
         launches_per_step = -(-npairs // min(args.batch_rho, npairs))
@@ -182,7 +201,20 @@ def main():
         b_launch = kernel_bytes(geom, dom, 0) + (kernel_bytes(geom, dom, 1) - kernel_bytes(geom, dom, 0)) * pairs_per_launch
     achieved = b_launch / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
     step_bytes = step_model_bytes(geom, nstate)
-    step_gbs = step_bytes / (ms * 1e-3) / 1e9
+    fft_ms = sum(kernels[k]["total_ms_per_step"] for k in ("zinv", "yinv", "xrho", "xvpsi", "yfwd", "zfwd")
+                 if k in kernels)
+
+    # MFMA contractions: algorithmic flops per step (K = 2 ngw real rows)
+    K = 2.0 * ngw
+    n = float(nstate)
+    gemm_flops = (2 * nat * n * K            # rnlsm1
+                  + 2 * K * n * nat          # fnonloc
+                  + 2 * n * n * K            # Lagrange: ovlap(c2, c0)
+                  + 2 * K * n * n            # Lagrange: rotate
+                  + n * n * K                # Loewdin: S = C^T C (symmetric)
+                  + 2 * K * n * n)           # Loewdin: C S^-1/2
+    gemm_ms = kernels.get("gemm", {}).get("total_ms_per_step", float("nan"))
+    gemm_tflops = gemm_flops / (gemm_ms * 1e-3) / 1e12
 
     traffic = None
     tf = ROOT / "profiles" / "traffic.json"
@@ -214,7 +246,8 @@ def main():
         "config": {
             "workload": f"{args.config}: {nstate} states, {n1}x{n2}x{n3} mesh, {ngw} Gamma plane waves, "
                         f"a={cfg['a']} bohr, ecut {cfg['ecut']} Ry",
-            "step": "rhoofr + vpsi (FP64 GPU path)",
+            "step": "rnlsm1 + rhoofr + vpsi + fnonloc + Lagrange projection + SD update + Loewdin (FP64 GPU path)",
+            "atoms": nat,
             "precision_path": "fp64",
             "batch_pairs": {"rhoofr": args.batch_rho, "vpsi": args.batch_vpsi},
         },
@@ -229,8 +262,13 @@ def main():
             "bytes_per_launch": b_launch,
             "avg_launch_ms": kernels[dom]["avg_ms"],
         },
-        "step_roofline": {"model_bytes": step_bytes, "achieved_GBs": round(step_gbs, 1),
-                          "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "fft_roofline": {"model_bytes": step_bytes, "ms_per_step": round(fft_ms, 4),
+                         "achieved": round(step_bytes / (fft_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(step_bytes / (fft_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "mfma_roofline": {"bound": "mfma", "flops_per_step": gemm_flops, "gemm_ms_per_step": gemm_ms,
+                          "achieved": round(gemm_tflops, 2), "peak": 78.6, "unit": "TFLOP/s",
+                          "frac": round(gemm_tflops / 78.6, 4)},
+        "lowdin_newton_schulz_iters": lowdin_iters[-args.steps:],
         "kernels": kernels,
         "cpu_baseline": cpu,
     }

@@ -35,6 +35,34 @@ def hipcc() -> str:
     return "hipcc"
 
 
+FORTRAN_DIR = PKG / "fortran"
+DRIVER_SRC = ROOT / "tests" / "fortran" / "shim_driver.F90"
+DRIVER = FORTRAN_DIR / "build" / "shim_driver.x"
+
+
+def amdflang() -> str:
+    for p in (os.environ.get("AMDFLANG"), "/opt/rocm/bin/amdflang"):
+        if p and Path(p).exists():
+            return p
+    return "amdflang"
+
+
+def build_fortran(verbose: bool = False) -> Path:
+    """Compiles the ISO_C_BINDING module and the test driver that calls the
+    library with CPMD's Fortran signatures."""
+    out = DRIVER.parent
+    out.mkdir(parents=True, exist_ok=True)
+    cmd = [amdflang(), "-O2", "-module-dir", str(out), str(FORTRAN_DIR / "micpmd.F90"), str(DRIVER_SRC),
+           "-L", str(PKG), "-lmicpmd", f"-Wl,-rpath,{PKG}", "-o", str(DRIVER)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("amdflang failed building the Fortran shim")
+    return DRIVER
+
+
 def build(verbose: bool = False) -> Path:
     srcs = [str(CSRC / s) for s in HIP_SOURCES]
     tmp = LIB.with_suffix(".so.tmp")
@@ -57,3 +85,5 @@ def build(verbose: bool = False) -> Path:
 if __name__ == "__main__":
     build(verbose=True)
     print(f"built {LIB}")
+    build_fortran(verbose=True)
+    print(f"built {DRIVER}")

@@ -203,6 +203,11 @@ __global__ void k_zero(double* __restrict__ x, size_t n) {
     x[i] = 0.0;
 }
 
+__global__ void k_axpy(double alpha, const double* __restrict__ x, double* __restrict__ y, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] += alpha * x[i];
+}
+
 void wfn_upload(mc_ctx* ctx, const double* host, int ldc, int nstate, double2* dev) {
   MC_CHECK(ldc >= ctx->ngw && nstate >= 0, MC_ERR_ARG, "wfn_upload: ldc < ngw or nstate < 0");
   if (nstate == 0) return;
@@ -419,6 +424,18 @@ int mc_wfn_zero(mc_ctx* ctx, void* c_dev, int nstate) {
   const size_t n = (size_t)ctx->ldg * nstate * 2;
   if (n) {
     hipLaunchKernelGGL(mc::k_zero, dim3(1024), dim3(256), 0, ctx->stream, (double*)c_dev, n);
+    MC_HIP(hipGetLastError());
+  }
+  MC_API_END
+}
+
+int mc_wfn_axpy(mc_ctx* ctx, double alpha, const void* x_dev, void* y_dev, int nstate) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && x_dev && y_dev && nstate >= 0, MC_ERR_ARG, "bad argument");
+  const size_t n = (size_t)ctx->ldg * nstate * 2;
+  if (n) {
+    hipLaunchKernelGGL(mc::k_axpy, dim3(2048), dim3(256), 0, ctx->stream, alpha, (const double*)x_dev,
+                       (double*)y_dev, n);
     MC_HIP(hipGetLastError());
   }
   MC_API_END

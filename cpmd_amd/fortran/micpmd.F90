@@ -1,0 +1,362 @@
+! ==================================================================
+! micpmd: CPMD's hot-path routines with their Fortran signatures, bound
+! to the micpmd C ABI (include/micpmd.h) through ISO_C_BINDING.
+!
+! The module keeps one library context (the role of CPMD's fft/cppt
+! module state, built by fftprp_default_init / setfftn) and forwards:
+!
+!   micpmd_vpsi    (c0,c2,f,vpot,psi,nstate,ikind,ispin,redist_c2)
+!                  = vpsi,    src/vpsi_utils.mod.F90:120
+!   micpmd_rhoofr  (c0,rhoe,psi,nstate,f)  = rhoofr, src/rhoofr_utils.mod.F90:122
+!                  (f = crge%f(:,1); ekin/csumg/csumr are left in
+!                   micpmd_ekin, micpmd_csumg, micpmd_csumr)
+!   micpmd_rgs     (cp,nstate,smat)        = rgs,    src/rgs_utils.mod.F90:31
+!   micpmd_lowdin  (cp,cn,nstate)          = lowdin, src/lowdin_utils.mod.F90:27
+!   micpmd_ovlap   (nstate,a,c1,c2)        = ovlap,  src/ovlap_utils.mod.F90:57
+!   micpmd_rotate  (a,c1,b,c2,gam,nstate)  = rotate, src/rotate_utils.mod.F90:63
+!   micpmd_rortog  (c0,cm,gam,tnon,nstate,epsog,maxit)
+!                  = rortog -> rortho, src/rortog_utils.mod.F90:44,315
+!   micpmd_rnlsm1  (c0,nstate,fnl)         = rnlsm1, src/rnlsm1_utils.mod.F90:42
+!   micpmd_fnonloc (c2,f,nstate,fnl)       = fnonloc, src/fnonloc_utils.mod.F90:55
+!   micpmd_lagrange(c0,c2,f,nstate)        = addup2 projection,
+!                                            src/forces_driver.mod.F90:344-370
+!
+! Every non-zero status of the C ABI is turned into a call of the
+! procedure registered with micpmd_set_stop (CPMD: stopgm(procedureN,
+! msg, __LINE__, __FILE__)); without one the program stops with the
+! library's message.
+! ==================================================================
+MODULE micpmd
+  USE, INTRINSIC :: iso_c_binding
+  IMPLICIT NONE
+
+  PRIVATE
+
+  INTEGER, PARAMETER :: real_8 = c_double
+
+  TYPE, BIND(C) :: mc_layout_t
+     INTEGER(c_int) :: nr1, nr2, nr3, kr1, kr2, kr3, ngw
+     TYPE(c_ptr)    :: inyh, hg
+     REAL(c_double) :: tpiba2, omega
+     INTEGER(c_int) :: geq0
+  END TYPE mc_layout_t
+
+  ABSTRACT INTERFACE
+     SUBROUTINE stop_proc(procedure_name, msg)
+       CHARACTER(*), INTENT(IN) :: procedure_name, msg
+     END SUBROUTINE stop_proc
+  END INTERFACE
+
+  TYPE(c_ptr), SAVE                       :: ctx = c_null_ptr
+  PROCEDURE(stop_proc), POINTER, SAVE     :: stop_hook => NULL()
+  INTEGER, SAVE                           :: ngw_s = 0, nat_s = 0, nhxs_s = 0
+
+  REAL(real_8), PUBLIC, SAVE :: micpmd_ekin = 0.0_real_8, micpmd_csumg = 0.0_real_8, &
+                                micpmd_csumr = 0.0_real_8
+
+  PUBLIC :: micpmd_init, micpmd_finalize, micpmd_set_stop
+  PUBLIC :: micpmd_vpsi, micpmd_rhoofr, micpmd_rgs, micpmd_lowdin, micpmd_ovlap
+  PUBLIC :: micpmd_rotate, micpmd_rortog, micpmd_lagrange
+  PUBLIC :: micpmd_set_projectors, micpmd_set_eigr, micpmd_rnlsm1, micpmd_fnonloc
+
+  INTERFACE
+     FUNCTION mc_last_error() BIND(C, name='mc_last_error') RESULT(p)
+       IMPORT :: c_ptr
+       TYPE(c_ptr) :: p
+     END FUNCTION mc_last_error
+     FUNCTION mc_ctx_create(lay, out) BIND(C, name='mc_ctx_create') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, mc_layout_t
+       TYPE(mc_layout_t), INTENT(IN) :: lay
+       TYPE(c_ptr), INTENT(OUT) :: out
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_ctx_create
+     FUNCTION mc_ctx_destroy(c) BIND(C, name='mc_ctx_destroy') RESULT(ierr)
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: c
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_ctx_destroy
+     FUNCTION mc_vpsi(c, c0, ldc, c2, f, vpot, nstate) BIND(C, name='mc_vpsi') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double_complex, c_double
+       TYPE(c_ptr), VALUE :: c
+       COMPLEX(c_double_complex), INTENT(IN) :: c0(*)
+       INTEGER(c_int), VALUE :: ldc, nstate
+       COMPLEX(c_double_complex), INTENT(INOUT) :: c2(*)
+       REAL(c_double), INTENT(IN) :: f(*), vpot(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_vpsi
+     FUNCTION mc_rhoofr(c, c0, ldc, f, nstate, rhoe, ekin, csumg, csumr) BIND(C, name='mc_rhoofr') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double_complex, c_double
+       TYPE(c_ptr), VALUE :: c
+       COMPLEX(c_double_complex), INTENT(IN) :: c0(*)
+       INTEGER(c_int), VALUE :: ldc, nstate
+       REAL(c_double), INTENT(IN) :: f(*)
+       REAL(c_double), INTENT(OUT) :: rhoe(*), ekin, csumg, csumr
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_rhoofr
+     FUNCTION mc_rgs(c, cp, ldc, nstate, smat) BIND(C, name='mc_rgs') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double_complex, c_double
+       TYPE(c_ptr), VALUE :: c
+       COMPLEX(c_double_complex), INTENT(INOUT) :: cp(*)
+       INTEGER(c_int), VALUE :: ldc, nstate
+       REAL(c_double), INTENT(OUT) :: smat(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_rgs
+     FUNCTION mc_lowdin(c, cp, ldc, nstate) BIND(C, name='mc_lowdin') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double_complex
+       TYPE(c_ptr), VALUE :: c
+       COMPLEX(c_double_complex), INTENT(INOUT) :: cp(*)
+       INTEGER(c_int), VALUE :: ldc, nstate
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_lowdin
+     FUNCTION mc_ovlap(c, nstate, a, c1, c2, ldc) BIND(C, name='mc_ovlap') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double_complex, c_double
+       TYPE(c_ptr), VALUE :: c
+       INTEGER(c_int), VALUE :: nstate, ldc
+       REAL(c_double), INTENT(OUT) :: a(*)
+       COMPLEX(c_double_complex), INTENT(IN) :: c1(*), c2(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_ovlap
+     FUNCTION mc_rotate(c, a, c1, b, c2, ldc, gam, nstate) BIND(C, name='mc_rotate') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double_complex, c_double
+       TYPE(c_ptr), VALUE :: c
+       REAL(c_double), VALUE :: a, b
+       COMPLEX(c_double_complex), INTENT(IN) :: c1(*)
+       COMPLEX(c_double_complex), INTENT(INOUT) :: c2(*)
+       INTEGER(c_int), VALUE :: ldc, nstate
+       REAL(c_double), INTENT(IN) :: gam(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_rotate
+     FUNCTION mc_rortog(c, c0, cm, ldc, gam, tnon, nstate, epsog, maxit, iters) BIND(C, name='mc_rortog') &
+          RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double_complex, c_double
+       TYPE(c_ptr), VALUE :: c
+       COMPLEX(c_double_complex), INTENT(IN) :: c0(*)
+       COMPLEX(c_double_complex), INTENT(INOUT) :: cm(*)
+       INTEGER(c_int), VALUE :: ldc, tnon, nstate, maxit
+       REAL(c_double), INTENT(OUT) :: gam(*)
+       REAL(c_double), VALUE :: epsog
+       INTEGER(c_int), INTENT(OUT) :: iters
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_rortog
+     FUNCTION mc_lagrange(c, c0, c2, ldc, f, nstate) BIND(C, name='mc_lagrange') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double_complex, c_double
+       TYPE(c_ptr), VALUE :: c
+       COMPLEX(c_double_complex), INTENT(IN) :: c0(*)
+       COMPLEX(c_double_complex), INTENT(INOUT) :: c2(*)
+       INTEGER(c_int), VALUE :: ldc, nstate
+       REAL(c_double), INTENT(IN) :: f(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_lagrange
+     FUNCTION mc_set_projectors(c, nsp, na, nhxs, ngh, lang, hmat, twnl, ldtw) &
+          BIND(C, name='mc_set_projectors') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: c
+       INTEGER(c_int), VALUE :: nsp, nhxs, ldtw
+       INTEGER(c_int), INTENT(IN) :: na(*), ngh(*), lang(*)
+       REAL(c_double), INTENT(IN) :: hmat(*), twnl(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_set_projectors
+     FUNCTION mc_set_eigr(c, eigr, lde, nat) BIND(C, name='mc_set_eigr') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double_complex
+       TYPE(c_ptr), VALUE :: c
+       COMPLEX(c_double_complex), INTENT(IN) :: eigr(*)
+       INTEGER(c_int), VALUE :: lde, nat
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_set_eigr
+     FUNCTION mc_rnlsm1(c, c0, ldc, nstate, fnl) BIND(C, name='mc_rnlsm1') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double_complex, c_double
+       TYPE(c_ptr), VALUE :: c
+       COMPLEX(c_double_complex), INTENT(IN) :: c0(*)
+       INTEGER(c_int), VALUE :: ldc, nstate
+       REAL(c_double), INTENT(OUT) :: fnl(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_rnlsm1
+     FUNCTION mc_fnonloc(c, c2, ldc, f, nstate, fnl) BIND(C, name='mc_fnonloc') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double_complex, c_double
+       TYPE(c_ptr), VALUE :: c
+       COMPLEX(c_double_complex), INTENT(INOUT) :: c2(*)
+       INTEGER(c_int), VALUE :: ldc, nstate
+       REAL(c_double), INTENT(IN) :: f(*), fnl(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_fnonloc
+     FUNCTION c_strlen(s) BIND(C, name='strlen') RESULT(n)
+       IMPORT :: c_ptr, c_size_t
+       TYPE(c_ptr), VALUE :: s
+       INTEGER(c_size_t) :: n
+     END FUNCTION c_strlen
+  END INTERFACE
+
+CONTAINS
+
+  ! ------------------------------------------------------------------
+  SUBROUTINE micpmd_set_stop(proc)
+    PROCEDURE(stop_proc) :: proc
+    stop_hook => proc
+  END SUBROUTINE micpmd_set_stop
+
+  SUBROUTINE fail(procedure_name, msg)
+    CHARACTER(*), INTENT(IN) :: procedure_name, msg
+    IF (ASSOCIATED(stop_hook)) THEN
+       CALL stop_hook(procedure_name, msg)
+    ELSE
+       WRITE(6,'(A,A,A,A)') ' micpmd| ', procedure_name, ': ', msg
+       ERROR STOP 1
+    ENDIF
+  END SUBROUTINE fail
+
+  SUBROUTINE check(ierr, procedure_name)
+    INTEGER(c_int), INTENT(IN) :: ierr
+    CHARACTER(*), INTENT(IN)   :: procedure_name
+    CHARACTER(kind=c_char), POINTER :: s(:)
+    CHARACTER(len=512) :: msg
+    TYPE(c_ptr) :: p
+    INTEGER :: i, n
+    IF (ierr == 0) RETURN
+    p = mc_last_error()
+    n = INT(c_strlen(p))
+    CALL c_f_pointer(p, s, [n])
+    msg = ' '
+    DO i = 1, MIN(n, LEN(msg))
+       msg(i:i) = s(i)
+    ENDDO
+    IF (ASSOCIATED(stop_hook)) THEN
+       CALL stop_hook(procedure_name, TRIM(msg))
+    ELSE
+       WRITE(6,'(A,A,A,I0,A,A)') ' micpmd| ', procedure_name, ': status ', ierr, ': ', TRIM(msg)
+       ERROR STOP 1
+    ENDIF
+  END SUBROUTINE check
+
+  ! ------------------------------------------------------------------
+  ! context from CPMD's layout: spar%nr1s.., fpar%kr1s.., ncpw%ngw,
+  ! cppt::inyh(3,ngw), cppt::hg(ngw), parm%tpiba2, parm%omega, geq0
+  SUBROUTINE micpmd_init(nr1, nr2, nr3, kr1, kr2, kr3, ngw, inyh, hg, tpiba2, omega, geq0)
+    INTEGER, INTENT(IN)                :: nr1, nr2, nr3, kr1, kr2, kr3, ngw
+    INTEGER(c_int), INTENT(IN), TARGET :: inyh(3, ngw)
+    REAL(real_8), INTENT(IN), TARGET   :: hg(ngw)
+    REAL(real_8), INTENT(IN)           :: tpiba2, omega
+    LOGICAL, INTENT(IN)                :: geq0
+    TYPE(mc_layout_t) :: lay
+    IF (C_ASSOCIATED(ctx)) CALL micpmd_finalize()
+    lay%nr1 = nr1; lay%nr2 = nr2; lay%nr3 = nr3
+    lay%kr1 = kr1; lay%kr2 = kr2; lay%kr3 = kr3
+    lay%ngw = ngw
+    lay%inyh = C_LOC(inyh)
+    lay%hg = C_LOC(hg)
+    lay%tpiba2 = tpiba2
+    lay%omega = omega
+    lay%geq0 = MERGE(1, 0, geq0)
+    CALL check(mc_ctx_create(lay, ctx), 'micpmd_init')
+    ngw_s = ngw
+  END SUBROUTINE micpmd_init
+
+  SUBROUTINE micpmd_finalize()
+    IF (C_ASSOCIATED(ctx)) CALL check(mc_ctx_destroy(ctx), 'micpmd_finalize')
+    ctx = c_null_ptr
+  END SUBROUTINE micpmd_finalize
+
+  ! ------------------------------------------------------------------
+  SUBROUTINE micpmd_vpsi(c0, c2, f, vpot, psi, nstate, ikind, ispin, redist_c2)
+    COMPLEX(real_8), INTENT(IN)    :: c0(:,:)
+    COMPLEX(real_8), INTENT(INOUT) :: c2(:,:)
+    REAL(real_8), INTENT(IN)       :: f(:), vpot(:,:)
+    COMPLEX(real_8)                :: psi(:)
+    INTEGER, INTENT(IN)            :: nstate, ikind, ispin
+    LOGICAL, INTENT(IN)            :: redist_c2
+    IF (ikind /= 1 .OR. ispin /= 1) CALL fail('vpsi', 'only the Gamma point with one spin (ikind=1, ispin=1)')
+    IF (SIZE(c2, 1) /= SIZE(c0, 1)) CALL fail('vpsi', 'c0 and c2 leading dimensions differ')
+    CALL check(mc_vpsi(ctx, c0, INT(SIZE(c0, 1), c_int), c2, f, vpot, INT(nstate, c_int)), 'vpsi')
+  END SUBROUTINE micpmd_vpsi
+
+  SUBROUTINE micpmd_rhoofr(c0, rhoe, psi, nstate, f)
+    COMPLEX(real_8), INTENT(IN) :: c0(:,:)
+    REAL(real_8), INTENT(OUT)   :: rhoe(:,:)
+    COMPLEX(real_8)             :: psi(:)
+    INTEGER, INTENT(IN)         :: nstate
+    REAL(real_8), INTENT(IN)    :: f(:)
+    CALL check(mc_rhoofr(ctx, c0, INT(SIZE(c0, 1), c_int), f, INT(nstate, c_int), rhoe, micpmd_ekin, &
+         micpmd_csumg, micpmd_csumr), 'rhoofr')
+  END SUBROUTINE micpmd_rhoofr
+
+  SUBROUTINE micpmd_rgs(cp, nstate, smat)
+    COMPLEX(real_8), INTENT(INOUT) :: cp(:,:)
+    INTEGER, INTENT(IN)            :: nstate
+    REAL(real_8), INTENT(OUT)      :: smat(nstate, nstate)
+    CALL check(mc_rgs(ctx, cp, INT(SIZE(cp, 1), c_int), INT(nstate, c_int), smat), 'rgs')
+  END SUBROUTINE micpmd_rgs
+
+  SUBROUTINE micpmd_lowdin(cp, cn, nstate)
+    COMPLEX(real_8), INTENT(INOUT) :: cp(:,:)
+    COMPLEX(real_8)                :: cn(:,:)
+    INTEGER, INTENT(IN)            :: nstate
+    CALL check(mc_lowdin(ctx, cp, INT(SIZE(cp, 1), c_int), INT(nstate, c_int)), 'lowdin')
+  END SUBROUTINE micpmd_lowdin
+
+  SUBROUTINE micpmd_ovlap(nstate, a, c1, c2)
+    INTEGER, INTENT(IN)         :: nstate
+    REAL(real_8), INTENT(OUT)   :: a(nstate, nstate)
+    COMPLEX(real_8), INTENT(IN) :: c1(:,:), c2(:,:)
+    CALL check(mc_ovlap(ctx, INT(nstate, c_int), a, c1, c2, INT(SIZE(c1, 1), c_int)), 'ovlap')
+  END SUBROUTINE micpmd_ovlap
+
+  SUBROUTINE micpmd_rotate(a, c1, b, c2, gam, nstate)
+    REAL(real_8), INTENT(IN)       :: a, b
+    COMPLEX(real_8), INTENT(IN)    :: c1(:,:)
+    COMPLEX(real_8), INTENT(INOUT) :: c2(:,:)
+    INTEGER, INTENT(IN)            :: nstate
+    REAL(real_8), INTENT(IN)       :: gam(nstate, nstate)
+    CALL check(mc_rotate(ctx, a, c1, b, c2, INT(SIZE(c1, 1), c_int), gam, INT(nstate, c_int)), 'rotate')
+  END SUBROUTINE micpmd_rotate
+
+  SUBROUTINE micpmd_rortog(c0, cm, gam, tnon, nstate, epsog, maxit)
+    COMPLEX(real_8), INTENT(IN)    :: c0(:,:)
+    COMPLEX(real_8), INTENT(INOUT) :: cm(:,:)
+    REAL(real_8), INTENT(OUT)      :: gam(nstate, nstate)
+    LOGICAL, INTENT(IN)            :: tnon
+    INTEGER, INTENT(IN)            :: nstate, maxit
+    REAL(real_8), INTENT(IN)       :: epsog
+    INTEGER(c_int) :: iters
+    CALL check(mc_rortog(ctx, c0, cm, INT(SIZE(c0, 1), c_int), gam, MERGE(1_c_int, 0_c_int, tnon), &
+         INT(nstate, c_int), epsog, INT(maxit, c_int), iters), 'rortog')
+  END SUBROUTINE micpmd_rortog
+
+  SUBROUTINE micpmd_lagrange(c0, c2, f, nstate)
+    COMPLEX(real_8), INTENT(IN)    :: c0(:,:)
+    COMPLEX(real_8), INTENT(INOUT) :: c2(:,:)
+    REAL(real_8), INTENT(IN)       :: f(:)
+    INTEGER, INTENT(IN)            :: nstate
+    CALL check(mc_lagrange(ctx, c0, c2, INT(SIZE(c0, 1), c_int), f, INT(nstate, c_int)), 'lagrange')
+  END SUBROUTINE micpmd_lagrange
+
+  ! ------------------------------------------------------------------
+  SUBROUTINE micpmd_set_projectors(na, nhxs, ngh, lang, hmat, twnl)
+    INTEGER(c_int), INTENT(IN) :: na(:), ngh(:), lang(:,:)
+    INTEGER, INTENT(IN)        :: nhxs
+    REAL(real_8), INTENT(IN)   :: hmat(:,:,:), twnl(:,:,:)
+    CALL check(mc_set_projectors(ctx, INT(SIZE(na), c_int), na, INT(nhxs, c_int), ngh, lang, hmat, twnl, &
+         INT(SIZE(twnl, 1), c_int)), 'set_projectors')
+    nat_s = SUM(na)
+    nhxs_s = nhxs
+  END SUBROUTINE micpmd_set_projectors
+
+  SUBROUTINE micpmd_set_eigr(eigr)
+    COMPLEX(real_8), INTENT(IN) :: eigr(:,:)
+    CALL check(mc_set_eigr(ctx, eigr, INT(SIZE(eigr, 1), c_int), INT(SIZE(eigr, 2), c_int)), 'set_eigr')
+  END SUBROUTINE micpmd_set_eigr
+
+  SUBROUTINE micpmd_rnlsm1(c0, nstate, fnl)
+    COMPLEX(real_8), INTENT(IN) :: c0(:,:)
+    INTEGER, INTENT(IN)         :: nstate
+    REAL(real_8), INTENT(OUT)   :: fnl(:,:,:)
+    CALL check(mc_rnlsm1(ctx, c0, INT(SIZE(c0, 1), c_int), INT(nstate, c_int), fnl), 'rnlsm1')
+  END SUBROUTINE micpmd_rnlsm1
+
+  SUBROUTINE micpmd_fnonloc(c2, f, nstate, fnl)
+    COMPLEX(real_8), INTENT(INOUT) :: c2(:,:)
+    REAL(real_8), INTENT(IN)       :: f(:)
+    INTEGER, INTENT(IN)            :: nstate
+    REAL(real_8), INTENT(IN)       :: fnl(:,:,:)
+    CALL check(mc_fnonloc(ctx, c2, INT(SIZE(c2, 1), c_int), f, INT(nstate, c_int), fnl), 'fnonloc')
+  END SUBROUTINE micpmd_fnonloc
+
+END MODULE micpmd

@@ -126,6 +126,9 @@ class Context:
     def zero_wfn(self, dev: DeviceArray, nstate: int):
         check(lib().mc_wfn_zero(self.h, dev.ptr, nstate))
 
+    def axpy_wfn(self, alpha: float, x: DeviceArray, y: DeviceArray, nstate: int):
+        check(lib().mc_wfn_axpy(self.h, alpha, x.ptr, y.ptr, nstate))
+
     def upload(self, c: np.ndarray, dev: DeviceArray | None = None) -> DeviceArray:
         c = np.asfortranarray(c, dtype=np.complex128)
         n = c.shape[1]

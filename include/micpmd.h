@@ -102,6 +102,10 @@ int mc_memset_d(void* dst, int value, size_t bytes);
  * src/forces_driver.mod.F90:175). */
 int mc_wfn_zero(mc_ctx* ctx, void* c_dev, int nstate);
 
+/* y += alpha x over nstate device wavefunctions (the steepest-descent update
+ * c0 <- c0 + dt c2 of updwf, src/updwf_utils.mod.F90:116-128). */
+int mc_wfn_axpy(mc_ctx* ctx, double alpha, const void* x_dev, void* y_dev, int nstate);
+
 /* Wavefunctions host (CPMD order, leading dim ldc) <-> device layout. */
 int mc_wfn_upload(mc_ctx* ctx, const double* c_host, int ldc, int nstate, void* c_dev);
 int mc_wfn_download(mc_ctx* ctx, const void* c_dev, int nstate, double* c_host, int ldc);
