@@ -390,6 +390,15 @@ int mc_ctx_set_batch(mc_ctx* ctx, int pairs_rhoofr, int pairs_vpsi) {
 
 size_t mc_wfn_ld(const mc_ctx* ctx) { return ctx ? (size_t)ctx->ldg : 0; }
 
+int mc_set_precision(mc_ctx* ctx, int fft_precision) {
+  MC_API_BEGIN
+  MC_CHECK(ctx, MC_ERR_ARG, "null context");
+  MC_CHECK(fft_precision == MC_PREC_FP64 || fft_precision == MC_PREC_FP32, MC_ERR_ARG, "unknown precision mode");
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->fft_fp32 = fft_precision == MC_PREC_FP32;
+  MC_API_END
+}
+
 int mc_dev_alloc(size_t bytes, void** ptr) {
   MC_API_BEGIN
   MC_CHECK(ptr, MC_ERR_ARG, "null pointer");

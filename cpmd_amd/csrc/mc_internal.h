@@ -167,6 +167,8 @@ struct mc_ctx {
   mc::DevBuf w_red2, w_info, w_wfn, w_f2, w_dd;
   mc::Projectors proj;
 
+  bool fft_fp32 = false;   // rhoofr/vpsi transforms and workspaces in FP32
+
   int batch_pairs_vpsi = 8;
   int batch_pairs_rho = 16;
 

@@ -9,12 +9,17 @@
 //   xvpsi: B (all P pairs), V(r)   -> B       (x^-1, *V, x^+1; V read once per batch)
 //   yfwd : B                       -> A       (y^+1)
 //   zfwd : A, C0, C2               -> C2      (z^+1, 1/(n1 n2 n3), decode, kinetic, c2 +=)
+// Every kernel is templated on the arithmetic type T of the transforms and of
+// the A/B workspaces (double: FP64 path; float: mixed-precision path; C0, C2,
+// V and rho stay FP64 in HBM) and on the compile-time length N of its axis
+// (0 = runtime plan).
 // Reference: vpsi src/vpsi_utils.mod.F90:376-675 (one pair per iteration through
 // invfftn/fwfftn, src/fftmain_utils.mod.F90:71-157) and rhoofr
 // src/rhoofr_utils.mod.F90:305-409.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "fft_device.h"
@@ -47,28 +52,43 @@ __device__ __forceinline__ double warp_sum(double v) {
   return v;
 }
 
+template <typename T>
+__device__ __forceinline__ void load_tw(v2<T>* tw, const double2* src, int n) {
+  for (int t = threadIdx.x; t < n; t += kThreads) {
+    const double2 w = src[t];
+    tw[t] = cmk<v2<T>>((T)w.x, (T)w.y);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ v2<T>* lds_base() {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  return reinterpret_cast<v2<T>*>(smem_raw);
+}
+
 // ---------------------------------------------------------------------------
 // zinv: Gamma pack of a state pair (c1 + i c2 at +G, conj(c1) + i conj(c2) at
 // -G, G=0 once; src/state_utils.mod.F90:171-189) into `lo` owner sticks and
 // their mirrors, inverse z transform, write to A.  Optionally accumulates the
 // per-state norms sum_G w_G |c|^2 (dotp, src/dotp_utils.mod.F90:58-101) and
-// sum_G hg |c|^2 (kin_energy, src/kin_energy_utils.mod.F90:95-103).
+// sum_G hg |c|^2 (kin_energy, src/kin_energy_utils.mod.F90:95-103) in FP64.
 // ---------------------------------------------------------------------------
+template <typename T, int N>
 __global__ __launch_bounds__(kThreads) void k_zinv(WParams w, const double2* __restrict__ c0, int nstate,
-                                                   int pair0, double2* __restrict__ A, int lo,
+                                                   int pair0, v2<T>* __restrict__ A, int lo,
                                                    double* __restrict__ red) {
-  extern __shared__ double2 smem[];
-  const int n3 = w.n3;
-  double2* tw = smem;
-  double2* buf = smem + n3;
+  using V = v2<T>;
+  const int n3 = N > 0 ? N : w.n3;
+  V* tw = lds_base<T>();
+  V* buf = tw + n3;
   const int pb = blockIdx.y;
   const int is1 = 2 * (pair0 + pb), is2 = is1 + 1;
   const bool has2 = is2 < nstate;
   const int o0 = blockIdx.x * lo;
   const int o1 = min(o0 + lo, w.n_own);
   const int nl = 2 * (o1 - o0);
-  for (int t = threadIdx.x; t < n3; t += kThreads) tw[t] = w.tw3[t];
-  for (int e = threadIdx.x; e < nl * n3; e += kThreads) buf[e] = make_double2(0.0, 0.0);
+  load_tw<T>(tw, w.tw3, n3);
+  for (int e = threadIdx.x; e < nl * n3; e += kThreads) buf[e] = cmk<V>((T)0, (T)0);
   __syncthreads();
   const int j0 = w.own_start[o0], j1 = w.own_start[o1];
   const double2* ca = c0 + (size_t)is1 * w.ldg;
@@ -80,11 +100,11 @@ __global__ __launch_bounds__(kThreads) void k_zinv(WParams w, const double2* __r
     const int t = o - o0;
     const double2 a = ca[j];
     const double2 b = has2 ? cb[j] : make_double2(0.0, 0.0);
-    buf[2 * t * n3 + l] = make_double2(a.x - b.y, a.y + b.x);
+    buf[2 * t * n3 + l] = cmk<V>((T)(a.x - b.y), (T)(a.y + b.x));
     const bool g0 = (j == 0) && w.geq0;
     if (!g0) {
       const int ml = (o == 0) ? 2 * t : 2 * t + 1;
-      buf[ml * n3 + (l == 0 ? 0 : n3 - l)] = make_double2(a.x + b.y, b.x - a.y);
+      buf[ml * n3 + (l == 0 ? 0 : n3 - l)] = cmk<V>((T)(a.x + b.y), (T)(b.x - a.y));
     }
     if (red) {
       const double na = a.x * a.x + a.y * a.y, nb = b.x * b.x + b.y * b.y;
@@ -96,7 +116,7 @@ __global__ __launch_bounds__(kThreads) void k_zinv(WParams w, const double2* __r
     }
   }
   __syncthreads();
-  fft_lines<1>(buf, nl, n3, w.p3, tw);
+  fft_any<T, N, 1>(buf, nl, n3, w.p3, tw);
   for (int e = threadIdx.x; e < nl * n3; e += kThreads) {
     const int line = e / n3, z = e - line * n3;
     const int o = o0 + (line >> 1);
@@ -124,22 +144,23 @@ __global__ __launch_bounds__(kThreads) void k_zinv(WParams w, const double2* __r
 // y pass over one x-frequency column and a tile of z planes.
 // DIR=+1: A (sticks of the column) -> B; DIR=-1: B -> A.
 // ---------------------------------------------------------------------------
-template <int DIR>
-__global__ __launch_bounds__(kThreads) void k_ypass(WParams w, double2* __restrict__ A,
-                                                    double2* __restrict__ B, int tz) {
-  extern __shared__ double2 smem[];
-  const int n2 = w.n2, n3 = w.n3;
-  double2* tw = smem;
-  double2* buf = smem + n2;
+template <typename T, int N, int DIR>
+__global__ __launch_bounds__(kThreads) void k_ypass(WParams w, v2<T>* __restrict__ A, v2<T>* __restrict__ B,
+                                                    int tz) {
+  using V = v2<T>;
+  const int n2 = N > 0 ? N : w.n2;
+  const int n3 = w.n3;
+  V* tw = lds_base<T>();
+  V* buf = tw + n2;
   const int c = blockIdx.x;
   const int z0 = blockIdx.y * tz;
   const int pb = blockIdx.z;
   const int nz = min(tz, n3 - z0);
   const int k0 = w.col_start[c], ncol = w.col_start[c + 1] - k0;
-  for (int t = threadIdx.x; t < n2; t += kThreads) tw[t] = w.tw2[t];
+  load_tw<T>(tw, w.tw2, n2);
   const size_t abase = (size_t)pb * w.n_slots * n3;
   if (DIR > 0) {
-    for (int e = threadIdx.x; e < nz * n2; e += kThreads) buf[e] = make_double2(0.0, 0.0);
+    for (int e = threadIdx.x; e < nz * n2; e += kThreads) buf[e] = cmk<V>((T)0, (T)0);
     __syncthreads();
     for (int e = threadIdx.x; e < ncol * nz; e += kThreads) {
       const int k = e / nz, z = e - k * nz;
@@ -153,7 +174,7 @@ __global__ __launch_bounds__(kThreads) void k_ypass(WParams w, double2* __restri
     }
   }
   __syncthreads();
-  fft_lines<DIR>(buf, nz, n2, w.p2, tw);
+  fft_any<T, N, DIR>(buf, nz, n2, w.p2, tw);
   if (DIR > 0) {
     for (int e = threadIdx.x; e < nz * n2; e += kThreads) {
       const int z = e / n2, y = e - z * n2;
@@ -169,10 +190,10 @@ __global__ __launch_bounds__(kThreads) void k_ypass(WParams w, double2* __restri
 }
 
 // gather the columns of (z, y-tile) of pair pb into ny zero-padded x lines
-__device__ __forceinline__ void load_xlines(const WParams& w, const double2* __restrict__ B, int pb,
-                                            int z, int y0, int ny, double2* buf) {
-  const int n1 = w.n1;
-  for (int e = threadIdx.x; e < ny * n1; e += kThreads) buf[e] = make_double2(0.0, 0.0);
+template <typename T>
+__device__ __forceinline__ void load_xlines(const WParams& w, const v2<T>* __restrict__ B, int pb, int z, int y0,
+                                            int ny, int n1, v2<T>* buf) {
+  for (int e = threadIdx.x; e < ny * n1; e += kThreads) buf[e] = cmk<v2<T>>((T)0, (T)0);
   __syncthreads();
   const size_t bbase = ((size_t)pb * w.n3 + z) * w.ncols;
   for (int e = threadIdx.x; e < w.ncols * ny; e += kThreads) {
@@ -185,19 +206,21 @@ __device__ __forceinline__ void load_xlines(const WParams& w, const double2* __r
 // ---------------------------------------------------------------------------
 // xrho: inverse x pass of every pair of the batch for one (z, y-tile) and
 // rho(r) += f1/omega Re(psi)^2 + f2/omega Im(psi)^2 (build_density_sum,
-// src/density_utils.mod.F90:61-80); pairs with f1 = f2 = 0 are skipped.
+// src/density_utils.mod.F90:61-80), accumulated in FP64; pairs with
+// f1 = f2 = 0 are skipped.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_xrho(WParams w, const double2* __restrict__ B, int npairs,
-                                                   int pair0, int nstate, const double* __restrict__ f,
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_xrho(WParams w, const v2<T>* __restrict__ B, int npairs, int pair0,
+                                                   int nstate, const double* __restrict__ f,
                                                    double* __restrict__ rho, int ty) {
-  extern __shared__ double2 smem[];
-  const int n1 = w.n1;
-  double2* tw = smem;
-  double2* buf = smem + n1;
+  using V = v2<T>;
+  const int n1 = N > 0 ? N : w.n1;
+  V* tw = lds_base<T>();
+  V* buf = tw + n1;
   const int y0 = blockIdx.x * ty, z = blockIdx.y;
   const int ny = min(ty, w.n2 - y0);
   const int ne = ny * n1;
-  for (int t = threadIdx.x; t < n1; t += kThreads) tw[t] = w.tw1[t];
+  load_tw<T>(tw, w.tw1, n1);
   double acc[kMaxElem];
 #pragma unroll
   for (int k = 0; k < kMaxElem; ++k) acc[k] = 0.0;
@@ -208,14 +231,15 @@ __global__ __launch_bounds__(kThreads) void k_xrho(WParams w, const double2* __r
     const double fb = is2 < nstate ? f[is2] : 0.0;
     if (fa == 0.0 && fb == 0.0) continue;
     const double c3 = fa * rom, c4 = fb * rom;
-    load_xlines(w, B, pb, z, y0, ny, buf);
-    fft_lines<1>(buf, ny, n1, w.p1, tw);
+    load_xlines<T>(w, B, pb, z, y0, ny, n1, buf);
+    fft_any<T, N, 1>(buf, ny, n1, w.p1, tw);
 #pragma unroll
     for (int k = 0; k < kMaxElem; ++k) {
       const int e = threadIdx.x + k * kThreads;
       if (e < ne) {
-        const double2 v = buf[e];
-        acc[k] = acc[k] + c3 * v.x * v.x + c4 * v.y * v.y;
+        const V v = buf[e];
+        const double vx = (double)v.x, vy = (double)v.y;
+        acc[k] = acc[k] + c3 * vx * vx + c4 * vy * vy;
       }
     }
     __syncthreads();
@@ -235,39 +259,37 @@ __global__ __launch_bounds__(kThreads) void k_xrho(WParams w, const double2* __r
 // xvpsi: inverse x pass, psi(r) *= V(r), forward x pass, for every pair of the
 // batch (vpsi:485-493 between invfftn and fwfftn); V is read once per batch.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_xvpsi(WParams w, double2* __restrict__ B, int npairs,
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_xvpsi(WParams w, v2<T>* __restrict__ B, int npairs,
                                                     const double* __restrict__ vpot, int ty) {
-  extern __shared__ double2 smem[];
-  const int n1 = w.n1;
-  double2* tw = smem;
-  double2* buf = smem + n1;
+  using V = v2<T>;
+  const int n1 = N > 0 ? N : w.n1;
+  V* tw = lds_base<T>();
+  V* buf = tw + n1;
   const int y0 = blockIdx.x * ty, z = blockIdx.y;
   const int ny = min(ty, w.n2 - y0);
   const int ne = ny * n1;
-  for (int t = threadIdx.x; t < n1; t += kThreads) tw[t] = w.tw1[t];
-  double vr[kMaxElem];
+  load_tw<T>(tw, w.tw1, n1);
+  T vr[kMaxElem];
 #pragma unroll
   for (int k = 0; k < kMaxElem; ++k) {
     const int e = threadIdx.x + k * kThreads;
-    vr[k] = 0.0;
+    vr[k] = (T)0;
     if (e < ne) {
       const int y = e / n1, x = e - y * n1;
-      vr[k] = vpot[(size_t)x + (size_t)w.kr1 * ((size_t)(y0 + y) + (size_t)w.kr2 * z)];
+      vr[k] = (T)vpot[(size_t)x + (size_t)w.kr1 * ((size_t)(y0 + y) + (size_t)w.kr2 * z)];
     }
   }
   for (int pb = 0; pb < npairs; ++pb) {
-    load_xlines(w, B, pb, z, y0, ny, buf);
-    fft_lines<1>(buf, ny, n1, w.p1, tw);
+    load_xlines<T>(w, B, pb, z, y0, ny, n1, buf);
+    fft_any<T, N, 1>(buf, ny, n1, w.p1, tw);
 #pragma unroll
     for (int k = 0; k < kMaxElem; ++k) {
       const int e = threadIdx.x + k * kThreads;
-      if (e < ne) {
-        const double2 v = buf[e];
-        buf[e] = make_double2(vr[k] * v.x, vr[k] * v.y);
-      }
+      if (e < ne) buf[e] = cscale(buf[e], vr[k]);
     }
     __syncthreads();
-    fft_lines<-1>(buf, ny, n1, w.p1, tw);
+    fft_any<T, N, -1>(buf, ny, n1, w.p1, tw);
     const size_t bbase = ((size_t)pb * w.n3 + z) * w.ncols;
     for (int e = threadIdx.x; e < w.ncols * ny; e += kThreads) {
       const int c = e / ny, y = e - c * ny;
@@ -279,31 +301,32 @@ __global__ __launch_bounds__(kThreads) void k_xvpsi(WParams w, double2* __restri
 
 // ---------------------------------------------------------------------------
 // zfwd: forward z pass of owner sticks + mirrors, scale 1/(n1 n2 n3), Gamma
-// decode fp = psi(G) + psi(-G), fm = psi(G) - psi(-G) and
+// decode fp = psi(G) + psi(-G), fm = psi(G) - psi(-G) and (in FP64)
 //   c2(:,is1) += -fi   (tpiba2 hg c0(:,is1) + (Re fp, Im fm))
 //   c2(:,is2) += -fip1 (tpiba2 hg c0(:,is2) + (Im fp, -Re fm))
 // with fi = f/2 (f = 0 -> 1)  (vpsi_utils.mod.F90:627-672, add_wfn :717).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_zfwd(WParams w, const double2* __restrict__ A,
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_zfwd(WParams w, const v2<T>* __restrict__ A,
                                                    const double2* __restrict__ c0, double2* __restrict__ c2,
-                                                   const double* __restrict__ f, int nstate, int pair0,
-                                                   int lo, double scale) {
-  extern __shared__ double2 smem[];
-  const int n3 = w.n3;
-  double2* tw = smem;
-  double2* buf = smem + n3;
+                                                   const double* __restrict__ f, int nstate, int pair0, int lo,
+                                                   double scale) {
+  using V = v2<T>;
+  const int n3 = N > 0 ? N : w.n3;
+  V* tw = lds_base<T>();
+  V* buf = tw + n3;
   const int pb = blockIdx.y;
   const int is1 = 2 * (pair0 + pb), is2 = is1 + 1;
   const bool has2 = is2 < nstate;
   const int o0 = blockIdx.x * lo;
   const int o1 = min(o0 + lo, w.n_own);
   const int nl = 2 * (o1 - o0);
-  for (int t = threadIdx.x; t < n3; t += kThreads) tw[t] = w.tw3[t];
+  load_tw<T>(tw, w.tw3, n3);
   for (int e = threadIdx.x; e < nl * n3; e += kThreads) {
     const int line = e / n3, z = e - line * n3;
     const int o = o0 + (line >> 1);
     const bool mir = line & 1;
-    double2 v = make_double2(0.0, 0.0);
+    V v = cmk<V>((T)0, (T)0);
     if (!(mir && o == 0)) {
       const int slot = mir ? w.n_own + o - 1 : o;
       v = A[((size_t)pb * w.n_slots + slot) * n3 + z];
@@ -311,7 +334,7 @@ __global__ __launch_bounds__(kThreads) void k_zfwd(WParams w, const double2* __r
     buf[e] = v;
   }
   __syncthreads();
-  fft_lines<-1>(buf, nl, n3, w.p3, tw);
+  fft_any<T, N, -1>(buf, nl, n3, w.p3, tw);
   double fi = f[is1] * 0.5;
   if (fi == 0.0) fi = 1.0;
   double fip1 = has2 ? f[is2] * 0.5 : 0.0;
@@ -325,11 +348,11 @@ __global__ __launch_bounds__(kThreads) void k_zfwd(WParams w, const double2* __r
     const int gi = w.ginfo[j];
     const int o = gi >> 12, l = gi & 4095;
     const int t = o - o0;
-    const double2 a = buf[2 * t * n3 + l];
+    const V a = buf[2 * t * n3 + l];
     const int ml = (o == 0) ? 2 * t : 2 * t + 1;
-    const double2 b = buf[ml * n3 + (l == 0 ? 0 : n3 - l)];
-    const double2 psin = make_double2(a.x * scale, a.y * scale);
-    const double2 psii = make_double2(b.x * scale, b.y * scale);
+    const V b = buf[ml * n3 + (l == 0 ? 0 : n3 - l)];
+    const double2 psin = make_double2((double)a.x * scale, (double)a.y * scale);
+    const double2 psii = make_double2((double)b.x * scale, (double)b.y * scale);
     const double2 fp = cadd(psin, psii);
     const double2 fm = csub(psin, psii);
     const double g2 = w.tpiba2 * w.hg[j];
@@ -406,10 +429,12 @@ static WParams make_params(mc_ctx* ctx) {
 
 static constexpr int kLdsBudget = 65536;
 
-// lines per workgroup for a transform of length n (register, LDS and 16-line caps)
+// lines per workgroup for a transform of length n: engine limits, LDS (lines + table) and 16
+template <typename T>
 static int lines_for(const FftPlan& pl) {
-  int l = max_lines(pl);
-  l = std::min(l, kLdsBudget / (int)(sizeof(double2) * pl.n) - 1);
+  const int a = fast_a(pl.n);
+  int l = a > 0 ? kThreads / a : max_lines(pl);
+  l = std::min(l, kLdsBudget / (int)(sizeof(v2<T>) * pl.n) - 1);
   l = std::min(l, 16);
   MC_CHECK(l >= 1, MC_ERR_UNSUPPORTED, "transform length too large for one workgroup");
   return l;
@@ -422,26 +447,41 @@ struct Shapes {
   int gx;   // zinv/zfwd workgroups per pair
 };
 
+template <typename T>
 static Shapes shapes(mc_ctx* ctx) {
   Shapes s;
-  MC_CHECK(lines_for(ctx->plan3) >= 2, MC_ERR_UNSUPPORTED, "nr3 too large for the z pass");
-  s.lo = lines_for(ctx->plan3) / 2;
-  s.tz = lines_for(ctx->plan2);
-  s.ty = std::min(lines_for(ctx->plan1), (kMaxElem * kThreads) / ctx->n1);
+  MC_CHECK(lines_for<T>(ctx->plan3) >= 2, MC_ERR_UNSUPPORTED, "nr3 too large for the z pass");
+  s.lo = lines_for<T>(ctx->plan3) / 2;
+  s.tz = lines_for<T>(ctx->plan2);
+  s.ty = std::min(lines_for<T>(ctx->plan1), (kMaxElem * kThreads) / ctx->n1);
   MC_CHECK(s.ty >= 1, MC_ERR_UNSUPPORTED, "nr1 too large for the x pass");
   s.gx = (ctx->n_own + s.lo - 1) / s.lo;
   return s;
 }
 
+template <typename T>
 static void ensure_workspace(mc_ctx* ctx, int P) {
-  ctx->w_a.ensure(sizeof(double2) * (size_t)P * ctx->n_slots * ctx->n3);
-  ctx->w_b.ensure(sizeof(double2) * (size_t)P * ctx->n3 * ctx->ncols * ctx->n2);
+  ctx->w_a.ensure(sizeof(v2<T>) * (size_t)P * ctx->n_slots * ctx->n3);
+  ctx->w_b.ensure(sizeof(v2<T>) * (size_t)P * ctx->n3 * ctx->ncols * ctx->n2);
 }
 
 static void upload_f(mc_ctx* ctx, const double* f_host, int nstate) {
   ctx->w_f.ensure(sizeof(double) * std::max(nstate, 1));
   if (nstate)
     MC_HIP(hipMemcpyAsync(ctx->w_f.p, f_host, sizeof(double) * nstate, hipMemcpyHostToDevice, ctx->stream));
+}
+
+// calls f(std::integral_constant<int, N>) with N = n if n has a compile-time
+// engine, N = 0 otherwise
+template <class F>
+static void with_length(int n, F&& f) {
+  switch (n) {
+#define MC_CASE(x) \
+  case x: f(std::integral_constant<int, x>()); return;
+    MC_FAST_SIZES(MC_CASE)
+#undef MC_CASE
+    default: f(std::integral_constant<int, 0>()); return;
+  }
 }
 
 #define LAUNCH(kid, kern, grid, block, shm, ...)                                  \
@@ -452,14 +492,16 @@ static void upload_f(mc_ctx* ctx, const double* f_host, int nstate) {
     timer_end(ctx, kid);                                                        \
   } while (0)
 
-void rhoofr_dev(mc_ctx* ctx, const double2* c0, const double* f_host, int nstate, double* rho,
-                double* ekin, double* csumg, double* csumr) {
+template <typename T>
+static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, int nstate, double* rho,
+                        double* ekin, double* csumg, double* csumr) {
+  using V = v2<T>;
   const size_t nnr1 = (size_t)ctx->kr1 * ctx->kr2 * ctx->kr3;
   const WParams w = make_params(ctx);
-  const Shapes s = shapes(ctx);
+  const Shapes s = shapes<T>(ctx);
   const int npairs = (nstate + 1) / 2;
   const int P = std::max(1, std::min(ctx->batch_pairs_rho, npairs));
-  ensure_workspace(ctx, P);
+  ensure_workspace<T>(ctx, P);
   upload_f(ctx, f_host, nstate);
   const int nblk_sum = 1024;
   ctx->w_red.ensure(sizeof(double) * ((size_t)std::max(npairs, 1) * s.gx * 4 + (size_t)npairs * 4 + nblk_sum));
@@ -467,20 +509,27 @@ void rhoofr_dev(mc_ctx* ctx, const double2* c0, const double* f_host, int nstate
   double* pair_sums = red + (size_t)std::max(npairs, 1) * s.gx * 4;
   double* rsum_part = pair_sums + (size_t)npairs * 4;
   LAUNCH(MC_KID_MISC, k_fill, dim3(1024), dim3(256), 0, rho, nnr1, 0.0);
-  const size_t shz = sizeof(double2) * (size_t)ctx->n3 * (1 + 2 * s.lo);
-  const size_t shy = sizeof(double2) * (size_t)ctx->n2 * (1 + s.tz);
-  const size_t shx = sizeof(double2) * (size_t)ctx->n1 * (1 + s.ty);
+  const size_t shz = sizeof(V) * (size_t)ctx->n3 * (1 + 2 * s.lo);
+  const size_t shy = sizeof(V) * (size_t)ctx->n2 * (1 + s.tz);
+  const size_t shx = sizeof(V) * (size_t)ctx->n1 * (1 + s.ty);
+  V* A = ctx->w_a.as<V>();
+  V* B = ctx->w_b.as<V>();
   for (int pair0 = 0; pair0 < npairs; pair0 += P) {
     const int np = std::min(P, npairs - pair0);
-    LAUNCH(MC_KID_ZINV, k_zinv, dim3(s.gx, np), dim3(kThreads), shz, w, c0, nstate, pair0,
-           ctx->w_a.as<double2>(), s.lo, red + (size_t)pair0 * s.gx * 4);
-    LAUNCH(MC_KID_YINV, k_ypass<1>, dim3(ctx->ncols, (ctx->n3 + s.tz - 1) / s.tz, np), dim3(kThreads), shy, w,
-           ctx->w_a.as<double2>(), ctx->w_b.as<double2>(), s.tz);
-    LAUNCH(MC_KID_XRHO, k_xrho, dim3((ctx->n2 + s.ty - 1) / s.ty, ctx->n3), dim3(kThreads), shx, w,
-           ctx->w_b.as<double2>(), np, pair0, nstate, ctx->w_f.as<double>(), rho, s.ty);
+    with_length(ctx->n3, [&](auto NN) {
+      LAUNCH(MC_KID_ZINV, (k_zinv<T, decltype(NN)::value>), dim3(s.gx, np), dim3(kThreads), shz, w, c0, nstate,
+             pair0, A, s.lo, red + (size_t)pair0 * s.gx * 4);
+    });
+    with_length(ctx->n2, [&](auto NN) {
+      LAUNCH(MC_KID_YINV, (k_ypass<T, decltype(NN)::value, 1>), dim3(ctx->ncols, (ctx->n3 + s.tz - 1) / s.tz, np),
+             dim3(kThreads), shy, w, A, B, s.tz);
+    });
+    with_length(ctx->n1, [&](auto NN) {
+      LAUNCH(MC_KID_XRHO, (k_xrho<T, decltype(NN)::value>), dim3((ctx->n2 + s.ty - 1) / s.ty, ctx->n3),
+             dim3(kThreads), shx, w, B, np, pair0, nstate, ctx->w_f.as<double>(), rho, s.ty);
+    });
   }
-  if (npairs > 0)
-    LAUNCH(MC_KID_MISC, k_reduce_pairs, dim3(npairs), dim3(64), 0, red, s.gx, pair_sums);
+  if (npairs > 0) LAUNCH(MC_KID_MISC, k_reduce_pairs, dim3(npairs), dim3(64), 0, red, s.gx, pair_sums);
   LAUNCH(MC_KID_MISC, k_sum_partial, dim3(nblk_sum), dim3(kThreads), 0, rho, nnr1, rsum_part);
   std::vector<double> hs((size_t)npairs * 4 + nblk_sum);
   MC_HIP(hipMemcpyAsync(hs.data(), pair_sums, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, ctx->stream));
@@ -497,32 +546,61 @@ void rhoofr_dev(mc_ctx* ctx, const double2* c0, const double* f_host, int nstate
   if (csumr) *csumr = sr * ctx->omega / ((double)ctx->n1 * ctx->n2 * ctx->n3);
 }
 
-void vpsi_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, const double* vpot,
-              int nstate) {
+template <typename T>
+static void vpsi_impl(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, const double* vpot,
+                      int nstate) {
+  using V = v2<T>;
   const WParams w = make_params(ctx);
-  const Shapes s = shapes(ctx);
+  const Shapes s = shapes<T>(ctx);
   const int npairs = (nstate + 1) / 2;
   if (npairs == 0) return;
   const int P = std::max(1, std::min(ctx->batch_pairs_vpsi, npairs));
-  ensure_workspace(ctx, P);
+  ensure_workspace<T>(ctx, P);
   upload_f(ctx, f_host, nstate);
-  const size_t shz = sizeof(double2) * (size_t)ctx->n3 * (1 + 2 * s.lo);
-  const size_t shy = sizeof(double2) * (size_t)ctx->n2 * (1 + s.tz);
-  const size_t shx = sizeof(double2) * (size_t)ctx->n1 * (1 + s.ty);
+  const size_t shz = sizeof(V) * (size_t)ctx->n3 * (1 + 2 * s.lo);
+  const size_t shy = sizeof(V) * (size_t)ctx->n2 * (1 + s.tz);
+  const size_t shx = sizeof(V) * (size_t)ctx->n1 * (1 + s.ty);
   const double scale = 1.0 / ((double)ctx->n1 * ctx->n2 * ctx->n3);
+  V* A = ctx->w_a.as<V>();
+  V* B = ctx->w_b.as<V>();
   for (int pair0 = 0; pair0 < npairs; pair0 += P) {
     const int np = std::min(P, npairs - pair0);
-    LAUNCH(MC_KID_ZINV, k_zinv, dim3(s.gx, np), dim3(kThreads), shz, w, c0, nstate, pair0,
-           ctx->w_a.as<double2>(), s.lo, (double*)nullptr);
-    LAUNCH(MC_KID_YINV, k_ypass<1>, dim3(ctx->ncols, (ctx->n3 + s.tz - 1) / s.tz, np), dim3(kThreads), shy, w,
-           ctx->w_a.as<double2>(), ctx->w_b.as<double2>(), s.tz);
-    LAUNCH(MC_KID_XVPSI, k_xvpsi, dim3((ctx->n2 + s.ty - 1) / s.ty, ctx->n3), dim3(kThreads), shx, w,
-           ctx->w_b.as<double2>(), np, vpot, s.ty);
-    LAUNCH(MC_KID_YFWD, k_ypass<-1>, dim3(ctx->ncols, (ctx->n3 + s.tz - 1) / s.tz, np), dim3(kThreads), shy, w,
-           ctx->w_a.as<double2>(), ctx->w_b.as<double2>(), s.tz);
-    LAUNCH(MC_KID_ZFWD, k_zfwd, dim3(s.gx, np), dim3(kThreads), shz, w, ctx->w_a.as<double2>(), c0, c2,
-           ctx->w_f.as<double>(), nstate, pair0, s.lo, scale);
+    with_length(ctx->n3, [&](auto NN) {
+      LAUNCH(MC_KID_ZINV, (k_zinv<T, decltype(NN)::value>), dim3(s.gx, np), dim3(kThreads), shz, w, c0, nstate,
+             pair0, A, s.lo, (double*)nullptr);
+    });
+    with_length(ctx->n2, [&](auto NN) {
+      LAUNCH(MC_KID_YINV, (k_ypass<T, decltype(NN)::value, 1>), dim3(ctx->ncols, (ctx->n3 + s.tz - 1) / s.tz, np),
+             dim3(kThreads), shy, w, A, B, s.tz);
+    });
+    with_length(ctx->n1, [&](auto NN) {
+      LAUNCH(MC_KID_XVPSI, (k_xvpsi<T, decltype(NN)::value>), dim3((ctx->n2 + s.ty - 1) / s.ty, ctx->n3),
+             dim3(kThreads), shx, w, B, np, vpot, s.ty);
+    });
+    with_length(ctx->n2, [&](auto NN) {
+      LAUNCH(MC_KID_YFWD, (k_ypass<T, decltype(NN)::value, -1>), dim3(ctx->ncols, (ctx->n3 + s.tz - 1) / s.tz, np),
+             dim3(kThreads), shy, w, A, B, s.tz);
+    });
+    with_length(ctx->n3, [&](auto NN) {
+      LAUNCH(MC_KID_ZFWD, (k_zfwd<T, decltype(NN)::value>), dim3(s.gx, np), dim3(kThreads), shz, w, A, c0, c2,
+             ctx->w_f.as<double>(), nstate, pair0, s.lo, scale);
+    });
   }
+}
+
+void rhoofr_dev(mc_ctx* ctx, const double2* c0, const double* f_host, int nstate, double* rho, double* ekin,
+                double* csumg, double* csumr) {
+  if (ctx->fft_fp32)
+    rhoofr_impl<float>(ctx, c0, f_host, nstate, rho, ekin, csumg, csumr);
+  else
+    rhoofr_impl<double>(ctx, c0, f_host, nstate, rho, ekin, csumg, csumr);
+}
+
+void vpsi_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, const double* vpot, int nstate) {
+  if (ctx->fft_fp32)
+    vpsi_impl<float>(ctx, c0, c2, f_host, vpot, nstate);
+  else
+    vpsi_impl<double>(ctx, c0, c2, f_host, vpot, nstate);
 }
 
 }  // namespace mc

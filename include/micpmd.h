@@ -91,6 +91,15 @@ int mc_ctx_synchronize(mc_ctx* ctx);
 int mc_ctx_set_batch(mc_ctx* ctx, int pairs_rhoofr, int pairs_vpsi);
 size_t mc_wfn_ld(const mc_ctx* ctx);
 
+/* Arithmetic of the rhoofr/vpsi transforms (no counterpart in the reference,
+ * whose transforms are FP64 only):
+ *   MC_PREC_FP64  every transform, workspace and accumulation in FP64 (default);
+ *   MC_PREC_FP32  transforms and stick/column workspaces in FP32; wavefunctions,
+ *                 V(r), rho(r), the Gamma decode, the kinetic term, c2 and all
+ *                 charge/energy sums stay FP64. */
+enum { MC_PREC_FP64 = 0, MC_PREC_FP32 = 1 };
+int mc_set_precision(mc_ctx* ctx, int fft_precision);
+
 /* Device memory helpers (hipMalloc / hipFree / hipMemcpy). */
 int mc_dev_alloc(size_t bytes, void** ptr);
 int mc_dev_free(void* ptr);

@@ -131,3 +131,29 @@ def test_cfg4_full_size_properties(gpu):
     out = ctx.download(c2d, n)
     ref = -(sysm.f[None, :]) * (0.5 * g2[:, None] + v0) * c
     assert np.abs(out - ref).max() < 1e-11 * np.abs(ref).max()
+
+
+# ---------------------------------------------------------------------------
+# FP32 transform mode (mixed precision): same kernels in float, FP64 C0/C2/rho.
+# Tolerances are FP32 round-off over three passes: 2e-6 relative.
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name,nstate", [("cfg1", None), ("cfg2", 6)])
+def test_fp32_transforms(gpu, name, nstate):
+    sysm, c = config_system(name, nstate=nstate)
+    f = sysm.f[:c.shape[1]]
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    ctx.set_precision("fp32")
+    L = oracle.Layout(sysm.cell, sysm.gset)
+    rho_g, ek_g, cg_g, cr_g = ctx.rhoofr(c, f)
+    rho_o, ek_o, cg_o, cr_o = oracle.rhoofr(L, c, f)
+    assert np.abs(rho_g - rho_o).max() < 2e-6 * np.abs(rho_o).max()
+    assert ek_g == pytest.approx(ek_o, rel=1e-12)        # E_kin is summed from C0 in FP64
+    assert cg_g == pytest.approx(cg_o, rel=1e-12)
+    assert cr_g == pytest.approx(cr_o, rel=2e-6)
+    v = S.real_space_potential(sysm)
+    c2 = ctx.vpsi(c, np.zeros_like(c), f, v)
+    c2_o = oracle.vpsi(L, c, np.zeros_like(c), f, v)
+    assert np.abs(c2 - c2_o).max() < 2e-6 * np.abs(c2_o).max()
+    # and back to FP64 on the same context
+    ctx.set_precision("fp64")
+    assert np.abs(ctx.rhoofr(c, f)[0] - rho_o).max() < 1e-11
