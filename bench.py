@@ -13,6 +13,14 @@ updwf + ortho).  The local potential V(r) is a fixed seeded input (vofrho is
 not part of the measured path).  Wavefunctions: seeded complex Gaussian
 ~exp(-|G|^2/8), Loewdin-orthonormalised; one s projector per atom (GTH).
 
+Two precision paths are timed from the same start, the first one being the
+headline `value`:
+  mixed  rhoofr/vpsi transforms and their workspaces in FP32 (everything
+         else FP64);
+  fp64   every operation in FP64.
+The band energy sum_i f_i <c_i|H|c_i> of the starting coefficients is
+evaluated on both paths and its difference per atom is reported.
+
 Prints one JSON line (see the measurement contract in DESIGN.md).
 """
 from __future__ import annotations
@@ -37,36 +45,38 @@ def log(msg: str):
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(ctx_geom, kid: str, np_pairs: int) -> float:
+def kernel_bytes(ctx_geom, kid: str, np_pairs: float, cw: float = 16.0) -> float:
     """Algorithmic HBM bytes of one launch of kernel `kid` processing np_pairs
-    state pairs (FP64 complex = 16 B, real = 8 B)."""
+    state pairs.  cw = bytes of a workspace complex (16 FP64, 8 FP32);
+    wavefunctions are FP64 complex (16 B), V and rho FP64 (8 B)."""
     ngw, S, Zw, n1, n2, n3 = ctx_geom
     c, r = 16.0, 8.0
     nnn = n1 * n2 * n3
     col = n3 * Zw * n2          # column workspace per pair (complex)
     stick = S * n3              # stick workspace per pair (complex)
     if kid == "zinv":
-        return np_pairs * c * (2 * ngw + stick)
+        return np_pairs * (c * 2 * ngw + cw * stick)
     if kid == "yinv" or kid == "yfwd":
-        return np_pairs * c * (stick + col)
+        return np_pairs * cw * (stick + col)
     if kid == "xrho":
-        return np_pairs * c * col + 2 * r * nnn
+        return np_pairs * cw * col + 2 * r * nnn
     if kid == "xvpsi":
-        return np_pairs * c * 2 * col + r * nnn
+        return np_pairs * cw * 2 * col + r * nnn
     if kid == "zfwd":
-        return np_pairs * c * (stick + 6 * ngw)
+        return np_pairs * (cw * stick + c * 6 * ngw)
     raise KeyError(kid)
 
 
-def step_model_bytes(ctx_geom, nstate: int) -> float:
-    """Per-step algorithmic bytes of rhoofr + vpsi: per state pair
-    vpsi = c(4 S N + 4 N^2 Zw + 6 NGW) + r N^3, rhoofr = c(2 S N + 2 N^2 Zw + 2 NGW),
-    plus rho written once (r N^3)."""
+def step_model_bytes(ctx_geom, nstate: int, cw: float = 16.0) -> float:
+    """Per-step algorithmic bytes of rhoofr + vpsi (the survey's model with the
+    transform workspaces at cw bytes per complex): per state pair
+    vpsi = cw(4 S N + 4 N^2 Zw) + 16*6 NGW + 8 N^3, rhoofr = cw(2 S N + 2 N^2 Zw) + 16*2 NGW,
+    plus rho written once (8 N^3)."""
     ngw, S, Zw, n1, n2, n3 = ctx_geom
-    c, r = 16.0, 8.0
+    r = 8.0
     npairs = (nstate + 1) // 2
-    vp = c * (4 * S * n3 + 4 * n1 * n2 * Zw + 6 * ngw) + r * n1 * n2 * n3
-    rh = c * (2 * S * n3 + 2 * n1 * n2 * Zw + 2 * ngw)
+    vp = cw * (4 * S * n3 + 4 * n1 * n2 * Zw) + 16.0 * 6 * ngw + r * n1 * n2 * n3
+    rh = cw * (2 * S * n3 + 2 * n1 * n2 * Zw) + 16.0 * 2 * ngw
     return npairs * (vp + rh) + 2 * r * n1 * n2 * n3
 
 
@@ -93,6 +103,9 @@ def cpu_baseline(sysm, c0, v, max_pairs: int = 1):
     }
 
 
+FFT_KIDS = ("zinv", "yinv", "xrho", "xvpsi", "yfwd", "zfwd")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +116,7 @@ def main():
     ap.add_argument("--batch-rho", type=int, default=16)
     ap.add_argument("--batch-vpsi", type=int, default=8)
     ap.add_argument("--dt", type=float, default=0.02)
+    ap.add_argument("--paths", default="mixed,fp64", help="comma list of paths to time (first = headline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -134,77 +148,13 @@ def main():
     ctx.set_projectors(na, nhxs, ngh, lang, hmat, twnl)
     ctx.set_eigr(eigr)
     del eigr, twnl
-    d_c0 = ctx.upload(c0)
+    d_c0 = ctx.alloc_wfn(nstate)
     d_c2 = ctx.alloc_wfn(nstate)
     d_rho = ops.DeviceArray(8 * ctx.nnr1)
     d_v = ops.DeviceArray(8 * ctx.nnr1).from_host(v)
     f = sysm.f
     nat = int(np.sum(na))
-    dt = args.dt
-
-    energies = {}
-    lowdin_iters = []
-
-    def step():
-        ctx.rnlsm1_dev(d_c0, nstate)
-        ek, cg, cr = ctx.rhoofr_dev(d_c0, f, nstate, d_rho)
-        energies.update(ekin=ek, csumg=cg, csumr=cr)
-        ctx.zero_wfn(d_c2, nstate)
-        ctx.vpsi_dev(d_c0, d_c2, f, d_v, nstate)
-        ctx.fnonloc_dev(d_c2, f, nstate)
-        ctx.lagrange_dev(d_c0, d_c2, f, nstate)
-        ctx.axpy_wfn(dt, d_c2, d_c0, nstate)
-        lowdin_iters.append(ctx.lowdin_dev(d_c0, nstate))
-
-    for i in range(args.warmup):
-        step()
-        log(f"[bench] warmup {i + 1}/{args.warmup} done")
-    ctx.synchronize()
-
-    ctx.timer_reset()
-    ctx.timer_enable(True)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    ctx.timer_enable(False)
-    ms = (t1 - t0) * 1e3 / args.steps
-    log(f"[bench] {args.steps} steps: {ms:.3f} ms/step (ekin {energies['ekin']:.10f}, "
-        f"csumr-csumg {energies['csumr'] - energies['csumg']:.2e})")
-
-    # per-kernel HIP-event times over the timed region
     npairs = (nstate + 1) // 2
-    kernels = {}
-    for kid, P in (("zinv", None), ("yinv", None), ("xrho", args.batch_rho), ("xvpsi", args.batch_vpsi),
-                   ("yfwd", args.batch_vpsi), ("zfwd", args.batch_vpsi), ("misc", None), ("gemm", None)):
-        tot, cnt = ctx.timer_get(kid)
-        if cnt:
-            kernels[kid] = {"total_ms_per_step": tot / args.steps, "launches": cnt, "avg_ms": tot / cnt}
-    # dominant kernel by time per step
-    dom = max((k for k in kernels if k not in ("misc", "gemm")), key=lambda k: kernels[k]["total_ms_per_step"])
-    # pairs per launch of the dominant kernel (full batches dominate; average over the step)
-    if dom in ("xrho",):
-        launches_per_step = -(-npairs // min(args.batch_rho, npairs))
-        pairs_per_launch = npairs / launches_per_step
-    elif dom in ("xvpsi", "yfwd", "zfwd"):
-        launches_per_step = -(-npairs // min(args.batch_vpsi, npairs))
-        pairs_per_launch = npairs / launches_per_step
-    else:   # zinv / yinv run in both rhoofr and vpsi batches
-        lr = -(-npairs // min(args.batch_rho, npairs))
-        lv = -(-npairs // min(args.batch_vpsi, npairs))
-        pairs_per_launch = 2 * npairs / (lr + lv)
-    # bytes of the average launch: linear in pairs except the once-per-launch real-space term
-    b_launch = kernel_bytes(geom, dom, 1) * pairs_per_launch
-    if dom in ("xrho", "xvpsi"):
-        b_launch = kernel_bytes(geom, dom, 0) + (kernel_bytes(geom, dom, 1) - kernel_bytes(geom, dom, 0)) * pairs_per_launch
-    achieved = b_launch / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
-    step_bytes = step_model_bytes(geom, nstate)
-    fft_ms = sum(kernels[k]["total_ms_per_step"] for k in ("zinv", "yinv", "xrho", "xvpsi", "yfwd", "zfwd")
-                 if k in kernels)
-
-    # MFMA contractions: algorithmic flops per step (K = 2 ngw real rows)
     K = 2.0 * ngw
     n = float(nstate)
     gemm_flops = (2 * nat * n * K            # rnlsm1
@@ -213,6 +163,77 @@ def main():
                   + 2 * K * n * n            # Lagrange: rotate
                   + n * n * K                # Loewdin: S = C^T C (symmetric)
                   + 2 * K * n * n)           # Loewdin: C S^-1/2
+
+    def band_energy():
+        """E_band = sum_i f_i <c_i|T + V + V_NL|c_i> = -sum_i dotp(c0_i, c2_i) after vpsi + fnonloc."""
+        ctx.rnlsm1_dev(d_c0, nstate)
+        ctx.zero_wfn(d_c2, nstate)
+        ctx.vpsi_dev(d_c0, d_c2, f, d_v, nstate)
+        ctx.fnonloc_dev(d_c2, f, nstate)
+        return -float(np.sum(ctx.dotp_wfn(d_c0, d_c2, nstate)))
+
+    def run_path(path):
+        ctx.set_precision("fp32" if path == "mixed" else "fp64")
+        ctx.upload(c0, d_c0)                 # every path starts from the same coefficients
+        e_band = band_energy()
+        energies = {}
+        lowdin_iters = []
+
+        def step():
+            ctx.rnlsm1_dev(d_c0, nstate)
+            ek, cg, cr = ctx.rhoofr_dev(d_c0, f, nstate, d_rho)
+            energies.update(ekin=ek, csumg=cg, csumr=cr)
+            ctx.zero_wfn(d_c2, nstate)
+            ctx.vpsi_dev(d_c0, d_c2, f, d_v, nstate)
+            ctx.fnonloc_dev(d_c2, f, nstate)
+            ctx.lagrange_dev(d_c0, d_c2, f, nstate)
+            ctx.axpy_wfn(args.dt, d_c2, d_c0, nstate)
+            lowdin_iters.append(ctx.lowdin_dev(d_c0, nstate))
+
+        for i in range(args.warmup):
+            step()
+        ctx.synchronize()
+        log(f"[bench] {path}: {args.warmup} warmup steps done")
+        ctx.timer_reset()
+        ctx.timer_enable(True)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        ctx.timer_enable(False)
+        ms = (t1 - t0) * 1e3 / args.steps
+        kernels = {}
+        for kid in FFT_KIDS + ("misc", "gemm"):
+            tot, cnt = ctx.timer_get(kid)
+            if cnt:
+                kernels[kid] = {"total_ms_per_step": tot / args.steps, "launches": cnt, "avg_ms": tot / cnt}
+        log(f"[bench] {path}: {args.steps} steps: {ms:.3f} ms/step, E_band(start) {e_band:.10f}, "
+            f"csumr-csumg {energies['csumr'] - energies['csumg']:.2e}")
+        return {"ms": ms, "kernels": kernels, "e_band": e_band, "lowdin_iters": lowdin_iters[-args.steps:]}
+
+    paths = [p.strip() for p in args.paths.split(",") if p.strip()]
+    res = {p: run_path(p) for p in paths}
+    head = paths[0]
+    R = res[head]
+    cw = 8.0 if head == "mixed" else 16.0
+    kernels = R["kernels"]
+    # dominant transform kernel by time per step
+    dom = max((k for k in kernels if k in FFT_KIDS), key=lambda k: kernels[k]["total_ms_per_step"])
+    # pairs per launch of the dominant kernel (average over the step)
+    lr = -(-npairs // min(args.batch_rho, npairs))
+    lv = -(-npairs // min(args.batch_vpsi, npairs))
+    if dom == "xrho":
+        ppl = npairs / lr
+    elif dom in ("xvpsi", "yfwd", "zfwd"):
+        ppl = npairs / lv
+    else:   # zinv / yinv run in both rhoofr and vpsi batches
+        ppl = 2 * npairs / (lr + lv)
+    b_launch = kernel_bytes(geom, dom, 0, cw) + (kernel_bytes(geom, dom, 1, cw) - kernel_bytes(geom, dom, 0, cw)) * ppl
+    achieved = b_launch / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
+    step_bytes = step_model_bytes(geom, nstate, cw)
+    fft_ms = sum(kernels[k]["total_ms_per_step"] for k in FFT_KIDS if k in kernels)
     gemm_ms = kernels.get("gemm", {}).get("total_ms_per_step", float("nan"))
     gemm_tflops = gemm_flops / (gemm_ms * 1e-3) / 1e12
 
@@ -220,7 +241,7 @@ def main():
     tf = ROOT / "profiles" / "traffic.json"
     if tf.exists():
         try:
-            traffic = json.loads(tf.read_text()).get(dom)
+            traffic = json.loads(tf.read_text()).get(f"{head}:{dom}")
         except Exception:
             traffic = None
 
@@ -230,27 +251,35 @@ def main():
         cpu = cpu_baseline(sysm, c0, v)
         log(f"[bench] cpu baseline {cpu['value']:.1f} ms/step")
 
+    compare = {}
+    if "fp64" in res and head != "fp64":
+        compare = {
+            "fp64_ms_per_step": round(res["fp64"]["ms"], 4),
+            "speedup_vs_fp64": round(res["fp64"]["ms"] / R["ms"], 3),
+            "dE_band_per_atom_Ha": abs(R["e_band"] - res["fp64"]["e_band"]) / nat,
+            "E_band_fp64_Ha": res["fp64"]["e_band"],
+        }
     out = {
         "metric": METRIC,
-        "value": round(ms, 4),
+        "value": round(R["ms"], 4),
         "unit": "ms/step",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
+        "ms_per_step": round(R["ms"], 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 transforms / f64" if head == "mixed" else "f64",
         "data": "synthetic (seeded Gaussian wavefunctions, Loewdin-orthonormalised; seeded smooth V(r))",
         "config": {
             "workload": f"{args.config}: {nstate} states, {n1}x{n2}x{n3} mesh, {ngw} Gamma plane waves, "
-                        f"a={cfg['a']} bohr, ecut {cfg['ecut']} Ry",
-            "step": "rnlsm1 + rhoofr + vpsi + fnonloc + Lagrange projection + SD update + Loewdin (FP64 GPU path)",
-            "atoms": nat,
-            "precision_path": "fp64",
+                        f"{nat} atoms, a={cfg['a']} bohr, ecut {cfg['ecut']} Ry",
+            "step": "rnlsm1 + rhoofr + vpsi + fnonloc + Lagrange projection + SD update + Loewdin",
+            "precision_path": head,
             "batch_pairs": {"rhoofr": args.batch_rho, "vpsi": args.batch_vpsi},
         },
+        **compare,
         "roofline": {
             "bound": "hbm",
             "kernel": dom,
@@ -268,8 +297,9 @@ def main():
         "mfma_roofline": {"bound": "mfma", "flops_per_step": gemm_flops, "gemm_ms_per_step": gemm_ms,
                           "achieved": round(gemm_tflops, 2), "peak": 78.6, "unit": "TFLOP/s",
                           "frac": round(gemm_tflops / 78.6, 4)},
-        "lowdin_newton_schulz_iters": lowdin_iters[-args.steps:],
-        "kernels": kernels,
+        "paths": {p: {"ms_per_step": round(res[p]["ms"], 4), "E_band_Ha": res[p]["e_band"],
+                      "lowdin_newton_schulz_iters": res[p]["lowdin_iters"],
+                      "kernels": res[p]["kernels"]} for p in paths},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

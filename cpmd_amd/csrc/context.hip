@@ -208,6 +208,25 @@ __global__ void k_axpy(double alpha, const double* __restrict__ x, double* __res
     y[i] += alpha * x[i];
 }
 
+// out[i] = Re a0 Re b0 + 2 sum_{G != 0} Re(conj(a) b) for column i (dotp,
+// src/dotp_utils.mod.F90:58-101); one workgroup per state, fixed order
+__global__ __launch_bounds__(256) void k_dotp_cols(const double2* __restrict__ a, const double2* __restrict__ b,
+                                                   int ngw, int ldg, int geq0, double* __restrict__ out) {
+  const int i = blockIdx.x;
+  const double2* x = a + (size_t)ldg * i;
+  const double2* y = b + (size_t)ldg * i;
+  double s = 0.0;
+  for (int g = threadIdx.x + (geq0 ? 1 : 0); g < ngw; g += 256) s += x[g].x * y[g].x + x[g].y * y[g].y;
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  __shared__ double part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t = part[0] + part[1] + part[2] + part[3];
+    out[i] = 2.0 * t + (geq0 ? x[0].x * y[0].x : 0.0);
+  }
+}
+
 void wfn_upload(mc_ctx* ctx, const double* host, int ldc, int nstate, double2* dev) {
   MC_CHECK(ldc >= ctx->ngw && nstate >= 0, MC_ERR_ARG, "wfn_upload: ldc < ngw or nstate < 0");
   if (nstate == 0) return;
@@ -447,6 +466,19 @@ int mc_wfn_axpy(mc_ctx* ctx, double alpha, const void* x_dev, void* y_dev, int n
                        (double*)y_dev, n);
     MC_HIP(hipGetLastError());
   }
+  MC_API_END
+}
+
+int mc_wfn_dotp(mc_ctx* ctx, const void* a_dev, const void* b_dev, int nstate, double* out) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && a_dev && b_dev && out && nstate >= 0, MC_ERR_ARG, "bad argument");
+  if (nstate == 0) return MC_OK;
+  ctx->w_red2.ensure(sizeof(double) * nstate);
+  hipLaunchKernelGGL(mc::k_dotp_cols, dim3(nstate), dim3(256), 0, ctx->stream, (const double2*)a_dev,
+                     (const double2*)b_dev, ctx->ngw, ctx->ldg, ctx->geq0, ctx->w_red2.as<double>());
+  MC_HIP(hipGetLastError());
+  MC_HIP(hipMemcpyAsync(out, ctx->w_red2.p, sizeof(double) * nstate, hipMemcpyDeviceToHost, ctx->stream));
+  MC_HIP(hipStreamSynchronize(ctx->stream));
   MC_API_END
 }
 

@@ -13,11 +13,18 @@
 //    (k x n) small; optional upper-triangular G (trmm) and a fused complex
 //    epilogue for fnonloc.
 //
-// Each workgroup computes a 64x64 tile with 4 waves (2x2), each wave a 32x32
-// block = 2x2 MFMA tiles of 16x16.  Operands are staged through LDS in
-// K-slices of 32.  MFMA operand layout (16x16x4 f64, checked on gfx950 with a
-// lane probe): lane l holds A[i = l%16][k = l/16], B[k = l/16][j = l%16] and,
-// in accumulator register r, D[i = 4r + l/16][j = l%16].
+// Each workgroup computes a 128x128 tile with 4 waves (2x2), each wave a 64x64
+// block = 4x4 MFMA tiles of 16x16 (64 FP64 accumulators per lane).  Operands
+// are staged through LDS in K slices of 16; the next slice is fetched into
+// registers while the current one is multiplied.  Workgroup ids are remapped so
+// that tiles sharing an operand block run on the same XCD (ids are dealt to the
+// 8 XCDs round robin): consecutive remapped ids share the K chunk (gemm_tn) or
+// the row block of A (gemm_nn).  The tall output of gemm_nn is written through
+// an LDS stage as 512-byte column segments.
+//
+// MFMA operand layout (16x16x4 f64, checked on gfx950 with a lane probe):
+// lane l holds A[i = l%16][k = l/16], B[k = l/16][j = l%16] and, in
+// accumulator register r, D[i = 4r + l/16][j = l%16].
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,111 +36,120 @@ namespace mc {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int kGT = 64;    // tile edge
-constexpr int kGK = 32;    // K slice
-constexpr int kGPad = 2;   // LDS row padding (doubles)
-constexpr int kGLd = kGT + kGPad;
+constexpr int kBT = 128;          // tile edge
+constexpr int kBK = 16;           // K slice
+constexpr int kBLd = kBT + 4;     // LDS row stride (doubles)
+constexpr int kXcd = 8;           // XCDs of an MI355X
 
-__device__ __forceinline__ void mfma_block(const double* As, const double* Bs, int wr, int wc, d4 (&acc)[2][2]) {
+__device__ __forceinline__ void mfma_tile(const double* As, const double* Bs, int wr, int wc, d4 (&acc)[4][4]) {
   const int l = threadIdx.x & 63;
   const int li = l & 15, lk = l >> 4;
 #pragma unroll
-  for (int kk = 0; kk < kGK; kk += 4) {
-    double a0 = As[(kk + lk) * kGLd + wr + li];
-    double a1 = As[(kk + lk) * kGLd + wr + 16 + li];
-    double b0 = Bs[(kk + lk) * kGLd + wc + li];
-    double b1 = Bs[(kk + lk) * kGLd + wc + 16 + li];
-    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  for (int kk = 0; kk < kBK; kk += 4) {
+    double a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = As[(kk + lk) * kBLd + wr + 16 * i + li];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = Bs[(kk + lk) * kBLd + wc + 16 * j + li];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
   }
 }
 
-// A^T B partial over K-chunk blockIdx.z.
-//   A: (K x m) column-major, lda (doubles); if proj.eigr != nullptr, A is the
-//      projector operand instead (K = 2*ngw rows: 2g = Re, 2g+1 = Im).
-//   B: (K x n) column-major, ldb.
+// remapped workgroup index: XCD x = id % 8 gets the contiguous range [x*per, (x+1)*per)
+__device__ __forceinline__ int xcd_tile(int total) {
+  const int id = blockIdx.x;
+  const int per = (total + kXcd - 1) / kXcd;
+  return (id % kXcd) * per + id / kXcd;
+}
+
+// ---------------------------------------------------------------------------
+// A^T B partials.  Launch ceil(total/8)*8 workgroups, total = nsplit * tm * tn.
+//   A: (K x m) column-major, lda; if proj.eigr != nullptr, A is the projector
+//      operand (K = 2*ngw rows: 2g = Re, 2g+1 = Im).  B: (K x n), ldb.
 //   P: partials [nsplit][n][m] (column-major m x n per split).
-// upper: only tiles with tile_row <= tile_col.
+// upper: tiles with tile_row > tile_col are skipped.
+// ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_gemm_tn(const double* __restrict__ A, int lda,
                                                  const double* __restrict__ B, int ldb, int K, int m, int n,
-                                                 int kchunk, double* __restrict__ P, int upper, ProjA proj) {
-  __shared__ double As[kGK * kGLd];
-  __shared__ double Bs[kGK * kGLd];
-  const int ti = blockIdx.x, tj = blockIdx.y, s = blockIdx.z;
+                                                 int kchunk, double* __restrict__ P, int upper, ProjA proj, int tm,
+                                                 int tn, int total) {
+  __shared__ double As[kBK * kBLd];
+  __shared__ double Bs[kBK * kBLd];
+  const int t = xcd_tile(total);
+  if (t >= total) return;
+  const int s = t / (tm * tn);
+  const int tile = t - s * tm * tn;
+  const int ti = tile / tn, tj = tile - ti * tn;
   if (upper && ti > tj) return;
-  const int i0 = ti * kGT, j0 = tj * kGT;
+  const int i0 = ti * kBT, j0 = tj * kBT;
   const int kbeg = s * kchunk;
   const int kend = min(K, kbeg + kchunk);
   const int wave = threadIdx.x >> 6;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  d4 acc[2][2];
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  d4 acc[4][4];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  const int lc = threadIdx.x >> 2;        // column within the tile
-  const int lq = (threadIdx.x & 3) * 8;   // k offset within the slice
-  for (int kb = kbeg; kb < kend; kb += kGK) {
-    // ---- A slice: 32 k x 64 columns
-    {
-      const int col = i0 + lc;
-      double v[8];
-      if (proj.eigr) {
-        // rows kb+lq .. +8 = G vectors g0 .. g0+4 (re, im interleaved); kb, lq even
-        const int g0 = (kb + lq) >> 1;
+    for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  const int lc = threadIdx.x >> 1;        // column within the tile
+  const int lq = (threadIdx.x & 1) * 8;   // k offset within the slice
+  double ra[8], rb[8];
+  auto fetch = [&](int kb) {
+    const int acol = i0 + lc, bcol = j0 + lc;
+    if (proj.eigr) {
+      const int g0 = (kb + lq) >> 1;   // rows kb+lq .. +8 = G vectors g0 .. g0+3 (re, im)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int g = g0 + u;
-          double re = 0.0, im = 0.0;
-          if (col < m && 2 * g < kend) {
-            const double2 e = proj.eigr[(size_t)col * proj.ldg + g];
-            double t = proj.twnl[g];
-            if (proj.geq0 && g == 0) t *= 0.5;
-            // (-i)^l * t * e
-            switch (proj.lmod4) {
-              case 0: re = t * e.x; im = t * e.y; break;
-              case 1: re = t * e.y; im = -t * e.x; break;
-              case 2: re = -t * e.x; im = -t * e.y; break;
-              default: re = -t * e.y; im = t * e.x; break;
-            }
+      for (int u = 0; u < 4; ++u) {
+        const int g = g0 + u;
+        double re = 0.0, im = 0.0;
+        if (acol < m && 2 * g < kend) {
+          const double2 e = proj.eigr[(size_t)acol * proj.ldg + g];
+          double tw = proj.twnl[g];
+          if (proj.geq0 && g == 0) tw *= 0.5;
+          switch (proj.lmod4) {   // (-i)^l * tw * e
+            case 0: re = tw * e.x; im = tw * e.y; break;
+            case 1: re = tw * e.y; im = -tw * e.x; break;
+            case 2: re = -tw * e.x; im = -tw * e.y; break;
+            default: re = -tw * e.y; im = tw * e.x; break;
           }
-          v[2 * u] = re;
-          v[2 * u + 1] = im;
         }
-      } else {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int k = kb + lq + u;
-          v[u] = (col < m && k < kend) ? A[(size_t)col * lda + k] : 0.0;
-        }
+        ra[2 * u] = re;
+        ra[2 * u + 1] = im;
       }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) As[(lq + u) * kGLd + lc] = v[u];
-    }
-    // ---- B slice
-    {
-      const int col = j0 + lc;
-      double v[8];
+    } else {
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int k = kb + lq + u;
-        v[u] = (col < n && k < kend) ? B[(size_t)col * ldb + k] : 0.0;
+        ra[u] = (acol < m && k < kend) ? A[(size_t)acol * lda + k] : 0.0;
       }
+    }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) Bs[(lq + u) * kGLd + lc] = v[u];
+    for (int u = 0; u < 8; ++u) {
+      const int k = kb + lq + u;
+      rb[u] = (bcol < n && k < kend) ? B[(size_t)bcol * ldb + k] : 0.0;
+    }
+  };
+  if (kbeg < kend) fetch(kbeg);
+  for (int kb = kbeg; kb < kend; kb += kBK) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      As[(lq + u) * kBLd + lc] = ra[u];
+      Bs[(lq + u) * kBLd + lc] = rb[u];
     }
     __syncthreads();
-    mfma_block(As, Bs, wr, wc, acc);
+    if (kb + kBK < kend) fetch(kb + kBK);
+    mfma_tile(As, Bs, wr, wc, acc);
     __syncthreads();
   }
   const int l = threadIdx.x & 63;
   double* Ps = P + (size_t)s * m * n;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < 4; ++b) {
       const int j = j0 + wc + b * 16 + (l & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -144,7 +160,7 @@ __global__ __launch_bounds__(256) void k_gemm_tn(const double* __restrict__ A, i
 }
 
 // out(i,j) = alpha * sum_s P[s](i,j) - g0 * a0(i) b0(j); symmetric: only the
-// upper triangle was computed, mirrored into the lower one (dmatc).
+// upper triangle is used and mirrored into the lower one (dmatc).
 __global__ void k_reduce_tn(const double* __restrict__ P, int nsplit, int m, int n, double alpha,
                             const double* __restrict__ a0, int lda, const double* __restrict__ b0, int ldb,
                             int g0, int symmetric, double* __restrict__ out, int ldo) {
@@ -160,91 +176,101 @@ __global__ void k_reduce_tn(const double* __restrict__ P, int nsplit, int m, int
   out[(size_t)j * ldo + i] = v;
 }
 
-// D = alpha A G + beta D.  A (M x k) column-major lda; G (k x n) column-major ldgm;
-// D (M x n) column-major ldd.  upper_g: G is upper triangular (k-loop stops at
-// the diagonal block).  With epi.twnl, D (real view of c2) += scaled A G.
-__global__ __launch_bounds__(256) void k_gemm_nn(const double* __restrict__ A, int lda,
+// ---------------------------------------------------------------------------
+// D = alpha A G + beta D.  A (M x k) column-major lda; G (k x n) column-major
+// ldgm; D (M x n) column-major ldd.  upper_g: G upper triangular (the k loop
+// stops after the diagonal block).  With epi.twnl, D (real view of c2) +=
+// twnl * ctm * (A G) on (re, im) row pairs (fnonloc).
+// Launch ceil(total/8)*8 workgroups, total = tm * tn (column tile fastest).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256, 2) void k_gemm_nn(const double* __restrict__ A, int lda,
                                                  const double* __restrict__ G, int ldgm, int M, int k, int n,
                                                  double alpha, double beta, double* __restrict__ D, int ldd,
-                                                 int upper_g, FnlEpi epi) {
-  __shared__ double As[kGK * kGLd];
-  __shared__ double Gs[kGK * kGLd];
-  const int i0 = blockIdx.x * kGT, j0 = blockIdx.y * kGT;
+                                                 int upper_g, FnlEpi epi, int tn, int total) {
+  __shared__ double smem[2 * kBK * kBLd];
+  double* As = smem;
+  double* Gs = smem + kBK * kBLd;
+  const int t = xcd_tile(total);
+  if (t >= total) return;
+  const int rt = t / tn, ct = t - rt * tn;
+  const int i0 = rt * kBT, j0 = ct * kBT;
   const int wave = threadIdx.x >> 6;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  d4 acc[2][2];
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  d4 acc[4][4];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  const int kmax = upper_g ? min(k, j0 + kGT) : k;
-  const int ak = threadIdx.x >> 3, ai = (threadIdx.x & 7) * 8;   // A slice: 32 k x 64 rows
-  const int gj = threadIdx.x >> 2, gq = (threadIdx.x & 3) * 8;   // G slice: 32 k x 64 cols
-  for (int kb = 0; kb < kmax; kb += kGK) {
-    {
-      const int kk = kb + ak;
-      const double* src = A + (size_t)kk * lda + i0 + ai;
+    for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  const int kmax = upper_g ? min(k, j0 + kBT) : k;
+  const int ak = threadIdx.x >> 4, ai = (threadIdx.x & 15) * 8;   // A slice: 16 k x 128 rows
+  const int gj = threadIdx.x >> 1, gq = (threadIdx.x & 1) * 8;   // G slice: 16 k x 128 cols
+  double ra[8], rg[8];
+  auto fetch = [&](int kb) {
+    const int kk = kb + ak;
+    const double* src = A + (size_t)kk * lda + i0 + ai;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int row = i0 + ai + u;
-        As[ak * kGLd + ai + u] = (kk < kmax && row < M) ? src[u] : 0.0;
-      }
+    for (int u = 0; u < 8; ++u) ra[u] = (kk < kmax && i0 + ai + u < M) ? src[u] : 0.0;
+    const int col = j0 + gj;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int kq = kb + gq + u;
+      rg[u] = (kq < kmax && col < n) ? G[(size_t)col * ldgm + kq] : 0.0;
     }
-    {
-      const int col = j0 + gj;
+  };
+  if (kmax > 0) fetch(0);
+  for (int kb = 0; kb < kmax; kb += kBK) {
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int kk = kb + gq + u;
-        Gs[(gq + u) * kGLd + gj] = (kk < kmax && col < n) ? G[(size_t)col * ldgm + kk] : 0.0;
-      }
+    for (int u = 0; u < 8; ++u) {
+      As[ak * kBLd + ai + u] = ra[u];
+      Gs[(gq + u) * kBLd + gj] = rg[u];
     }
     __syncthreads();
-    mfma_block(As, Gs, wr, wc, acc);
+    if (kb + kBK < kmax) fetch(kb + kBK);
+    mfma_tile(As, Gs, wr, wc, acc);
     __syncthreads();
   }
+  // epilogue through LDS: per wave, one 64-row x 16-column strip at a time
+  // (8 KB per wave inside the 33 KB of smem, free after the last barrier)
   const int l = threadIdx.x & 63;
-  const int q = l >> 4;
+  double* stage = smem + wave * (64 * 16);
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int b = 0; b < 4; ++b) {
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int j = j0 + wc + b * 16 + (l & 15);
-      const int ib = i0 + wr + a * 16 + q;          // rows ib + 4r; ib has the parity of q
-      if (epi.twnl) {
-        double ffi = (j < n) ? epi.f[j] : 1.0;
-        if (ffi < 1e-5) ffi = 1.0;
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          // the imaginary row 2g+1 of G vector g lives in lane l^16 (q odd)
-          const double xr = acc[a][b][r];
-          const double xi = __shfl_xor(xr, 16, 64);
-          const int i = ib + 4 * r;
-          if ((q & 1) == 0 && j < n && i + 1 < M) {
-            const double t = epi.twnl[i >> 1];
-            double yr, yi;
-            // ctm = -ffi (-i)^l ; y = t * ctm * x
-            switch (epi.lmod4) {
-              case 0: yr = -ffi * t * xr; yi = -ffi * t * xi; break;
-              case 1: yr = -ffi * t * xi; yi = ffi * t * xr; break;
-              case 2: yr = ffi * t * xr; yi = ffi * t * xi; break;
-              default: yr = ffi * t * xi; yi = -ffi * t * xr; break;
-            }
-            double* d = D + (size_t)j * ldd + i;
-            d[0] += yr;
-            d[1] += yi;
+      for (int r = 0; r < 4; ++r) stage[(l & 15) * 64 + a * 16 + (l >> 4) + 4 * r] = acc[a][b][r];
+    __syncthreads();
+    // 32 lanes per column, each one (2p, 2p+1) row pair; 2 columns per sweep
+#pragma unroll 1
+    for (int sw = 0; sw < 8; ++sw) {
+      const int c = 2 * sw + (l >> 5);
+      const int p = l & 31;
+      const int j = j0 + wc + 16 * b + c;
+      const int i = i0 + wr + 2 * p;
+      const double x0 = stage[c * 64 + 2 * p], x1 = stage[c * 64 + 2 * p + 1];
+      if (j < n && i < M) {
+        double* d = D + (size_t)j * ldd + i;
+        if (epi.twnl) {
+          double f = epi.f[j];
+          if (f < 1e-5) f = 1.0;
+          const double tw = epi.twnl[i >> 1];
+          double yr, yi;
+          switch (epi.lmod4) {   // ctm = -f (-i)^l ; y = tw * ctm * x
+            case 0: yr = -f * tw * x0; yi = -f * tw * x1; break;
+            case 1: yr = -f * tw * x1; yi = f * tw * x0; break;
+            case 2: yr = f * tw * x0; yi = f * tw * x1; break;
+            default: yr = f * tw * x1; yi = -f * tw * x0; break;
           }
-        }
-      } else if (j < n) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = ib + 4 * r;
-          if (i < M) {
-            double* d = D + (size_t)j * ldd + i;
-            *d = (beta == 0.0 ? 0.0 : beta * *d) + alpha * acc[a][b][r];
-          }
+          d[0] += yr;
+          if (i + 1 < M) d[1] += yi;
+        } else {
+          d[0] = (beta == 0.0 ? 0.0 : beta * d[0]) + alpha * x0;
+          if (i + 1 < M) d[1] = (beta == 0.0 ? 0.0 : beta * d[1]) + alpha * x1;
         }
       }
     }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -255,19 +281,21 @@ __global__ __launch_bounds__(256) void k_gemm_nn(const double* __restrict__ A, i
 void gemm_tn(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, int K, int m, int n, double alpha,
              bool g0corr, bool symmetric, double* out, int ldo, const ProjA* proj) {
   if (m <= 0 || n <= 0) return;
-  const int tm = (m + kGT - 1) / kGT, tn = (n + kGT - 1) / kGT;
-  int tiles = symmetric ? (tm * (tm + 1)) / 2 : tm * tn;
-  // aim at ~4 workgroups per CU (256 CUs) with K chunks of at least 1024 rows
-  int nsplit = std::max(1, std::min((1024 + tiles - 1) / tiles, std::max(1, K / 1024)));
+  const int tm = (m + kBT - 1) / kBT, tn = (n + kBT - 1) / kBT;
+  const int tiles = symmetric ? (tm * (tm + 1)) / 2 : tm * tn;
+  // about 1024 busy workgroups (4 per CU), K chunks of at least 2048 rows
+  int nsplit = std::max(1, std::min((1024 + tiles - 1) / tiles, std::max(1, K / 2048)));
   int kchunk = (K + nsplit - 1) / nsplit;
-  kchunk = (kchunk + kGK - 1) / kGK * kGK;
+  kchunk = (kchunk + kBK - 1) / kBK * kBK;
   nsplit = (K + kchunk - 1) / kchunk;
   ctx->w_gemm.ensure(sizeof(double) * (size_t)nsplit * m * n);
   ProjA pa{};
   if (proj) pa = *proj;
+  const int total = nsplit * tm * tn;
+  const int launch = (total + kXcd - 1) / kXcd * kXcd;
   timer_begin(ctx, MC_KID_GEMM);
-  hipLaunchKernelGGL(k_gemm_tn, dim3(tm, tn, nsplit), dim3(256), 0, ctx->stream, A, lda, B, ldb, K, m, n, kchunk,
-                     ctx->w_gemm.as<double>(), symmetric ? 1 : 0, pa);
+  hipLaunchKernelGGL(k_gemm_tn, dim3(launch), dim3(256), 0, ctx->stream, A, lda, B, ldb, K, m, n, kchunk,
+                     ctx->w_gemm.as<double>(), symmetric ? 1 : 0, pa, tm, tn, total);
   MC_HIP(hipGetLastError());
   timer_end(ctx, MC_KID_GEMM);
   timer_begin(ctx, MC_KID_MISC);
@@ -282,9 +310,12 @@ void gemm_nn(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, i
   if (M <= 0 || n <= 0) return;
   FnlEpi e{};
   if (epi) e = *epi;
+  const int tm = (M + kBT - 1) / kBT, tn = (n + kBT - 1) / kBT;
+  const int total = tm * tn;
+  const int launch = (total + kXcd - 1) / kXcd * kXcd;
   timer_begin(ctx, MC_KID_GEMM);
-  hipLaunchKernelGGL(k_gemm_nn, dim3((M + kGT - 1) / kGT, (n + kGT - 1) / kGT), dim3(256), 0, ctx->stream, A, lda,
-                     G, ldgm, M, k, n, alpha, beta, D, ldd, upper_g ? 1 : 0, e);
+  hipLaunchKernelGGL(k_gemm_nn, dim3(launch), dim3(256), 0, ctx->stream, A, lda, G, ldgm, M, k, n, alpha, beta, D,
+                     ldd, upper_g ? 1 : 0, e, tn, total);
   MC_HIP(hipGetLastError());
   timer_end(ctx, MC_KID_GEMM);
 }

@@ -133,6 +133,11 @@ class Context:
     def axpy_wfn(self, alpha: float, x: DeviceArray, y: DeviceArray, nstate: int):
         check(lib().mc_wfn_axpy(self.h, alpha, x.ptr, y.ptr, nstate))
 
+    def dotp_wfn(self, a: DeviceArray, b: DeviceArray, nstate: int) -> np.ndarray:
+        out = np.zeros(nstate)
+        check(lib().mc_wfn_dotp(self.h, a.ptr, b.ptr, nstate, _p(out)))
+        return out
+
     def upload(self, c: np.ndarray, dev: DeviceArray | None = None) -> DeviceArray:
         c = np.asfortranarray(c, dtype=np.complex128)
         n = c.shape[1]

@@ -115,6 +115,11 @@ int mc_wfn_zero(mc_ctx* ctx, void* c_dev, int nstate);
  * c0 <- c0 + dt c2 of updwf, src/updwf_utils.mod.F90:116-128). */
 int mc_wfn_axpy(mc_ctx* ctx, double alpha, const void* x_dev, void* y_dev, int nstate);
 
+/* out[i] = dotp(a_i, b_i), the Gamma-point inner product of state i
+ * (src/dotp_utils.mod.F90:58-101); with b = c2 after vpsi + fnonloc,
+ * -sum_i out[i] is the band energy sum_i f_i <c_i|H|c_i>. */
+int mc_wfn_dotp(mc_ctx* ctx, const void* a_dev, const void* b_dev, int nstate, double* out);
+
 /* Wavefunctions host (CPMD order, leading dim ldc) <-> device layout. */
 int mc_wfn_upload(mc_ctx* ctx, const double* c_host, int ldc, int nstate, void* c_dev);
 int mc_wfn_download(mc_ctx* ctx, const void* c_dev, int nstate, double* c_host, int ldc);
