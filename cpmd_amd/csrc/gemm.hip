@@ -13,14 +13,19 @@
 //    (k x n) small; optional upper-triangular G (trmm) and a fused complex
 //    epilogue for fnonloc.
 //
-// Each workgroup computes a 128x128 tile with 4 waves (2x2), each wave a 64x64
-// block = 4x4 MFMA tiles of 16x16 (64 FP64 accumulators per lane).  Operands
+// Each workgroup computes a 128x128 tile (64x64 for small problems) with 4
+// waves (2x2), each wave a 64x64 block = 4x4 MFMA tiles of 16x16 (64 FP64
+// accumulators per lane).  Operands
 // are staged through LDS in K slices of 16; the next slice is fetched into
 // registers while the current one is multiplied.  Workgroup ids are remapped so
 // that tiles sharing an operand block run on the same XCD (ids are dealt to the
 // 8 XCDs round robin): consecutive remapped ids share the K chunk (gemm_tn) or
 // the row block of A (gemm_nn).  The tall output of gemm_nn is written through
 // an LDS stage as 512-byte column segments.
+//
+// With ctx->gemm_x3 (mixed precision) the tall products (K or M of at least
+// kX3Min rows, i.e. the wavefunction contractions) go to the split-bf16 kernels
+// of gemm_x3.hip; the small n x n products stay FP64.
 //
 // MFMA operand layout (16x16x4 f64, checked on gfx950 with a lane probe):
 // lane l holds A[i = l%16][k = l/16], B[k = l/16][j = l%16] and, in
@@ -36,25 +41,36 @@ namespace mc {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int kBT = 128;          // tile edge
 constexpr int kBK = 16;           // K slice
-constexpr int kBLd = kBT + 4;     // LDS row stride (doubles)
 constexpr int kXcd = 8;           // XCDs of an MI355X
 
-__device__ __forceinline__ void mfma_tile(const double* As, const double* Bs, int wr, int wc, d4 (&acc)[4][4]) {
+// Tile geometry: BT x BT per workgroup (BT = 128 for the tall products, 64
+// for small n x n ones), 2 x 2 waves of WB = BT/2, NB = WB/16 MFMA tiles per
+// wave edge, LDS row stride LD.
+template <int BT>
+struct Tile {
+  static constexpr int WB = BT / 2, NB = WB / 16, LD = BT + 4;
+  static constexpr int PER = BT / 16;          // doubles per thread and operand slice
+  static constexpr int TPC = 256 / BT;         // threads per tile column in a slice
+};
+
+template <int BT>
+__device__ __forceinline__ void mfma_tile(const double* As, const double* Bs, int wr, int wc,
+                                          d4 (&acc)[Tile<BT>::NB][Tile<BT>::NB]) {
+  constexpr int NB = Tile<BT>::NB, LD = Tile<BT>::LD;
   const int l = threadIdx.x & 63;
   const int li = l & 15, lk = l >> 4;
 #pragma unroll
   for (int kk = 0; kk < kBK; kk += 4) {
-    double a[4], b[4];
+    double a[NB], b[NB];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = As[(kk + lk) * kBLd + wr + 16 * i + li];
+    for (int i = 0; i < NB; ++i) a[i] = As[(kk + lk) * LD + wr + 16 * i + li];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) b[j] = Bs[(kk + lk) * kBLd + wc + 16 * j + li];
+    for (int j = 0; j < NB; ++j) b[j] = Bs[(kk + lk) * LD + wc + 16 * j + li];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < NB; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < NB; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
   }
 }
 
@@ -72,37 +88,40 @@ __device__ __forceinline__ int xcd_tile(int total) {
 //   P: partials [nsplit][n][m] (column-major m x n per split).
 // upper: tiles with tile_row > tile_col are skipped.
 // ---------------------------------------------------------------------------
+template <int BT>
 __global__ __launch_bounds__(256) void k_gemm_tn(const double* __restrict__ A, int lda,
                                                  const double* __restrict__ B, int ldb, int K, int m, int n,
                                                  int kchunk, double* __restrict__ P, int upper, ProjA proj, int tm,
                                                  int tn, int total) {
-  __shared__ double As[kBK * kBLd];
-  __shared__ double Bs[kBK * kBLd];
+  using TL = Tile<BT>;
+  constexpr int NB = TL::NB, LD = TL::LD, PER = TL::PER, WB = TL::WB;
+  __shared__ double As[kBK * LD];
+  __shared__ double Bs[kBK * LD];
   const int t = xcd_tile(total);
   if (t >= total) return;
   const int s = t / (tm * tn);
   const int tile = t - s * tm * tn;
   const int ti = tile / tn, tj = tile - ti * tn;
   if (upper && ti > tj) return;
-  const int i0 = ti * kBT, j0 = tj * kBT;
+  const int i0 = ti * BT, j0 = tj * BT;
   const int kbeg = s * kchunk;
   const int kend = min(K, kbeg + kchunk);
   const int wave = threadIdx.x >> 6;
-  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
-  d4 acc[4][4];
+  const int wr = (wave >> 1) * WB, wc = (wave & 1) * WB;
+  d4 acc[NB][NB];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < NB; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  const int lc = threadIdx.x >> 1;        // column within the tile
-  const int lq = (threadIdx.x & 1) * 8;   // k offset within the slice
-  double ra[8], rb[8];
+    for (int b = 0; b < NB; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  const int lc = threadIdx.x / TL::TPC;            // column within the tile
+  const int lq = (threadIdx.x % TL::TPC) * PER;    // k offset within the slice (even)
+  double ra[PER], rb[PER];
   auto fetch = [&](int kb) {
     const int acol = i0 + lc, bcol = j0 + lc;
     if (proj.eigr) {
-      const int g0 = (kb + lq) >> 1;   // rows kb+lq .. +8 = G vectors g0 .. g0+3 (re, im)
+      const int g0 = (kb + lq) >> 1;   // rows kb+lq .. +PER = G vectors g0 .. (re, im)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < PER / 2; ++u) {
         const int g = g0 + u;
         double re = 0.0, im = 0.0;
         if (acol < m && 2 * g < kend) {
@@ -121,13 +140,13 @@ __global__ __launch_bounds__(256) void k_gemm_tn(const double* __restrict__ A, i
       }
     } else {
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < PER; ++u) {
         const int k = kb + lq + u;
         ra[u] = (acol < m && k < kend) ? A[(size_t)acol * lda + k] : 0.0;
       }
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < PER; ++u) {
       const int k = kb + lq + u;
       rb[u] = (bcol < n && k < kend) ? B[(size_t)bcol * ldb + k] : 0.0;
     }
@@ -135,21 +154,21 @@ __global__ __launch_bounds__(256) void k_gemm_tn(const double* __restrict__ A, i
   if (kbeg < kend) fetch(kbeg);
   for (int kb = kbeg; kb < kend; kb += kBK) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      As[(lq + u) * kBLd + lc] = ra[u];
-      Bs[(lq + u) * kBLd + lc] = rb[u];
+    for (int u = 0; u < PER; ++u) {
+      As[(lq + u) * LD + lc] = ra[u];
+      Bs[(lq + u) * LD + lc] = rb[u];
     }
     __syncthreads();
     if (kb + kBK < kend) fetch(kb + kBK);
-    mfma_tile(As, Bs, wr, wc, acc);
+    mfma_tile<BT>(As, Bs, wr, wc, acc);
     __syncthreads();
   }
   const int l = threadIdx.x & 63;
   double* Ps = P + (size_t)s * m * n;
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < NB; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
+    for (int b = 0; b < NB; ++b) {
       const int j = j0 + wc + b * 16 + (l & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -183,36 +202,39 @@ __global__ void k_reduce_tn(const double* __restrict__ P, int nsplit, int m, int
 // twnl * ctm * (A G) on (re, im) row pairs (fnonloc).
 // Launch ceil(total/8)*8 workgroups, total = tm * tn (column tile fastest).
 // ---------------------------------------------------------------------------
+template <int BT>
 __global__ __launch_bounds__(256, 2) void k_gemm_nn(const double* __restrict__ A, int lda,
-                                                 const double* __restrict__ G, int ldgm, int M, int k, int n,
-                                                 double alpha, double beta, double* __restrict__ D, int ldd,
-                                                 int upper_g, FnlEpi epi, int tn, int total) {
-  __shared__ double smem[2 * kBK * kBLd];
+                                                    const double* __restrict__ G, int ldgm, int M, int k, int n,
+                                                    double alpha, double beta, double* __restrict__ D, int ldd,
+                                                    int upper_g, FnlEpi epi, int tn, int total) {
+  using TL = Tile<BT>;
+  constexpr int NB = TL::NB, LD = TL::LD, PER = TL::PER, WB = TL::WB;
+  __shared__ double smem[2 * kBK * LD];
   double* As = smem;
-  double* Gs = smem + kBK * kBLd;
+  double* Gs = smem + kBK * LD;
   const int t = xcd_tile(total);
   if (t >= total) return;
   const int rt = t / tn, ct = t - rt * tn;
-  const int i0 = rt * kBT, j0 = ct * kBT;
+  const int i0 = rt * BT, j0 = ct * BT;
   const int wave = threadIdx.x >> 6;
-  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
-  d4 acc[4][4];
+  const int wr = (wave >> 1) * WB, wc = (wave & 1) * WB;
+  d4 acc[NB][NB];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < NB; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-  const int kmax = upper_g ? min(k, j0 + kBT) : k;
-  const int ak = threadIdx.x >> 4, ai = (threadIdx.x & 15) * 8;   // A slice: 16 k x 128 rows
-  const int gj = threadIdx.x >> 1, gq = (threadIdx.x & 1) * 8;   // G slice: 16 k x 128 cols
-  double ra[8], rg[8];
+    for (int b = 0; b < NB; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  const int kmax = upper_g ? min(k, j0 + BT) : k;
+  const int ak = threadIdx.x >> 4, ai = (threadIdx.x & 15) * PER;       // A slice: 16 k x BT rows
+  const int gj = threadIdx.x / TL::TPC, gq = (threadIdx.x % TL::TPC) * PER;   // G slice: 16 k x BT cols
+  double ra[PER], rg[PER];
   auto fetch = [&](int kb) {
     const int kk = kb + ak;
     const double* src = A + (size_t)kk * lda + i0 + ai;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) ra[u] = (kk < kmax && i0 + ai + u < M) ? src[u] : 0.0;
+    for (int u = 0; u < PER; ++u) ra[u] = (kk < kmax && i0 + ai + u < M) ? src[u] : 0.0;
     const int col = j0 + gj;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < PER; ++u) {
       const int kq = kb + gq + u;
       rg[u] = (kq < kmax && col < n) ? G[(size_t)col * ldgm + kq] : 0.0;
     }
@@ -220,34 +242,35 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(const double* __restrict__ A
   if (kmax > 0) fetch(0);
   for (int kb = 0; kb < kmax; kb += kBK) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      As[ak * kBLd + ai + u] = ra[u];
-      Gs[(gq + u) * kBLd + gj] = rg[u];
+    for (int u = 0; u < PER; ++u) {
+      As[ak * LD + ai + u] = ra[u];
+      Gs[(gq + u) * LD + gj] = rg[u];
     }
     __syncthreads();
     if (kb + kBK < kmax) fetch(kb + kBK);
-    mfma_tile(As, Gs, wr, wc, acc);
+    mfma_tile<BT>(As, Gs, wr, wc, acc);
     __syncthreads();
   }
-  // epilogue through LDS: per wave, one 64-row x 16-column strip at a time
-  // (8 KB per wave inside the 33 KB of smem, free after the last barrier)
+  // epilogue through LDS: per wave, one WB-row x 16-column strip at a time
+  // (WB*16 doubles per wave inside smem, free after the last barrier)
   const int l = threadIdx.x & 63;
-  double* stage = smem + wave * (64 * 16);
+  double* stage = smem + wave * (WB * 16);
+  constexpr int LPC = WB / 2;          // lanes per column (one row pair each)
+  constexpr int CPS = 64 / LPC;        // columns per sweep
 #pragma unroll
-  for (int b = 0; b < 4; ++b) {
+  for (int b = 0; b < NB; ++b) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < NB; ++a)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) stage[(l & 15) * 64 + a * 16 + (l >> 4) + 4 * r] = acc[a][b][r];
+      for (int r = 0; r < 4; ++r) stage[(l & 15) * WB + a * 16 + (l >> 4) + 4 * r] = acc[a][b][r];
     __syncthreads();
-    // 32 lanes per column, each one (2p, 2p+1) row pair; 2 columns per sweep
 #pragma unroll 1
-    for (int sw = 0; sw < 8; ++sw) {
-      const int c = 2 * sw + (l >> 5);
-      const int p = l & 31;
+    for (int sw = 0; sw < 16 / CPS; ++sw) {
+      const int c = CPS * sw + l / LPC;
+      const int p = l % LPC;
       const int j = j0 + wc + 16 * b + c;
       const int i = i0 + wr + 2 * p;
-      const double x0 = stage[c * 64 + 2 * p], x1 = stage[c * 64 + 2 * p + 1];
+      const double x0 = stage[c * WB + 2 * p], x1 = stage[c * WB + 2 * p + 1];
       if (j < n && i < M) {
         double* d = D + (size_t)j * ldd + i;
         if (epi.twnl) {
@@ -281,10 +304,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(const double* __restrict__ A
 void gemm_tn(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, int K, int m, int n, double alpha,
              bool g0corr, bool symmetric, double* out, int ldo, const ProjA* proj) {
   if (m <= 0 || n <= 0) return;
-  const int tm = (m + kBT - 1) / kBT, tn = (n + kBT - 1) / kBT;
+  if (ctx->gemm_x3 && K >= kX3Min) {
+    gemm_tn_x3(ctx, A, lda, B, ldb, K, m, n, alpha, g0corr, symmetric, out, ldo, proj);
+    return;
+  }
+  // 128-tiles for long K, 64-tiles (more workgroups) for short K
+  const int BT = K >= 16384 ? 128 : 64;
+  const int tm = (m + BT - 1) / BT, tn = (n + BT - 1) / BT;
   const int tiles = symmetric ? (tm * (tm + 1)) / 2 : tm * tn;
-  // about 1024 busy workgroups (4 per CU), K chunks of at least 2048 rows
-  int nsplit = std::max(1, std::min((1024 + tiles - 1) / tiles, std::max(1, K / 2048)));
+  // about 1024 busy workgroups (4 per CU), K chunks of at least 512 rows
+  int nsplit = std::max(1, std::min((1024 + tiles - 1) / tiles, std::max(1, K / 512)));
   int kchunk = (K + nsplit - 1) / nsplit;
   kchunk = (kchunk + kBK - 1) / kBK * kBK;
   nsplit = (K + kchunk - 1) / kchunk;
@@ -294,10 +323,19 @@ void gemm_tn(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, in
   const int total = nsplit * tm * tn;
   const int launch = (total + kXcd - 1) / kXcd * kXcd;
   timer_begin(ctx, MC_KID_GEMM);
-  hipLaunchKernelGGL(k_gemm_tn, dim3(launch), dim3(256), 0, ctx->stream, A, lda, B, ldb, K, m, n, kchunk,
-                     ctx->w_gemm.as<double>(), symmetric ? 1 : 0, pa, tm, tn, total);
+  if (BT == 128)
+    hipLaunchKernelGGL(k_gemm_tn<128>, dim3(launch), dim3(256), 0, ctx->stream, A, lda, B, ldb, K, m, n, kchunk,
+                       ctx->w_gemm.as<double>(), symmetric ? 1 : 0, pa, tm, tn, total);
+  else
+    hipLaunchKernelGGL(k_gemm_tn<64>, dim3(launch), dim3(256), 0, ctx->stream, A, lda, B, ldb, K, m, n, kchunk,
+                       ctx->w_gemm.as<double>(), symmetric ? 1 : 0, pa, tm, tn, total);
   MC_HIP(hipGetLastError());
   timer_end(ctx, MC_KID_GEMM);
+  reduce_tn(ctx, nsplit, m, n, alpha, A, lda, B, ldb, g0corr, symmetric, out, ldo);
+}
+
+void reduce_tn(mc_ctx* ctx, int nsplit, int m, int n, double alpha, const double* A, int lda, const double* B, int ldb,
+               bool g0corr, bool symmetric, double* out, int ldo) {
   timer_begin(ctx, MC_KID_MISC);
   hipLaunchKernelGGL(k_reduce_tn, dim3((m + 255) / 256, n), dim3(256), 0, ctx->stream, ctx->w_gemm.as<double>(),
                      nsplit, m, n, alpha, A, lda, B, ldb, g0corr ? 1 : 0, symmetric ? 1 : 0, out, ldo);
@@ -308,14 +346,26 @@ void gemm_tn(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, in
 void gemm_nn(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n, double alpha,
              double beta, double* D, int ldd, bool upper_g, const FnlEpi* epi) {
   if (M <= 0 || n <= 0) return;
+  if (ctx->gemm_x3 && M >= kX3Min && M % 2 == 0 && lda % 2 == 0 && ldd % 2 == 0 &&
+      (reinterpret_cast<uintptr_t>(A) & 15) == 0) {
+    gemm_nn_x3(ctx, A, lda, G, ldgm, M, k, n, alpha, beta, D, ldd, upper_g, epi);
+    return;
+  }
   FnlEpi e{};
   if (epi) e = *epi;
-  const int tm = (M + kBT - 1) / kBT, tn = (n + kBT - 1) / kBT;
+  // 128-tiles when they already give >= 1024 workgroups, 64-tiles otherwise
+  const int t128 = ((M + 127) / 128) * ((n + 127) / 128);
+  const int BT = t128 >= 1024 ? 128 : 64;
+  const int tm = (M + BT - 1) / BT, tn = (n + BT - 1) / BT;
   const int total = tm * tn;
   const int launch = (total + kXcd - 1) / kXcd * kXcd;
   timer_begin(ctx, MC_KID_GEMM);
-  hipLaunchKernelGGL(k_gemm_nn, dim3(launch), dim3(256), 0, ctx->stream, A, lda, G, ldgm, M, k, n, alpha, beta, D,
-                     ldd, upper_g ? 1 : 0, e, tn, total);
+  if (BT == 128)
+    hipLaunchKernelGGL(k_gemm_nn<128>, dim3(launch), dim3(256), 0, ctx->stream, A, lda, G, ldgm, M, k, n, alpha, beta,
+                       D, ldd, upper_g ? 1 : 0, e, tn, total);
+  else
+    hipLaunchKernelGGL(k_gemm_nn<64>, dim3(launch), dim3(256), 0, ctx->stream, A, lda, G, ldgm, M, k, n, alpha, beta,
+                       D, ldd, upper_g ? 1 : 0, e, tn, total);
   MC_HIP(hipGetLastError());
   timer_end(ctx, MC_KID_GEMM);
 }

@@ -168,6 +168,7 @@ struct mc_ctx {
   mc::Projectors proj;
 
   bool fft_fp32 = false;   // rhoofr/vpsi transforms and workspaces in FP32
+  bool gemm_x3 = false;    // tall wavefunction contractions as split-bf16 MFMA products
 
   int batch_pairs_vpsi = 8;
   int batch_pairs_rho = 16;
@@ -188,6 +189,15 @@ void gemm_tn(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, in
              bool g0corr, bool symmetric, double* out, int ldo, const ProjA* proj);
 void gemm_nn(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n, double alpha,
              double beta, double* D, int ldd, bool upper_g, const FnlEpi* epi);
+// out = alpha * sum of the nsplit partials in ctx->w_gemm (- G=0 term), mirrored if symmetric
+void reduce_tn(mc_ctx* ctx, int nsplit, int m, int n, double alpha, const double* A, int lda, const double* B, int ldb,
+               bool g0corr, bool symmetric, double* out, int ldo);
+// split-bf16 (3-term, FP32 accumulation) variants used in mixed precision (gemm_x3.hip)
+constexpr int kX3Min = 16384;   // rows of K (tn) or M (nn) from which the split kernels are used
+void gemm_tn_x3(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, int K, int m, int n, double alpha,
+                bool g0corr, bool symmetric, double* out, int ldo, const ProjA* proj);
+void gemm_nn_x3(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n, double alpha,
+                double beta, double* D, int ldd, bool upper_g, const FnlEpi* epi);
 
 // orthonormalisation and projectors (ortho.hip)
 void ovlap_dev(mc_ctx* ctx, int n, double* a_dev, int lda, const double2* c1, const double2* c2);

@@ -215,11 +215,22 @@ void ovlap_dev(mc_ctx* ctx, int n, double* a_dev, int lda, const double2* c1, co
           ctx->geq0 != 0, c1 == c2, a_dev, lda, nullptr);
 }
 
-// C <- C X with X (n x n) on the device; through the workspace copy of C.
+// C <- C X with X (n x n) on the device; through the workspace copy of C.  With
+// split-bf16 products C X is formed as C + C (X - I), so that the rounding of the
+// product scales with |X - I| (small near orthonormality), not with |X|.
 static void apply_right(mc_ctx* ctx, double2* c, const double* X, int n, bool upper) {
   const size_t bytes = sizeof(double2) * (size_t)ctx->ldg * n;
   ctx->w_wfn.ensure(bytes);
   MC_HIP(hipMemcpyAsync(ctx->w_wfn.p, c, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  if (ctx->gemm_x3) {
+    double* XmI = small(ctx, 6, n);
+    hipLaunchKernelGGL(k_axpby_eye, dim3((n + 255) / 256, n), dim3(256), 0, ctx->stream, X, 1.0, -1.0,
+                       (const double*)nullptr, 0.0, XmI, n);
+    MC_HIP(hipGetLastError());
+    gemm_nn(ctx, ctx->w_wfn.as<double>(), wfn_ld(ctx), XmI, n, wfn_ld(ctx), n, n, 1.0, 1.0, (double*)c, wfn_ld(ctx),
+            upper, nullptr);
+    return;
+  }
   gemm_nn(ctx, ctx->w_wfn.as<double>(), wfn_ld(ctx), X, n, wfn_ld(ctx), n, n, 1.0, 0.0, (double*)c, wfn_ld(ctx),
           upper, nullptr);
 }

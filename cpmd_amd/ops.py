@@ -116,9 +116,10 @@ class Context:
     def set_batch(self, pairs_rhoofr: int, pairs_vpsi: int):
         check(lib().mc_ctx_set_batch(self.h, pairs_rhoofr, pairs_vpsi))
 
-    def set_precision(self, fft: str):
-        """'fp64' (default) or 'fp32' for the rhoofr/vpsi transforms."""
-        check(lib().mc_set_precision(self.h, {"fp64": 0, "fp32": 1}[fft]))
+    def set_precision(self, mode: str):
+        """'fp64' (default), 'fp32' (FP32 rhoofr/vpsi transforms) or 'mixed'
+        (FP32 transforms and split-bf16 MFMA wavefunction contractions)."""
+        check(lib().mc_set_precision(self.h, {"fp64": 0, "fp32": 1, "mixed": 2}[mode]))
 
     def synchronize(self):
         check(lib().mc_ctx_synchronize(self.h))

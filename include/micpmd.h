@@ -91,13 +91,18 @@ int mc_ctx_synchronize(mc_ctx* ctx);
 int mc_ctx_set_batch(mc_ctx* ctx, int pairs_rhoofr, int pairs_vpsi);
 size_t mc_wfn_ld(const mc_ctx* ctx);
 
-/* Arithmetic of the rhoofr/vpsi transforms (no counterpart in the reference,
- * whose transforms are FP64 only):
- *   MC_PREC_FP64  every transform, workspace and accumulation in FP64 (default);
- *   MC_PREC_FP32  transforms and stick/column workspaces in FP32; wavefunctions,
- *                 V(r), rho(r), the Gamma decode, the kinetic term, c2 and all
- *                 charge/energy sums stay FP64. */
-enum { MC_PREC_FP64 = 0, MC_PREC_FP32 = 1 };
+/* Arithmetic mode (no counterpart in the reference, which is FP64 only):
+ *   MC_PREC_FP64   every transform, contraction and accumulation in FP64 (default);
+ *   MC_PREC_FP32   rhoofr/vpsi transforms and stick/column workspaces in FP32;
+ *                  wavefunctions, V(r), rho(r), the Gamma decode, the kinetic
+ *                  term, c2 and all charge/energy sums stay FP64;
+ *   MC_PREC_MIXED  MC_PREC_FP32 plus the tall wavefunction contractions (ovlap,
+ *                  rnlsm1, fnonloc, rotate, the C X products of rgs/lowdin) as
+ *                  3-term bf16 split MFMA products with FP32 accumulation per K
+ *                  chunk and FP64 chunk sums; n x n algebra (Cholesky, inverse,
+ *                  Newton-Schulz) stays FP64.  lowdin/rgs apply C X as
+ *                  C + C (X - I). */
+enum { MC_PREC_FP64 = 0, MC_PREC_FP32 = 1, MC_PREC_MIXED = 2 };
 int mc_set_precision(mc_ctx* ctx, int fft_precision);
 
 /* Device memory helpers (hipMalloc / hipFree / hipMemcpy). */

@@ -118,6 +118,11 @@ def _setup_dense(L):
     L.or_rortho.restype = _ci
     L.or_rnlsm1.argtypes = [_ci, _ci, _ci, _ip, _ci, _ip, _ip, _dp, _ci, _dp, _ci, _dp, _ci, _ci, _dp]
     L.or_fnonloc.argtypes = [_ci, _ci, _ip, _ci, _ip, _ip, _dp, _dp, _ci, _dp, _ci, _dp, _dp, _dp, _ci, _ci]
+    L.or_rnlsm2.argtypes = [_ci, _ci, C.c_double, _dp, _ci, _ip, _ci, _ip, _ip, _dp, _ci, _dp, _ci, _dp, _ci, _ci,
+                            _dp]
+    L.or_rnlfor.argtypes = [_ci, _ip, _ci, _ip, _dp, _dp, _dp, _dp, _ci, _dp]
+    L.or_enl.argtypes = [_ci, _ip, _ci, _ip, _dp, _dp, _dp, _ci]
+    L.or_enl.restype = C.c_double
 
 
 def _L():
@@ -236,3 +241,32 @@ def fnonloc(P: Projectors, c2, f, fnl):
                     _p(P.twnl), P.twnl.shape[0], _p(P.eigr), P.eigr.shape[0], _p(fnl), _p(f), _p(out), out.shape[0],
                     out.shape[1])
     return out
+
+
+def rnlsm2(P: Projectors, c0, gk, tpiba, geq0=True):
+    """dfnl (nat, nhxs, 3, nstate); gk (3, ngw) in units of 2 pi / alat."""
+    c0 = np.asfortranarray(c0, dtype=np.complex128)
+    gk = np.asfortranarray(gk, dtype=np.float64)
+    n = c0.shape[1]
+    dfnl = np.zeros((P.nat, P.nhxs, 3, n), order="F")
+    _L().or_rnlsm2(c0.shape[0], int(geq0), float(tpiba), _p(gk), P.nsp, _p(P.na, _ip), P.nhxs, _p(P.ngh, _ip),
+                   _p(P.lang, _ip), _p(P.twnl), P.twnl.shape[0], _p(P.eigr), P.eigr.shape[0], _p(c0), c0.shape[0], n,
+                   _p(dfnl))
+    return dfnl
+
+
+def rnlfor(P: Projectors, fnl, dfnl, f, fion=None):
+    """fion (3, nat) -= the nonlocal forces (rnlfor accumulates)."""
+    fnl = np.asfortranarray(fnl, dtype=np.float64)
+    dfnl = np.asfortranarray(dfnl, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros((3, P.nat), order="F") if fion is None else _F(fion, np.float64)
+    _L().or_rnlfor(P.nsp, _p(P.na, _ip), P.nhxs, _p(P.ngh, _ip), _p(P.hmat), _p(fnl), _p(dfnl), _p(f), len(f),
+                   _p(out))
+    return out
+
+
+def enl(P: Projectors, fnl, f):
+    fnl = np.asfortranarray(fnl, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return _L().or_enl(P.nsp, _p(P.na, _ip), P.nhxs, _p(P.ngh, _ip), _p(P.hmat), _p(fnl), _p(f), len(f))

@@ -540,3 +540,88 @@ void or_fnonloc(int ngw, int nsp, const int* na, int nhxs, const int* ngh, const
     isa0 += na[is];
   }
 }
+
+/* rnlsm2, src/rnlsm2_utils.mod.F90:321-482 (Gamma, imagp = 1):
+ * dfnl(isa,iv,k,i) = 2 tpiba sum_G Re(conj(eiscr) c), eiscr = gk(k,G) twnl (-i)^(l+1) eigr,
+ * G=0 halved (:441); gk (3 x ldgk) in units of 2 pi / alat */
+void or_rnlsm2(int ngw, int geq0, double tpiba, const double* gk, int nsp, const int* na, int nhxs, const int* ngh,
+               const int* lang, const double* twnl, int ldtw, const double* eigrd, int lde, const double* c0d, int ldc,
+               int n, double* dfnl) {
+  const cplx* eigr = (const cplx*)eigrd;
+  const cplx* c0 = (const cplx*)c0d;
+  int nat = 0;
+  for (int is = 0; is < nsp; ++is) nat += na[is];
+  memset(dfnl, 0, sizeof(double) * (size_t)nat * nhxs * 3 * n);
+  for (int k = 0; k < 3; ++k) {
+    int isa0 = 0;
+    for (int is = 0; is < nsp; ++is) {
+      for (int iv = 0; iv < ngh[is]; ++iv) {
+        cplx ci = mi_pow(lang[iv + nhxs * is] + 1);
+        for (int ia = 0; ia < na[is]; ++ia)
+          for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int g = 0; g < ngw; ++g) {
+              cplx e = gk[k + 3 * (size_t)g] * twnl[g + (size_t)ldtw * (iv + nhxs * is)] * ci *
+                       eigr[g + (size_t)lde * (isa0 + ia)];
+              if (geq0 && g == 0) e *= 0.5;
+              s += creal(e) * creal(c0[g + (size_t)ldc * i]) + cimag(e) * cimag(c0[g + (size_t)ldc * i]);
+            }
+            dfnl[(isa0 + ia) + (size_t)nat * (iv + (size_t)nhxs * (k + 3 * (size_t)i))] = 2.0 * tpiba * s;
+          }
+      }
+      isa0 += na[is];
+    }
+  }
+}
+
+/* rnlfor, src/rnlfor_utils.mod.F90:201-280 (Goedecker branch; BHS/TM :281+ is
+ * the diagonal case of h): fion(k,isa) -= sum_i 2 f_i sum_{iv,jv} h(iv,jv)
+ * dfnl(isa,iv,k,i) fnl(isa,jv,i), states with |f_i| <= 1e-12 skipped */
+void or_rnlfor(int nsp, const int* na, int nhxs, const int* ngh, const double* hmat, const double* fnl,
+               const double* dfnl, const double* f, int n, double* fion) {
+  int nat = 0;
+  for (int is = 0; is < nsp; ++is) nat += na[is];
+  int isa0 = 0;
+  for (int is = 0; is < nsp; ++is) {
+    for (int iv = 0; iv < ngh[is]; ++iv)
+      for (int jv = 0; jv < ngh[is]; ++jv) {
+        const double h = hmat[iv + nhxs * (jv + (size_t)nhxs * is)];
+        if (h == 0.0) continue;
+        for (int i = 0; i < n; ++i) {
+          if (fabs(f[i]) <= 1e-12) continue;
+          const double tt = 2.0 * f[i] * h;
+          for (int ia = 0; ia < na[is]; ++ia) {
+            const int isa = isa0 + ia;
+            const double fj = fnl[isa + (size_t)nat * (jv + (size_t)nhxs * i)];
+            for (int k = 0; k < 3; ++k)
+              fion[k + 3 * (size_t)isa] -= tt * dfnl[isa + (size_t)nat * (iv + (size_t)nhxs * (k + 3 * (size_t)i))] * fj;
+          }
+        }
+      }
+    isa0 += na[is];
+  }
+}
+
+/* E_NL = sum_i f_i sum_isa sum_{iv,jv} h(iv,jv) fnl(isa,iv,i) fnl(isa,jv,i)
+ * (the energy whose position derivative rnlfor returns) */
+double or_enl(int nsp, const int* na, int nhxs, const int* ngh, const double* hmat, const double* fnl,
+              const double* f, int n) {
+  int nat = 0;
+  for (int is = 0; is < nsp; ++is) nat += na[is];
+  double e = 0.0;
+  int isa0 = 0;
+  for (int is = 0; is < nsp; ++is) {
+    for (int iv = 0; iv < ngh[is]; ++iv)
+      for (int jv = 0; jv < ngh[is]; ++jv) {
+        const double h = hmat[iv + nhxs * (jv + (size_t)nhxs * is)];
+        for (int i = 0; i < n; ++i)
+          for (int ia = 0; ia < na[is]; ++ia) {
+            const int isa = isa0 + ia;
+            e += f[i] * h * fnl[isa + (size_t)nat * (iv + (size_t)nhxs * i)] *
+                 fnl[isa + (size_t)nat * (jv + (size_t)nhxs * i)];
+          }
+      }
+    isa0 += na[is];
+  }
+  return e;
+}

@@ -1,0 +1,143 @@
+"""GPU checks of the mixed-precision contractions (MC_PREC_MIXED): 3-term
+bf16 split MFMA products with FP32 accumulation per K chunk.
+
+Error model.  hi = bf16(x) and lo = bf16(x - hi) leave |x - hi| <= 2^-8 |x| and
+|d| = |x - hi - lo| <= 2^-16 |x|; with the dropped lo*lo term one product
+carries a relative error of at most 3 * 2^-16; the FP32 accumulation of a K
+chunk of length L adds at most L * 2^-24 of the sum of |terms|.  So every entry
+obeys
+    |out_mixed - out_fp64| <= (3 * 2^-16 + L * 2^-24) * sum_k |a_k| |b_k|
+(L <= 2048 for the split-K products, L = n for the n-column rotations).  The
+tests check that bound entrywise against the oracle, and tighter typical-error
+bounds for near-orthonormal sets: the 3-term product biases every correlated
+diagonal term by -lo^2 (mean 2^-16/3 E[1/m^2] = 2.8e-6 relative for mantissas m
+in [1, 2)), the symmetric overlap keeps the lo*lo term and is unbiased."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from cpmd_amd import ops
+from cpmd_amd import system as S
+
+pytestmark = pytest.mark.gpu
+
+U_SPLIT = 3 * 2.0 ** -16
+U32 = 2.0 ** -24
+
+
+def real_rows(c):
+    """(2*ngw x n) real view of the half-sphere coefficients."""
+    c = np.asarray(c)
+    out = np.empty((2 * c.shape[0], c.shape[1]))
+    out[0::2] = c.real
+    out[1::2] = c.imag
+    return out
+
+
+@pytest.fixture(scope="module")
+def cfg2_130():
+    """cfg2 plane-wave set (68,338 PW, 136,676 real rows) with 130 states."""
+    sysm = S.make_system(24.0, 64, 70.0, nstate=130)
+    c = S.initial_coefficients(sysm, seed=31)
+    rng = np.random.default_rng(32)
+    d = (rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape)) * np.exp(
+        -sysm.gset.hg[:sysm.gset.ngw, None] * sysm.cell.tpiba2 / 8)
+    d[0] = d[0].real
+    cp = np.asfortranarray(c + 0.02 * d / np.sqrt(np.sum(np.abs(d) ** 2, axis=0)))
+    return sysm, c, cp
+
+
+def test_mixed_ovlap(gpu, cfg2_130):
+    sysm, c, cp = cfg2_130
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    ctx.set_precision("mixed")
+    for a, b in ((cp, cp), (cp, c)):
+        got = ctx.ovlap(a, b)
+        ref = oracle.ovlap(a, b)
+        bound = 2.0 * np.abs(real_rows(a)).T @ np.abs(real_rows(b))
+        err = np.abs(got - ref)
+        assert np.all(err <= (U_SPLIT + 2048 * U32) * bound + 1e-300)
+        dg = np.diag(got - ref)
+        msg = f"diag err mean {dg.mean():.3e} std {dg.std():.3e} max {err.max():.3e}"
+        if a is b:
+            # 4-term: the rounding left is random in sign, its mean over the
+            # 130 diagonal entries is small next to its spread
+            assert err.max() < 1e-6, msg
+            assert abs(dg.mean()) < 0.3 * dg.std() + 1e-9, msg
+        else:
+            # 3-term: every correlated diagonal term loses lo^2 (mean 2.8e-6 relative)
+            assert err.max() < 6e-6, msg
+            assert -4e-6 < dg.mean() < -1.5e-6, msg
+        assert err.max() > 0.0                         # the split kernels ran
+    s = ctx.ovlap(cp, cp)
+    assert np.array_equal(s, s.T)
+    ctx.set_precision("fp64")
+    assert np.abs(ctx.ovlap(cp, c) - oracle.ovlap(cp, c)).max() < 1e-13
+
+
+def test_mixed_rotate(gpu, cfg2_130):
+    sysm, c, cp = cfg2_130
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    ctx.set_precision("mixed")
+    rng = np.random.default_rng(5)
+    n = c.shape[1]
+    gam = rng.standard_normal((n, n))
+    got = ctx.rotate(0.5, c, -2.0, cp, gam)
+    ref = oracle.rotate(0.5, c, -2.0, cp, gam)
+    bound = 0.5 * np.abs(real_rows(c)) @ np.abs(gam)
+    err = np.abs(real_rows(got) - real_rows(ref))
+    assert np.all(err <= (U_SPLIT + n * U32) * bound + 1e-15 * np.abs(real_rows(ref)))
+    assert err.max() > 0.0
+
+
+def test_mixed_lowdin_rgs(gpu, cfg2_130):
+    """Orthonormality after the mixed-precision factorisations, measured in FP64."""
+    sysm, c, cp = cfg2_130
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    n = c.shape[1]
+    ref = oracle.lowdin(cp)[0]
+    ctx.set_precision("mixed")
+    g = ctx.lowdin(cp)
+    r = ctx.rgs(cp)[0]
+    for out in (g, r):
+        assert np.abs(S.dotp_gamma(out, out) - np.eye(n)).max() < 1e-6
+    # C (I + E) with E = S^-1/2 - I: the product error scales with |E| ~ 0.02
+    assert np.abs(g - ref).max() < 1e-6 * np.abs(ref).max()
+
+
+def test_mixed_projectors(gpu):
+    sysm = S.make_system(24.0, 64, 70.0, nstate=64)
+    c = S.initial_coefficients(sysm, seed=41)
+    ngw = sysm.gset.ngw
+    na, nhxs, ngh, lang, hmat, twnl, eigr = S.projector_set(sysm, sysm.gset.miller[:ngw], sysm.gset.hg[:ngw])
+    P = dict(na=na, nhxs=nhxs, ngh=ngh, lang=lang, hmat=hmat, twnl=twnl, eigr=eigr)
+    OP = oracle.Projectors(**P)
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    ctx.set_projectors(na, nhxs, ngh, lang, hmat, twnl)
+    ctx.set_eigr(eigr)
+    ctx.set_precision("mixed")
+    fnl = ctx.rnlsm1(c)
+    ofnl = oracle.rnlsm1(OP, c)
+    # |Re/Im of (-i)^l twnl eigr| <= |twnl|: sum_k |a||b| <= 2 sum_G |twnl| (|Re c| + |Im c|)
+    tw = np.abs(twnl[:, 0, 0])
+    b = 2.0 * (tw[:, None] * (np.abs(c.real) + np.abs(c.imag))).sum(0)        # (nstate,)
+    err = np.abs(fnl - ofnl)
+    assert np.all(err <= (U_SPLIT + 2048 * U32) * b[None, None, :] * math.sqrt(2) + 1e-300)
+    assert err.max() > 0.0
+    f = np.full(c.shape[1], 2.0)
+    f[3] = 0.0
+    rng = np.random.default_rng(7)
+    c2 = np.asfortranarray(rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape))
+    out = ctx.fnonloc(c2, f, ofnl)
+    oout = oracle.fnonloc(OP, c2, f, ofnl)
+    # term bound: f * |twnl(G)| * sum_ia |dd(ia, i)|, dd = h fnl; the (re, im)
+    # epilogue mixes the two components of the product (factor 2)
+    h = float(hmat[0, 0, 0])
+    fe = np.where(f < 1e-5, 1.0, f)
+    bnd = (fe[None, :] * tw[:, None] * (abs(h) * np.abs(ofnl[:, 0, :]).sum(0))[None, :]) * 2.0
+    nat = int(np.sum(na))
+    for part in (np.real, np.imag):
+        err2 = np.abs(part(out) - part(oout))
+        assert np.all(err2 <= (U_SPLIT + nat * U32) * bnd + 1e-15 * np.abs(oout).max())

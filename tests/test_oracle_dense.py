@@ -190,3 +190,67 @@ def test_fnonloc_is_gradient(lval):
     h = 1e-5
     de = (energy(c + h * d) - energy(c - h * d)) / (2 * h)
     assert np.trace(S.dotp_gamma(c2, d)) == pytest.approx(-0.5 * de, rel=1e-7)
+
+
+def _proj_case_multi():
+    """One species on 2 atoms with projectors s, s' (coupled by h12) and p_y."""
+    cell, gs, c = small_system("ortho", 3, seed=12)
+    ngw = gs.ngw
+    g = gs.miller[:ngw] @ cell.bvec * cell.tpiba
+    g2 = (g ** 2).sum(1)
+    tw = np.zeros((ngw, 3, 1))
+    tw[:, 0, 0] = np.exp(-0.3 * g2)
+    tw[:, 1, 0] = g2 * np.exp(-0.35 * g2)
+    tw[:, 2, 0] = g[:, 1] * np.exp(-0.3 * g2)
+    h = np.zeros((3, 3, 1))
+    h[:2, :2, 0] = [[1.7, 0.4], [0.4, -0.6]]
+    h[2, 2, 0] = 0.9
+
+    def proj(tau):
+        frac = tau @ np.linalg.inv(cell.avec)
+        eigr = np.exp(-2j * math.pi * (gs.miller[:ngw] @ frac.T))
+        return oracle.Projectors(na=[2], nhxs=3, ngh=[3], lang=[0, 0, 1], hmat=h, twnl=tw, eigr=eigr)
+
+    tau = np.array([[1.0, 2.0, 0.5], [3.0, 0.2, 4.0]])
+    gk = (gs.miller[:ngw] @ cell.bvec).T              # (3, ngw), units of 2 pi / alat
+    return cell, gs, c, proj, tau, gk
+
+
+def test_rnlsm2_is_position_derivative():
+    """dfnl(ia, iv, k, i) = d fnl(ia, iv, i) / d tau_k(ia) (central differences)."""
+    cell, gs, c, proj, tau, gk = _proj_case_multi()
+    dfnl = oracle.rnlsm2(proj(tau), c, gk, cell.tpiba)
+    h = 1e-5
+    for ia in range(2):
+        for k in range(3):
+            tp, tm = tau.copy(), tau.copy()
+            tp[ia, k] += h
+            tm[ia, k] -= h
+            fd = (oracle.rnlsm1(proj(tp), c)[ia] - oracle.rnlsm1(proj(tm), c)[ia]) / (2 * h)
+            assert np.abs(dfnl[ia, :, k, :] - fd).max() < 1e-8 * max(1.0, np.abs(fd).max())
+    # the other atom's fnl does not move
+    tp = tau.copy()
+    tp[0, 0] += 1e-3
+    assert np.array_equal(oracle.rnlsm1(proj(tp), c)[1], oracle.rnlsm1(proj(tau), c)[1])
+
+
+def test_rnlfor_is_force():
+    """rnlfor = -dE_NL/dtau with E_NL = sum_i f_i fnl^T h fnl (central differences)."""
+    cell, gs, c, proj, tau, gk = _proj_case_multi()
+    f = np.array([2.0, 0.0, 1.5])
+    P = proj(tau)
+    fnl = oracle.rnlsm1(P, c)
+    fion = oracle.rnlfor(P, fnl, oracle.rnlsm2(P, c, gk, cell.tpiba), f)
+    # accumulates into an existing array
+    base = np.arange(6.0).reshape(3, 2, order="F")
+    assert np.allclose(oracle.rnlfor(P, fnl, oracle.rnlsm2(P, c, gk, cell.tpiba), f, base) - base, fion,
+                       rtol=0, atol=1e-14)
+    h = 1e-5
+    for ia in range(2):
+        for k in range(3):
+            tp, tm = tau.copy(), tau.copy()
+            tp[ia, k] += h
+            tm[ia, k] -= h
+            ep = oracle.enl(proj(tp), oracle.rnlsm1(proj(tp), c), f)
+            em = oracle.enl(proj(tm), oracle.rnlsm1(proj(tm), c), f)
+            assert fion[k, ia] == pytest.approx(-(ep - em) / (2 * h), rel=1e-7, abs=1e-10)
