@@ -77,6 +77,10 @@ SIGNATURES = {
     "mc_set_eigr": (C.c_int, [_vp, _dp, C.c_int, C.c_int]),
     "mc_rnlsm1": (C.c_int, [_vp, _dp, C.c_int, C.c_int, _dp]),
     "mc_rnlsm1_dev": (C.c_int, [_vp, _vp, C.c_int, _dp]),
+    "mc_set_gk": (C.c_int, [_vp, _dp]),
+    "mc_rnlsm2": (C.c_int, [_vp, _dp, C.c_int, C.c_int, _dp]),
+    "mc_rnlsm2_dev": (C.c_int, [_vp, _vp, C.c_int, _dp]),
+    "mc_rnlfor": (C.c_int, [_vp, _dp, C.c_int, _dp, _dp, _dp]),
     "mc_fnonloc": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp]),
     "mc_fnonloc_dev": (C.c_int, [_vp, _vp, _dp, C.c_int, _dp]),
     "mc_timer_enable": (C.c_int, [_vp, C.c_int]),

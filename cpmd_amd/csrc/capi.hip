@@ -197,6 +197,40 @@ int mc_rnlsm1(mc_ctx* ctx, const double* c0, int ldc, int nstate, double* fnl) {
   MC_API_END
 }
 
+int mc_set_gk(mc_ctx* ctx, const double* gk) {
+  MC_API_BEGIN
+  check_ctx(ctx);
+  MC_CHECK(gk || ctx->ngw == 0, MC_ERR_ARG, "set_gk: null argument");
+  mc::set_gk(ctx, gk);
+  MC_API_END
+}
+
+int mc_rnlsm2_dev(mc_ctx* ctx, const void* c0_dev, int nstate, double* dfnl) {
+  MC_API_BEGIN
+  check_ctx(ctx);
+  MC_CHECK(c0_dev && nstate >= 0, MC_ERR_ARG, "rnlsm2: bad argument");
+  mc::rnlsm2_dev(ctx, (const double2*)c0_dev, nstate, dfnl);
+  MC_API_END
+}
+
+int mc_rnlsm2(mc_ctx* ctx, const double* c0, int ldc, int nstate, double* dfnl) {
+  MC_API_BEGIN
+  check_ctx(ctx);
+  MC_CHECK(c0 && dfnl && nstate >= 0, MC_ERR_ARG, "rnlsm2: bad argument");
+  HostWfn h;
+  double2* d = h.upload(ctx, c0, ldc, nstate);
+  mc::rnlsm2_dev(ctx, d, nstate, dfnl);
+  MC_API_END
+}
+
+int mc_rnlfor(mc_ctx* ctx, const double* f, int nstate, const double* fnl, const double* dfnl, double* fion) {
+  MC_API_BEGIN
+  check_ctx(ctx);
+  MC_CHECK(f && fion && nstate >= 0, MC_ERR_ARG, "rnlfor: bad argument");
+  mc::rnlfor_dev(ctx, f, nstate, fnl, dfnl, fion);
+  MC_API_END
+}
+
 int mc_fnonloc_dev(mc_ctx* ctx, void* c2_dev, const double* f, int nstate, const double* fnl) {
   MC_API_BEGIN
   check_ctx(ctx);

@@ -127,6 +127,7 @@ __global__ __launch_bounds__(256) void k_gemm_tn(const double* __restrict__ A, i
         if (acol < m && 2 * g < kend) {
           const double2 e = proj.eigr[(size_t)acol * proj.ldg + g];
           double tw = proj.twnl[g];
+          if (proj.gkk) tw *= proj.gkk[g];
           if (proj.geq0 && g == 0) tw *= 0.5;
           switch (proj.lmod4) {   // (-i)^l * tw * e
             case 0: re = tw * e.x; im = tw * e.y; break;

@@ -95,6 +95,7 @@ struct ProjA {
   int ldg;
   int lmod4;             // l mod 4 of (-i)^l
   int geq0;
+  const double* gkk = nullptr;   // (ldg) one Cartesian component of G (rnlsm2), or nullptr
 };
 
 // Epilogue of fnonloc (src/fnonloc_utils.mod.F90:272-300):
@@ -115,6 +116,10 @@ struct Projectors {
   DevBuf d_h;      // (nhxs, nhxs, nsp)
   DevBuf d_eigr;   // (ldg, nat) complex
   DevBuf d_fnl;    // (nat, nhxs, nstate)
+  DevBuf d_gk;     // (ldg, 3) G vectors in units of 2 pi / alat (rnlsm2)
+  bool gk_ready = false;
+  DevBuf d_dfnl;   // (nat, nhxs, 3, nstate)
+  int dfnl_nstate = -1;
 };
 
 }  // namespace mc
@@ -211,6 +216,10 @@ void set_projectors(mc_ctx* ctx, int nsp, const int* na, int nhxs, const int* ng
                     const double* hmat, const double* twnl, int ldtw);
 void set_eigr(mc_ctx* ctx, const double* eigr, int lde, int nat);
 void rnlsm1_dev(mc_ctx* ctx, const double2* c0, int n, double* fnl_host);
+void set_gk(mc_ctx* ctx, const double* gk);
+void rnlsm2_dev(mc_ctx* ctx, const double2* c0, int n, double* dfnl_host);
+void rnlfor_dev(mc_ctx* ctx, const double* f_host, int n, const double* fnl_host, const double* dfnl_host,
+                double* fion_host);
 void fnonloc_dev(mc_ctx* ctx, double2* c2, const double* f_host, int n, const double* fnl_host);
 void wfn_upload(mc_ctx* ctx, const double* host, int ldc, int nstate, double2* dev);
 void wfn_download(mc_ctx* ctx, const double2* dev, int nstate, double* host, int ldc);

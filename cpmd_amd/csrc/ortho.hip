@@ -207,6 +207,34 @@ static double max_abs_diff(mc_ctx* ctx, const double* a, const double* b, size_t
   return m;
 }
 
+// rnlfor (src/rnlfor_utils.mod.F90:201-280, Gamma): one thread per (k, isa);
+// fion(k, isa) = -sum_i 2 f_i sum_{iv,jv} h(iv,jv) dfnl(isa,iv,k,i) fnl(isa,jv,i),
+// states with |f_i| <= 1e-12 skipped.  out (3, nat).
+__global__ void k_rnlfor(const double* __restrict__ fnl, const double* __restrict__ dfnl, const double* __restrict__ f,
+                         int n, int nat, int nhxs, const int* __restrict__ sp_of, const int* __restrict__ ngh,
+                         const double* __restrict__ h, double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * nat) return;
+  const int k = e % 3, isa = e / 3;
+  const int is = sp_of[isa];
+  const int nh = ngh[is];
+  const double* hs = h + (size_t)nhxs * nhxs * is;
+  double acc = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double fi = f[i];
+    if (fabs(fi) <= 1e-12) continue;
+    double s = 0.0;
+    for (int iv = 0; iv < nh; ++iv) {
+      const double d = dfnl[isa + (size_t)nat * (iv + (size_t)nhxs * (k + 3 * (size_t)i))];
+      double t = 0.0;
+      for (int jv = 0; jv < nh; ++jv) t += hs[iv + nhxs * jv] * fnl[isa + (size_t)nat * (jv + (size_t)nhxs * i)];
+      s += d * t;
+    }
+    acc += 2.0 * fi * s;
+  }
+  out[k + 3 * (size_t)isa] = -acc;
+}
+
 // ---------------------------------------------------------------------------
 // drivers (device-resident wavefunctions)
 // ---------------------------------------------------------------------------
@@ -483,6 +511,100 @@ void rnlsm1_dev(mc_ctx* ctx, const double2* c0, int n, double* fnl_host) {
                           ctx->stream));
     MC_HIP(hipStreamSynchronize(ctx->stream));
   }
+}
+
+void set_gk(mc_ctx* ctx, const double* gk) {
+  auto& P = ctx->proj;
+  std::vector<double> t((size_t)ctx->ngw * 3);
+  for (int g = 0; g < ctx->ngw; ++g)
+    for (int k = 0; k < 3; ++k) t[g + (size_t)ctx->ngw * k] = gk[k + 3 * (size_t)g];
+  P.d_gk.ensure(sizeof(double) * (size_t)ctx->ldg * 3);
+  DevBuf stage;
+  stage.ensure(sizeof(double) * t.size());
+  MC_HIP(hipMemcpy(stage.p, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_permute_real_in, dim3((ctx->ldg + 255) / 256, 3), dim3(256), 0, ctx->stream,
+                     stage.as<double>(), ctx->ngw, P.d_gk.as<double>(), ctx->ldg, ctx->d_perm.as<int>(), ctx->ngw, 3);
+  MC_HIP(hipGetLastError());
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  P.gk_ready = true;
+}
+
+// dfnl(isa, iv, k, i) = 2 tpiba Re sum_G conj(gk_k twnl (-i)^(l+1) eigr) c (G=0
+// halved): the rnlsm1 contraction with the operand scaled by one G component.
+void rnlsm2_dev(mc_ctx* ctx, const double2* c0, int n, double* dfnl_host) {
+  auto& P = ctx->proj;
+  MC_CHECK(P.ready, MC_ERR_ARG, "rnlsm2: projectors not set (mc_set_projectors)");
+  MC_CHECK(P.eigr_nat == P.nat, MC_ERR_ARG, "rnlsm2: structure factors not set for all atoms (mc_set_eigr)");
+  MC_CHECK(P.gk_ready, MC_ERR_ARG, "rnlsm2: G vectors not set (mc_set_gk)");
+  const size_t nf = (size_t)P.nat * P.nhxs * 3 * std::max(n, 1);
+  P.d_dfnl.ensure(sizeof(double) * nf);
+  P.dfnl_nstate = n;
+  hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, ctx->stream, P.d_dfnl.as<double>(), nf);
+  MC_HIP(hipGetLastError());
+  const double tpiba = std::sqrt(ctx->tpiba2);
+  for (int k = 0; k < 3; ++k) {
+    int isa0 = 0;
+    for (int is = 0; is < P.nsp; ++is) {
+      for (int iv = 0; iv < P.ngh[is]; ++iv) {
+        if (P.na[is] == 0 || n == 0) continue;
+        ProjA pa;
+        pa.eigr = P.d_eigr.as<double2>() + (size_t)isa0 * ctx->ldg;
+        pa.twnl = P.d_twnl.as<double>() + (size_t)(is * P.nhxs + iv) * ctx->ldg;
+        pa.ldg = ctx->ldg;
+        pa.lmod4 = ((P.lang[iv + (size_t)P.nhxs * is] + 1) % 4 + 4) % 4;
+        pa.geq0 = ctx->geq0;
+        pa.gkk = P.d_gk.as<double>() + (size_t)k * ctx->ldg;
+        gemm_tn(ctx, nullptr, 0, (const double*)c0, wfn_ld(ctx), wfn_rows(ctx), P.na[is], n, 2.0 * tpiba, false,
+                false, P.d_dfnl.as<double>() + isa0 + (size_t)P.nat * (iv + (size_t)P.nhxs * k),
+                P.nat * P.nhxs * 3, &pa);
+      }
+      isa0 += P.na[is];
+    }
+  }
+  if (dfnl_host) {
+    MC_HIP(hipMemcpyAsync(dfnl_host, P.d_dfnl.p, sizeof(double) * (size_t)P.nat * P.nhxs * 3 * n,
+                          hipMemcpyDeviceToHost, ctx->stream));
+    MC_HIP(hipStreamSynchronize(ctx->stream));
+  }
+}
+
+void rnlfor_dev(mc_ctx* ctx, const double* f_host, int n, const double* fnl_host, const double* dfnl_host,
+                double* fion_host) {
+  auto& P = ctx->proj;
+  MC_CHECK(P.ready, MC_ERR_ARG, "rnlfor: projectors not set (mc_set_projectors)");
+  if (n <= 0 || P.nat == 0) return;
+  const size_t nf = (size_t)P.nat * P.nhxs * n;
+  if (fnl_host) {
+    P.d_fnl.ensure(sizeof(double) * nf);
+    MC_HIP(hipMemcpyAsync(P.d_fnl.p, fnl_host, sizeof(double) * nf, hipMemcpyHostToDevice, ctx->stream));
+    P.fnl_nstate = n;
+  }
+  if (dfnl_host) {
+    P.d_dfnl.ensure(sizeof(double) * nf * 3);
+    MC_HIP(hipMemcpyAsync(P.d_dfnl.p, dfnl_host, sizeof(double) * nf * 3, hipMemcpyHostToDevice, ctx->stream));
+    P.dfnl_nstate = n;
+  }
+  MC_CHECK(P.fnl_nstate == n, MC_ERR_ARG, "rnlfor: no fnl for this number of states (call rnlsm1 first)");
+  MC_CHECK(P.dfnl_nstate == n, MC_ERR_ARG, "rnlfor: no dfnl for this number of states (call rnlsm2 first)");
+  std::vector<int> sp_of(P.nat);
+  for (int is = 0, isa = 0; is < P.nsp; ++is)
+    for (int ia = 0; ia < P.na[is]; ++ia) sp_of[isa++] = is;
+  // small staging area: sp_of (nat ints), ngh (nsp ints), f (n), out (3 nat)
+  const size_t off_ngh = sizeof(int) * P.nat, off_f = (off_ngh + sizeof(int) * P.nsp + 15) / 16 * 16;
+  const size_t off_out = off_f + sizeof(double) * n, total = off_out + sizeof(double) * 3 * P.nat;
+  ctx->w_f.ensure(total);
+  char* base = (char*)ctx->w_f.p;
+  MC_HIP(hipMemcpyAsync(base, sp_of.data(), sizeof(int) * P.nat, hipMemcpyHostToDevice, ctx->stream));
+  MC_HIP(hipMemcpyAsync(base + off_ngh, P.ngh.data(), sizeof(int) * P.nsp, hipMemcpyHostToDevice, ctx->stream));
+  MC_HIP(hipMemcpyAsync(base + off_f, f_host, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_rnlfor, dim3((3 * P.nat + 127) / 128), dim3(128), 0, ctx->stream, P.d_fnl.as<double>(),
+                     P.d_dfnl.as<double>(), (const double*)(base + off_f), n, P.nat, P.nhxs, (const int*)base,
+                     (const int*)(base + off_ngh), P.d_h.as<double>(), (double*)(base + off_out));
+  MC_HIP(hipGetLastError());
+  std::vector<double> out(3 * (size_t)P.nat);
+  MC_HIP(hipMemcpyAsync(out.data(), base + off_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, ctx->stream));
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  for (size_t e = 0; e < out.size(); ++e) fion_host[e] += out[e];
 }
 
 void fnonloc_dev(mc_ctx* ctx, double2* c2, const double* f_host, int n, const double* fnl_host) {

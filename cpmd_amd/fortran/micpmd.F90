@@ -18,6 +18,9 @@
 !                  = rortog -> rortho, src/rortog_utils.mod.F90:44,315
 !   micpmd_rnlsm1  (c0,nstate,fnl)         = rnlsm1, src/rnlsm1_utils.mod.F90:42
 !   micpmd_fnonloc (c2,f,nstate,fnl)       = fnonloc, src/fnonloc_utils.mod.F90:55
+!   micpmd_set_gk  (gk)                    = the cppt::gk G vectors used by rnlsm2
+!   micpmd_rnlsm2  (c0,nstate,dfnl)        = rnlsm2, src/rnlsm2_utils.mod.F90:321
+!   micpmd_rnlfor  (fion,f,nstate,fnl,dfnl)= rnlfor, src/rnlfor_utils.mod.F90:41
 !   micpmd_lagrange(c0,c2,f,nstate)        = addup2 projection,
 !                                            src/forces_driver.mod.F90:344-370
 !
@@ -58,6 +61,7 @@ MODULE micpmd
   PUBLIC :: micpmd_vpsi, micpmd_rhoofr, micpmd_rgs, micpmd_lowdin, micpmd_ovlap
   PUBLIC :: micpmd_rotate, micpmd_rortog, micpmd_lagrange
   PUBLIC :: micpmd_set_projectors, micpmd_set_eigr, micpmd_rnlsm1, micpmd_fnonloc
+  PUBLIC :: micpmd_set_gk, micpmd_rnlsm2, micpmd_rnlfor
 
   INTERFACE
      FUNCTION mc_last_error() BIND(C, name='mc_last_error') RESULT(p)
@@ -171,6 +175,28 @@ MODULE micpmd
        REAL(c_double), INTENT(OUT) :: fnl(*)
        INTEGER(c_int) :: ierr
      END FUNCTION mc_rnlsm1
+     FUNCTION mc_set_gk(c, gk) BIND(C, name='mc_set_gk') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: c
+       REAL(c_double), INTENT(IN) :: gk(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_set_gk
+     FUNCTION mc_rnlsm2(c, c0, ldc, nstate, dfnl) BIND(C, name='mc_rnlsm2') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double_complex, c_double
+       TYPE(c_ptr), VALUE :: c
+       COMPLEX(c_double_complex), INTENT(IN) :: c0(*)
+       INTEGER(c_int), VALUE :: ldc, nstate
+       REAL(c_double), INTENT(OUT) :: dfnl(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_rnlsm2
+     FUNCTION mc_rnlfor(c, f, nstate, fnl, dfnl, fion) BIND(C, name='mc_rnlfor') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: c
+       REAL(c_double), INTENT(IN) :: f(*), fnl(*), dfnl(*)
+       INTEGER(c_int), VALUE :: nstate
+       REAL(c_double), INTENT(INOUT) :: fion(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_rnlfor
      FUNCTION mc_fnonloc(c, c2, ldc, f, nstate, fnl) BIND(C, name='mc_fnonloc') RESULT(ierr)
        IMPORT :: c_int, c_ptr, c_double_complex, c_double
        TYPE(c_ptr), VALUE :: c
@@ -358,5 +384,26 @@ CONTAINS
     REAL(real_8), INTENT(IN)       :: fnl(:,:,:)
     CALL check(mc_fnonloc(ctx, c2, INT(SIZE(c2, 1), c_int), f, INT(nstate, c_int), fnl), 'fnonloc')
   END SUBROUTINE micpmd_fnonloc
+
+  SUBROUTINE micpmd_set_gk(gk)
+    REAL(real_8), INTENT(IN) :: gk(:,:)
+    CALL check(mc_set_gk(ctx, gk), 'set_gk')
+  END SUBROUTINE micpmd_set_gk
+
+  SUBROUTINE micpmd_rnlsm2(c0, nstate, dfnl)
+    COMPLEX(real_8), INTENT(IN) :: c0(:,:)
+    INTEGER, INTENT(IN)         :: nstate
+    REAL(real_8), INTENT(OUT)   :: dfnl(:,:,:,:)
+    CALL check(mc_rnlsm2(ctx, c0, INT(SIZE(c0, 1), c_int), INT(nstate, c_int), dfnl), 'rnlsm2')
+  END SUBROUTINE micpmd_rnlsm2
+
+  ! fion(3, nat) in isa order (CPMD's fion(:, ia, is) flattened by species)
+  SUBROUTINE micpmd_rnlfor(fion, f, nstate, fnl, dfnl)
+    REAL(real_8), INTENT(INOUT) :: fion(:,:)
+    REAL(real_8), INTENT(IN)    :: f(:)
+    INTEGER, INTENT(IN)         :: nstate
+    REAL(real_8), INTENT(IN)    :: fnl(:,:,:), dfnl(:,:,:,:)
+    CALL check(mc_rnlfor(ctx, f, INT(nstate, c_int), fnl, dfnl, fion), 'rnlfor')
+  END SUBROUTINE micpmd_rnlfor
 
 END MODULE micpmd

@@ -292,13 +292,40 @@ class Context:
         check(lib().mc_rnlsm1_dev(self.h, c0.ptr, nstate, _p(fnl) if want_fnl else None))
         return fnl
 
+    def set_gk(self, gk: np.ndarray):
+        """gk (3, ngw) in units of 2 pi / alat, CPMD G order."""
+        self._gk = np.asfortranarray(gk, dtype=np.float64)
+        check(lib().mc_set_gk(self.h, _p(self._gk)))
+
+    def rnlsm2(self, c0: np.ndarray) -> np.ndarray:
+        c0 = np.asfortranarray(c0, dtype=np.complex128)
+        n = c0.shape[1]
+        dfnl = np.zeros((self.nat, self.nhxs, 3, n), order="F")
+        check(lib().mc_rnlsm2(self.h, _p(c0), c0.shape[0], n, _p(dfnl)))
+        return dfnl
+
+    def rnlsm2_dev(self, c0: DeviceArray, nstate: int, want_dfnl: bool = False):
+        dfnl = np.zeros((self.nat, self.nhxs, 3, nstate), order="F") if want_dfnl else None
+        check(lib().mc_rnlsm2_dev(self.h, c0.ptr, nstate, _p(dfnl) if want_dfnl else None))
+        return dfnl
+
+    def rnlfor(self, f: np.ndarray, fnl: np.ndarray | None = None, dfnl: np.ndarray | None = None,
+               fion: np.ndarray | None = None) -> np.ndarray:
+        """fion (3, nat) with the nonlocal forces subtracted (zeros if not given);
+        fnl / dfnl None: the device copies of the last rnlsm1 / rnlsm2."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        out = np.zeros((3, self.nat), order="F") if fion is None else np.array(fion, dtype=np.float64, order="F")
+        a = np.asfortranarray(fnl, dtype=np.float64) if fnl is not None else None
+        b = np.asfortranarray(dfnl, dtype=np.float64) if dfnl is not None else None
+        check(lib().mc_rnlfor(self.h, _p(f), len(f), _p(a) if a is not None else None,
+                              _p(b) if b is not None else None, _p(out)))
+        return out
+
     def fnonloc(self, c2: np.ndarray, f: np.ndarray, fnl: np.ndarray | None = None) -> np.ndarray:
         out = np.array(c2, dtype=np.complex128, order="F", copy=True)
         f = np.ascontiguousarray(f, dtype=np.float64)
-        fp = _p(np.asfortranarray(fnl, dtype=np.float64)) if fnl is not None else None
-        keep = fnl
-        check(lib().mc_fnonloc(self.h, _p(out), out.shape[0], _p(f), out.shape[1], fp))
-        del keep
+        a = np.asfortranarray(fnl, dtype=np.float64) if fnl is not None else None
+        check(lib().mc_fnonloc(self.h, _p(out), out.shape[0], _p(f), out.shape[1], _p(a) if a is not None else None))
         return out
 
     def fnonloc_dev(self, c2: DeviceArray, f: np.ndarray, nstate: int):

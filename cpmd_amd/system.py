@@ -331,6 +331,31 @@ def projector_set(sys: SyntheticSystem, miller: np.ndarray, hg: np.ndarray):
             np.full((1, 1, 1), sys.pp.h11), np.asfortranarray(twnl), eigr)
 
 
+def local_pp_set(sys: SyntheticSystem, hg: np.ndarray, raggio: float = 1.2):
+    """Local pseudopotential and Gaussian ion charge of the synthetic species on
+    the density G set: (zv, raggio, vps, rhops), vps / rhops (nsp=1, nhg).
+
+    vps = voo / Omega with CPMD's Goedecker local form factor
+    (src/formf_utils.mod.F90:131-187: the difference of the erf-screened
+    Coulomb tail and the Gaussian ion charge of radius raggio plus the C1, C2
+    terms), evaluated directly instead of through CPMD's radial spline;
+    rhops = -Zv exp(-G^2 raggio^2 / 4) / Omega (putps,
+    src/rinforce_utils.mod.F90:430-457).  raggio 1.2 is CPMD's default radius."""
+    pp = sys.pp
+    om = sys.cell.omega
+    gg = hg * sys.cell.tpiba2
+    rc2 = pp.rloc ** 2
+    r2 = raggio ** 2
+    rs1 = math.sqrt((2.0 * math.pi) ** 3) * pp.rloc ** 3 * np.exp(-0.5 * gg * rc2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        voo = np.where(gg < 1e-8, math.pi * pp.zion * (2.0 * rc2 - r2),
+                       -4.0 * math.pi * pp.zion / gg * (np.exp(-0.5 * gg * rc2) - np.exp(-0.25 * gg * r2)))
+    voo = voo + pp.c1 * rs1 + pp.c2 * (3.0 - rc2 * gg) * rs1
+    vps = (voo / om)[None, :]
+    rhops = (-pp.zion * np.exp(-0.25 * r2 * gg) / om)[None, :]
+    return np.array([pp.zion]), np.array([raggio]), np.asfortranarray(vps), np.asfortranarray(rhops)
+
+
 def real_space_potential(sys: SyntheticSystem, seed: int = 99, amplitude: float = 1.0) -> np.ndarray:
     """A smooth, seeded real potential V(r) on the padded mesh, layout
     v[ix + kr1*(iy + kr2*iz)] (CPMD's vpot(nnr1)).  Padding entries are zero."""

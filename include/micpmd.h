@@ -236,6 +236,28 @@ int mc_rnlsm1_dev(mc_ctx* ctx, const void* c0_dev, int nstate, double* fnl);
 int mc_fnonloc(mc_ctx* ctx, double* c2, int ldc, const double* f, int nstate, const double* fnl);
 int mc_fnonloc_dev(mc_ctx* ctx, void* c2_dev, const double* f, int nstate, const double* fnl);
 
+/* G vectors gk(3, ngw) in units of 2 pi / alat, CPMD G order (cppt::gk, the
+ * first ngw columns; set once per cell).  Needed by mc_rnlsm2. */
+int mc_set_gk(mc_ctx* ctx, const double* gk);
+
+/* rnlsm2: dfnl(isa, iv, k, i) = 2 tpiba Re sum_G conj(gk_k twnl (-i)^(l+1) eigr) c
+ * (G=0 halved), the derivative of fnl with respect to the position of atom
+ * isa; dfnl is (nat, nhxs, 3, nstate) float64 (sfac::dfnl with imagp = 1).
+ * The library keeps a device copy for mc_rnlfor.  dfnl may be NULL in
+ * mc_rnlsm2_dev.
+ * Replaces rnlsm2(c0,nstate,ikpt,ikind), src/rnlsm2_utils.mod.F90:321-482 (Gamma). */
+int mc_rnlsm2(mc_ctx* ctx, const double* c0, int ldc, int nstate, double* dfnl);
+int mc_rnlsm2_dev(mc_ctx* ctx, const void* c0_dev, int nstate, double* dfnl);
+
+/* rnlfor: fion(k, isa) -= sum_i 2 f_i sum_{iv,jv} h(iv,jv) dfnl(isa,iv,k,i)
+ * fnl(isa,jv,i) (states with |f_i| <= 1e-12 skipped), i.e. adds the nonlocal
+ * force -dE_NL/dR.  fion is (3, nat) float64 in isa order (CPMD's
+ * fion(3, nax, nsp) flattened by species); fnl / dfnl: host arrays or NULL for
+ * the device copies of the last mc_rnlsm1 / mc_rnlsm2.
+ * Replaces rnlfor(fion,f,wk,nstate,nkpoint), src/rnlfor_utils.mod.F90:41-360
+ * (Gamma, wk = 1). */
+int mc_rnlfor(mc_ctx* ctx, const double* f, int nstate, const double* fnl, const double* dfnl, double* fion);
+
 /* Optional per-kernel timing with HIP events on the context stream. */
 int mc_timer_enable(mc_ctx* ctx, int on);
 int mc_timer_reset(mc_ctx* ctx);

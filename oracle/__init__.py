@@ -123,6 +123,12 @@ def _setup_dense(L):
     L.or_rnlfor.argtypes = [_ci, _ip, _ci, _ip, _dp, _dp, _dp, _dp, _ci, _dp]
     L.or_enl.argtypes = [_ci, _ip, _ci, _ip, _dp, _dp, _dp, _ci]
     L.or_enl.restype = C.c_double
+    L.or_lda_pade.argtypes = [C.c_double, _dp, _dp]
+    L.or_xcener.argtypes = [_ip, _ip, _dp, _dp, _dp, _dp]
+    L.or_ewald.argtypes = [_ci, _ip, _dp, _dp, _dp, _dp, _ci, _dp, _dp, _dp]
+    L.or_eicalc.argtypes = [_ci, _ip, _ip, _ci, _ip, _dp, _dp, _dp, _dp, _dp]
+    L.or_vofrho.argtypes = [_ip, _ci, _ip, _dp, _dp, C.c_double, C.c_double, _ci, _ci, _ip, _dp, _dp, _dp, _dp,
+                            _dp, _dp, _dp, _dp]
 
 
 def _L():
@@ -270,3 +276,62 @@ def enl(P: Projectors, fnl, f):
     fnl = np.asfortranarray(fnl, dtype=np.float64)
     f = np.ascontiguousarray(f, dtype=np.float64)
     return _L().or_enl(P.nsp, _p(P.na, _ip), P.nhxs, _p(P.ngh, _ip), _p(P.hmat), _p(fnl), _p(f), len(f))
+
+
+# --------------------------------------------------------------------------
+# local potential / Hartree / LDA / Ewald
+# --------------------------------------------------------------------------
+VOFRHO_KEYS = ("ehep", "ehii", "ehee", "epseu", "vploc", "exc", "vxc")
+
+
+def lda_pade(rho):
+    """(eps_xc, V_xc) per point, Goedecker-Teter-Hutter Pade LDA."""
+    rho = np.atleast_1d(np.asarray(rho, dtype=np.float64))
+    e = np.empty_like(rho)
+    v = np.empty_like(rho)
+    L = _L()
+    for i, r in enumerate(rho):
+        a = C.c_double(); b = C.c_double()
+        L.or_lda_pade(float(r), C.byref(a), C.byref(b))
+        e[i] = a.value
+        v[i] = b.value
+    return e, v
+
+
+def ewald(na, zv, raggio, tau, ht, iesr=2, forces=True):
+    """(esr, eself, fion (3, nat)); tau (nat, 3) bohr, ht rows = lattice vectors."""
+    na = np.ascontiguousarray(na, dtype=np.int32)
+    zv = np.ascontiguousarray(zv, dtype=np.float64)
+    rg = np.ascontiguousarray(raggio, dtype=np.float64)
+    t = np.ascontiguousarray(np.asarray(tau, dtype=np.float64))          # (nat, 3) C = (3, nat) F
+    h = np.asfortranarray(np.asarray(ht, dtype=np.float64))
+    fion = np.zeros((3, int(na.sum())), order="F")
+    esr = C.c_double(); es = C.c_double()
+    _L().or_ewald(len(na), _p(na, _ip), _p(zv), _p(rg), _p(t), _p(h), int(iesr), C.byref(esr), C.byref(es),
+                  _p(fion) if forces else None)
+    return esr.value, es.value, fion
+
+
+def vofrho(L: Layout, inyh, hg, gk, na, vps, rhops, frac, rho, geq0=True, forces=True):
+    """vofrhoa + vofrhob (closed shell, LDA Pade, no core/gradient correction).
+
+    inyh (nhg, 3) centred 1-based indices, hg (nhg,), gk (nhg, 3) units 2 pi/alat,
+    vps / rhops (nsp, nhg), frac (nat, 3).  Returns (v(r), energies dict,
+    fion (3, nat) from potfor, rho(G))."""
+    inyh = np.ascontiguousarray(inyh, dtype=np.int32)
+    hg = np.ascontiguousarray(hg, dtype=np.float64)
+    gk = np.ascontiguousarray(gk, dtype=np.float64)
+    na = np.ascontiguousarray(na, dtype=np.int32)
+    vps = np.asfortranarray(vps, dtype=np.float64)
+    rhops = np.asfortranarray(rhops, dtype=np.float64)
+    frac = np.ascontiguousarray(frac, dtype=np.float64)
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    nhg = hg.shape[0]
+    v = np.zeros(L.nnr1)
+    out = np.zeros(7)
+    fion = np.zeros((3, int(na.sum())), order="F")
+    rhog = np.zeros(nhg, dtype=np.complex128)
+    _L().or_vofrho(_p(L.dims, _ip), nhg, _p(inyh, _ip), _p(hg), _p(gk), L.tpiba2, L.omega, int(geq0), len(na),
+                   _p(na, _ip), _p(vps), _p(rhops), _p(frac), _p(rho), _p(v), _p(out),
+                   _p(fion) if forces else None, _p(rhog))
+    return v, dict(zip(VOFRHO_KEYS, out)), fion, rhog

@@ -625,3 +625,264 @@ double or_enl(int nsp, const int* na, int nhxs, const int* ngh, const double* hm
   }
   return e;
 }
+
+/* ------------------------------------------------------------------------- */
+/* local potential, Hartree, LDA and Ewald terms (vofrhoa / vofrhob / rpiiint) */
+/* ------------------------------------------------------------------------- */
+
+/* e^{-i G.R} for centred index triple (1-based inyh) and fractional position */
+static cplx phase(const int* nr, const int* iny, const double* fr) {
+  double t = 0.0;
+  for (int k = 0; k < 3; ++k) t += (double)(iny[k] - (nr[k] / 2 + 1)) * fr[k];
+  return cexp(-2.0 * M_PI * I * t);
+}
+
+/* eicalc, src/eicalc_utils.mod.F90:33-173: eivps(G) = sum_isa vps(is,G) e^{-iG.R},
+ * eirop(G) = sum_isa rhops(is,G) e^{-iG.R}; vps, rhops (nsp, nhg), frac (3, nat) */
+void or_eicalc(int nhg, const int* nr, const int* inyh, int nsp, const int* na, const double* vps,
+               const double* rhops, const double* frac, double* eivpsd, double* eiropd) {
+  cplx* eivps = (cplx*)eivpsd;
+  cplx* eirop = (cplx*)eiropd;
+  for (int g = 0; g < nhg; ++g) {
+    cplx a = 0, b = 0;
+    int isa = 0;
+    for (int is = 0; is < nsp; ++is)
+      for (int ia = 0; ia < na[is]; ++ia, ++isa) {
+        cplx e = phase(nr, inyh + 3 * (size_t)g, frac + 3 * (size_t)isa);
+        a += vps[is + (size_t)nsp * g] * e;
+        b += rhops[is + (size_t)nsp * g] * e;
+      }
+    eivps[g] = a;
+    eirop[g] = b;
+  }
+}
+
+/* ppener, src/ppener_utils.mod.F90:23-108 (real parts of eh, ei, ee, eps);
+ * vtemp(G) = scg rho_tot + eivps (G != 0), scg(0) rho_tot (G = 0) */
+void or_ppener(int nhg, int geq0, const double* rhogd, const double* eivpsd, const double* eiropd, const double* scg,
+               double* eh, double* ei, double* ee, double* eps, double* vploc, double* vtempd) {
+  const cplx* rhog = (const cplx*)rhogd;
+  const cplx* eivps = (const cplx*)eivpsd;
+  const cplx* eirop = (const cplx*)eiropd;
+  cplx* vtemp = (cplx*)vtempd;
+  double sh = 0, si = 0, se = 0, sp = 0;
+  int g1 = 0;
+  *vploc = 0.0;
+  if (geq0) {
+    cplx vp = eivps[0], rp = eirop[0], rhet = rhog[0], rt = rhet + rp;
+    *vploc = creal(vp);
+    sp = creal(0.5 * vp * conj(rhet));
+    sh = 0.5 * scg[0] * creal(rt) * creal(rt);
+    si = creal(0.5 * scg[0] * rp * rp);
+    se = creal(0.5 * scg[0] * rhet * rhet);
+    vtemp[0] = scg[0] * rt;
+    g1 = 1;
+  }
+  for (int g = g1; g < nhg; ++g) {
+    cplx vp = eivps[g], rp = eirop[g], rhet = rhog[g], rt = rhet + rp;
+    cplx vcg = scg[g] * rt;
+    vtemp[g] = vcg + vp;
+    sh += creal(vcg * conj(rt));
+    si += scg[g] * creal(rp * conj(rp));
+    se += scg[g] * creal(rhet * conj(rhet));
+    sp += creal(conj(rhet) * vp);
+  }
+  *eh = sh;
+  *ei = si;
+  *ee = se;
+  *eps = sp;
+}
+
+/* potfor, src/potfor_utils.mod.F90:38-201: fion(k,isa) += 2 Omega tpiba
+ * Re(e^{-iG.R} (rhops scg conj(rho_tot) + vps conj(rho_e)) i gk_k), G != 0 */
+void or_potfor(int nhg, int geq0, double omega, double tpiba, const double* gk, const int* nr, const int* inyh,
+               int nsp, const int* na, const double* vps, const double* rhops, const double* scg, const double* frac,
+               const double* rhogd, const double* eiropd, double* fion) {
+  const cplx* rhog = (const cplx*)rhogd;
+  const cplx* eirop = (const cplx*)eiropd;
+  const double omtp = 2.0 * omega * tpiba;
+  int isa = 0;
+  for (int is = 0; is < nsp; ++is)
+    for (int ia = 0; ia < na[is]; ++ia, ++isa) {
+      double f[3] = {0, 0, 0};
+      for (int g = geq0 ? 1 : 0; g < nhg; ++g) {
+        cplx rhet = rhog[g], rt = rhet + eirop[g];
+        cplx t = phase(nr, inyh + 3 * (size_t)g, frac + 3 * (size_t)isa) *
+                 (rhops[is + (size_t)nsp * g] * scg[g] * conj(rt) + vps[is + (size_t)nsp * g] * conj(rhet));
+        for (int k = 0; k < 3; ++k) f[k] += creal(t * (I * gk[k + 3 * (size_t)g]));
+      }
+      for (int k = 0; k < 3; ++k) fion[k + 3 * (size_t)isa] += omtp * f[k];
+    }
+}
+
+/* LDA exchange-correlation per electron and potential, Goedecker-Teter-Hutter
+ * Pade form (CPMD's FUNCTIONAL LDA = mfxcc_is_pade, src/nfunc_utils.mod.F90:251-419;
+ * PRB 54, 1703 (1996)): eps(rs) = -(a0 + a1 rs + a2 rs^2 + a3 rs^3) /
+ * (b1 rs + b2 rs^2 + b3 rs^3 + b4 rs^4), V = eps - rs/3 d eps/d rs, rho floored at 1e-24 */
+void or_lda_pade(double rho, double* eps, double* vxc) {
+  static const double a0 = 0.4581652932831429, a1 = 2.217058676663745, a2 = 0.7405551735357053,
+                      a3 = 0.01968227878617998;
+  static const double b1 = 1.0, b2 = 4.504130959426697, b3 = 1.110667363742916, b4 = 0.02359291751427506;
+  const double r = rho > 1e-24 ? rho : 1e-24;
+  const double rs = cbrt(3.0 / (4.0 * M_PI * r));
+  const double num = a0 + rs * (a1 + rs * (a2 + rs * a3));
+  const double den = rs * (b1 + rs * (b2 + rs * (b3 + rs * b4)));
+  const double dnum = a1 + rs * (2.0 * a2 + rs * 3.0 * a3);
+  const double dden = b1 + rs * (2.0 * b2 + rs * (3.0 * b3 + rs * 4.0 * b4));
+  const double e = -num / den;
+  const double de = -(dnum * den - num * dden) / (den * den);
+  *eps = e;
+  *vxc = e - rs / 3.0 * de;
+}
+
+/* xcener over the valid mesh (nr1 x nr2 x nr3 of the kr1 x kr2 x kr3 array):
+ * v += V_xc(rho); sxc = sum eps rho_floored; vsum = sum V_xc max(rho, 0) */
+void or_xcener(const int* nr, const int* kr, const double* rho, double* v, double* sxc, double* vsum) {
+  double s = 0.0, w = 0.0;
+  for (int z = 0; z < nr[2]; ++z)
+    for (int y = 0; y < nr[1]; ++y)
+      for (int x = 0; x < nr[0]; ++x) {
+        const size_t e = x + (size_t)kr[0] * (y + (size_t)kr[1] * z);
+        const double r = rho[e] > 1e-24 ? rho[e] : 1e-24;
+        double ex, vx;
+        or_lda_pade(r, &ex, &vx);
+        v[e] += vx;
+        s += ex * r;
+        w += vx * (rho[e] > 0.0 ? rho[e] : 0.0);
+      }
+  *sxc = s;
+  *vsum = w;
+}
+
+/* Ewald real-space sum esr and self energy eself (rpiiint, src/rpiiint_utils.mod.F90:29-200;
+ * eself src/rinforce_utils.mod.F90:239-249); ht(3,3) column-major with ht(i,:) = a_i;
+ * fion (3, nat) accumulates; pairs further than 20 rckj skipped */
+void or_ewald(int nsp, const int* na, const double* zv, const double* raggio, const double* tau, const double* ht,
+              int iesr, double* esr, double* eself, double* fion) {
+  int nat = 0;
+  for (int is = 0; is < nsp; ++is) nat += na[is];
+  int* sp = malloc(sizeof(int) * (nat > 0 ? nat : 1));
+  for (int is = 0, isa = 0; is < nsp; ++is)
+    for (int ia = 0; ia < na[is]; ++ia) sp[isa++] = is;
+  /* inverse cell for minimum-image differences */
+  double a[3][3], inv[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) a[i][j] = ht[i + 3 * j];   /* a[i] = lattice vector i */
+  const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                     a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+  inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
+  inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+  inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
+  inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
+  inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
+  inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+  inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
+  inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+  inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+  double e = 0.0;
+  for (int l = 0; l < nat; ++l)
+    for (int m = l; m < nat; ++m) {
+      const int k = sp[l], j = sp[m];
+      const double zv2 = zv[k] * zv[j];
+      if (fabs(zv2) < 1e-10) continue;
+      const double rckj = sqrt(raggio[k] * raggio[k] + raggio[j] * raggio[j]);
+      double d[3] = {0, 0, 0};
+      const int tzero = (l == m);
+      if (!tzero) {
+        double dd[3], fr[3];
+        for (int q = 0; q < 3; ++q) dd[q] = tau[q + 3 * l] - tau[q + 3 * m];
+        for (int i = 0; i < 3; ++i) fr[i] = dd[0] * inv[0][i] + dd[1] * inv[1][i] + dd[2] * inv[2][i];
+        for (int i = 0; i < 3; ++i) fr[i] -= nearbyint(fr[i]);
+        for (int q = 0; q < 3; ++q) d[q] = fr[0] * a[0][q] + fr[1] * a[1][q] + fr[2] * a[2][q];
+      }
+      for (int ix = -iesr; ix <= iesr; ++ix)
+        for (int iy = -iesr; iy <= iesr; ++iy)
+          for (int iz = -iesr; iz <= iesr; ++iz) {
+            if (tzero && ix == 0 && iy == 0 && iz == 0) continue;
+            double r[3];
+            for (int q = 0; q < 3; ++q) r[q] = d[q] + ix * a[0][q] + iy * a[1][q] + iz * a[2][q];
+            const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            const double rl = sqrt(r2);
+            const double arg = rl / rckj;
+            if (arg > 20.0) continue;
+            const double w = tzero ? 0.5 : 1.0;
+            const double add = zv2 * erfc(arg) / rl;
+            e += add * w;
+            if (fion) {
+              const double pre = 2.0 * zv2 / sqrt(M_PI) * exp(-arg * arg) / rckj;
+              const double rep = w * (add + pre) / r2;
+              for (int q = 0; q < 3; ++q) {
+                fion[q + 3 * l] += rep * r[q];
+                fion[q + 3 * m] -= rep * r[q];
+              }
+            }
+          }
+    }
+  double s = 0.0;
+  for (int is = 0; is < nsp; ++is) s += na[is] * zv[is] * zv[is] / raggio[is];
+  *esr = e;
+  *eself = s / sqrt(2.0 * M_PI);
+  free(sp);
+}
+
+/* vofrho for a closed-shell Gamma density (vofrhoa + vofrhob, no core
+ * correction, no gradient correction), dims = (nr1,nr2,nr3,kr1,kr2,kr3):
+ *   rho (kr1,kr2,kr3) -> rho(G) (forward FFT, 1/N), eicalc, ppener, potfor (if
+ *   fion), v(r) = Re invfft(vtemp on +-G) + V_xc(rho).
+ * out[0..6] = ehep, ei*Omega, ee*Omega, epseu, vploc, exc, vxc (src/vofrhoa:96-122, vofrhob:459-461) */
+void or_vofrho(const int* dims, int nhg, const int* inyh, const double* hg, const double* gk, double tpiba2,
+               double omega, int geq0, int nsp, const int* na, const double* vps, const double* rhops,
+               const double* frac, const double* rho, double* v, double* out, double* fion, double* rhog_out) {
+  const int nr[3] = {dims[0], dims[1], dims[2]};
+  const int kr[3] = {dims[3], dims[4], dims[5]};
+  const size_t nnr = (size_t)kr[0] * kr[1] * kr[2];
+  const double nn = (double)nr[0] * nr[1] * nr[2];
+  cplx* w = calloc(nnr, sizeof(cplx));
+  for (size_t e = 0; e < nnr; ++e) w[e] = rho[e];
+  or_fft3d(nr[0], nr[1], nr[2], kr[0], kr[1], (double*)w, -1);
+  cplx* rhog = malloc(sizeof(cplx) * nhg);
+  cplx* eivps = malloc(sizeof(cplx) * nhg);
+  cplx* eirop = malloc(sizeof(cplx) * nhg);
+  cplx* vtemp = malloc(sizeof(cplx) * nhg);
+  double* scg = malloc(sizeof(double) * nhg);
+  for (int g = 0; g < nhg; ++g) {
+    int idx[3];
+    for (int k = 0; k < 3; ++k) {
+      int m = inyh[k + 3 * g] - (nr[k] / 2 + 1);
+      idx[k] = ((m % nr[k]) + nr[k]) % nr[k];
+    }
+    rhog[g] = w[idx[0] + (size_t)kr[0] * (idx[1] + (size_t)kr[1] * idx[2])] / nn;
+    scg[g] = (geq0 && g == 0) ? 0.0 : 4.0 * M_PI / (tpiba2 * hg[g]);
+  }
+  or_eicalc(nhg, nr, inyh, nsp, na, vps, rhops, frac, (double*)eivps, (double*)eirop);
+  double eh, ei, ee, eps, vploc;
+  or_ppener(nhg, geq0, (double*)rhog, (double*)eivps, (double*)eirop, scg, &eh, &ei, &ee, &eps, &vploc,
+            (double*)vtemp);
+  if (fion)
+    or_potfor(nhg, geq0, omega, sqrt(tpiba2), gk, nr, inyh, nsp, na, vps, rhops, scg, frac, (double*)rhog,
+              (double*)eirop, fion);
+  memset(w, 0, sizeof(cplx) * nnr);
+  for (int g = 0; g < nhg; ++g) {
+    int ip[3], im[3];
+    for (int k = 0; k < 3; ++k) {
+      int m = inyh[k + 3 * g] - (nr[k] / 2 + 1);
+      ip[k] = ((m % nr[k]) + nr[k]) % nr[k];
+      im[k] = ((-m % nr[k]) + nr[k]) % nr[k];
+    }
+    w[im[0] + (size_t)kr[0] * (im[1] + (size_t)kr[1] * im[2])] = conj(vtemp[g]);
+    w[ip[0] + (size_t)kr[0] * (ip[1] + (size_t)kr[1] * ip[2])] = vtemp[g];
+  }
+  or_fft3d(nr[0], nr[1], nr[2], kr[0], kr[1], (double*)w, +1);
+  for (size_t e = 0; e < nnr; ++e) v[e] = creal(w[e]);
+  double sxc, vsum;
+  or_xcener(nr, kr, rho, v, &sxc, &vsum);
+  out[0] = eh * omega;
+  out[1] = ei * omega;
+  out[2] = ee * omega;
+  out[3] = 2.0 * eps * omega;
+  out[4] = vploc;
+  out[5] = sxc * omega / nn;
+  out[6] = vsum * omega / nn;
+  if (rhog_out) memcpy(rhog_out, rhog, sizeof(cplx) * nhg);
+  free(w); free(rhog); free(eivps); free(eirop); free(vtemp); free(scg);
+}

@@ -181,3 +181,37 @@ def test_device_resident_ortho_cfg4(gpu):
     ref = a @ z @ np.diag(w ** -0.5) @ z.T
     sub = ctx.download(cp, n)[:2000]
     assert np.abs(sub - c[:2000] @ ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["cfg1", "two_species"])
+def test_rnlsm2_rnlfor(gpu, kind):
+    """dfnl and the nonlocal forces against the oracle (itself pinned by finite
+    differences of fnl and E_NL), host and device-resident entry points."""
+    sysm, c = config_system("cfg1")
+    ngw = sysm.gset.ngw
+    if kind == "cfg1":
+        na, nhxs, ngh, lang, hmat, twnl, eigr = S.projector_set(sysm, sysm.gset.miller[:ngw], sysm.gset.hg[:ngw])
+        P = dict(na=na, nhxs=nhxs, ngh=ngh, lang=lang, hmat=hmat, twnl=twnl, eigr=eigr)
+    else:
+        P = _two_species_projectors(sysm, ngw)
+    gk = (sysm.gset.miller[:ngw] @ sysm.cell.bvec).T
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    ctx.set_projectors(P["na"], P["nhxs"], P["ngh"], P["lang"], P["hmat"], P["twnl"])
+    ctx.set_eigr(P["eigr"])
+    ctx.set_gk(gk)
+    OP = oracle.Projectors(**P)
+    dfnl = ctx.rnlsm2(c)
+    odfnl = oracle.rnlsm2(OP, c, gk, sysm.cell.tpiba)
+    assert rel(dfnl, odfnl) < 1e-12
+    f = np.full(c.shape[1], 2.0)
+    f[2] = 0.0
+    ofnl = oracle.rnlsm1(OP, c)
+    ofion = oracle.rnlfor(OP, ofnl, odfnl, f)
+    assert rel(ctx.rnlfor(f, ofnl, odfnl), ofion) < 1e-12          # host fnl / dfnl
+    d = ctx.upload(c)
+    n = c.shape[1]
+    ctx.rnlsm1_dev(d, n)
+    ctx.rnlsm2_dev(d, n)
+    base = np.ones((3, OP.nat))
+    got = ctx.rnlfor(f, fion=base)                                   # device copies, accumulates
+    assert rel(got - base, ofion) < 1e-12

@@ -1,0 +1,190 @@
+"""Pins of the oracle's local-potential / Hartree / LDA / Ewald terms (the
+vofrhoa + vofrhob + rpiiint path) by the mathematics, not by restating the
+oracle's loops:
+
+* the Pade LDA reduces to Slater exchange for rs -> 0, its potential is
+  d(rho eps)/d rho, and it agrees with the Perdew-Zunger (1981) fit of the same
+  Ceperley-Alder data (textbook formula written out here);
+* Ewald: the ion-ion energy of a simple cubic lattice of point charges in a
+  neutralising background is the Madelung value -1.4186487397 Z^2/a, and for a
+  random cell the Ewald energy does not depend on the Gaussian radius raggio;
+* the total electrostatic + local energy of a neutral system does not depend on
+  raggio (the G = 0 terms of vps and of the Gaussian ion charge cancel);
+* v(r) from vofrho is the functional derivative of E_H + E_loc + E_xc
+  (central differences along zero-mean density changes);
+* potfor + the Ewald forces are -dE/dR of E_H + E_loc + E_esr - E_self at
+  fixed electron density (central differences);
+* the ion-ion part of E_H does not see the electrons (ei = ehep at rho_e = 0).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from cpmd_amd import system as S
+
+
+# --------------------------------------------------------------------------- LDA
+def _pz_lda(rs):
+    """Slater exchange + Perdew-Zunger correlation, PRB 23, 5048 (1981), unpolarised."""
+    ex = -0.75 * (9.0 / (4.0 * math.pi ** 2)) ** (1.0 / 3.0) / rs
+    if rs < 1.0:
+        ec = 0.0311 * math.log(rs) - 0.048 + 0.0020 * rs * math.log(rs) - 0.0116 * rs
+    else:
+        ec = -0.1423 / (1.0 + 1.0529 * math.sqrt(rs) + 0.3334 * rs)
+    return ex + ec
+
+
+def _rho_of_rs(rs):
+    return 3.0 / (4.0 * math.pi * rs ** 3)
+
+
+def test_lda_pade_limits_and_potential():
+    # rs -> 0: pure Slater exchange, eps rs -> -(3/4)(9/(4 pi^2))^(1/3)
+    rs = 1e-7
+    e, _ = oracle.lda_pade(_rho_of_rs(rs))
+    assert e[0] * rs == pytest.approx(-0.75 * (9.0 / (4.0 * math.pi ** 2)) ** (1.0 / 3.0), rel=1e-5)
+    # V = d(rho eps)/d rho
+    for rs in (0.3, 1.0, 2.5, 7.0):
+        r = _rho_of_rs(rs)
+        h = 1e-6 * r
+        ep, _ = oracle.lda_pade(r + h)
+        em, _ = oracle.lda_pade(r - h)
+        _, v = oracle.lda_pade(r)
+        assert v[0] == pytest.approx(((r + h) * ep[0] - (r - h) * em[0]) / (2 * h), rel=1e-7)
+    # same Ceperley-Alder data as Perdew-Zunger: agreement within the fits' accuracy
+    for rs in (0.5, 1.0, 2.0, 4.0, 10.0):
+        e, _ = oracle.lda_pade(_rho_of_rs(rs))
+        assert abs(e[0] - _pz_lda(rs)) < 1e-3
+
+
+# --------------------------------------------------------------------------- Ewald
+def _ion_energy_g(zv, raggio, tau, a, nmax=12):
+    """ei * Omega of ppener for rho_e = 0 (Gaussian ions, G != 0), cubic cell a."""
+    om = a ** 3
+    frac = tau / a
+    e = 0.0
+    rng = range(-nmax, nmax + 1)
+    for i in rng:
+        for j in rng:
+            for k in rng:
+                if (i, j, k) <= (0, 0, 0):
+                    continue                                  # half sphere, G != 0
+                m = np.array([i, j, k], dtype=float)
+                g2 = (2 * math.pi / a) ** 2 * (m @ m)
+                rho = -zv * math.exp(-0.25 * raggio ** 2 * g2) / om * np.sum(np.exp(-2j * math.pi * (frac @ m)))
+                e += 4 * math.pi / g2 * abs(rho) ** 2
+    return e * om
+
+
+def test_ewald_madelung_simple_cubic():
+    a, z = 7.0, 3.0
+    for raggio in (1.0, 1.4):
+        esr, eself, _ = oracle.ewald([1], [z], [raggio], np.zeros((1, 3)), np.eye(3) * a, iesr=3)
+        ei = _ion_energy_g(z, raggio, np.zeros((1, 3)), a)
+        background = -math.pi * z * z * raggio ** 2 / a ** 3   # G = 0 of the neutralising background
+        assert ei + esr - eself + background == pytest.approx(-1.4186487397 * z * z / a, rel=1e-9)
+
+
+def test_ewald_split_independent_and_forces():
+    rng = np.random.default_rng(3)
+    a = 8.0
+    tau = rng.uniform(0, a, size=(4, 3))
+    zv = 2.0
+    q = 4 * zv
+    tot = []
+    for raggio in (0.9, 1.3):
+        esr, eself, _ = oracle.ewald([4], [zv], [raggio], tau, np.eye(3) * a, iesr=3)
+        ei = _ion_energy_g(zv, raggio, tau, a, nmax=10)
+        tot.append(ei + esr - eself - math.pi * q * q * raggio ** 2 / a ** 3)
+    assert tot[0] == pytest.approx(tot[1], rel=1e-9)
+    # esr forces are -d esr / d tau
+    esr, _, f = oracle.ewald([4], [zv], [1.1], tau, np.eye(3) * a, iesr=2)
+    h = 1e-5
+    for ia in range(4):
+        for k in range(3):
+            tp, tm = tau.copy(), tau.copy()
+            tp[ia, k] += h
+            tm[ia, k] -= h
+            ep = oracle.ewald([4], [zv], [1.1], tp, np.eye(3) * a, iesr=2, forces=False)[0]
+            em = oracle.ewald([4], [zv], [1.1], tm, np.eye(3) * a, iesr=2, forces=False)[0]
+            assert f[k, ia] == pytest.approx(-(ep - em) / (2 * h), rel=1e-6, abs=1e-9)
+
+
+# --------------------------------------------------------------------------- vofrho
+def _small_case(raggio=1.2, seed=5):
+    """a = 9 bohr, 3 atoms (Zv = 4), 25 Ry: mesh 36^3, density sphere on the full mesh."""
+    sysm = S.make_system(9.0, 3, 25.0, nstate=6, seed=seed, with_density=True)
+    gs = sysm.gset
+    L = oracle.Layout(sysm.cell, gs)
+    c = S.initial_coefficients(sysm, seed=seed)
+    rho, _, _, _ = oracle.rhoofr(L, c, sysm.f)
+    zv, rg, vps, rhops = S.local_pp_set(sysm, gs.hg, raggio=raggio)
+    nh = np.array([n // 2 + 1 for n in sysm.cell.nr])
+    inyh = gs.miller + nh
+    gk = gs.miller @ sysm.cell.bvec
+    frac = sysm.tau @ np.linalg.inv(sysm.cell.avec)
+    return sysm, L, rho, inyh, gk, zv, rg, vps, rhops, frac
+
+
+def _e_hloc_xc(en):
+    return en["ehep"] + en["epseu"] + en["exc"]
+
+
+def test_vofrho_potential_is_functional_derivative():
+    sysm, L, rho, inyh, gk, zv, rg, vps, rhops, frac = _small_case()
+    gs = sysm.gset
+    v, en, _, _ = oracle.vofrho(L, inyh, gs.hg, gk, [3], vps, rhops, frac, rho, forces=False)
+    nr, kr = sysm.cell.nr, sysm.cell.kr
+    rng = np.random.default_rng(1)
+    d = np.zeros((kr[2], kr[1], kr[0]))
+    # a smooth, zero-mean change (low Fourier components only)
+    x = np.arange(nr[0]) / nr[0]
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    dd = np.cos(2 * math.pi * (X + 2 * Y)) + 0.5 * np.sin(2 * math.pi * (3 * Z - Y))
+    d[:nr[2], :nr[1], :nr[0]] = dd.transpose(2, 1, 0) * 1e-3
+    d = d.ravel()
+    h = 1e-3
+    ep = _e_hloc_xc(oracle.vofrho(L, inyh, gs.hg, gk, [3], vps, rhops, frac, rho + h * d, forces=False)[1])
+    em = _e_hloc_xc(oracle.vofrho(L, inyh, gs.hg, gk, [3], vps, rhops, frac, rho - h * d, forces=False)[1])
+    nn = nr[0] * nr[1] * nr[2]
+    assert (ep - em) / (2 * h) == pytest.approx(sysm.cell.omega / nn * np.dot(v, d), rel=1e-7)
+    # charge and energies are sane: Nel = Omega rho(G=0), E_xc < 0
+    assert en["exc"] < 0
+
+
+def test_vofrho_forces_and_raggio():
+    sysm, L, rho, inyh, gk, zv, rg, vps, rhops, frac = _small_case()
+    gs = sysm.gset
+    a = sysm.cell.alat
+    tau = sysm.tau
+
+    def energy(t, raggio=1.2, forces=False):
+        fr = t @ np.linalg.inv(sysm.cell.avec)
+        zv_, rg_, vps_, rhops_ = S.local_pp_set(sysm, gs.hg, raggio=raggio)
+        _, en, fion, _ = oracle.vofrho(L, inyh, gs.hg, gk, [3], vps_, rhops_, fr, rho, forces=forces)
+        esr, eself, fe = oracle.ewald([3], zv_, rg_, t, sysm.cell.avec, iesr=3)
+        return en["ehep"] + en["epseu"] + esr - eself, fion + fe
+
+    e0, f = energy(tau, forces=True)
+    h = 1e-5
+    for ia in range(3):
+        for k in range(3):
+            tp, tm = tau.copy(), tau.copy()
+            tp[ia, k] += h
+            tm[ia, k] -= h
+            fd = -(energy(tp)[0] - energy(tm)[0]) / (2 * h)
+            assert f[k, ia] == pytest.approx(fd, rel=1e-6, abs=1e-8)
+    # neutral system: the electrostatic + local energy does not depend on raggio
+    nel = sysm.cell.omega * oracle.vofrho(L, inyh, gs.hg, gk, [3], vps, rhops, frac, rho, forces=False)[3][0].real
+    assert nel == pytest.approx(12.0, rel=1e-10)
+    assert energy(tau, raggio=1.0)[0] == pytest.approx(e0, abs=1e-8)
+
+
+def test_ppener_ion_part_without_electrons():
+    sysm, L, rho, inyh, gk, zv, rg, vps, rhops, frac = _small_case()
+    gs = sysm.gset
+    _, en0, _, _ = oracle.vofrho(L, inyh, gs.hg, gk, [3], vps, rhops, frac, np.zeros_like(rho), forces=False)
+    assert en0["ehep"] == pytest.approx(en0["ehii"], rel=1e-13)
+    assert en0["ehee"] == 0.0 and en0["epseu"] == 0.0
