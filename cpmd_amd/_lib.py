@@ -18,7 +18,7 @@ MC_OK = 0
 ERRORS = {1: "MC_ERR_ARG", 2: "MC_ERR_HIP", 3: "MC_ERR_NOTPOSDEF", 4: "MC_ERR_NOCONV",
           5: "MC_ERR_UNSUPPORTED", 6: "MC_ERR_INTERNAL"}
 
-KID = {"zinv": 0, "yinv": 1, "xrho": 2, "xvpsi": 3, "yfwd": 4, "zfwd": 5, "misc": 6, "gemm": 7}
+KID = {"zinv": 0, "yinv": 1, "xrho": 2, "xvpsi": 3, "yfwd": 4, "zfwd": 5, "misc": 6, "gemm": 7, "vofrho": 8}
 
 
 class McError(RuntimeError):
@@ -81,6 +81,12 @@ SIGNATURES = {
     "mc_rnlsm2": (C.c_int, [_vp, _dp, C.c_int, C.c_int, _dp]),
     "mc_rnlsm2_dev": (C.c_int, [_vp, _vp, C.c_int, _dp]),
     "mc_rnlfor": (C.c_int, [_vp, _dp, C.c_int, _dp, _dp, _dp]),
+    "mc_set_density_gset": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp]),
+    "mc_set_local_pp": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _dp, _dp]),
+    "mc_set_ions": (C.c_int, [_vp, _dp, _dp]),
+    "mc_vofrho": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "mc_vofrho_dev": (C.c_int, [_vp, _vp, _vp, _dp, _dp]),
+    "mc_ewald": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
     "mc_fnonloc": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp]),
     "mc_fnonloc_dev": (C.c_int, [_vp, _vp, _dp, C.c_int, _dp]),
     "mc_timer_enable": (C.c_int, [_vp, C.c_int]),

@@ -231,6 +231,62 @@ int mc_rnlfor(mc_ctx* ctx, const double* f, int nstate, const double* fnl, const
   MC_API_END
 }
 
+int mc_set_density_gset(mc_ctx* ctx, int nhg, const int* inyh, const double* hg, const double* gk) {
+  MC_API_BEGIN
+  check_ctx(ctx);
+  MC_CHECK(inyh && hg && gk, MC_ERR_ARG, "set_density_gset: null argument");
+  mc::set_density_gset(ctx, nhg, inyh, hg, gk);
+  MC_API_END
+}
+
+int mc_set_local_pp(mc_ctx* ctx, int nsp, const int* na, const double* zv, const double* raggio, const double* vps,
+                    const double* rhops) {
+  MC_API_BEGIN
+  check_ctx(ctx);
+  MC_CHECK(na && zv && raggio && vps && rhops, MC_ERR_ARG, "set_local_pp: null argument");
+  mc::set_local_pp(ctx, nsp, na, zv, raggio, vps, rhops);
+  MC_API_END
+}
+
+int mc_set_ions(mc_ctx* ctx, const double* tau, const double* ht) {
+  MC_API_BEGIN
+  check_ctx(ctx);
+  MC_CHECK(ht && (tau || ctx->dens.nat == 0), MC_ERR_ARG, "set_ions: null argument");
+  mc::set_ions(ctx, tau, ht);
+  MC_API_END
+}
+
+int mc_vofrho_dev(mc_ctx* ctx, const void* rho_dev, void* v_dev, double* energies, double* fion) {
+  MC_API_BEGIN
+  check_ctx(ctx);
+  MC_CHECK(rho_dev && v_dev && rho_dev != v_dev, MC_ERR_ARG, "vofrho: bad rho / v");
+  mc::vofrho_dev(ctx, (const double*)rho_dev, (double*)v_dev, energies, fion);
+  MC_API_END
+}
+
+int mc_vofrho(mc_ctx* ctx, const double* rho, double* v, double* energies, double* fion) {
+  MC_API_BEGIN
+  check_ctx(ctx);
+  MC_CHECK(rho && v, MC_ERR_ARG, "vofrho: null argument");
+  const size_t nnr = (size_t)ctx->kr1 * ctx->kr2 * ctx->kr3;
+  mc::DevBuf r, o;
+  r.ensure(sizeof(double) * nnr);
+  o.ensure(sizeof(double) * nnr);
+  MC_HIP(hipMemcpyAsync(r.p, rho, sizeof(double) * nnr, hipMemcpyHostToDevice, ctx->stream));
+  mc::vofrho_dev(ctx, r.as<double>(), o.as<double>(), energies, fion);
+  MC_HIP(hipMemcpyAsync(v, o.p, sizeof(double) * nnr, hipMemcpyDeviceToHost, ctx->stream));
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  MC_API_END
+}
+
+int mc_ewald(mc_ctx* ctx, int iesr, double* esr, double* eself, double* fion) {
+  MC_API_BEGIN
+  check_ctx(ctx);
+  MC_CHECK(esr && eself, MC_ERR_ARG, "ewald: null argument");
+  mc::ewald(ctx, iesr, esr, eself, fion);
+  MC_API_END
+}
+
 int mc_fnonloc_dev(mc_ctx* ctx, void* c2_dev, const double* f, int nstate, const double* fnl) {
   MC_API_BEGIN
   check_ctx(ctx);

@@ -122,6 +122,19 @@ struct Projectors {
   int dfnl_nstate = -1;
 };
 
+// Density G set, local pseudopotential and ions of vofrho / ewald (vofrho.hip).
+struct DensitySet {
+  bool ready = false, pp_ready = false, ions_ready = false;
+  int nhg = 0, nsp = 0, nat = 0;
+  std::vector<int> na;
+  std::vector<double> zv, raggio, tau;
+  double a[9] = {}, inv[9] = {};   // lattice vectors a[3i+q]; fractional f_i = sum_q d_q inv[3q+i]
+  DevBuf d_gm, d_gidx, d_scg, d_gk;
+  DevBuf d_sp_start, d_sp_of, d_zv, d_raggio, d_vps, d_rhops;
+  DevBuf d_tau, d_frac, d_ph1, d_ph2, d_ph3;
+  DevBuf w_W, w_g, w_part;
+};
+
 }  // namespace mc
 
 // The context: static layout of the plane-wave set on one GPU plus workspaces.
@@ -174,6 +187,7 @@ struct mc_ctx {
 
   bool fft_fp32 = false;   // rhoofr/vpsi transforms and workspaces in FP32
   bool gemm_x3 = false;    // tall wavefunction contractions as split-bf16 MFMA products
+  mc::DensitySet dens;     // vofrho / ewald data
 
   int batch_pairs_vpsi = 8;
   int batch_pairs_rho = 16;
@@ -188,6 +202,15 @@ void timer_end(mc_ctx* ctx, int kid);
 void timer_flush(mc_ctx* ctx);
 
 __global__ void k_zero(double* __restrict__ x, size_t n);
+__global__ void k_fill(double* __restrict__ x, size_t n, double v);
+
+// vofrho / ewald (vofrho.hip)
+void set_density_gset(mc_ctx* ctx, int nhg, const int* inyh, const double* hg, const double* gk);
+void set_local_pp(mc_ctx* ctx, int nsp, const int* na, const double* zv, const double* raggio, const double* vps,
+                  const double* rhops);
+void set_ions(mc_ctx* ctx, const double* tau, const double* ht);
+void vofrho_dev(mc_ctx* ctx, const double* rho, double* v, double* energies, double* fion_host);
+void ewald(mc_ctx* ctx, int iesr, double* esr, double* eself, double* fion_host);
 
 // contractions (gemm.hip)
 void gemm_tn(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, int K, int m, int n, double alpha,

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "fft_device.h"
+#include "fft_launch.h"
 #include "mc_internal.h"
 
 namespace mc {
@@ -45,26 +46,6 @@ struct WParams {
 };
 
 constexpr int kMaxElem = 16;  // real-space elements per thread kept in registers (x pass)
-
-__device__ __forceinline__ double warp_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
-template <typename T>
-__device__ __forceinline__ void load_tw(v2<T>* tw, const double2* src, int n) {
-  for (int t = threadIdx.x; t < n; t += kThreads) {
-    const double2 w = src[t];
-    tw[t] = cmk<v2<T>>((T)w.x, (T)w.y);
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ v2<T>* lds_base() {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  return reinterpret_cast<v2<T>*>(smem_raw);
-}
 
 // ---------------------------------------------------------------------------
 // zinv: Gamma pack of a state pair (c1 + i c2 at +G, conj(c1) + i conj(c2) at
@@ -427,19 +408,6 @@ static WParams make_params(mc_ctx* ctx) {
   return w;
 }
 
-static constexpr int kLdsBudget = 65536;
-
-// lines per workgroup for a transform of length n: engine limits, LDS (lines + table) and 16
-template <typename T>
-static int lines_for(const FftPlan& pl) {
-  const int a = fast_a(pl.n);
-  int l = a > 0 ? kThreads / a : max_lines(pl);
-  l = std::min(l, kLdsBudget / (int)(sizeof(v2<T>) * pl.n) - 1);
-  l = std::min(l, 16);
-  MC_CHECK(l >= 1, MC_ERR_UNSUPPORTED, "transform length too large for one workgroup");
-  return l;
-}
-
 struct Shapes {
   int lo;   // owner sticks per zinv/zfwd workgroup
   int tz;   // z planes per y-pass workgroup
@@ -470,27 +438,6 @@ static void upload_f(mc_ctx* ctx, const double* f_host, int nstate) {
   if (nstate)
     MC_HIP(hipMemcpyAsync(ctx->w_f.p, f_host, sizeof(double) * nstate, hipMemcpyHostToDevice, ctx->stream));
 }
-
-// calls f(std::integral_constant<int, N>) with N = n if n has a compile-time
-// engine, N = 0 otherwise
-template <class F>
-static void with_length(int n, F&& f) {
-  switch (n) {
-#define MC_CASE(x) \
-  case x: f(std::integral_constant<int, x>()); return;
-    MC_FAST_SIZES(MC_CASE)
-#undef MC_CASE
-    default: f(std::integral_constant<int, 0>()); return;
-  }
-}
-
-#define LAUNCH(kid, kern, grid, block, shm, ...)                                  \
-  do {                                                                          \
-    timer_begin(ctx, kid);                                                      \
-    hipLaunchKernelGGL(kern, grid, block, shm, ctx->stream, __VA_ARGS__);       \
-    MC_HIP(hipGetLastError());                                                  \
-    timer_end(ctx, kid);                                                        \
-  } while (0)
 
 template <typename T>
 static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, int nstate, double* rho,

@@ -21,6 +21,10 @@
 !   micpmd_set_gk  (gk)                    = the cppt::gk G vectors used by rnlsm2
 !   micpmd_rnlsm2  (c0,nstate,dfnl)        = rnlsm2, src/rnlsm2_utils.mod.F90:321
 !   micpmd_rnlfor  (fion,f,nstate,fnl,dfnl)= rnlfor, src/rnlfor_utils.mod.F90:41
+!   micpmd_set_density(nhg,inyh,hg,gk), micpmd_set_local_pp(na,zv,raggio,vps,rhops),
+!   micpmd_set_ions(tau,ht)                = the cppt / ions / metr_com state of vofrho
+!   micpmd_vofrho  (rhoe,v,ener,fion)      = vofrhoa + vofrhob, src/vofrhoa_utils.mod.F90:41
+!   micpmd_ewald   (iesr,esr,eself,fion)   = rpiiint, src/rpiiint_utils.mod.F90:29
 !   micpmd_lagrange(c0,c2,f,nstate)        = addup2 projection,
 !                                            src/forces_driver.mod.F90:344-370
 !
@@ -62,6 +66,7 @@ MODULE micpmd
   PUBLIC :: micpmd_rotate, micpmd_rortog, micpmd_lagrange
   PUBLIC :: micpmd_set_projectors, micpmd_set_eigr, micpmd_rnlsm1, micpmd_fnonloc
   PUBLIC :: micpmd_set_gk, micpmd_rnlsm2, micpmd_rnlfor
+  PUBLIC :: micpmd_set_density, micpmd_set_local_pp, micpmd_set_ions, micpmd_vofrho, micpmd_ewald
 
   INTERFACE
      FUNCTION mc_last_error() BIND(C, name='mc_last_error') RESULT(p)
@@ -197,6 +202,44 @@ MODULE micpmd
        REAL(c_double), INTENT(INOUT) :: fion(*)
        INTEGER(c_int) :: ierr
      END FUNCTION mc_rnlfor
+     FUNCTION mc_set_density_gset(c, nhg, inyh, hg, gk) BIND(C, name='mc_set_density_gset') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: c
+       INTEGER(c_int), VALUE :: nhg
+       INTEGER(c_int), INTENT(IN) :: inyh(*)
+       REAL(c_double), INTENT(IN) :: hg(*), gk(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_set_density_gset
+     FUNCTION mc_set_local_pp(c, nsp, na, zv, raggio, vps, rhops) BIND(C, name='mc_set_local_pp') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: c
+       INTEGER(c_int), VALUE :: nsp
+       INTEGER(c_int), INTENT(IN) :: na(*)
+       REAL(c_double), INTENT(IN) :: zv(*), raggio(*), vps(*), rhops(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_set_local_pp
+     FUNCTION mc_set_ions(c, tau, ht) BIND(C, name='mc_set_ions') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: c
+       REAL(c_double), INTENT(IN) :: tau(*), ht(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_set_ions
+     FUNCTION mc_vofrho(c, rho, v, ener, fion) BIND(C, name='mc_vofrho') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: c
+       REAL(c_double), INTENT(IN) :: rho(*)
+       REAL(c_double), INTENT(OUT) :: v(*), ener(*)
+       REAL(c_double), INTENT(INOUT) :: fion(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_vofrho
+     FUNCTION mc_ewald(c, iesr, esr, eself, fion) BIND(C, name='mc_ewald') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: c
+       INTEGER(c_int), VALUE :: iesr
+       REAL(c_double), INTENT(OUT) :: esr, eself
+       REAL(c_double), INTENT(INOUT) :: fion(*)
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_ewald
      FUNCTION mc_fnonloc(c, c2, ldc, f, nstate, fnl) BIND(C, name='mc_fnonloc') RESULT(ierr)
        IMPORT :: c_int, c_ptr, c_double_complex, c_double
        TYPE(c_ptr), VALUE :: c
@@ -405,5 +448,39 @@ CONTAINS
     REAL(real_8), INTENT(IN)    :: fnl(:,:,:), dfnl(:,:,:,:)
     CALL check(mc_rnlfor(ctx, f, INT(nstate, c_int), fnl, dfnl, fion), 'rnlfor')
   END SUBROUTINE micpmd_rnlfor
+
+  SUBROUTINE micpmd_set_density(nhg, inyh, hg, gk)
+    INTEGER, INTENT(IN)      :: nhg
+    INTEGER, INTENT(IN)      :: inyh(:,:)
+    REAL(real_8), INTENT(IN) :: hg(:), gk(:,:)
+    CALL check(mc_set_density_gset(ctx, INT(nhg, c_int), inyh, hg, gk), 'set_density_gset')
+  END SUBROUTINE micpmd_set_density
+
+  SUBROUTINE micpmd_set_local_pp(na, zv, raggio, vps, rhops)
+    INTEGER, INTENT(IN)      :: na(:)
+    REAL(real_8), INTENT(IN) :: zv(:), raggio(:), vps(:,:), rhops(:,:)
+    CALL check(mc_set_local_pp(ctx, INT(SIZE(na), c_int), na, zv, raggio, vps, rhops), 'set_local_pp')
+  END SUBROUTINE micpmd_set_local_pp
+
+  ! tau(3, nat) in isa order; ht(3, 3) with ht(i, :) = lattice vector i
+  SUBROUTINE micpmd_set_ions(tau, ht)
+    REAL(real_8), INTENT(IN) :: tau(:,:), ht(:,:)
+    CALL check(mc_set_ions(ctx, tau, ht), 'set_ions')
+  END SUBROUTINE micpmd_set_ions
+
+  ! ener(7) = ehep, ehii, ehee, epseu, vploc, exc, vxc; fion(3, nat) += potfor
+  SUBROUTINE micpmd_vofrho(rhoe, v, ener, fion)
+    REAL(real_8), INTENT(IN)    :: rhoe(:)
+    REAL(real_8), INTENT(OUT)   :: v(:), ener(7)
+    REAL(real_8), INTENT(INOUT) :: fion(:,:)
+    CALL check(mc_vofrho(ctx, rhoe, v, ener, fion), 'vofrho')
+  END SUBROUTINE micpmd_vofrho
+
+  SUBROUTINE micpmd_ewald(iesr, esr, eself, fion)
+    INTEGER, INTENT(IN)         :: iesr
+    REAL(real_8), INTENT(OUT)   :: esr, eself
+    REAL(real_8), INTENT(INOUT) :: fion(:,:)
+    CALL check(mc_ewald(ctx, INT(iesr, c_int), esr, eself, fion), 'ewald')
+  END SUBROUTINE micpmd_ewald
 
 END MODULE micpmd

@@ -321,6 +321,57 @@ class Context:
                               _p(b) if b is not None else None, _p(out)))
         return out
 
+    # ------------------------------------------------------------ vofrho / ewald
+    VOFRHO_KEYS = ("ehep", "ehii", "ehee", "epseu", "vploc", "exc", "vxc")
+
+    def set_density_gset(self, inyh: np.ndarray, hg: np.ndarray, gk: np.ndarray):
+        """inyh (nhg, 3) centred 1-based indices, hg (nhg,), gk (nhg, 3) in 2 pi / alat."""
+        self._inyh = np.ascontiguousarray(inyh, dtype=np.int32)
+        self._hgd = np.ascontiguousarray(hg, dtype=np.float64)
+        self._gkd = np.ascontiguousarray(gk, dtype=np.float64)
+        check(lib().mc_set_density_gset(self.h, len(self._hgd), self._inyh.ctypes.data_as(C.POINTER(C.c_int)),
+                                        _p(self._hgd), _p(self._gkd)))
+
+    def set_local_pp(self, na, zv, raggio, vps: np.ndarray, rhops: np.ndarray):
+        """vps / rhops (nsp, nhg)."""
+        na = np.ascontiguousarray(na, dtype=np.int32)
+        zv = np.ascontiguousarray(zv, dtype=np.float64)
+        rg = np.ascontiguousarray(raggio, dtype=np.float64)
+        vp = np.asfortranarray(vps, dtype=np.float64)
+        rp = np.asfortranarray(rhops, dtype=np.float64)
+        self.nat_ions = int(na.sum())
+        check(lib().mc_set_local_pp(self.h, len(na), na.ctypes.data_as(C.POINTER(C.c_int)), _p(zv), _p(rg), _p(vp),
+                                    _p(rp)))
+
+    def set_ions(self, tau: np.ndarray, ht: np.ndarray):
+        """tau (nat, 3) bohr; ht (3, 3) with rows = lattice vectors."""
+        t = np.ascontiguousarray(tau, dtype=np.float64)
+        h = np.asfortranarray(ht, dtype=np.float64)
+        check(lib().mc_set_ions(self.h, _p(t), _p(h)))
+
+    def vofrho(self, rho: np.ndarray, forces: bool = True):
+        """(v(r), energies dict, fion (3, nat) from potfor or None)."""
+        r = np.ascontiguousarray(rho, dtype=np.float64)
+        v = np.zeros_like(r)
+        e = np.zeros(7)
+        f = np.zeros((3, self.nat_ions), order="F") if forces else None
+        check(lib().mc_vofrho(self.h, _p(r), _p(v), _p(e), _p(f) if forces else None))
+        return v, dict(zip(self.VOFRHO_KEYS, e)), f
+
+    def vofrho_dev(self, rho: DeviceArray, v: DeviceArray, forces: bool = False):
+        e = np.zeros(7)
+        f = np.zeros((3, self.nat_ions), order="F") if forces else None
+        check(lib().mc_vofrho_dev(self.h, rho.ptr, v.ptr, _p(e), _p(f) if forces else None))
+        return dict(zip(self.VOFRHO_KEYS, e)), f
+
+    def ewald(self, iesr: int = 2, forces: bool = True):
+        """(esr, eself, fion (3, nat) or None)."""
+        esr = C.c_double()
+        es = C.c_double()
+        f = np.zeros((3, self.nat_ions), order="F") if forces else None
+        check(lib().mc_ewald(self.h, int(iesr), C.byref(esr), C.byref(es), _p(f) if forces else None))
+        return esr.value, es.value, f
+
     def fnonloc(self, c2: np.ndarray, f: np.ndarray, fnl: np.ndarray | None = None) -> np.ndarray:
         out = np.array(c2, dtype=np.complex128, order="F", copy=True)
         f = np.ascontiguousarray(f, dtype=np.float64)

@@ -50,8 +50,9 @@ enum {
   MC_KID_YFWD = 4,   /* forward y pass                             */
   MC_KID_ZFWD = 5,   /* forward z pass + Gamma decode + kinetic    */
   MC_KID_MISC = 6,   /* reductions, permutes, small kernels        */
-  MC_KID_GEMM = 7,   /* FP64 MFMA contractions                     */
-  MC_KID_COUNT = 8
+  MC_KID_GEMM = 7,   /* MFMA contractions (FP64 or split bf16)      */
+  MC_KID_VOFRHO = 8, /* dense density transforms, per-G, LDA, Ewald  */
+  MC_KID_COUNT = 9
 };
 
 typedef struct mc_ctx mc_ctx;
@@ -257,6 +258,40 @@ int mc_rnlsm2_dev(mc_ctx* ctx, const void* c0_dev, int nstate, double* dfnl);
  * Replaces rnlfor(fion,f,wk,nstate,nkpoint), src/rnlfor_utils.mod.F90:41-360
  * (Gamma, wk = 1). */
 int mc_rnlfor(mc_ctx* ctx, const double* f, int nstate, const double* fnl, const double* dfnl, double* fion);
+
+/* ------------------------------------------------------------------------
+ * Density -> potential (vofrhoa + vofrhob, closed shell, LDA) and Ewald
+ * ------------------------------------------------------------------------ */
+
+/* Density G set: nhg vectors (G = 0 first when geq0), centred 1-based indices
+ * inyh(3, nhg), hg(nhg) in (2 pi / alat)^2 and gk(3, nhg) in 2 pi / alat (cppt
+ * inyh / hg / gk, all nhg columns).  scg = 4 pi / (tpiba2 hg) is formed here
+ * (0 at G = 0). */
+int mc_set_density_gset(mc_ctx* ctx, int nhg, const int* inyh, const double* hg, const double* gk);
+/* Local pseudopotential per species on the density set: vps(nsp, nhg) and the
+ * Gaussian ion charges rhops(nsp, nhg) (cppt::vps / rhops from putps,
+ * src/rinforce_utils.mod.F90:414-460), valence charges zv(nsp), Gaussian
+ * radii raggio(nsp) and atoms per species na(nsp) (isa order). */
+int mc_set_local_pp(mc_ctx* ctx, int nsp, const int* na, const double* zv, const double* raggio, const double* vps,
+                    const double* rhops);
+/* Ionic positions tau(3, nat) in bohr (isa order; CPMD's tau0(:, ia, is)
+ * flattened by species) and the cell ht(3, 3) with ht(i, :) = lattice vector i
+ * (metr_com%ht); called after every ionic move. */
+int mc_set_ions(mc_ctx* ctx, const double* tau, const double* ht);
+/* vofrhoa + vofrhob for a closed-shell density with the Pade LDA (CPMD's
+ * FUNCTIONAL LDA): rho(r) (nnr1) -> v(r) (nnr1) = V_H + V_loc (G != 0) + V_xc;
+ * energies[7] = ehep, ehii (ei Omega), ehee, epseu, vploc, exc, vxc
+ * (ener_com, src/vofrhoa_utils.mod.F90:105-122, src/vofrhob_utils.mod.F90:459-461);
+ * fion (3, nat) += the local / Hartree forces (potfor) when not NULL.
+ * Replaces vofrhoa(tau0,fion,rhoe,v,vtemp,tfor,.FALSE.) + vofrhob
+ * (src/vofrhoa_utils.mod.F90:41-144, src/vofrhob_utils.mod.F90:80-461). */
+int mc_vofrho(mc_ctx* ctx, const double* rho, double* v, double* energies, double* fion);
+int mc_vofrho_dev(mc_ctx* ctx, const void* rho_dev, void* v_dev, double* energies, double* fion);
+/* Ewald real-space sum over (2 iesr + 1)^3 cells and self energy: esr, eself
+ * (ener_com%esr / eself), fion (3, nat) += the esr forces when not NULL.
+ * Replaces rpiiint(esr,tau0,fion,iesr,tfor), src/rpiiint_utils.mod.F90:29-200,
+ * and the eself sum of src/rinforce_utils.mod.F90:239-249. */
+int mc_ewald(mc_ctx* ctx, int iesr, double* esr, double* eself, double* fion);
 
 /* Optional per-kernel timing with HIP events on the context stream. */
 int mc_timer_enable(mc_ctx* ctx, int on);
