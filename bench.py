@@ -1,25 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark of the plane-wave inner loop on one MI355X.
 
-A step is one wavefunction-optimisation step of the 512-state workload on the
-192^3 mesh (198,204 Gamma plane waves, cubic a = 32 bohr, 256 atoms, ecut
-80 Ry) with every input resident in HBM:
+A step is one Car-Parrinello MD step of the 512-state workload on the 192^3
+mesh (198,204 Gamma plane waves, 1.6M density G vectors, cubic a = 32 bohr,
+256 atoms, ecut 80 Ry, dt = 4 a.u., fictitious mass 400 a.u.) with every
+input resident in HBM:
 
-  rnlsm1 -> rhoofr -> vpsi -> fnonloc -> Lagrange projection
-         -> steepest-descent update c0 += dt c2 -> Loewdin orthonormalisation
+  phfac -> rnlsm1 + rnlsm2 -> rhoofr -> vofrho (+ potfor) -> Ewald
+        -> vpsi + fnonloc (c2 = -f H c) -> rnlfor
+        -> velocity Verlet of the ions and of the coefficients (velupa, posupa)
+        -> rortog (orthonormality constraint)
 
-(CPMD: forces_driver rnlsm / rscpot(rhoofr) / vpsi / fnonloc / addup2, then
-updwf + ortho).  The local potential V(r) is a fixed seeded input (vofrho is
-not part of the measured path).  Wavefunctions: seeded complex Gaussian
-~exp(-|G|^2/8), Loewdin-orthonormalised; one s projector per atom (GTH).
+(CPMD's forces driver, src/forces_driver.mod.F90:175-384, and the CP update
+of src/posupa_utils.mod.F90 / src/velupa_utils.mod.F90 / src/rortog_utils.mod.F90).
+--step wfopt times a steepest-descent wavefunction-optimisation step instead
+(Lagrange projection, update, Loewdin).  Wavefunctions: seeded complex
+Gaussian ~exp(-|G|^2/8), Loewdin-orthonormalised; GTH pseudopotential with one
+s projector per atom; LDA (Pade).
 
 Two precision paths are timed from the same start, the first one being the
 headline `value`:
-  mixed  rhoofr/vpsi transforms and their workspaces in FP32 (everything
-         else FP64);
+  mixed  rhoofr/vpsi transforms and their workspaces in FP32, the wavefunction
+         contractions as 3-term bf16 split MFMA products (FP32 accumulation);
+         vofrho, Ewald, n x n algebra and every energy sum in FP64;
   fp64   every operation in FP64.
-The band energy sum_i f_i <c_i|H|c_i> of the starting coefficients is
-evaluated on both paths and its difference per atom is reported.
+Reported errors: E_KS and F_ion of the headline path against FP64 on the same
+coefficients and positions, the same after the warmup + timed steps from the
+same start (both evaluated in FP64), and ||C^T C - I||.
 
 Prints one JSON line (see the measurement contract in DESIGN.md).
 """
@@ -104,6 +111,8 @@ def cpu_baseline(sysm, c0, v, max_pairs: int = 1):
 
 
 FFT_KIDS = ("zinv", "yinv", "xrho", "xvpsi", "yfwd", "zfwd")
+AMU = 1822.888486                  # electron masses per atomic mass unit
+ION_MASS = 12.0 * AMU             # the synthetic species (Z = 6)
 
 
 def main():
@@ -115,7 +124,9 @@ def main():
     ap.add_argument("--nstate", type=int, default=0)
     ap.add_argument("--batch-rho", type=int, default=16)
     ap.add_argument("--batch-vpsi", type=int, default=8)
-    ap.add_argument("--dt", type=float, default=0.02)
+    ap.add_argument("--step", choices=("cp", "wfopt"), default="cp")
+    ap.add_argument("--dt", type=float, default=0.0, help="time step (a.u.; cp default 4, wfopt default 0.02)")
+    ap.add_argument("--emass", type=float, default=400.0)
     ap.add_argument("--paths", default="mixed,fp64", help="comma list of paths to time (first = headline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -129,68 +140,77 @@ def main():
 
     from cpmd_amd import ops
     from cpmd_amd import system as S
+    from cpmd_amd.ks import KSDevice
 
     cfg = S.CONFIGS[args.config]
     nstate = args.nstate or cfg["nstate"]
+    dt = args.dt or (4.0 if args.step == "cp" else 0.02)
     log(f"[bench] building {args.config}: a={cfg['a']} nat={cfg['nat']} ecut={cfg['ecut']} nstate={nstate}")
-    sysm = S.make_system(cfg["a"], cfg["nat"], cfg["ecut"], nstate=nstate)
+    sysm = S.make_system(cfg["a"], cfg["nat"], cfg["ecut"], nstate=nstate, with_density=True)
     c0 = S.initial_coefficients(sysm)
-    v = S.real_space_potential(sysm)
     ngw = sysm.gset.ngw
     n1, n2, n3 = sysm.cell.nr
+    ks = KSDevice(sysm)
+    ctx = ks.ctx
+    ctx.set_batch(args.batch_rho, args.batch_vpsi)
     sticks, cols, _ = ops.layout_stats(sysm.cell, sysm.gset)
     geom = (ngw, sticks, cols, n1, n2, n3)
-    log(f"[bench] mesh {sysm.cell.nr} ngw {ngw} sticks {sticks} columns {cols}")
-
-    ctx = ops.Context(sysm.cell, sysm.gset)
-    ctx.set_batch(args.batch_rho, args.batch_vpsi)
-    na, nhxs, ngh, lang, hmat, twnl, eigr = S.projector_set(sysm, sysm.gset.miller[:ngw], sysm.gset.hg[:ngw])
-    ctx.set_projectors(na, nhxs, ngh, lang, hmat, twnl)
-    ctx.set_eigr(eigr)
-    del eigr, twnl
-    d_c0 = ctx.alloc_wfn(nstate)
-    d_c2 = ctx.alloc_wfn(nstate)
-    d_rho = ops.DeviceArray(8 * ctx.nnr1)
-    d_v = ops.DeviceArray(8 * ctx.nnr1).from_host(v)
+    log(f"[bench] mesh {sysm.cell.nr} ngw {ngw} nhg {sysm.gset.nhg} sticks {sticks} columns {cols}")
+    buf = {k: ctx.alloc_wfn(nstate) for k in ("c0", "cm", "cp", "c2")}
     f = sysm.f
-    nat = int(np.sum(na))
+    nat = sysm.nat
     npairs = (nstate + 1) // 2
     K = 2.0 * ngw
     n = float(nstate)
-    gemm_flops = (2 * nat * n * K            # rnlsm1
-                  + 2 * K * n * nat          # fnonloc
-                  + 2 * n * n * K            # Lagrange: ovlap(c2, c0)
-                  + 2 * K * n * n            # Lagrange: rotate
-                  + n * n * K                # Loewdin: S = C^T C (symmetric)
-                  + 2 * K * n * n)           # Loewdin: C S^-1/2
+    if args.step == "cp":
+        gemm_flops = (4 * 2 * nat * n * K        # rnlsm1 + rnlsm2 (3 components)
+                      + 2 * K * n * nat          # fnonloc
+                      + 2 * n * n * K            # rortog: <c0|cp>
+                      + n * n * K                # rortog: <cp|cp> (symmetric)
+                      + 2 * K * n * n)           # rortog: cp += c0 X
+        sym_flops = n * n * K
+        step_desc = ("phfac + rnlsm1 + rnlsm2 + rhoofr + vofrho(+potfor) + Ewald + vpsi + fnonloc + rnlfor "
+                     "+ velocity Verlet (ions, electrons) + rortog")
+    else:
+        gemm_flops = (2 * nat * n * K + 2 * K * n * nat + 2 * n * n * K + 2 * K * n * n + n * n * K + 2 * K * n * n)
+        sym_flops = n * n * K
+        step_desc = "rnlsm1 + rhoofr + vofrho + vpsi + fnonloc + Lagrange projection + SD update + Loewdin"
 
-    def band_energy():
-        """E_band = sum_i f_i <c_i|T + V + V_NL|c_i> = -sum_i dotp(c0_i, c2_i) after vpsi + fnonloc."""
-        ctx.rnlsm1_dev(d_c0, nstate)
-        ctx.zero_wfn(d_c2, nstate)
-        ctx.vpsi_dev(d_c0, d_c2, f, d_v, nstate)
-        ctx.fnonloc_dev(d_c2, f, nstate)
-        return -float(np.sum(ctx.dotp_wfn(d_c0, d_c2, nstate)))
+    def fp64_eval(tau):
+        ctx.set_precision("fp64")
+        return ks.energy(buf["c0"], f, tau=tau, gradient=buf["c2"], forces=True)
 
     def run_path(path):
         ctx.set_precision("mixed" if path == "mixed" else "fp64")
-        ctx.upload(c0, d_c0)                 # every path starts from the same coefficients
-        e_band = band_energy()
-        energies = {}
-        lowdin_iters = []
+        ctx.upload(c0, buf["c0"])                 # every path starts from the same coefficients
+        ctx.axpby_wfn(0.0, buf["c0"], 0.0, buf["cm"], nstate)
+        tau = sysm.tau.copy()
+        vion = np.zeros_like(tau)
+        start = ks.energy(buf["c0"], f, tau=tau, gradient=buf["c2"], forces=True)
+        iters = []
 
-        def step():
-            ctx.rnlsm1_dev(d_c0, nstate)
-            ek, cg, cr = ctx.rhoofr_dev(d_c0, f, nstate, d_rho)
-            energies.update(ekin=ek, csumg=cg, csumr=cr)
-            ctx.zero_wfn(d_c2, nstate)
-            ctx.vpsi_dev(d_c0, d_c2, f, d_v, nstate)
-            ctx.fnonloc_dev(d_c2, f, nstate)
-            ctx.lagrange_dev(d_c0, d_c2, f, nstate)
-            ctx.axpy_wfn(args.dt, d_c2, d_c0, nstate)
-            lowdin_iters.append(ctx.lowdin_dev(d_c0, nstate))
+        def step_cp():
+            nonlocal tau, vion
+            r = ks.energy(buf["c0"], f, tau=tau, gradient=buf["c2"], forces=True)
+            vion = vion + dt / ION_MASS * r["fion"].T
+            tau = tau + dt * vion
+            ctx.axpy_wfn(dt / args.emass, buf["c2"], buf["cm"], nstate)            # velupa
+            ctx.axpby_wfn(1.0, buf["c0"], 0.0, buf["cp"], nstate)                 # posupa
+            ctx.axpy_wfn(dt, buf["cm"], buf["cp"], nstate)
+            _, it = ctx.rortog_dev(buf["c0"], buf["cp"], nstate, epsog=1e-8, maxit=60)
+            iters.append(it)
+            ctx.axpby_wfn(1.0 / dt, buf["cp"], 0.0, buf["cm"], nstate)            # cm = (cp - c0) / dt
+            ctx.axpy_wfn(-1.0 / dt, buf["c0"], buf["cm"], nstate)
+            buf["c0"], buf["cp"] = buf["cp"], buf["c0"]
 
-        for i in range(args.warmup):
+        def step_wfopt():
+            ks.energy(buf["c0"], f, tau=tau, gradient=buf["c2"])
+            ctx.lagrange_dev(buf["c0"], buf["c2"], f, nstate)
+            ctx.axpy_wfn(dt, buf["c2"], buf["c0"], nstate)
+            iters.append(ctx.lowdin_dev(buf["c0"], nstate))
+
+        step = step_cp if args.step == "cp" else step_wfopt
+        for _ in range(args.warmup):
             step()
         ctx.synchronize()
         log(f"[bench] {path}: {args.warmup} warmup steps done")
@@ -205,19 +225,17 @@ def main():
         ctx.timer_enable(False)
         ms = (t1 - t0) * 1e3 / args.steps
         kernels = {}
-        for kid in FFT_KIDS + ("misc", "gemm"):
+        for kid in FFT_KIDS + ("misc", "gemm", "vofrho"):
             tot, cnt = ctx.timer_get(kid)
             if cnt:
                 kernels[kid] = {"total_ms_per_step": tot / args.steps, "launches": cnt, "avg_ms": tot / cnt}
-        # after warmup + steps updates: E_band of the final C and ||C^T C - I||, both in FP64
-        ctx.set_precision("fp64")
-        e_end = band_energy()
-        orth = float(np.abs(ctx.ovlap_dev(d_c0, d_c0, nstate) - np.eye(nstate)).max())
-        log(f"[bench] {path}: {args.steps} steps: {ms:.3f} ms/step, E_band(start) {e_band:.10f}, "
-            f"E_band(end, fp64) {e_end:.10f}, |C^T C - I| {orth:.2e}, "
-            f"csumr-csumg {energies['csumr'] - energies['csumg']:.2e}")
-        return {"ms": ms, "kernels": kernels, "e_band": e_band, "e_end": e_end, "orth": orth,
-                "csum_diff": energies["csumr"] - energies["csumg"], "lowdin_iters": lowdin_iters[-args.steps:]}
+        # the state after warmup + steps, evaluated in FP64
+        end = fp64_eval(tau)
+        orth = float(np.abs(ctx.ovlap_dev(buf["c0"], buf["c0"], nstate) - np.eye(nstate)).max())
+        log(f"[bench] {path}: {args.steps} steps: {ms:.3f} ms/step, E_KS(start) {start['etot']:.10f}, "
+            f"E_KS(end, fp64) {end['etot']:.10f}, |C^T C - I| {orth:.2e}, iters {iters[-args.steps:]}")
+        return {"ms": ms, "kernels": kernels, "start": start, "end": end, "orth": orth,
+                "iters": iters[-args.steps:]}
 
     paths = [p.strip() for p in args.paths.split(",") if p.strip()]
     res = {p: run_path(p) for p in paths}
@@ -225,9 +243,7 @@ def main():
     R = res[head]
     cw = 8.0 if head == "mixed" else 16.0
     kernels = R["kernels"]
-    # dominant transform kernel by time per step
     dom = max((k for k in kernels if k in FFT_KIDS), key=lambda k: kernels[k]["total_ms_per_step"])
-    # pairs per launch of the dominant kernel (average over the step)
     lr = -(-npairs // min(args.batch_rho, npairs))
     lv = -(-npairs // min(args.batch_vpsi, npairs))
     if dom == "xrho":
@@ -248,7 +264,7 @@ def main():
         ms = r["kernels"].get("gemm", {}).get("total_ms_per_step", float("nan"))
         useful = gemm_flops / (ms * 1e-3) / 1e12
         if path == "mixed":
-            issued = 3 * gemm_flops + n * n * K      # + the lo*lo term of S = C^T C
+            issued = 3 * gemm_flops + sym_flops
             tf = issued / (ms * 1e-3) / 1e12
             return {"bound": "mfma", "dtype": "bf16 (3-term split)", "flops_per_step": issued,
                     "gemm_ms_per_step": ms, "achieved": round(tf, 2), "peak": 2500.0, "unit": "TFLOP/s",
@@ -267,19 +283,24 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle, 1 pair sample) ...")
-        cpu = cpu_baseline(sysm, c0, v)
+        cpu = cpu_baseline(sysm, c0, np.ascontiguousarray(ks.d_v.to_host(ks.nnr1)))
         log(f"[bench] cpu baseline {cpu['value']:.1f} ms/step")
 
     compare = {}
     if "fp64" in res and head != "fp64":
+        F = res["fp64"]
         compare = {
-            "fp64_ms_per_step": round(res["fp64"]["ms"], 4),
-            "speedup_vs_fp64": round(res["fp64"]["ms"] / R["ms"], 3),
-            "dE_band_per_atom_Ha": abs(R["e_band"] - res["fp64"]["e_band"]) / nat,
-            "E_band_fp64_Ha": res["fp64"]["e_band"],
-            "dE_band_after_steps_per_atom_Ha": abs(R["e_end"] - res["fp64"]["e_end"]) / nat,
+            "fp64_ms_per_step": round(F["ms"], 4),
+            "speedup_vs_fp64": round(F["ms"] / R["ms"], 3),
+            # same C and R, the headline path's arithmetic against FP64
+            "dE_KS_per_atom_Ha": abs(R["start"]["etot"] - F["start"]["etot"]) / nat,
+            "dF_max_Ha_per_bohr": float(np.abs(R["start"]["fion"] - F["start"]["fion"]).max()),
+            # after warmup + steps from the same start, both evaluated in FP64
+            "dE_KS_after_steps_per_atom_Ha": abs(R["end"]["etot"] - F["end"]["etot"]) / nat,
+            "dF_after_steps_max_Ha_per_bohr": float(np.abs(R["end"]["fion"] - F["end"]["fion"]).max()),
             "orthonormality_err": R["orth"],
-            "orthonormality_err_fp64": res["fp64"]["orth"],
+            "orthonormality_err_fp64": F["orth"],
+            "E_KS_fp64_Ha": F["start"]["etot"],
         }
     out = {
         "metric": METRIC,
@@ -293,11 +314,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 transforms, bf16-split MFMA contractions, f64 sums" if head == "mixed" else "f64",
-        "data": "synthetic (seeded Gaussian wavefunctions, Loewdin-orthonormalised; seeded smooth V(r))",
+        "data": "synthetic (seeded Gaussian wavefunctions, Loewdin-orthonormalised; seeded atom positions, "
+                "analytic GTH pseudopotential)",
         "config": {
             "workload": f"{args.config}: {nstate} states, {n1}x{n2}x{n3} mesh, {ngw} Gamma plane waves, "
-                        f"{nat} atoms, a={cfg['a']} bohr, ecut {cfg['ecut']} Ry",
-            "step": "rnlsm1 + rhoofr + vpsi + fnonloc + Lagrange projection + SD update + Loewdin",
+                        f"{sysm.gset.nhg} density G, {nat} atoms, a={cfg['a']} bohr, ecut {cfg['ecut']} Ry",
+            "step": step_desc,
+            "step_kind": args.step,
+            "dt": dt,
+            "emass": args.emass,
             "precision_path": head,
             "batch_pairs": {"rhoofr": args.batch_rho, "vpsi": args.batch_vpsi},
         },
@@ -317,10 +342,9 @@ def main():
                          "achieved": round(step_bytes / (fft_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(step_bytes / (fft_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "mfma_roofline": mfma_roofline(head, R),
-        "paths": {p: {"ms_per_step": round(res[p]["ms"], 4), "E_band_Ha": res[p]["e_band"],
-                      "E_band_after_steps_Ha": res[p]["e_end"], "orthonormality_err": res[p]["orth"],
-                      "csumr_minus_csumg": res[p]["csum_diff"],
-                      "lowdin_newton_schulz_iters": res[p]["lowdin_iters"],
+        "paths": {p: {"ms_per_step": round(res[p]["ms"], 4), "E_KS_start_Ha": res[p]["start"]["etot"],
+                      "E_KS_end_fp64_Ha": res[p]["end"]["etot"], "orthonormality_err": res[p]["orth"],
+                      "constraint_iters": res[p]["iters"],
                       "mfma_roofline": mfma_roofline(p, res[p]),
                       "kernels": res[p]["kernels"]} for p in paths},
         "cpu_baseline": cpu,

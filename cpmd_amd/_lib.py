@@ -54,6 +54,7 @@ SIGNATURES = {
     "mc_memset_d": (C.c_int, [_vp, C.c_int, C.c_size_t]),
     "mc_wfn_zero": (C.c_int, [_vp, _vp, C.c_int]),
     "mc_wfn_axpy": (C.c_int, [_vp, C.c_double, _vp, _vp, C.c_int]),
+    "mc_wfn_axpby": (C.c_int, [_vp, C.c_double, _vp, C.c_double, _vp, C.c_int]),
     "mc_wfn_dotp": (C.c_int, [_vp, _vp, _vp, C.c_int, _dp]),
     "mc_wfn_upload": (C.c_int, [_vp, _dp, C.c_int, C.c_int, _vp]),
     "mc_wfn_download": (C.c_int, [_vp, _vp, C.c_int, _dp, C.c_int]),
@@ -87,6 +88,7 @@ SIGNATURES = {
     "mc_vofrho": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),
     "mc_vofrho_dev": (C.c_int, [_vp, _vp, _vp, _dp, _dp]),
     "mc_ewald": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
+    "mc_phfac": (C.c_int, [_vp]),
     "mc_fnonloc": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp]),
     "mc_fnonloc_dev": (C.c_int, [_vp, _vp, _dp, C.c_int, _dp]),
     "mc_timer_enable": (C.c_int, [_vp, C.c_int]),

@@ -279,6 +279,13 @@ int mc_vofrho(mc_ctx* ctx, const double* rho, double* v, double* energies, doubl
   MC_API_END
 }
 
+int mc_phfac(mc_ctx* ctx) {
+  MC_API_BEGIN
+  check_ctx(ctx);
+  mc::phfac(ctx);
+  MC_API_END
+}
+
 int mc_ewald(mc_ctx* ctx, int iesr, double* esr, double* eself, double* fion) {
   MC_API_BEGIN
   check_ctx(ctx);

@@ -78,6 +78,7 @@ static inline int wrap(int m, int n) { int r = m % n; return r < 0 ? r + n : r; 
 struct HostLayout {
   int n_own = 0, n_slots = 0, ncols = 0, max_col_len = 0;
   std::vector<int> perm, ginfo, own_start, col_start, col_slot, col_gy, col_gx;
+  std::vector<int> gmw;   // (3, ngw) centred indices of the device rows
   std::vector<double> hg;
 };
 
@@ -134,6 +135,12 @@ static HostLayout build_layout(const mc_layout* lay) {
     return std::get<2>(mill[a]) < std::get<2>(mill[b]);
   });
   L.perm = idx;
+  L.gmw.resize(3 * (size_t)ngw);
+  for (int j = 0; j < ngw; ++j) {
+    L.gmw[3 * (size_t)j] = std::get<0>(mill[idx[j]]);
+    L.gmw[3 * (size_t)j + 1] = std::get<1>(mill[idx[j]]);
+    L.gmw[3 * (size_t)j + 2] = std::get<2>(mill[idx[j]]);
+  }
   L.ginfo.resize(ngw);
   L.hg.resize(ngw);
   L.own_start.assign(L.n_own + 1, 0);
@@ -206,6 +213,12 @@ __global__ void k_zero(double* __restrict__ x, size_t n) {
 __global__ void k_axpy(double alpha, const double* __restrict__ x, double* __restrict__ y, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     y[i] += alpha * x[i];
+}
+
+// y = alpha x + beta y (beta = 0: y is not read)
+__global__ void k_axpby(double alpha, const double* __restrict__ x, double beta, double* __restrict__ y, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] = beta == 0.0 ? alpha * x[i] : alpha * x[i] + beta * y[i];
 }
 
 // out[i] = Re a0 Re b0 + 2 sum_{G != 0} Re(conj(a) b) for column i (dotp,
@@ -346,6 +359,7 @@ int mc_ctx_create(const mc_layout* lay, mc_ctx** out) {
     MC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     ctx->own_stream = true;
     mc::upload_vec(ctx->d_perm, L.perm);
+    mc::upload_vec(ctx->d_gmw, L.gmw);
     mc::upload_vec(ctx->d_ginfo, L.ginfo);
     mc::upload_vec(ctx->d_hg, L.hg);
     mc::upload_vec(ctx->d_own_start, L.own_start);
@@ -465,6 +479,18 @@ int mc_wfn_axpy(mc_ctx* ctx, double alpha, const void* x_dev, void* y_dev, int n
   const size_t n = (size_t)ctx->ldg * nstate * 2;
   if (n) {
     hipLaunchKernelGGL(mc::k_axpy, dim3(2048), dim3(256), 0, ctx->stream, alpha, (const double*)x_dev,
+                       (double*)y_dev, n);
+    MC_HIP(hipGetLastError());
+  }
+  MC_API_END
+}
+
+int mc_wfn_axpby(mc_ctx* ctx, double alpha, const void* x_dev, double beta, void* y_dev, int nstate) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && x_dev && y_dev && nstate >= 0, MC_ERR_ARG, "bad argument");
+  const size_t n = (size_t)ctx->ldg * nstate * 2;
+  if (n) {
+    hipLaunchKernelGGL(mc::k_axpby, dim3(2048), dim3(256), 0, ctx->stream, alpha, (const double*)x_dev, beta,
                        (double*)y_dev, n);
     MC_HIP(hipGetLastError());
   }

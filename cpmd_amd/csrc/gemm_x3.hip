@@ -6,10 +6,17 @@
 // round-to-nearest from the FP32 value of x: |x - hi| <= 2^-8 |x|, |x - hi - lo|
 // <= 2^-16 |x|), and a product is accumulated as hi*hi + hi*lo + lo*hi; the
 // dropped lo*lo term and the split residuals leave at most 3 * 2^-16 of |x y|.
-// The symmetric overlap C^T C also accumulates lo*lo (its diagonal would
-// otherwise carry the positive bias sum lo_k^2).  Each K chunk accumulates in
-// FP32; split-K partials are written and reduced in FP64 (k_reduce_tn of
-// gemm.hip), so the FP32 accumulation error is that of one chunk.
+// Each K chunk accumulates in FP32; split-K partials are written and reduced
+// in FP64 (k_reduce_tn of gemm.hip), so the FP32 accumulation error is that of
+// one chunk.
+//
+// Diagonal of square overlaps.  On a diagonal entry sum_k a_k b_k with
+// correlated a, b (S = C^T C, or <c0|H c0>) every product loses the same-signed
+// lo*lo, and the FP32 accumulation of a chunk shrinks the sum by a few 1e-7
+// (measured); both bias the norms that orthonormalisation relies on.  The
+// diagonal of every square wavefunction overlap is therefore recomputed in
+// FP64 (k_diag_tn, one pass over the two operands), which leaves only the
+// small, sign-random errors of the off-diagonal entries.
 //
 // Tiles are 128 x 128 per workgroup, 4 waves of 64 x 64 (4 x 4 MFMA tiles, 64
 // FP32 accumulators per lane), K slices of 32 (one MFMA depth).  Operand
@@ -58,9 +65,8 @@ __device__ __forceinline__ void split8(const double* v, bf8& hi, bf8& lo) {
 }
 
 // one K slice of a 64 x 64 wave block: acc += A_s B_s with A_s rows
-// [wr, wr+64) and B_s columns [wc, wc+64), both stored as [row][k] bf16.
-// NT = 3: hi*hi + hi*lo + lo*hi; NT = 4 adds lo*lo.
-template <int NT>
+// [wr, wr+64) and B_s columns [wc, wc+64), both stored as [row][k] bf16:
+// hi*hi + hi*lo + lo*hi.
 __device__ __forceinline__ void x3_slice(const __bf16* Ahi, const __bf16* Alo, const __bf16* Bhi, const __bf16* Blo,
                                          int wr, int wc, f4 (&acc)[4][4]) {
   const int l = threadIdx.x & 63;
@@ -80,7 +86,6 @@ __device__ __forceinline__ void x3_slice(const __bf16* Ahi, const __bf16* Alo, c
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      if (NT == 4) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bl[b], acc[a][b], 0, 0, 0);
       acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
       acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
       acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
@@ -106,7 +111,6 @@ __device__ __forceinline__ void put16(__bf16* hi, __bf16* lo, int row, int kq, c
 // B (K x n); P[s] (m x n) FP64 partials; upper: skip tiles below the diagonal.
 // kchunk is a multiple of 32.
 // ---------------------------------------------------------------------------
-template <int NT>
 __global__ __launch_bounds__(256, 2) void k_x3_tn(const double* __restrict__ A, int lda, const double* __restrict__ B,
                                                   int ldb, int K, int m, int n, int kchunk, double* __restrict__ P,
                                                   int upper, ProjA proj, int tm, int tn, int total) {
@@ -172,7 +176,7 @@ __global__ __launch_bounds__(256, 2) void k_x3_tn(const double* __restrict__ A, 
     put16(Bhi, Blo, lc, lq, rb);
     __syncthreads();
     if (kb + kK < kend) fetch(kb + kK);
-    x3_slice<NT>(Ahi, Alo, Bhi, Blo, wr, wc, acc);
+    x3_slice(Ahi, Alo, Bhi, Blo, wr, wc, acc);
     __syncthreads();
   }
   const int l = threadIdx.x & 63;
@@ -258,7 +262,7 @@ __global__ __launch_bounds__(256, 2) void k_x3_nn(const double* __restrict__ A, 
     put16(Ghi, Glo, gc, gq, rg);
     __syncthreads();
     if (kb + kK < kmax) fetch(kb + kK);
-    x3_slice<3>(Ahi, Alo, Ghi, Glo, wr, wc, acc);
+    x3_slice(Ahi, Alo, Ghi, Glo, wr, wc, acc);
     __syncthreads();
   }
   // epilogue: lane holds rows i .. i+3 (i = 4(l/16) + 16a + wr + i0) of column j
@@ -302,6 +306,28 @@ __global__ __launch_bounds__(256, 2) void k_x3_nn(const double* __restrict__ A, 
   }
 }
 
+// out(i, i) = alpha sum_k A(k, i) B(k, i) [- A(0, i) B(0, i)] in FP64, one
+// workgroup per column, fixed reduction order
+__global__ __launch_bounds__(256) void k_diag_tn(const double* __restrict__ A, int lda, const double* __restrict__ B,
+                                                 int ldb, int K, double alpha, int g0corr, double* __restrict__ out,
+                                                 int ldo) {
+  const int i = blockIdx.x;
+  const double* a = A + (size_t)lda * i;
+  const double* b = B + (size_t)ldb * i;
+  double s = 0.0;
+  for (int k = threadIdx.x; k < K; k += 256) s += a[k] * b[k];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  __shared__ double part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = alpha * (part[0] + part[1] + part[2] + part[3]);
+    if (g0corr) v -= a[0] * b[0];
+    out[(size_t)ldo * i + i] = v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host wrappers
 // ---------------------------------------------------------------------------
@@ -321,18 +347,18 @@ void gemm_tn_x3(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb,
   const int total = nsplit * tm * tn;
   const int launch = (total + kXcd - 1) / kXcd * kXcd;
   timer_begin(ctx, MC_KID_GEMM);
-  // S = C^T C: every diagonal term a_k^2 loses lo_k^2 >= 0 in the 3-term
-  // product (a bias of ~2.8e-6 relative on S_ii that would show up as a norm
-  // error after orthonormalisation); the symmetric product keeps the lo*lo term.
-  if (symmetric)
-    hipLaunchKernelGGL(k_x3_tn<4>, dim3(launch), dim3(256), 0, ctx->stream, A, lda, B, ldb, K, m, n, kchunk,
-                       ctx->w_gemm.as<double>(), 1, pa, tm, tn, total);
-  else
-    hipLaunchKernelGGL(k_x3_tn<3>, dim3(launch), dim3(256), 0, ctx->stream, A, lda, B, ldb, K, m, n, kchunk,
-                       ctx->w_gemm.as<double>(), 0, pa, tm, tn, total);
+  hipLaunchKernelGGL(k_x3_tn, dim3(launch), dim3(256), 0, ctx->stream, A, lda, B, ldb, K, m, n, kchunk,
+                     ctx->w_gemm.as<double>(), symmetric ? 1 : 0, pa, tm, tn, total);
   MC_HIP(hipGetLastError());
   timer_end(ctx, MC_KID_GEMM);
   reduce_tn(ctx, nsplit, m, n, alpha, A, lda, B, ldb, g0corr, symmetric, out, ldo);
+  if (!proj && m == n) {
+    timer_begin(ctx, MC_KID_GEMM);
+    hipLaunchKernelGGL(k_diag_tn, dim3(n), dim3(256), 0, ctx->stream, A, lda, B, ldb, K, alpha, g0corr ? 1 : 0, out,
+                       ldo);
+    MC_HIP(hipGetLastError());
+    timer_end(ctx, MC_KID_GEMM);
+  }
 }
 
 void gemm_nn_x3(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n, double alpha,

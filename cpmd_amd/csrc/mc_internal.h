@@ -163,6 +163,7 @@ struct mc_ctx {
 
   // device tables
   mc::DevBuf d_perm;       // int[ngw]     stick-order -> CPMD order
+  mc::DevBuf d_gmw;        // int[3 ngw]   centred indices (m1, m2, m3) of the device rows
   mc::DevBuf d_ginfo;      // int[ngw]     (owner << 12) | z index
   mc::DevBuf d_hg;         // double[ngw]  |G|^2 (tpiba2 units), stick order
   mc::DevBuf d_own_start;  // int[n_own+1]
@@ -211,6 +212,7 @@ void set_local_pp(mc_ctx* ctx, int nsp, const int* na, const double* zv, const d
 void set_ions(mc_ctx* ctx, const double* tau, const double* ht);
 void vofrho_dev(mc_ctx* ctx, const double* rho, double* v, double* energies, double* fion_host);
 void ewald(mc_ctx* ctx, int iesr, double* esr, double* eself, double* fion_host);
+void phfac(mc_ctx* ctx);
 
 // contractions (gemm.hip)
 void gemm_tn(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, int K, int m, int n, double alpha,

@@ -402,6 +402,23 @@ __global__ __launch_bounds__(kThreads) void k_ewald(EParams p, double* __restric
   block_sum_store<4>(acc, out + 4 * (size_t)l, 4);
 }
 
+// projector structure factors on the wavefunction rows (device order):
+// eigr(j, isa) = e^{-iG_j.R_isa} from the phase tables, 0 on padding rows
+__global__ void k_eigr(const int* __restrict__ gmw, int ngw, int ldg, int n1, int n2, int n3,
+                       const double2* __restrict__ ph1, const double2* __restrict__ ph2,
+                       const double2* __restrict__ ph3, double2* __restrict__ eigr) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, isa = blockIdx.y;
+  if (j >= ldg) return;
+  double2 e = make_double2(0.0, 0.0);
+  if (j < ngw) {
+    const double2 a = ph1[(size_t)isa * (n1 + 1) + gmw[3 * j] + n1 / 2];
+    const double2 b = ph2[(size_t)isa * (n2 + 1) + gmw[3 * j + 1] + n2 / 2];
+    const double2 c = ph3[(size_t)isa * (n3 + 1) + gmw[3 * j + 2] + n3 / 2];
+    e = cmul(cmul(a, b), c);
+  }
+  eigr[(size_t)isa * ldg + j] = e;
+}
+
 // ---------------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------------
@@ -700,6 +717,26 @@ void ewald(mc_ctx* ctx, int iesr, double* esr, double* eself, double* fion_host)
       for (int q = 0; q < 3; ++q) fion_host[3 * (size_t)l + q] += h[4 * (size_t)l + 1 + q];
   }
   if (esr) *esr = e;
+}
+
+}  // namespace mc
+
+namespace mc {
+
+// phfac on the device: the projector structure factors of the current ions
+// (replaces the host eigr upload after an ionic move)
+void phfac(mc_ctx* ctx) {
+  auto& D = ctx->dens;
+  auto& P = ctx->proj;
+  MC_CHECK(D.ions_ready, MC_ERR_ARG, "phfac: ions not set (mc_set_ions)");
+  MC_CHECK(P.ready && P.nat == D.nat, MC_ERR_ARG, "phfac: projector atoms differ from the ions of mc_set_ions");
+  P.d_eigr.ensure(sizeof(double2) * (size_t)ctx->ldg * std::max(D.nat, 1));
+  P.eigr_nat = D.nat;
+  if (D.nat == 0) return;
+  LAUNCH(MC_KID_VOFRHO, k_eigr, dim3((ctx->ldg + 255) / 256, D.nat), dim3(256), 0, ctx->d_gmw.as<int>(), ctx->ngw,
+         ctx->ldg, ctx->n1, ctx->n2, ctx->n3, D.d_ph1.as<double2>(), D.d_ph2.as<double2>(), D.d_ph3.as<double2>(),
+         P.d_eigr.as<double2>());
+  MC_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 }  // namespace mc

@@ -170,7 +170,43 @@ __global__ __launch_bounds__(kThreads) void k_ypass(WParams w, v2<T>* __restrict
   }
 }
 
-// gather the columns of (z, y-tile) of pair pb into ny zero-padded x lines
+// The x passes gather, per pair, the ncols columns of one (z, y-tile) into ny
+// zero-padded x lines.  On the FP32 path each thread owns up to kMaxCol column
+// elements: they are fetched into registers one pair ahead (the loads of pair
+// pb+1 are in flight during the transforms of pair pb) and scattered into LDS.
+constexpr int kMaxCol = 8;
+
+template <typename T>
+__device__ __forceinline__ void fetch_cols(const WParams& w, const v2<T>* __restrict__ B, int pb, int z, int y0,
+                                           int ny, v2<T> (&r)[kMaxCol]) {
+  const size_t bbase = ((size_t)pb * w.n3 + z) * w.ncols;
+  const int m = w.ncols * ny;
+#pragma unroll
+  for (int k = 0; k < kMaxCol; ++k) {
+    const int e = threadIdx.x + k * kThreads;
+    if (e < m) {
+      const int c = e / ny, y = e - c * ny;
+      r[k] = B[(bbase + c) * w.n2 + y0 + y];
+    }
+  }
+}
+
+// LDS position of each owned column element: y * n1 + x(column)
+__device__ __forceinline__ void col_slots(const WParams& w, int ny, int n1, int (&dst)[kMaxCol]) {
+  const int m = w.ncols * ny;
+#pragma unroll
+  for (int k = 0; k < kMaxCol; ++k) {
+    const int e = threadIdx.x + k * kThreads;
+    dst[k] = -1;
+    if (e < m) {
+      const int c = e / ny, y = e - c * ny;
+      dst[k] = y * n1 + w.col_gx[c];
+    }
+  }
+}
+
+// direct gather (no register prefetch): the FP64 path, where the prefetch
+// registers would cost occupancy
 template <typename T>
 __device__ __forceinline__ void load_xlines(const WParams& w, const v2<T>* __restrict__ B, int pb, int z, int y0,
                                             int ny, int n1, v2<T>* buf) {
@@ -181,6 +217,16 @@ __device__ __forceinline__ void load_xlines(const WParams& w, const v2<T>* __res
     const int c = e / ny, y = e - c * ny;
     buf[y * n1 + w.col_gx[c]] = B[(bbase + c) * w.n2 + y0 + y];
   }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ void put_cols(int ne, const int (&dst)[kMaxCol], const v2<T> (&r)[kMaxCol], v2<T>* buf) {
+  for (int e = threadIdx.x; e < ne; e += kThreads) buf[e] = cmk<v2<T>>((T)0, (T)0);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kMaxCol; ++k)
+    if (dst[k] >= 0) buf[dst[k]] = r[k];
   __syncthreads();
 }
 
@@ -206,13 +252,26 @@ __global__ __launch_bounds__(kThreads) void k_xrho(WParams w, const v2<T>* __res
 #pragma unroll
   for (int k = 0; k < kMaxElem; ++k) acc[k] = 0.0;
   const double rom = 1.0 / w.omega;
+  constexpr bool kPre = sizeof(T) == 4;   // register prefetch of the next pair (FP32 path)
+  int dst[kMaxCol];
+  V r[kMaxCol];
+  if (kPre) {
+    col_slots(w, ny, n1, dst);
+    if (npairs > 0) fetch_cols<T>(w, B, 0, z, y0, ny, r);
+  }
   for (int pb = 0; pb < npairs; ++pb) {
     const int is1 = 2 * (pair0 + pb), is2 = is1 + 1;
     const double fa = f[is1];
     const double fb = is2 < nstate ? f[is2] : 0.0;
-    if (fa == 0.0 && fb == 0.0) continue;
+    const bool skip = (fa == 0.0 && fb == 0.0);
+    if (kPre) {
+      if (!skip) put_cols<T>(ne, dst, r, buf);
+      if (pb + 1 < npairs) fetch_cols<T>(w, B, pb + 1, z, y0, ny, r);
+    } else if (!skip) {
+      load_xlines<T>(w, B, pb, z, y0, ny, n1, buf);
+    }
+    if (skip) continue;
     const double c3 = fa * rom, c4 = fb * rom;
-    load_xlines<T>(w, B, pb, z, y0, ny, n1, buf);
     fft_any<T, N, 1>(buf, ny, n1, w.p1, tw);
 #pragma unroll
     for (int k = 0; k < kMaxElem; ++k) {
@@ -261,8 +320,21 @@ __global__ __launch_bounds__(kThreads) void k_xvpsi(WParams w, v2<T>* __restrict
       vr[k] = (T)vpot[(size_t)x + (size_t)w.kr1 * ((size_t)(y0 + y) + (size_t)w.kr2 * z)];
     }
   }
+  constexpr bool kPre = sizeof(T) == 4;   // register prefetch of the next pair (FP32 path)
+  int dst[kMaxCol];
+  V r[kMaxCol];
+  if (kPre) {
+    col_slots(w, ny, n1, dst);
+    if (npairs > 0) fetch_cols<T>(w, B, 0, z, y0, ny, r);
+  }
+  const int m = w.ncols * ny;
   for (int pb = 0; pb < npairs; ++pb) {
-    load_xlines<T>(w, B, pb, z, y0, ny, n1, buf);
+    if (kPre) {
+      put_cols<T>(ne, dst, r, buf);
+      if (pb + 1 < npairs) fetch_cols<T>(w, B, pb + 1, z, y0, ny, r);
+    } else {
+      load_xlines<T>(w, B, pb, z, y0, ny, n1, buf);
+    }
     fft_any<T, N, 1>(buf, ny, n1, w.p1, tw);
 #pragma unroll
     for (int k = 0; k < kMaxElem; ++k) {
@@ -272,9 +344,20 @@ __global__ __launch_bounds__(kThreads) void k_xvpsi(WParams w, v2<T>* __restrict
     __syncthreads();
     fft_any<T, N, -1>(buf, ny, n1, w.p1, tw);
     const size_t bbase = ((size_t)pb * w.n3 + z) * w.ncols;
-    for (int e = threadIdx.x; e < w.ncols * ny; e += kThreads) {
-      const int c = e / ny, y = e - c * ny;
-      B[(bbase + c) * w.n2 + y0 + y] = buf[y * n1 + w.col_gx[c]];
+    if (kPre) {
+#pragma unroll
+      for (int k = 0; k < kMaxCol; ++k) {
+        const int e = threadIdx.x + k * kThreads;
+        if (e < m) {
+          const int c = e / ny, y = e - c * ny;
+          B[(bbase + c) * w.n2 + y0 + y] = buf[dst[k]];
+        }
+      }
+    } else {
+      for (int e = threadIdx.x; e < m; e += kThreads) {
+        const int c = e / ny, y = e - c * ny;
+        B[(bbase + c) * w.n2 + y0 + y] = buf[y * n1 + w.col_gx[c]];
+      }
     }
     __syncthreads();
   }
@@ -422,6 +505,7 @@ static Shapes shapes(mc_ctx* ctx) {
   s.lo = lines_for<T>(ctx->plan3) / 2;
   s.tz = lines_for<T>(ctx->plan2);
   s.ty = std::min(lines_for<T>(ctx->plan1), (kMaxElem * kThreads) / ctx->n1);
+  s.ty = std::min(s.ty, (kMaxCol * kThreads) / std::max(ctx->ncols, 1));
   MC_CHECK(s.ty >= 1, MC_ERR_UNSUPPORTED, "nr1 too large for the x pass");
   s.gx = (ctx->n_own + s.lo - 1) / s.lo;
   return s;

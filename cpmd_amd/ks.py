@@ -1,0 +1,88 @@
+"""Kohn-Sham energy, electronic gradient (H psi) and ionic forces on the GPU,
+composed from the library's entry points in the order of CPMD's forces driver
+for the Gamma, closed-shell, LDA, norm-conserving path
+(src/forces_driver.mod.F90:175-384):
+
+    phfac -> rnlsm1 (+ rnlsm2) -> rhoofr -> vofrho (+ potfor) -> Ewald
+          -> vpsi + fnonloc (c2 = -f H c) -> rnlfor
+
+E_KS = ekin + ehep + epseu + exc + esr - eself + enl (ener_com%etot).  The
+wavefunctions stay on the device; only energies, fnl and forces come back.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import ops
+
+
+class KSDevice:
+    """One system on one GPU: layout, projectors, density set, local PP, ions."""
+
+    def __init__(self, sysm, raggio: float = 1.2, iesr: int = 2):
+        from . import system as S
+        gs = sysm.gset
+        if gs.nhg == gs.ngw:
+            raise ValueError("KSDevice needs the density G set (make_system(..., with_density=True))")
+        self.sysm = sysm
+        self.iesr = iesr
+        ctx = ops.Context(sysm.cell, gs)
+        ngw = gs.ngw
+        na, nhxs, ngh, lang, hmat, twnl, _ = S.projector_set(sysm, gs.miller[:ngw], gs.hg[:ngw], with_eigr=False)
+        ctx.set_projectors(na, nhxs, ngh, lang, hmat, twnl)
+        ctx.set_gk((gs.miller[:ngw] @ sysm.cell.bvec).T)
+        zv, rg, vps, rhops = S.local_pp_set(sysm, gs.hg, raggio=raggio)
+        nh = np.array([n // 2 + 1 for n in sysm.cell.nr])
+        ctx.set_density_gset(gs.miller + nh, gs.hg, gs.miller @ sysm.cell.bvec)
+        ctx.set_local_pp(na, zv, rg, vps, rhops)
+        self.ctx = ctx
+        self.hmat = np.asarray(hmat, dtype=np.float64).reshape(nhxs, nhxs, -1, order="F")
+        self.na = np.asarray(na)
+        self.nhxs = nhxs
+        self.ngh = np.asarray(ngh)
+        self.nnr1 = ctx.nnr1
+        self.d_rho = ops.DeviceArray(8 * self.nnr1)
+        self.d_v = ops.DeviceArray(8 * self.nnr1)
+        self._tau = None
+
+    def set_ions(self, tau: np.ndarray):
+        self.ctx.set_ions(tau, self.sysm.cell.avec)
+        self.ctx.phfac()
+        self._tau = np.array(tau, dtype=np.float64)
+
+    def _enl(self, fnl: np.ndarray, f: np.ndarray) -> float:
+        """sum_i f_i sum_isa sum_{iv,jv} h(iv,jv) fnl(isa,iv,i) fnl(isa,jv,i)."""
+        e = 0.0
+        isa0 = 0
+        for is_, na in enumerate(self.na):
+            h = self.hmat[:, :, is_]
+            blk = fnl[isa0:isa0 + na]                                   # (na, nhxs, n)
+            e += float(np.einsum("i,aji,jk,aki->", f, blk, h, blk))
+            isa0 += na
+        return e
+
+    def energy(self, d_c0: ops.DeviceArray, f: np.ndarray, tau: np.ndarray | None = None,
+               gradient: ops.DeviceArray | None = None, forces: bool = False) -> dict:
+        """E_KS terms of the device coefficients d_c0; with gradient=d_c2,
+        d_c2 <- -f H c; with forces, 'fion' (3, nat)."""
+        ctx = self.ctx
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        n = len(f)
+        if tau is not None and (self._tau is None or not np.array_equal(tau, self._tau)):
+            self.set_ions(tau)
+        fnl = ctx.rnlsm1_dev(d_c0, n, want_fnl=True)
+        ekin, _, _ = ctx.rhoofr_dev(d_c0, f, n, self.d_rho)
+        en, floc = ctx.vofrho_dev(self.d_rho, self.d_v, forces=forces)
+        esr, eself, fesr = ctx.ewald(self.iesr, forces=forces)
+        out = dict(ekin=ekin, ehep=en["ehep"], epseu=en["epseu"], exc=en["exc"], esr=esr, eself=eself,
+                   enl=self._enl(fnl, f))
+        out["etot"] = (out["ekin"] + out["ehep"] + out["epseu"] + out["exc"] + out["esr"] - out["eself"]
+                       + out["enl"])
+        if gradient is not None:
+            ctx.zero_wfn(gradient, n)
+            ctx.vpsi_dev(d_c0, gradient, f, self.d_v, n)
+            ctx.fnonloc_dev(gradient, f, n)
+        if forces:
+            ctx.rnlsm2_dev(d_c0, n)
+            out["fion"] = ctx.rnlfor(f, fion=floc + fesr)
+        return out

@@ -134,6 +134,10 @@ class Context:
     def axpy_wfn(self, alpha: float, x: DeviceArray, y: DeviceArray, nstate: int):
         check(lib().mc_wfn_axpy(self.h, alpha, x.ptr, y.ptr, nstate))
 
+    def axpby_wfn(self, alpha: float, x: DeviceArray, beta: float, y: DeviceArray, nstate: int):
+        """y = alpha x + beta y."""
+        check(lib().mc_wfn_axpby(self.h, alpha, x.ptr, beta, y.ptr, nstate))
+
     def dotp_wfn(self, a: DeviceArray, b: DeviceArray, nstate: int) -> np.ndarray:
         out = np.zeros(nstate)
         check(lib().mc_wfn_dotp(self.h, a.ptr, b.ptr, nstate, _p(out)))
@@ -363,6 +367,10 @@ class Context:
         f = np.zeros((3, self.nat_ions), order="F") if forces else None
         check(lib().mc_vofrho_dev(self.h, rho.ptr, v.ptr, _p(e), _p(f) if forces else None))
         return dict(zip(self.VOFRHO_KEYS, e)), f
+
+    def phfac(self):
+        """Projector structure factors of the ions of set_ions, built on the device."""
+        check(lib().mc_phfac(self.h))
 
     def ewald(self, iesr: int = 2, forces: bool = True):
         """(esr, eself, fion (3, nat) or None)."""

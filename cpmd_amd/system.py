@@ -322,11 +322,12 @@ def structure_factors(sys: SyntheticSystem, miller: np.ndarray) -> np.ndarray:
     return np.asfortranarray(np.exp(1j * phase))
 
 
-def projector_set(sys: SyntheticSystem, miller: np.ndarray, hg: np.ndarray):
+def projector_set(sys: SyntheticSystem, miller: np.ndarray, hg: np.ndarray, with_eigr: bool = True):
     """The one-species, one-s-projector Goedecker set of the synthetic systems as
-    the projector arguments of the library: (na, nhxs, ngh, lang, hmat, twnl, eigr)."""
+    the projector arguments of the library: (na, nhxs, ngh, lang, hmat, twnl, eigr);
+    eigr is None when with_eigr is False (the device builds it, mc_phfac)."""
     twnl = gth_projector_g(sys, hg)[:, None, None]            # (ngw, nhxs=1, nsp=1)
-    eigr = structure_factors(sys, miller)
+    eigr = structure_factors(sys, miller) if with_eigr else None
     return (np.array([sys.nat], np.int32), 1, np.array([1], np.int32), np.zeros((1, 1), np.int32),
             np.full((1, 1, 1), sys.pp.h11), np.asfortranarray(twnl), eigr)
 

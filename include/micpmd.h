@@ -120,6 +120,10 @@ int mc_wfn_zero(mc_ctx* ctx, void* c_dev, int nstate);
 /* y += alpha x over nstate device wavefunctions (the steepest-descent update
  * c0 <- c0 + dt c2 of updwf, src/updwf_utils.mod.F90:116-128). */
 int mc_wfn_axpy(mc_ctx* ctx, double alpha, const void* x_dev, void* y_dev, int nstate);
+/* y = alpha x + beta y (beta = 0: y is only written) over nstate device
+ * wavefunctions (the Verlet position / velocity updates of posupa and velupa,
+ * src/posupa_utils.mod.F90:44-187, src/velupa_utils.mod.F90:25-72). */
+int mc_wfn_axpby(mc_ctx* ctx, double alpha, const void* x_dev, double beta, void* y_dev, int nstate);
 
 /* out[i] = dotp(a_i, b_i), the Gamma-point inner product of state i
  * (src/dotp_utils.mod.F90:58-101); with b = c2 after vpsi + fnonloc,
@@ -278,6 +282,11 @@ int mc_set_local_pp(mc_ctx* ctx, int nsp, const int* na, const double* zv, const
  * flattened by species) and the cell ht(3, 3) with ht(i, :) = lattice vector i
  * (metr_com%ht); called after every ionic move. */
 int mc_set_ions(mc_ctx* ctx, const double* tau, const double* ht);
+/* Projector structure factors eigr(G, isa) = e^{-iG.R} of the ions of
+ * mc_set_ions, built on the device (in place of mc_set_eigr after an ionic
+ * move).  Requires mc_set_projectors with the same atoms.
+ * Replaces the eigr part of phfac(tau0), src/phfac_utils.mod.F90:45-261. */
+int mc_phfac(mc_ctx* ctx);
 /* vofrhoa + vofrhob for a closed-shell density with the Pade LDA (CPMD's
  * FUNCTIONAL LDA): rho(r) (nnr1) -> v(r) (nnr1) = V_H + V_loc (G != 0) + V_xc;
  * energies[7] = ehep, ehii (ei Omega), ehee, epseu, vploc, exc, vxc

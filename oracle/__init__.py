@@ -335,3 +335,53 @@ def vofrho(L: Layout, inyh, hg, gk, na, vps, rhops, frac, rho, geq0=True, forces
                    _p(na, _ip), _p(vps), _p(rhops), _p(frac), _p(rho), _p(v), _p(out),
                    _p(fion) if forces else None, _p(rhog))
     return v, dict(zip(VOFRHO_KEYS, out)), fion, rhog
+
+
+# --------------------------------------------------------------------------
+# Kohn-Sham energy, gradient and forces composed from the routines above
+# (the Gamma, closed-shell, LDA path of CPMD's forces driver,
+# src/forces_driver.mod.F90:175-384)
+# --------------------------------------------------------------------------
+class KSSystem:
+    """Everything the KS energy needs besides C: layout, density set, local and
+    nonlocal pseudopotential data and the ions."""
+
+    def __init__(self, L, inyh, hg, gk, na, zv, raggio, vps, rhops, avec, bvec, tpiba, miller_w, nl_args,
+                 iesr=2):
+        self.L = L
+        self.inyh, self.hg, self.gk = inyh, hg, gk
+        self.na, self.zv, self.raggio, self.vps, self.rhops = na, zv, raggio, vps, rhops
+        self.avec = np.asarray(avec, dtype=np.float64)
+        self.miller_w = miller_w          # (ngw, 3) wavefunction G set, for eigr
+        self.gkw = np.asfortranarray((miller_w.astype(np.float64) @ np.asarray(bvec)).T)   # (3, ngw), 2 pi / alat
+        self.tpiba = float(tpiba)
+        self.nl = nl_args                 # dict(nhxs, ngh, lang, hmat, twnl)
+        self.iesr = iesr
+
+    def projectors(self, tau):
+        frac = tau @ np.linalg.inv(self.avec)
+        eigr = np.exp(-2j * np.pi * (self.miller_w.astype(np.float64) @ frac.T))
+        return Projectors(na=self.na, eigr=eigr, **self.nl)
+
+
+def ks_energy(K: KSSystem, c, f, tau, forces=False, gradient=False):
+    """E_KS = ekin + ehep + epseu + exc + esr - eself + enl (ener_com%etot of
+    CPMD for this path).  gradient: c2 = -f H c (vpsi + fnonloc); forces: fion
+    (3, nat) = potfor + Ewald + rnlfor."""
+    frac = tau @ np.linalg.inv(K.avec)
+    rho, ekin, _, _ = rhoofr(K.L, c, f)
+    v, en, floc, _ = vofrho(K.L, K.inyh, K.hg, K.gk, K.na, K.vps, K.rhops, frac, rho, forces=forces)
+    esr, eself, fesr = ewald(K.na, K.zv, K.raggio, tau, K.avec, iesr=K.iesr, forces=forces)
+    P = K.projectors(tau)
+    fnl = rnlsm1(P, c)
+    e_nl = enl(P, fnl, f)
+    out = {"ekin": ekin, "ehep": en["ehep"], "epseu": en["epseu"], "exc": en["exc"], "esr": esr, "eself": eself,
+           "enl": e_nl, "v": v, "rho": rho}
+    out["etot"] = ekin + en["ehep"] + en["epseu"] + en["exc"] + esr - eself + e_nl
+    if gradient:
+        c2 = vpsi(K.L, c, np.zeros_like(c), f, v)
+        out["c2"] = fnonloc(P, c2, f, fnl)
+    if forces:
+        dfnl = rnlsm2(P, c, K.gkw, K.tpiba)
+        out["fion"] = floc + fesr + rnlfor(P, fnl, dfnl, f)
+    return out

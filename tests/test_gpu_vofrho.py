@@ -45,8 +45,9 @@ def test_vofrho_parity(gpu, cfg):
     v, en, f = ctx.vofrho(rho)
     ov, oen, of, _ = oracle.vofrho(L, inyh, gs.hg, gk, [nat], vps, rhops, frac, rho)
     assert rel(v, ov) < 1e-12
+    # sums over up to 5e5 G vectors / 2e6 mesh points in a different order
     for k in ("ehep", "ehii", "ehee", "epseu", "exc", "vxc"):
-        assert en[k] == pytest.approx(oen[k], rel=1e-12), k
+        assert en[k] == pytest.approx(oen[k], rel=1e-11), k
     assert en["vploc"] == pytest.approx(oen["vploc"], rel=1e-12, abs=1e-15)
     assert rel(f, of) < 1e-10
     # padding of v(r) is zero and the device-resident entry point agrees

@@ -14,7 +14,11 @@ oracle's loops:
   (central differences along zero-mean density changes);
 * potfor + the Ewald forces are -dE/dR of E_H + E_loc + E_esr - E_self at
   fixed electron density (central differences);
-* the ion-ion part of E_H does not see the electrons (ei = ehep at rho_e = 0).
+* the ion-ion part of E_H does not see the electrons (ei = ehep at rho_e = 0);
+* the assembled E_KS = ekin + ehep + epseu + exc + esr - eself + enl has the
+  gradient -2 <d|c2> with c2 = vpsi + fnonloc (the H psi adjoint) along tangent
+  coefficient directions, and -dE_KS/dR = potfor + Ewald + rnlfor forces
+  (central differences, ionic step 1e-4 bohr).
 """
 import math
 
@@ -188,3 +192,62 @@ def test_ppener_ion_part_without_electrons():
     _, en0, _, _ = oracle.vofrho(L, inyh, gs.hg, gk, [3], vps, rhops, frac, np.zeros_like(rho), forces=False)
     assert en0["ehep"] == pytest.approx(en0["ehii"], rel=1e-13)
     assert en0["ehee"] == 0.0 and en0["epseu"] == 0.0
+
+
+# --------------------------------------------------------------------------- E_KS
+def ks_system(a=9.0, nat=3, ecut=25.0, nstate=6, seed=5):
+    """The KS functional's inputs for a seeded system (also used by the GPU tests)."""
+    sysm = S.make_system(a, nat, ecut, nstate=nstate, seed=seed, with_density=True)
+    gs = sysm.gset
+    L = oracle.Layout(sysm.cell, gs)
+    zv, rg, vps, rhops = S.local_pp_set(sysm, gs.hg)
+    nh = np.array([n // 2 + 1 for n in sysm.cell.nr])
+    ngw = gs.ngw
+    na, nhxs, ngh, lang, hmat, twnl, _ = S.projector_set(sysm, gs.miller[:ngw], gs.hg[:ngw])
+    K = oracle.KSSystem(L, gs.miller + nh, gs.hg, gs.miller @ sysm.cell.bvec, [nat], zv, rg, vps, rhops,
+                        sysm.cell.avec, sysm.cell.bvec, sysm.cell.tpiba, gs.miller[:ngw],
+                        dict(nhxs=nhxs, ngh=ngh, lang=lang, hmat=hmat, twnl=twnl))
+    c = S.initial_coefficients(sysm, seed=seed)
+    return sysm, K, c
+
+
+def tangent_directions(c, k, seed):
+    """k random directions d with <c_i|d_i> = 0 (Gamma metric), G=0 real."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(k):
+        d = rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape)
+        d[0] = d[0].real
+        for i in range(c.shape[1]):
+            d[:, i] -= c[:, i] * S.dotp_gamma(c[:, i:i + 1], d[:, i:i + 1])[0, 0]
+        d /= np.sqrt(np.sum(S.dotp_gamma(d, d).diagonal()))
+        out.append(np.asfortranarray(d))
+    return out
+
+
+def test_ks_gradient_is_h_psi():
+    """dE_KS along tangent directions = -2 sum_i <d_i|c2_i>, c2 = vpsi + fnonloc (= -f H c)."""
+    sysm, K, c = ks_system()
+    f = sysm.f
+    r = oracle.ks_energy(K, c, f, sysm.tau, gradient=True)
+    h = 1e-4
+    for d in tangent_directions(c, 3, seed=11):
+        ep = oracle.ks_energy(K, np.asfortranarray(c + h * d), f, sysm.tau)["etot"]
+        em = oracle.ks_energy(K, np.asfortranarray(c - h * d), f, sysm.tau)["etot"]
+        g = -2.0 * np.trace(S.dotp_gamma(d, r["c2"]))
+        assert (ep - em) / (2 * h) == pytest.approx(g, rel=1e-6)
+
+
+def test_ks_forces_are_position_derivatives():
+    """F_ion = potfor + Ewald + rnlfor = -dE_KS/dR at fixed C (plane waves: no Pulay term)."""
+    sysm, K, c = ks_system()
+    f = sysm.f
+    r = oracle.ks_energy(K, c, f, sysm.tau, forces=True)
+    h = 1e-4
+    for ia in range(3):
+        for k in range(3):
+            tp, tm = sysm.tau.copy(), sysm.tau.copy()
+            tp[ia, k] += h
+            tm[ia, k] -= h
+            fd = -(oracle.ks_energy(K, c, f, tp)["etot"] - oracle.ks_energy(K, c, f, tm)["etot"]) / (2 * h)
+            assert r["fion"][k, ia] == pytest.approx(fd, rel=1e-6, abs=1e-7)
