@@ -305,7 +305,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(const double* __restrict__ A
 void gemm_tn(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, int K, int m, int n, double alpha,
              bool g0corr, bool symmetric, double* out, int ldo, const ProjA* proj) {
   if (m <= 0 || n <= 0) return;
-  if (ctx->gemm_x3 && K >= kX3Min) {
+  const auto al16 = [](const double* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  if (ctx->gemm_x3 && K >= kX3Min && ldb % 2 == 0 && al16(B) && (proj || (lda % 2 == 0 && al16(A)))) {
     gemm_tn_x3(ctx, A, lda, B, ldb, K, m, n, alpha, g0corr, symmetric, out, ldo, proj);
     return;
   }

@@ -163,12 +163,30 @@ __global__ __launch_bounds__(256, 2) void k_x3_tn(const double* __restrict__ A, 
       }
     } else {
       const double* src = A + (size_t)acol * lda + kb + lq;
+      if (acol < m && kb + lq + 16 <= kend) {   // 16-byte vector loads (lda even, chunk 32-aligned)
 #pragma unroll
-      for (int u = 0; u < 16; ++u) ra[u] = (acol < m && kb + lq + u < kend) ? src[u] : 0.0;
+        for (int u = 0; u < 8; ++u) {
+          const double2 v = reinterpret_cast<const double2*>(src)[u];
+          ra[2 * u] = v.x;
+          ra[2 * u + 1] = v.y;
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) ra[u] = (acol < m && kb + lq + u < kend) ? src[u] : 0.0;
+      }
     }
     const double* src = B + (size_t)bcol * ldb + kb + lq;
+    if (bcol < n && kb + lq + 16 <= kend) {
 #pragma unroll
-    for (int u = 0; u < 16; ++u) rb[u] = (bcol < n && kb + lq + u < kend) ? src[u] : 0.0;
+      for (int u = 0; u < 8; ++u) {
+        const double2 v = reinterpret_cast<const double2*>(src)[u];
+        rb[2 * u] = v.x;
+        rb[2 * u + 1] = v.y;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) rb[u] = (bcol < n && kb + lq + u < kend) ? src[u] : 0.0;
+    }
   };
   if (kbeg < kend) fetch(kbeg);
   for (int kb = kbeg; kb < kend; kb += kK) {
@@ -223,6 +241,7 @@ __global__ __launch_bounds__(256, 2) void k_x3_nn(const double* __restrict__ A, 
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
   const int kmax = upper_g ? min(k, j0 + kT) : k;
+  const bool gvec = (ldgm % 2 == 0) && ((reinterpret_cast<uintptr_t>(G) & 15) == 0);
   // A slice (128 rows x 32 k): wave w loads k = 8w .. 8w+7 of rows i0+2l, i0+2l+1
   const int ar = 2 * l, ak = 8 * wave;
   // G slice (32 k x 128 columns): column gc, k = gq .. gq+15
@@ -239,8 +258,17 @@ __global__ __launch_bounds__(256, 2) void k_x3_nn(const double* __restrict__ A, 
     }
     const int col = j0 + gc;
     const double* src = G + (size_t)col * ldgm + kb + gq;
+    if (gvec && col < n && kb + gq + 16 <= kmax) {
 #pragma unroll
-    for (int u = 0; u < 16; ++u) rg[u] = (col < n && kb + gq + u < kmax) ? src[u] : 0.0;
+      for (int u = 0; u < 8; ++u) {
+        const double2 v = reinterpret_cast<const double2*>(src)[u];
+        rg[2 * u] = v.x;
+        rg[2 * u + 1] = v.y;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) rg[u] = (col < n && kb + gq + u < kmax) ? src[u] : 0.0;
+    }
   };
   if (kmax > 0) fetch(0);
   for (int kb = 0; kb < kmax; kb += kK) {
@@ -334,6 +362,9 @@ __global__ __launch_bounds__(256) void k_diag_tn(const double* __restrict__ A, i
 void gemm_tn_x3(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, int K, int m, int n, double alpha,
                 bool g0corr, bool symmetric, double* out, int ldo, const ProjA* proj) {
   if (m <= 0 || n <= 0) return;
+  MC_CHECK(ldb % 2 == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0 &&
+               (proj || (lda % 2 == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0)),
+           MC_ERR_INTERNAL, "split-bf16 gemm_tn needs even lda, ldb and 16-byte aligned operands");
   const int tm = (m + kT - 1) / kT, tn = (n + kT - 1) / kT;
   const int tiles = symmetric ? (tm * (tm + 1)) / 2 : tm * tn;
   // about 1024 busy workgroups, K chunks of at least 1024 rows

@@ -415,6 +415,7 @@ void rortog_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, i
     }
     MC_HIP(hipGetLastError());
     difgam = max_abs_diff(ctx, gamn, gam, (size_t)n * n);
+    MC_CHECK(std::isfinite(difgam), MC_ERR_NOCONV, "RORTHO: iteration diverged (non-finite gamma)");
     MC_HIP(hipMemcpyAsync(gam, gamn, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->stream));
     if (!(difgam > epsog * 0.5 && it <= maxit)) break;
   }
