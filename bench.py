@@ -174,12 +174,12 @@ def main():
                       + 2 * n * n * K            # rortog: <c0|cp>
                       + n * n * K                # rortog: <cp|cp> (symmetric)
                       + 2 * K * n * n)           # rortog: cp += c0 X
-        sym_flops = n * n * K
+        tn_gemm_flops = 4 * 2 * nat * n * K + 2 * n * n * K + n * n * K   # the A^T B products
         step_desc = ("phfac + rnlsm1 + rnlsm2 + rhoofr + vofrho(+potfor) + Ewald + vpsi + fnonloc + rnlfor "
                      "+ velocity Verlet (ions, electrons) + rortog")
     else:
         gemm_flops = (2 * nat * n * K + 2 * K * n * nat + 2 * n * n * K + 2 * K * n * n + n * n * K + 2 * K * n * n)
-        sym_flops = n * n * K
+        tn_gemm_flops = 2 * nat * n * K + 2 * n * n * K + n * n * K
         step_desc = "rnlsm1 + rhoofr + vofrho + vpsi + fnonloc + Lagrange projection + SD update + Loewdin"
 
     if args.step == "cp" and args.prep_steps > 0:
@@ -207,7 +207,8 @@ def main():
         ctx.set_precision("fp64")
         return ks.energy(buf["c0"], f, tau=tau, gradient=buf["c2"], forces=True)
 
-    modes = {"mixed": "mixed", "fp64": "fp64", "fp32fft": "fp32"}   # fp32fft: FP32 transforms, FP64 contractions
+    # fp32fft: FP32 transforms, FP64 contractions; bf16x3: every split product 3-term
+    modes = {"mixed": "mixed", "fp64": "fp64", "fp32fft": "fp32", "bf16x3": "mixed_bf16x3"}
 
     def run_path(path):
         ctx.set_precision(modes[path])
@@ -287,15 +288,18 @@ def main():
     fft_ms = sum(kernels[k]["total_ms_per_step"] for k in FFT_KIDS if k in kernels)
 
     def mfma_roofline(path, r):
-        """FP64 path: FP64 MFMA flops / 78.6 TF.  Mixed path: bf16 MFMA issue
-        (3 products per FP64-equivalent product, 4 on the symmetric overlap)
-        against the 2.5 PF dense bf16 peak; 'useful' counts each product once."""
+        """FP64 path: FP64 MFMA flops / 78.6 TF.  Mixed paths: bf16 MFMA issue
+        (6 products per FP64-equivalent product for the 3-part A^T B products,
+        3 for the 2-part ones) against the 2.5 PF dense bf16 peak; 'useful'
+        counts each product once."""
         ms = r["kernels"].get("gemm", {}).get("total_ms_per_step", float("nan"))
         useful = gemm_flops / (ms * 1e-3) / 1e12
-        if path == "mixed":
-            issued = 3 * gemm_flops + sym_flops
+        if path in ("mixed", "bf16x3"):
+            # 3-part A^T B products issue 6 bf16 products per FP64 product, 2-part 3
+            tn_flops = tn_gemm_flops if path == "mixed" else 0.0
+            issued = 3 * gemm_flops + 3 * tn_flops
             tf = issued / (ms * 1e-3) / 1e12
-            return {"bound": "mfma", "dtype": "bf16 (3-term split)", "flops_per_step": issued,
+            return {"bound": "mfma", "dtype": "bf16 split (3 or 6 products)", "flops_per_step": issued,
                     "gemm_ms_per_step": ms, "achieved": round(tf, 2), "peak": 2500.0, "unit": "TFLOP/s",
                     "frac": round(tf / 2500.0, 4), "useful_fp64_equiv_tflops": round(useful, 2)}
         return {"bound": "mfma", "dtype": "f64", "flops_per_step": gemm_flops, "gemm_ms_per_step": ms,

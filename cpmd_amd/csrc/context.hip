@@ -426,11 +426,12 @@ size_t mc_wfn_ld(const mc_ctx* ctx) { return ctx ? (size_t)ctx->ldg : 0; }
 int mc_set_precision(mc_ctx* ctx, int fft_precision) {
   MC_API_BEGIN
   MC_CHECK(ctx, MC_ERR_ARG, "null context");
-  MC_CHECK(fft_precision == MC_PREC_FP64 || fft_precision == MC_PREC_FP32 || fft_precision == MC_PREC_MIXED,
-           MC_ERR_ARG, "unknown precision mode");
+  MC_CHECK(fft_precision >= MC_PREC_FP64 && fft_precision <= MC_PREC_MIXED_BF16X3, MC_ERR_ARG,
+           "unknown precision mode");
   MC_HIP(hipStreamSynchronize(ctx->stream));
   ctx->fft_fp32 = fft_precision != MC_PREC_FP64;
-  ctx->gemm_x3 = fft_precision == MC_PREC_MIXED;
+  ctx->gemm_x3 = fft_precision == MC_PREC_MIXED || fft_precision == MC_PREC_MIXED_BF16X3;
+  ctx->x3_parts = fft_precision == MC_PREC_MIXED_BF16X3 ? 2 : 3;
   MC_API_END
 }
 

@@ -188,6 +188,7 @@ struct mc_ctx {
 
   bool fft_fp32 = false;   // rhoofr/vpsi transforms and workspaces in FP32
   bool gemm_x3 = false;    // tall wavefunction contractions as split-bf16 MFMA products
+  int x3_parts = 3;        // bf16 parts per operand of the split A^T B products (2 or 3)
   mc::DensitySet dens;     // vofrho / ewald data
 
   int batch_pairs_vpsi = 8;

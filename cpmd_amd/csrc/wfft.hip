@@ -435,6 +435,11 @@ __global__ __launch_bounds__(kThreads) void k_zfwd(WParams w, const v2<T>* __res
   }
 }
 
+__global__ void k_scale(double* __restrict__ x, size_t n, double v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] *= v;
+}
+
 __global__ void k_fill(double* __restrict__ x, size_t n, double v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     x[i] = v;
@@ -572,9 +577,19 @@ static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, in
     if (is2 < nstate && f_host[is2] != 0.0) { sg += f_host[is2] * hs[4 * p + 2]; sk += f_host[is2] * hs[4 * p + 3]; }
   }
   for (int b = 0; b < nblk_sum; ++b) sr += hs[(size_t)npairs * 4 + b];
+  double cr = sr * ctx->omega / ((double)ctx->n1 * ctx->n2 * ctx->n3);
+  if (std::is_same<T, float>::value && cr > 0.0 && sg > 0.0) {
+    // FP32 transforms lose ~1e-7 of the charge (a systematic rounding of the
+    // norm, csumr - csumg ~ -1.6e-4 at 1024 electrons); rescale rho so that
+    // its integral is the FP64 G-space charge csumg (the invariant CPMD's
+    // csumr/csumg check enforces, rhoofr:625-635)
+    const double scale = sg / cr;
+    LAUNCH(MC_KID_MISC, k_scale, dim3(1024), dim3(256), 0, rho, nnr1, scale);
+    cr *= scale;
+  }
   if (ekin) *ekin = sk * ctx->tpiba2;
   if (csumg) *csumg = sg;
-  if (csumr) *csumr = sr * ctx->omega / ((double)ctx->n1 * ctx->n2 * ctx->n3);
+  if (csumr) *csumr = cr;
 }
 
 template <typename T>

@@ -117,9 +117,10 @@ class Context:
         check(lib().mc_ctx_set_batch(self.h, pairs_rhoofr, pairs_vpsi))
 
     def set_precision(self, mode: str):
-        """'fp64' (default), 'fp32' (FP32 rhoofr/vpsi transforms) or 'mixed'
-        (FP32 transforms and split-bf16 MFMA wavefunction contractions)."""
-        check(lib().mc_set_precision(self.h, {"fp64": 0, "fp32": 1, "mixed": 2}[mode]))
+        """'fp64' (default), 'fp32' (FP32 rhoofr/vpsi transforms), 'mixed' (FP32
+        transforms and split-bf16 MFMA wavefunction contractions) or
+        'mixed_bf16x3' (mixed with every split product 3-term)."""
+        check(lib().mc_set_precision(self.h, {"fp64": 0, "fp32": 1, "mixed": 2, "mixed_bf16x3": 3}[mode]))
 
     def synchronize(self):
         check(lib().mc_ctx_synchronize(self.h))

@@ -97,13 +97,18 @@ size_t mc_wfn_ld(const mc_ctx* ctx);
  *   MC_PREC_FP32   rhoofr/vpsi transforms and stick/column workspaces in FP32;
  *                  wavefunctions, V(r), rho(r), the Gamma decode, the kinetic
  *                  term, c2 and all charge/energy sums stay FP64;
- *   MC_PREC_MIXED  MC_PREC_FP32 plus the tall wavefunction contractions (ovlap,
- *                  rnlsm1, fnonloc, rotate, the C X products of rgs/lowdin) as
- *                  3-term bf16 split MFMA products with FP32 accumulation per K
- *                  chunk and FP64 chunk sums; n x n algebra (Cholesky, inverse,
- *                  Newton-Schulz) stays FP64.  lowdin/rgs apply C X as
- *                  C + C (X - I). */
-enum { MC_PREC_FP64 = 0, MC_PREC_FP32 = 1, MC_PREC_MIXED = 2 };
+ *   MC_PREC_MIXED  MC_PREC_FP32 plus the tall wavefunction contractions as
+ *                  bf16 split MFMA products with FP32 accumulation per K chunk
+ *                  and FP64 chunk sums: the A^T B products (ovlap, rnlsm1,
+ *                  rnlsm2) split each operand in three bf16 parts (six
+ *                  products, FP32-level accuracy), the A X products (rotate,
+ *                  fnonloc, the C X of rgs/lowdin) in two (three products,
+ *                  hi hi + hi lo + lo hi); the diagonal of square overlaps is
+ *                  summed in FP64; n x n algebra (Cholesky, inverse,
+ *                  Newton-Schulz) stays FP64; lowdin/rgs apply C X as
+ *                  C + C (X - I);
+ *   MC_PREC_MIXED_BF16X3  as MC_PREC_MIXED with every split product 3-term. */
+enum { MC_PREC_FP64 = 0, MC_PREC_FP32 = 1, MC_PREC_MIXED = 2, MC_PREC_MIXED_BF16X3 = 3 };
 int mc_set_precision(mc_ctx* ctx, int fft_precision);
 
 /* Device memory helpers (hipMalloc / hipFree / hipMemcpy). */
@@ -142,7 +147,10 @@ int mc_wfn_download(mc_ctx* ctx, const void* c_dev, int nstate, double* c_host, 
  * src/kin_energy_utils.mod.F90:28-110 and build_density_sum
  * src/density_utils.mod.F90:61-80.  Pairs with f1 = f2 = 0 are skipped
  * (rhoofr:313-316).  The charge-sum check (rhoofr:625-635) is left to the
- * caller, which receives csumg and csumr. */
+ * caller, which receives csumg and csumr.  With FP32 transforms
+ * (MC_PREC_FP32 / MC_PREC_MIXED) rhoe is rescaled by csumg / csumr (removing
+ * the ~1e-7 systematic charge loss of the FP32 rounding), and csumr is that of
+ * the returned rhoe. */
 int mc_rhoofr(mc_ctx* ctx, const double* c0, int ldc, const double* f, int nstate,
               double* rhoe, double* ekin, double* csumg, double* csumr);
 int mc_rhoofr_dev(mc_ctx* ctx, const void* c0_dev, const double* f, int nstate,

@@ -150,6 +150,7 @@ def test_fp32_transforms(gpu, name, nstate):
     assert ek_g == pytest.approx(ek_o, rel=1e-12)        # E_kin is summed from C0 in FP64
     assert cg_g == pytest.approx(cg_o, rel=1e-12)
     assert cr_g == pytest.approx(cr_o, rel=2e-6)
+    assert cr_g == pytest.approx(cg_g, rel=1e-12)        # FP32 density rescaled to the FP64 charge
     v = S.real_space_potential(sysm)
     c2 = ctx.vpsi(c, np.zeros_like(c), f, v)
     c2_o = oracle.vpsi(L, c, np.zeros_like(c), f, v)
