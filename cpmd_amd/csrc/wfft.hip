@@ -364,6 +364,275 @@ __global__ __launch_bounds__(kThreads) void k_xvpsi(WParams w, v2<T>* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// Fused x pass for the compile-time lengths N = A*B (FastShape).  One
+// workgroup holds L consecutive y lines of one z plane; lane t works on line
+// l = t % L (so 16 lanes read or write 16 consecutive y, one 128-byte run of
+// a column of B) and on group g = t / L of the line:
+//   phase 1 (g = n2 < B):  x[B n1 + n2], n1 < A, straight from B into
+//            registers, DFT_A, twiddle w^{n2 k1}, to LDS buffer 1
+//   phase 2 (g = k1 < A):  DFT_B of buffer 1 row k1 -> psi(x = k1 + A k2)
+//            in registers (|psi|^2 accumulation or the V(r) multiply here)
+//   phase 3 (g = k1):      forward DFT_B over k2, twiddle w^{k1 k1'}, to LDS
+//            buffer 2 at A k1' + k1
+//   phase 4 (g = k1' < B): DFT_A of buffer 2 row k1' -> psi(x = k1' + B k2'),
+//            straight from registers to the sphere columns of B.
+// Each phase is the four-step split of fft_fast (the forward one with A and B
+// exchanged, so that phase 2's registers feed phase 3 directly).  With two
+// buffers a pair takes two barriers (phase 1 -> 2, phase 3 -> 4): the next
+// pair's phase 1 writes buffer 1 after every lane has passed the 3 -> 4
+// barrier, hence after its phase 2 reads.  NB = 1 shares one buffer and adds
+// the two barriers that sharing needs (the FP64 path, whose buffers are twice
+// as large, and any length whose two buffers would pass 64 KiB).  LDS element
+// (row group q, row r of the group, line l) sits at row index * L + l + 16 q:
+// the 16-element shift per group puts the two 32-lane halves of a wave, which
+// hold consecutive groups in phases 2 and 4, on disjoint 8-byte bank groups.
+// ---------------------------------------------------------------------------
+template <int N>
+struct XTile {
+  static constexpr int A = FastShape<N>::A, B = FastShape<N>::B;
+  static constexpr int G = A > B ? A : B;
+  static constexpr int L = kThreads / G;        // lines per workgroup
+  static constexpr int kBuf = N * L + 16 * G;   // elements of one LDS buffer
+};
+
+// LDS bytes of the fused x kernels: tw, NB buffers, x -> column map
+template <typename T, int N>
+constexpr size_t xlds(int nb) {
+  return sizeof(v2<T>) * (size_t)(N + nb * XTile<N>::kBuf) + sizeof(int) * N;
+}
+template <typename T, int N>
+constexpr int xbufs() { return (sizeof(T) == 4 && xlds<T, N>(2) <= 65536) ? 2 : 1; }
+template <typename T, int N>
+constexpr size_t xfused_lds() { return xlds<T, N>(xbufs<T, N>()); }
+// lengths whose fused kernels fit the 64 KiB of LDS of one workgroup
+template <typename T, int N>
+constexpr bool xfused_ok() {
+  if constexpr (N > 0) return xfused_lds<T, N>() <= 65536;
+  else return false;
+}
+
+template <typename T, int N>
+struct XFused {
+  using V = v2<T>;
+  static constexpr int A = XTile<N>::A, B = XTile<N>::B, L = XTile<N>::L;
+  static constexpr int NB = xbufs<T, N>();
+  V* tw;
+  V* buf[2];
+  int* xcol;
+  int l, g, ny;
+
+  __device__ __forceinline__ void init(const WParams& w, int y0) {
+    tw = lds_base<T>();
+    buf[0] = tw + N;
+    buf[1] = NB == 2 ? buf[0] + XTile<N>::kBuf : buf[0];
+    xcol = reinterpret_cast<int*>(buf[0] + NB * XTile<N>::kBuf);
+    l = threadIdx.x % L;
+    g = threadIdx.x / L;
+    ny = min(L, w.n2 - y0);
+    load_tw<T>(tw, w.tw1, N);
+    for (int x = threadIdx.x; x < N; x += kThreads) xcol[x] = -1;
+    __syncthreads();
+    for (int c = threadIdx.x; c < w.ncols; c += kThreads) xcol[w.col_gx[c]] = c;
+  }
+  // Byte offsets, within one pair's (z) slab of B, of the elements
+  // x = B k + g of this lane's line (the positions of phase 1's inputs and
+  // phase 4's outputs alike), kNoCol off the sphere columns: fixed for the
+  // whole kernel.  32-bit unsigned bytes, so that the accesses take the
+  // scalar-base + vector-offset form (a slab is far below 4 GiB).
+  static constexpr unsigned kNoCol = 0xffffffffu;
+  __device__ __forceinline__ void offsets(const WParams& w, int y0, unsigned (&off)[A]) const {
+#pragma unroll
+    for (int k = 0; k < A; ++k) {
+      const int c = (g < B && l < ny) ? xcol[B * k + g] : -1;
+      off[k] = c >= 0 ? (unsigned)((c * w.n2 + y0 + l) * sizeof(V)) : kNoCol;
+    }
+  }
+  // phase 1 loads of one pair (slab base Bs): column elements, zero elsewhere
+  __device__ __forceinline__ void fetch(const V* __restrict__ Bs, const unsigned (&off)[A], V (&r)[A]) const {
+#pragma unroll
+    for (int k = 0; k < A; ++k) r[k] = off[k] != kNoCol ? at(Bs, off[k]) : cmk<V>((T)0, (T)0);
+  }
+  __device__ __forceinline__ static const V& at(const V* p, unsigned b) {
+    return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(p) + b);
+  }
+  __device__ __forceinline__ static V& at(V* p, unsigned b) {
+    return *reinterpret_cast<V*>(reinterpret_cast<char*>(p) + b);
+  }
+  template <int DIR>
+  __device__ __forceinline__ void phase1(V (&v)[A], V* bf) const {
+    if (g < B) {
+      dftn<A, DIR>(v);
+#pragma unroll
+      for (int k1 = 1; k1 < A; ++k1) v[k1] = twmul<DIR>(v[k1], tw[g * k1]);
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) bf[(B * k1 + g) * L + l + 16 * k1] = v[k1];
+    }
+  }
+  // u[k2] = psi(x = g + A k2), lanes g < A
+  template <int DIR>
+  __device__ __forceinline__ void phase2(V (&u)[B], const V* bf) const {
+    if (g < A) {
+#pragma unroll
+      for (int m = 0; m < B; ++m) u[m] = bf[(B * g + m) * L + l + 16 * g];
+      dftn<B, DIR>(u);
+    }
+  }
+  template <int DIR>
+  __device__ __forceinline__ void phase3(V (&u)[B], V* bf) const {
+    if (g < A) {
+      dftn<B, DIR>(u);
+#pragma unroll
+      for (int k1 = 1; k1 < B; ++k1) u[k1] = twmul<DIR>(u[k1], tw[g * k1]);
+#pragma unroll
+      for (int k1 = 0; k1 < B; ++k1) bf[(A * k1 + g) * L + l + 16 * k1] = u[k1];
+    }
+  }
+  // v[k2] = psi(x = g + B k2), lanes g < B
+  template <int DIR>
+  __device__ __forceinline__ void phase4(V (&v)[A], const V* bf) const {
+    if (g < B) {
+#pragma unroll
+      for (int m = 0; m < A; ++m) v[m] = bf[(A * g + m) * L + l + 16 * g];
+      dftn<A, DIR>(v);
+    }
+  }
+};
+
+// xrho, fused form (same contract as k_xrho).  With two buffers the pairs
+// alternate between them, one barrier per pair.
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_xrho_f(WParams w, const v2<T>* __restrict__ Bg, int npairs, int pair0,
+                                                     int nstate, const double* __restrict__ f,
+                                                     double* __restrict__ rho) {
+  using V = v2<T>;
+  using X = XFused<T, N>;
+  constexpr int A = X::A, B = X::B, L = X::L;
+  X s;
+  const int y0 = blockIdx.x * L, z = blockIdx.y;
+  s.init(w, y0);
+  __syncthreads();
+  auto occupied = [&](int p) {
+    const int i1 = 2 * (pair0 + p), i2 = i1 + 1;
+    return f[i1] != 0.0 || (i2 < nstate && f[i2] != 0.0);
+  };
+  const double rom = 1.0 / w.omega;
+  double acc[B];
+#pragma unroll
+  for (int k = 0; k < B; ++k) acc[k] = 0.0;
+  unsigned off[A];
+  s.offsets(w, y0, off);
+  const size_t slab = (size_t)w.ncols * w.n2;   // elements of one (pair, z) slab of B
+  auto base = [&](int p) { return Bg + ((size_t)p * w.n3 + z) * slab; };
+  constexpr bool kPre = sizeof(T) == 4;
+  V r[A];
+  int pb = 0;
+  while (pb < npairs && !occupied(pb)) ++pb;
+  if (kPre && pb < npairs) s.fetch(base(pb), off, r);
+  int par = 0;
+  while (pb < npairs) {   // uniform over the workgroup
+    int nx = pb + 1;
+    while (nx < npairs && !occupied(nx)) ++nx;
+    const int is1 = 2 * (pair0 + pb), is2 = is1 + 1;
+    const double c3 = f[is1] * rom, c4 = (is2 < nstate ? f[is2] : 0.0) * rom;
+    V v[A];
+    if (kPre) {
+#pragma unroll
+      for (int k = 0; k < A; ++k) v[k] = r[k];
+      if (nx < npairs) s.fetch(base(nx), off, r);
+    } else {
+      s.fetch(base(pb), off, v);
+    }
+    V* bf = par ? s.buf[1] : s.buf[0];
+    s.template phase1<1>(v, bf);
+    __syncthreads();
+    V u[B];
+    s.template phase2<1>(u, bf);
+    if (s.g < A) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        const double vx = (double)u[k].x, vy = (double)u[k].y;
+        acc[k] = acc[k] + c3 * vx * vx + c4 * vy * vy;
+      }
+    }
+    if (X::NB == 1) __syncthreads();   // the next pair rewrites the only buffer
+    par ^= (X::NB - 1);
+    pb = nx;
+  }
+  // transpose the accumulators through LDS (rows of N + 1 doubles per line)
+  // so that the read-modify-write of rho runs along x
+  __syncthreads();
+  double* st = reinterpret_cast<double*>(s.buf[0]);
+  constexpr int LD = N + 1;
+  static_assert(sizeof(double) * L * LD <= sizeof(V) * X::NB * XTile<N>::kBuf, "rho staging exceeds the buffers");
+  if (s.g < A) {
+#pragma unroll
+    for (int k = 0; k < B; ++k) st[s.l * LD + s.g + A * k] = acc[k];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < s.ny * N; e += kThreads) {
+    const int y = e / N, x = e - y * N;
+    const size_t idx = (size_t)x + (size_t)w.kr1 * ((size_t)(y0 + y) + (size_t)w.kr2 * z);
+    rho[idx] += st[y * LD + x];
+  }
+}
+
+// xvpsi, fused form (same contract as k_xvpsi)
+template <typename T, int N>
+__global__ __launch_bounds__(kThreads) void k_xvpsi_f(WParams w, v2<T>* __restrict__ Bg, int npairs,
+                                                      const double* __restrict__ vpot) {
+  using V = v2<T>;
+  using X = XFused<T, N>;
+  constexpr int A = X::A, B = X::B, L = X::L;
+  X s;
+  const int y0 = blockIdx.x * L, z = blockIdx.y;
+  s.init(w, y0);
+  // V(r) of this tile, staged x-major (row stride L + 1) through buffer 1 so
+  // that lane (l, g) picks up V(x = g + A k2, y0 + l) for its phase-2 values
+  T* vst = reinterpret_cast<T*>(s.buf[1]);
+  for (int e = threadIdx.x; e < L * N; e += kThreads) {
+    const int y = e / N, x = e - y * N;
+    vst[x * (L + 1) + y] =
+        y < s.ny ? (T)vpot[(size_t)x + (size_t)w.kr1 * ((size_t)(y0 + y) + (size_t)w.kr2 * z)] : (T)0;
+  }
+  __syncthreads();
+  T vr[B];
+#pragma unroll
+  for (int k = 0; k < B; ++k) vr[k] = s.g < A ? vst[(s.g + A * k) * (L + 1) + s.l] : (T)0;
+  if (X::NB == 1) __syncthreads();
+  unsigned off[A];
+  s.offsets(w, y0, off);
+  const size_t slab = (size_t)w.ncols * w.n2;   // elements of one (pair, z) slab of B
+  constexpr bool kPre = sizeof(T) == 4;
+  V r[A];
+  if (kPre && npairs > 0) s.fetch(Bg + (size_t)z * slab, off, r);
+  for (int pb = 0; pb < npairs; ++pb) {
+    V* Bs = Bg + ((size_t)pb * w.n3 + z) * slab;
+    V v[A];
+    if (kPre) {
+#pragma unroll
+      for (int k = 0; k < A; ++k) v[k] = r[k];
+      if (pb + 1 < npairs) s.fetch(Bs + (size_t)w.n3 * slab, off, r);
+    } else {
+      s.fetch(Bs, off, v);
+    }
+    s.template phase1<1>(v, s.buf[0]);
+    __syncthreads();
+    V u[B];
+    s.template phase2<1>(u, s.buf[0]);
+#pragma unroll
+    for (int k = 0; k < B; ++k) u[k] = cscale(u[k], vr[k]);
+    if (X::NB == 1) __syncthreads();
+    s.template phase3<-1>(u, s.buf[1]);
+    __syncthreads();
+    s.template phase4<-1>(v, s.buf[1]);
+#pragma unroll
+    for (int k = 0; k < A; ++k)
+      if (off[k] != X::kNoCol) X::at(Bs, off[k]) = v[k];
+    if (X::NB == 1) __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // zfwd: forward z pass of owner sticks + mirrors, scale 1/(n1 n2 n3), Gamma
 // decode fp = psi(G) + psi(-G), fm = psi(G) - psi(-G) and (in FP64)
 //   c2(:,is1) += -fi   (tpiba2 hg c0(:,is1) + (Re fp, Im fm))
@@ -561,8 +830,14 @@ static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, in
              dim3(kThreads), shy, w, A, B, s.tz);
     });
     with_length(ctx->n1, [&](auto NN) {
-      LAUNCH(MC_KID_XRHO, (k_xrho<T, decltype(NN)::value>), dim3((ctx->n2 + s.ty - 1) / s.ty, ctx->n3),
-             dim3(kThreads), shx, w, B, np, pair0, nstate, ctx->w_f.as<double>(), rho, s.ty);
+      constexpr int N1 = decltype(NN)::value;
+      if constexpr (xfused_ok<T, N1>()) {
+        LAUNCH(MC_KID_XRHO, (k_xrho_f<T, N1>), dim3((ctx->n2 + XTile<N1>::L - 1) / XTile<N1>::L, ctx->n3),
+               dim3(kThreads), (xfused_lds<T, N1>()), w, B, np, pair0, nstate, ctx->w_f.as<double>(), rho);
+      } else {
+        LAUNCH(MC_KID_XRHO, (k_xrho<T, N1>), dim3((ctx->n2 + s.ty - 1) / s.ty, ctx->n3), dim3(kThreads), shx, w, B,
+               np, pair0, nstate, ctx->w_f.as<double>(), rho, s.ty);
+      }
     });
   }
   if (npairs > 0) LAUNCH(MC_KID_MISC, k_reduce_pairs, dim3(npairs), dim3(64), 0, red, s.gx, pair_sums);
@@ -620,8 +895,14 @@ static void vpsi_impl(mc_ctx* ctx, const double2* c0, double2* c2, const double*
              dim3(kThreads), shy, w, A, B, s.tz);
     });
     with_length(ctx->n1, [&](auto NN) {
-      LAUNCH(MC_KID_XVPSI, (k_xvpsi<T, decltype(NN)::value>), dim3((ctx->n2 + s.ty - 1) / s.ty, ctx->n3),
-             dim3(kThreads), shx, w, B, np, vpot, s.ty);
+      constexpr int N1 = decltype(NN)::value;
+      if constexpr (xfused_ok<T, N1>()) {
+        LAUNCH(MC_KID_XVPSI, (k_xvpsi_f<T, N1>), dim3((ctx->n2 + XTile<N1>::L - 1) / XTile<N1>::L, ctx->n3),
+               dim3(kThreads), (xfused_lds<T, N1>()), w, B, np, vpot);
+      } else {
+        LAUNCH(MC_KID_XVPSI, (k_xvpsi<T, N1>), dim3((ctx->n2 + s.ty - 1) / s.ty, ctx->n3), dim3(kThreads), shx, w,
+               B, np, vpot, s.ty);
+      }
     });
     with_length(ctx->n2, [&](auto NN) {
       LAUNCH(MC_KID_YFWD, (k_ypass<T, decltype(NN)::value, -1>), dim3(ctx->ncols, (ctx->n3 + s.tz - 1) / s.tz, np),

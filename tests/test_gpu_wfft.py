@@ -39,7 +39,8 @@ def _check_vpsi(ctx, L, c, f, v, seed=3):
     assert err < C2_TOL * scale, (err, scale)
 
 
-@pytest.mark.parametrize("kind,nstate", [("cubic", 6), ("cubic", 5), ("ortho", 4), ("ortho2", 7), ("ortho2", 1)])
+@pytest.mark.parametrize("kind,nstate", [("cubic", 6), ("cubic", 5), ("ortho", 4), ("ortho2", 7), ("ortho2", 1),
+                                         ("ortho3", 5)])
 def test_rhoofr_small(gpu, kind, nstate):
     cell, gs, c = small_system(kind, nstate)
     ctx = ops.Context(cell, gs)
@@ -57,7 +58,8 @@ def test_rhoofr_zero_occupations(gpu):
     assert not rho.any() and ek == 0.0 and cg == 0.0 and cr == 0.0
 
 
-@pytest.mark.parametrize("kind,nstate", [("cubic", 6), ("cubic", 3), ("ortho", 4), ("ortho2", 5), ("ortho2", 1)])
+@pytest.mark.parametrize("kind,nstate", [("cubic", 6), ("cubic", 3), ("ortho", 4), ("ortho2", 5), ("ortho2", 1),
+                                         ("ortho3", 4)])
 def test_vpsi_small(gpu, kind, nstate):
     cell, gs, c = small_system(kind, nstate)
     ctx = ops.Context(cell, gs)
@@ -158,3 +160,23 @@ def test_fp32_transforms(gpu, name, nstate):
     # and back to FP64 on the same context
     ctx.set_precision("fp64")
     assert np.abs(ctx.rhoofr(c, f)[0] - rho_o).max() < 1e-11
+
+
+@pytest.mark.parametrize("kind,nstate", [("ortho", 5), ("ortho3", 4)])
+def test_fp32_transforms_small(gpu, kind, nstate):
+    """FP32 transforms on a ragged y tile (n2 = 18 < lines per workgroup) and on
+    the runtime-plan lengths (n1 = 30, n2 = n3 = 20)."""
+    cell, gs, c = small_system(kind, nstate)
+    f = np.linspace(2.0, 0.5, nstate)
+    ctx = ops.Context(cell, gs)
+    ctx.set_precision("fp32")
+    L = oracle.Layout(cell, gs)
+    rho_g, ek_g, cg_g, cr_g = ctx.rhoofr(c, f)
+    rho_o, ek_o, cg_o, cr_o = oracle.rhoofr(L, c, f)
+    assert np.abs(rho_g - rho_o).max() < 2e-6 * np.abs(rho_o).max()
+    assert cr_g == pytest.approx(cg_g, rel=1e-12)
+    sysm = S.SyntheticSystem(cell=cell, gset=gs, tau=np.zeros((1, 3)), nstate=nstate, f=np.full(nstate, 2.0))
+    v = S.real_space_potential(sysm, seed=11)
+    c2 = ctx.vpsi(c, np.zeros_like(c), f, v)
+    c2_o = oracle.vpsi(L, c, np.zeros_like(c), f, v)
+    assert np.abs(c2 - c2_o).max() < 2e-6 * np.abs(c2_o).max()

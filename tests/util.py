@@ -16,6 +16,8 @@ def small_system(kind: str = "cubic", nstate: int = 6, seed: int = 7):
         cell = S.orthorhombic_cell(6.0, 6.0, 8.0, 20.0)
     elif kind == "ortho2":
         cell = S.orthorhombic_cell(5.0, 6.0, 7.0, 15.0)
+    elif kind == "ortho3":   # n1 = 30: x pass through the runtime-plan transform
+        cell = S.orthorhombic_cell(9.0, 6.5, 7.0, 20.0)
     else:
         raise ValueError(kind)
     gs = S.gvectors(cell, with_density=False)
