@@ -47,6 +47,11 @@ struct WParams {
 
 constexpr int kMaxElem = 16;  // real-space elements per thread kept in registers (x pass)
 
+// element (pair pb, column c, y, z) of the column workspace B[pair][col][y][z]
+__device__ __forceinline__ size_t bidx(const WParams& w, int pb, int c, int y, int z) {
+  return (((size_t)pb * w.ncols + c) * w.n2 + y) * w.n3 + z;
+}
+
 // ---------------------------------------------------------------------------
 // zinv: Gamma pack of a state pair (c1 + i c2 at +G, conj(c1) + i conj(c2) at
 // -G, G=0 once; src/state_utils.mod.F90:171-189) into `lo` owner sticks and
@@ -151,7 +156,7 @@ __global__ __launch_bounds__(kThreads) void k_ypass(WParams w, v2<T>* __restrict
   } else {
     for (int e = threadIdx.x; e < nz * n2; e += kThreads) {
       const int z = e / n2, y = e - z * n2;
-      buf[e] = B[(((size_t)pb * n3 + z0 + z) * w.ncols + c) * n2 + y];
+      buf[e] = B[bidx(w, pb, c, y, z0 + z)];
     }
   }
   __syncthreads();
@@ -159,7 +164,7 @@ __global__ __launch_bounds__(kThreads) void k_ypass(WParams w, v2<T>* __restrict
   if (DIR > 0) {
     for (int e = threadIdx.x; e < nz * n2; e += kThreads) {
       const int z = e / n2, y = e - z * n2;
-      B[(((size_t)pb * n3 + z0 + z) * w.ncols + c) * n2 + y] = buf[e];
+      B[bidx(w, pb, c, y, z0 + z)] = buf[e];
     }
   } else {
     for (int e = threadIdx.x; e < ncol * nz; e += kThreads) {
@@ -179,14 +184,13 @@ constexpr int kMaxCol = 8;
 template <typename T>
 __device__ __forceinline__ void fetch_cols(const WParams& w, const v2<T>* __restrict__ B, int pb, int z, int y0,
                                            int ny, v2<T> (&r)[kMaxCol]) {
-  const size_t bbase = ((size_t)pb * w.n3 + z) * w.ncols;
   const int m = w.ncols * ny;
 #pragma unroll
   for (int k = 0; k < kMaxCol; ++k) {
     const int e = threadIdx.x + k * kThreads;
     if (e < m) {
       const int c = e / ny, y = e - c * ny;
-      r[k] = B[(bbase + c) * w.n2 + y0 + y];
+      r[k] = B[bidx(w, pb, c, y0 + y, z)];
     }
   }
 }
@@ -212,10 +216,9 @@ __device__ __forceinline__ void load_xlines(const WParams& w, const v2<T>* __res
                                             int ny, int n1, v2<T>* buf) {
   for (int e = threadIdx.x; e < ny * n1; e += kThreads) buf[e] = cmk<v2<T>>((T)0, (T)0);
   __syncthreads();
-  const size_t bbase = ((size_t)pb * w.n3 + z) * w.ncols;
   for (int e = threadIdx.x; e < w.ncols * ny; e += kThreads) {
     const int c = e / ny, y = e - c * ny;
-    buf[y * n1 + w.col_gx[c]] = B[(bbase + c) * w.n2 + y0 + y];
+    buf[y * n1 + w.col_gx[c]] = B[bidx(w, pb, c, y0 + y, z)];
   }
   __syncthreads();
 }
@@ -343,20 +346,19 @@ __global__ __launch_bounds__(kThreads) void k_xvpsi(WParams w, v2<T>* __restrict
     }
     __syncthreads();
     fft_any<T, N, -1>(buf, ny, n1, w.p1, tw);
-    const size_t bbase = ((size_t)pb * w.n3 + z) * w.ncols;
     if (kPre) {
 #pragma unroll
       for (int k = 0; k < kMaxCol; ++k) {
         const int e = threadIdx.x + k * kThreads;
         if (e < m) {
           const int c = e / ny, y = e - c * ny;
-          B[(bbase + c) * w.n2 + y0 + y] = buf[dst[k]];
+          B[bidx(w, pb, c, y0 + y, z)] = buf[dst[k]];
         }
       }
     } else {
       for (int e = threadIdx.x; e < m; e += kThreads) {
         const int c = e / ny, y = e - c * ny;
-        B[(bbase + c) * w.n2 + y0 + y] = buf[y * n1 + w.col_gx[c]];
+        B[bidx(w, pb, c, y0 + y, z)] = buf[y * n1 + w.col_gx[c]];
       }
     }
     __syncthreads();
@@ -364,116 +366,147 @@ __global__ __launch_bounds__(kThreads) void k_xvpsi(WParams w, v2<T>* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// Fused x pass for the compile-time lengths N = A*B (FastShape).  One
-// workgroup holds L consecutive y lines of one z plane; lane t works on line
-// l = t % L (so 16 lanes read or write 16 consecutive y, one 128-byte run of
-// a column of B) and on group g = t / L of the line:
-//   phase 1 (g = n2 < B):  x[B n1 + n2], n1 < A, straight from B into
-//            registers, DFT_A, twiddle w^{n2 k1}, to LDS buffer 1
-//   phase 2 (g = k1 < A):  DFT_B of buffer 1 row k1 -> psi(x = k1 + A k2)
-//            in registers (|psi|^2 accumulation or the V(r) multiply here)
-//   phase 3 (g = k1):      forward DFT_B over k2, twiddle w^{k1 k1'}, to LDS
-//            buffer 2 at A k1' + k1
-//   phase 4 (g = k1' < B): DFT_A of buffer 2 row k1' -> psi(x = k1' + B k2'),
-//            straight from registers to the sphere columns of B.
-// Each phase is the four-step split of fft_fast (the forward one with A and B
-// exchanged, so that phase 2's registers feed phase 3 directly).  With two
-// buffers a pair takes two barriers (phase 1 -> 2, phase 3 -> 4): the next
-// pair's phase 1 writes buffer 1 after every lane has passed the 3 -> 4
-// barrier, hence after its phase 2 reads.  NB = 1 shares one buffer and adds
-// the two barriers that sharing needs (the FP64 path, whose buffers are twice
-// as large, and any length whose two buffers would pass 64 KiB).  LDS element
-// (row group q, row r of the group, line l) sits at row index * L + l + 16 q:
-// the 16-element shift per group puts the two 32-lane halves of a wave, which
-// hold consecutive groups in phases 2 and 4, on disjoint 8-byte bank groups.
+// Fused y and x passes for the compile-time lengths N = A*B (FastShape).
+// A workgroup holds L consecutive z lines (z0 .. z0+L-1) of one (column, y)
+// or (x column, y row); lane t works on line l = t % L and on group g = t / L
+// of it.  With B laid out [pair][col][y][z], 16 lanes with consecutive l
+// touch one 128-byte run of B, so every global access is coalesced straight
+// from registers.  The phases are the four-step split of fft_fast:
+//   phase 1 (g = n2 < B):  x[B n1 + n2], n1 < A, in registers, DFT_A,
+//            twiddle w^{n2 k1}, to an LDS buffer
+//   phase 2 (g = k1 < A):  DFT_B of row group k1 -> X(k1 + A k2) in registers
+// and for xvpsi, whose registers after phase 2 feed the forward transform
+// directly (the split with A and B exchanged):
+//   phase 3 (g = k1):      DFT_B over k2, twiddle w^{k1 k1'}, to buffer 2
+//   phase 4 (g = k1' < B): DFT_A of row group k1' -> Y(k1' + B k2').
+// A buffer row (one index of the transform, L lines) is 16 elements for
+// L = 16; rows are stored with bit 0 flipped in odd row groups (groups of
+// even size), so that the two 32-lane halves of a wave, which hold
+// consecutive groups or consecutive rows, land on opposite halves of the
+// 64-bank LDS.  Two buffers alternate between pairs, one barrier per
+// transform; NB = 1 (FP64 lengths whose two buffers pass 64 KiB, and the FP64
+// path at large N) shares one buffer and adds the barriers that needs.
 // ---------------------------------------------------------------------------
+// register budget of the fused kernels: FP64 is held to 2 waves per SIMD
+// (256 VGPRs); FP32 keeps its register prefetch unconstrained
+#define F4_WAVES __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 1 : 2, 8)))
+
 template <int N>
-struct XTile {
+struct F4Tile {
   static constexpr int A = FastShape<N>::A, B = FastShape<N>::B;
   static constexpr int G = A > B ? A : B;
-  static constexpr int L = kThreads / G;        // lines per workgroup
-  static constexpr int kBuf = N * L + 16 * G;   // elements of one LDS buffer
+  static constexpr int L = kThreads / G;   // lines per workgroup
+  static constexpr int kBuf = N * L;       // elements of one LDS buffer
 };
 
-// LDS bytes of the fused x kernels: tw, NB buffers, x -> column map
+// LDS bytes of the fused kernels: the two twiddle tables, nb buffers, one
+// int map of length N
 template <typename T, int N>
-constexpr size_t xlds(int nb) {
-  return sizeof(v2<T>) * (size_t)(N + nb * XTile<N>::kBuf) + sizeof(int) * N;
+constexpr size_t f4_lds(int nb) {
+  return sizeof(v2<T>) * (size_t)(2 * N + nb * F4Tile<N>::kBuf) + sizeof(int) * N;
 }
 template <typename T, int N>
-constexpr int xbufs() { return (sizeof(T) == 4 && xlds<T, N>(2) <= 65536) ? 2 : 1; }
+constexpr int f4_bufs() { return f4_lds<T, N>(2) <= 65536 ? 2 : 1; }
 template <typename T, int N>
-constexpr size_t xfused_lds() { return xlds<T, N>(xbufs<T, N>()); }
-// lengths whose fused kernels fit the 64 KiB of LDS of one workgroup
+constexpr size_t f4_lds() { return f4_lds<T, N>(f4_bufs<T, N>()); }
+// lengths with a fused kernel that fits the 64 KiB of LDS of one workgroup
 template <typename T, int N>
-constexpr bool xfused_ok() {
-  if constexpr (N > 0) return xfused_lds<T, N>() <= 65536;
+constexpr bool f4_ok() {
+  if constexpr (N > 0) return f4_lds<T, N>() <= 65536;
   else return false;
 }
 
-template <typename T, int N>
-struct XFused {
-  using V = v2<T>;
-  static constexpr int A = XTile<N>::A, B = XTile<N>::B, L = XTile<N>::L;
-  static constexpr int NB = xbufs<T, N>();
-  V* tw;
-  V* buf[2];
-  int* xcol;
-  int l, g, ny;
+// Buffer addressing.  Logical row r of row group q (groups of P rows) is
+// stored at physical row r ^ (q & 1) when P is even (r otherwise), L lines
+// per row.  Every access of a phase is then one of two per-lane base
+// registers plus a compile-time offset:
+//   write row P k + g (k compile-time):  P k L + wb[k & 1],  wb = {g L + l, (g ^ 1) L + l}
+//   read row P g + m (m compile-time):   m L + rb[m & 1],    rb = {(P g + s) L + l, (P g - s) L + l}, s = g & 1
+struct F4Bases {
+  int wb[2], rb[2];
+};
+template <int P, int L>
+__device__ __forceinline__ F4Bases f4_bases(int g, int l) {
+  F4Bases b;
+  const int s = (P % 2 == 0) ? (g & 1) : 0;
+  b.wb[0] = g * L + l;
+  b.wb[1] = ((P % 2 == 0) ? (g ^ 1) : g) * L + l;
+  b.rb[0] = (P * g + s) * L + l;
+  b.rb[1] = (P * g - s) * L + l;
+  return b;
+}
 
-  __device__ __forceinline__ void init(const WParams& w, int y0) {
-    tw = lds_base<T>();
-    buf[0] = tw + N;
-    buf[1] = NB == 2 ? buf[0] + XTile<N>::kBuf : buf[0];
-    xcol = reinterpret_cast<int*>(buf[0] + NB * XTile<N>::kBuf);
+template <typename T, int N>
+struct F4 {
+  using V = v2<T>;
+  static constexpr int A = F4Tile<N>::A, B = F4Tile<N>::B, L = F4Tile<N>::L;
+  static constexpr int NB = f4_bufs<T, N>();
+  static constexpr unsigned kNone = 0xffffffffu;
+  V* t1;      // t1[k1 B + n2] = w^{n2 k1}, k1 < A, n2 < B (phase 1)
+  V* t3;      // t3[k1 A + n2] = w^{n2 k1}, k1 < B, n2 < A (phase 3)
+  V* buf[2];
+  int* map;   // N ints after the buffers
+  int l, g;
+  F4Bases b12, b34;   // buffer bases of phases 1/2 (groups of B rows) and 3/4 (groups of A rows)
+
+  __device__ __forceinline__ void init(const double2* table) {
+    t1 = lds_base<T>();
+    t3 = t1 + N;
+    buf[0] = t3 + N;
+    buf[1] = NB == 2 ? buf[0] + F4Tile<N>::kBuf : buf[0];
+    map = reinterpret_cast<int*>(buf[0] + NB * F4Tile<N>::kBuf);
     l = threadIdx.x % L;
     g = threadIdx.x / L;
-    ny = min(L, w.n2 - y0);
-    load_tw<T>(tw, w.tw1, N);
-    for (int x = threadIdx.x; x < N; x += kThreads) xcol[x] = -1;
-    __syncthreads();
-    for (int c = threadIdx.x; c < w.ncols; c += kThreads) xcol[w.col_gx[c]] = c;
-  }
-  // Byte offsets, within one pair's (z) slab of B, of the elements
-  // x = B k + g of this lane's line (the positions of phase 1's inputs and
-  // phase 4's outputs alike), kNoCol off the sphere columns: fixed for the
-  // whole kernel.  32-bit unsigned bytes, so that the accesses take the
-  // scalar-base + vector-offset form (a slab is far below 4 GiB).
-  static constexpr unsigned kNoCol = 0xffffffffu;
-  __device__ __forceinline__ void offsets(const WParams& w, int y0, unsigned (&off)[A]) const {
-#pragma unroll
-    for (int k = 0; k < A; ++k) {
-      const int c = (g < B && l < ny) ? xcol[B * k + g] : -1;
-      off[k] = c >= 0 ? (unsigned)((c * w.n2 + y0 + l) * sizeof(V)) : kNoCol;
+    b12 = f4_bases<B, L>(g, l);
+    b34 = f4_bases<A, L>(g, l);
+    for (int e = threadIdx.x; e < N; e += kThreads) {
+      const int k1 = e / B, n2 = e - k1 * B;
+      const double2 w1 = table[n2 * k1];
+      t1[e] = cmk<V>((T)w1.x, (T)w1.y);
+      const int j1 = e / A, m2 = e - j1 * A;
+      const double2 w3 = table[m2 * j1];
+      t3[e] = cmk<V>((T)w3.x, (T)w3.y);
     }
   }
-  // phase 1 loads of one pair (slab base Bs): column elements, zero elsewhere
-  __device__ __forceinline__ void fetch(const V* __restrict__ Bs, const unsigned (&off)[A], V (&r)[A]) const {
-#pragma unroll
-    for (int k = 0; k < A; ++k) r[k] = off[k] != kNoCol ? at(Bs, off[k]) : cmk<V>((T)0, (T)0);
-  }
-  __device__ __forceinline__ static const V& at(const V* p, unsigned b) {
+  // 32-bit byte offsets from a uniform base: scalar-base + vector-offset accesses
+  __device__ __forceinline__ static V ld(const V* p, unsigned b) {
     return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(p) + b);
   }
-  __device__ __forceinline__ static V& at(V* p, unsigned b) {
-    return *reinterpret_cast<V*>(reinterpret_cast<char*>(p) + b);
+  __device__ __forceinline__ static void st(V* p, unsigned b, V v) {
+    *reinterpret_cast<V*>(reinterpret_cast<char*>(p) + b) = v;
+  }
+  template <int K>
+  __device__ __forceinline__ static void fetch(const V* __restrict__ p, const unsigned (&off)[K], V (&r)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) r[k] = off[k] != kNone ? ld(p, off[k]) : cmk<V>((T)0, (T)0);
+  }
+  template <int K>
+  __device__ __forceinline__ static void store(V* __restrict__ p, const unsigned (&off)[K], const V (&r)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (off[k] != kNone) st(p, off[k], r[k]);
   }
   template <int DIR>
   __device__ __forceinline__ void phase1(V (&v)[A], V* bf) const {
     if (g < B) {
       dftn<A, DIR>(v);
+      const V* tp = t1 + g;
 #pragma unroll
-      for (int k1 = 1; k1 < A; ++k1) v[k1] = twmul<DIR>(v[k1], tw[g * k1]);
+      for (int k1 = 1; k1 < A; ++k1) v[k1] = twmul<DIR>(v[k1], tp[k1 * B]);
+      V* w0 = bf + b12.wb[0];
+      V* w1 = bf + b12.wb[1];
 #pragma unroll
-      for (int k1 = 0; k1 < A; ++k1) bf[(B * k1 + g) * L + l + 16 * k1] = v[k1];
+      for (int k1 = 0; k1 < A; ++k1) ((k1 & 1) ? w1 : w0)[B * k1 * L] = v[k1];
     }
   }
-  // u[k2] = psi(x = g + A k2), lanes g < A
+  // u[k2] = X(g + A k2), lanes g < A
   template <int DIR>
   __device__ __forceinline__ void phase2(V (&u)[B], const V* bf) const {
     if (g < A) {
+      const V* r0 = bf + b12.rb[0];
+      const V* r1 = bf + b12.rb[1];
 #pragma unroll
-      for (int m = 0; m < B; ++m) u[m] = bf[(B * g + m) * L + l + 16 * g];
+      for (int m = 0; m < B; ++m) u[m] = ((m & 1) ? r1 : r0)[m * L];
       dftn<B, DIR>(u);
     }
   }
@@ -481,69 +514,160 @@ struct XFused {
   __device__ __forceinline__ void phase3(V (&u)[B], V* bf) const {
     if (g < A) {
       dftn<B, DIR>(u);
+      const V* tp = t3 + g;
 #pragma unroll
-      for (int k1 = 1; k1 < B; ++k1) u[k1] = twmul<DIR>(u[k1], tw[g * k1]);
+      for (int k1 = 1; k1 < B; ++k1) u[k1] = twmul<DIR>(u[k1], tp[k1 * A]);
+      V* w0 = bf + b34.wb[0];
+      V* w1 = bf + b34.wb[1];
 #pragma unroll
-      for (int k1 = 0; k1 < B; ++k1) bf[(A * k1 + g) * L + l + 16 * k1] = u[k1];
+      for (int k1 = 0; k1 < B; ++k1) ((k1 & 1) ? w1 : w0)[A * k1 * L] = u[k1];
     }
   }
-  // v[k2] = psi(x = g + B k2), lanes g < B
+  // v[k2] = Y(g + B k2), lanes g < B
   template <int DIR>
   __device__ __forceinline__ void phase4(V (&v)[A], const V* bf) const {
     if (g < B) {
+      const V* r0 = bf + b34.rb[0];
+      const V* r1 = bf + b34.rb[1];
 #pragma unroll
-      for (int m = 0; m < A; ++m) v[m] = bf[(A * g + m) * L + l + 16 * g];
+      for (int m = 0; m < A; ++m) v[m] = ((m & 1) ? r1 : r0)[m * L];
       dftn<A, DIR>(v);
     }
   }
 };
 
-// xrho, fused form (same contract as k_xrho).  With two buffers the pairs
-// alternate between them, one barrier per pair.
+// ---------------------------------------------------------------------------
+// y pass, fused form: workgroup (column c, z tile, pairs p0 .. p0+ppb-1).
+// DIR=+1: sticks of column c in A -> B (dense y); DIR=-1: B -> sticks in A.
+// ---------------------------------------------------------------------------
+template <typename T, int N, int DIR>
+__global__ __launch_bounds__(kThreads) F4_WAVES void k_ypass_f(WParams w, v2<T>* __restrict__ Ag, v2<T>* __restrict__ Bg,
+                                                      int npairs, int ppb) {
+  using V = v2<T>;
+  using X = F4<T, N>;
+  constexpr int A = X::A, B = X::B, L = X::L;
+  X s;
+  s.init(w.tw2);
+  const int c = blockIdx.x, z0 = blockIdx.y * L;
+  const int p0 = blockIdx.z * ppb, p1 = min(npairs, p0 + ppb);
+  const bool zin = z0 + s.l < w.n3;
+  for (int y = threadIdx.x; y < N; y += kThreads) s.map[y] = -1;
+  __syncthreads();
+  const int k0 = w.col_start[c], k1 = w.col_start[c + 1];
+  for (int k = k0 + threadIdx.x; k < k1; k += kThreads) s.map[w.col_gy[k]] = w.col_slot[k];
+  __syncthreads();
+  const size_t aslab = (size_t)w.n_slots * w.n3;   // A elements per pair
+  const size_t bslab = (size_t)w.ncols * N * w.n3;  // B elements per pair
+  const unsigned zoff = (unsigned)(z0 + s.l);
+  // sparse (A) offsets: DIR>0 phase-1 inputs y = B k + g; DIR<0 phase-2 outputs y = g + A k
+  constexpr int KA = DIR > 0 ? A : B;
+  unsigned offa[KA];
+#pragma unroll
+  for (int k = 0; k < KA; ++k) {
+    const bool act = zin && (DIR > 0 ? s.g < B : s.g < A);
+    const int y = DIR > 0 ? B * k + s.g : s.g + A * k;
+    const int slot = act ? s.map[y] : -1;
+    offa[k] = slot >= 0 ? (unsigned)(((size_t)slot * w.n3 + zoff) * sizeof(V)) : X::kNone;
+  }
+  // dense (B) side, affine in k: DIR>0 phase-2 outputs y = g + A k; DIR<0
+  // phase-1 inputs y = B k + g
+  const bool actb = zin && (DIR > 0 ? s.g < A : s.g < B);
+  const unsigned offb0 = (unsigned)((((size_t)c * N + s.g) * w.n3 + zoff) * sizeof(V));
+  const unsigned dstep = (unsigned)((DIR > 0 ? A : B) * w.n3 * sizeof(V));
+  constexpr bool kPre = sizeof(T) == 4;   // register prefetch of the next pair (FP32)
+  V r[A];
+  auto load = [&](int p, V (&dst)[A]) {
+    if constexpr (DIR > 0) {
+      X::fetch(Ag + (size_t)p * aslab, offa, dst);
+    } else {
+      const V* bp = Bg + (size_t)p * bslab;
+#pragma unroll
+      for (int k = 0; k < A; ++k) dst[k] = actb ? X::ld(bp, offb0 + k * dstep) : cmk<V>((T)0, (T)0);
+    }
+  };
+  if (kPre && p0 < p1) load(p0, r);
+  int par = 0;
+  for (int p = p0; p < p1; ++p) {
+    if (!kPre) load(p, r);
+    V* bf = par ? s.buf[1] : s.buf[0];
+    s.template phase1<DIR>(r, bf);
+    if (kPre && p + 1 < p1) load(p + 1, r);   // in flight through the rest of this pair
+    __syncthreads();
+    V u[B];
+    s.template phase2<DIR>(u, bf);
+    if constexpr (DIR > 0) {
+      V* bp = Bg + (size_t)p * bslab;
+      if (actb) {
+#pragma unroll
+        for (int k = 0; k < B; ++k) X::st(bp, offb0 + k * dstep, u[k]);
+      }
+    } else {
+      X::store(Ag + (size_t)p * aslab, offa, u);
+    }
+    if (X::NB == 1) __syncthreads();
+    par ^= (X::NB - 1);
+  }
+}
+
+// x -> column map of the x kernels and the byte offsets, within one pair's B,
+// of lane (l, g)'s elements x = B k + g (phase 1 inputs, phase 4 outputs)
+template <typename X>
+__device__ __forceinline__ void x_offsets(const X& s, const WParams& w, int y, int z0, unsigned (&off)[X::A]) {
+  using V = typename X::V;
+  const bool act = s.g < X::B && z0 + s.l < w.n3;
+#pragma unroll
+  for (int k = 0; k < X::A; ++k) {
+    const int c = act ? s.map[X::B * k + s.g] : -1;
+    off[k] = c >= 0 ? (unsigned)((((size_t)c * w.n2 + y) * w.n3 + z0 + s.l) * sizeof(V)) : X::kNone;
+  }
+}
+template <typename X>
+__device__ __forceinline__ void x_map(const X& s, const WParams& w, int n1) {
+  for (int x = threadIdx.x; x < n1; x += kThreads) s.map[x] = -1;
+  __syncthreads();
+  for (int c = threadIdx.x; c < w.ncols; c += kThreads) s.map[w.col_gx[c]] = c;
+}
+
+// xrho, fused form (same contract as k_xrho): workgroup (z tile, y)
 template <typename T, int N>
-__global__ __launch_bounds__(kThreads) void k_xrho_f(WParams w, const v2<T>* __restrict__ Bg, int npairs, int pair0,
+__global__ __launch_bounds__(kThreads) F4_WAVES void k_xrho_f(WParams w, const v2<T>* __restrict__ Bg, int npairs, int pair0,
                                                      int nstate, const double* __restrict__ f,
                                                      double* __restrict__ rho) {
   using V = v2<T>;
-  using X = XFused<T, N>;
+  using X = F4<T, N>;
   constexpr int A = X::A, B = X::B, L = X::L;
   X s;
-  const int y0 = blockIdx.x * L, z = blockIdx.y;
-  s.init(w, y0);
+  s.init(w.tw1);
+  const int z0 = blockIdx.x * L, y = blockIdx.y;
+  const int nz = min(L, w.n3 - z0);
+  x_map(s, w, N);
   __syncthreads();
   auto occupied = [&](int p) {
     const int i1 = 2 * (pair0 + p), i2 = i1 + 1;
     return f[i1] != 0.0 || (i2 < nstate && f[i2] != 0.0);
   };
+  unsigned off[A];
+  x_offsets(s, w, y, z0, off);
+  const size_t bslab = (size_t)w.ncols * w.n2 * w.n3;
   const double rom = 1.0 / w.omega;
   double acc[B];
 #pragma unroll
   for (int k = 0; k < B; ++k) acc[k] = 0.0;
-  unsigned off[A];
-  s.offsets(w, y0, off);
-  const size_t slab = (size_t)w.ncols * w.n2;   // elements of one (pair, z) slab of B
-  auto base = [&](int p) { return Bg + ((size_t)p * w.n3 + z) * slab; };
   constexpr bool kPre = sizeof(T) == 4;
   V r[A];
   int pb = 0;
   while (pb < npairs && !occupied(pb)) ++pb;
-  if (kPre && pb < npairs) s.fetch(base(pb), off, r);
+  if (kPre && pb < npairs) X::fetch(Bg + (size_t)pb * bslab, off, r);
   int par = 0;
   while (pb < npairs) {   // uniform over the workgroup
     int nx = pb + 1;
     while (nx < npairs && !occupied(nx)) ++nx;
     const int is1 = 2 * (pair0 + pb), is2 = is1 + 1;
     const double c3 = f[is1] * rom, c4 = (is2 < nstate ? f[is2] : 0.0) * rom;
-    V v[A];
-    if (kPre) {
-#pragma unroll
-      for (int k = 0; k < A; ++k) v[k] = r[k];
-      if (nx < npairs) s.fetch(base(nx), off, r);
-    } else {
-      s.fetch(base(pb), off, v);
-    }
+    if (!kPre) X::fetch(Bg + (size_t)pb * bslab, off, r);
     V* bf = par ? s.buf[1] : s.buf[0];
-    s.template phase1<1>(v, bf);
+    s.template phase1<1>(r, bf);
+    if (kPre && nx < npairs) X::fetch(Bg + (size_t)nx * bslab, off, r);
     __syncthreads();
     V u[B];
     s.template phase2<1>(u, bf);
@@ -558,41 +682,46 @@ __global__ __launch_bounds__(kThreads) void k_xrho_f(WParams w, const v2<T>* __r
     par ^= (X::NB - 1);
     pb = nx;
   }
-  // transpose the accumulators through LDS (rows of N + 1 doubles per line)
-  // so that the read-modify-write of rho runs along x
+  // transpose the accumulators through LDS (one row per z line) so that the
+  // read-modify-write of rho runs along x
   __syncthreads();
-  double* st = reinterpret_cast<double*>(s.buf[0]);
-  constexpr int LD = N + 1;
-  static_assert(sizeof(double) * L * LD <= sizeof(V) * X::NB * XTile<N>::kBuf, "rho staging exceeds the buffers");
+  double* stg = reinterpret_cast<double*>(s.buf[0]);
+  constexpr int LD = (sizeof(double) * L * (N + 1) <= sizeof(V) * X::NB * F4Tile<N>::kBuf) ? N + 1 : N;
+  static_assert(sizeof(double) * L * LD <= sizeof(V) * X::NB * F4Tile<N>::kBuf, "rho staging exceeds the buffers");
   if (s.g < A) {
 #pragma unroll
-    for (int k = 0; k < B; ++k) st[s.l * LD + s.g + A * k] = acc[k];
+    for (int k = 0; k < B; ++k) stg[s.l * LD + s.g + A * k] = acc[k];
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < s.ny * N; e += kThreads) {
-    const int y = e / N, x = e - y * N;
-    const size_t idx = (size_t)x + (size_t)w.kr1 * ((size_t)(y0 + y) + (size_t)w.kr2 * z);
-    rho[idx] += st[y * LD + x];
+  for (int e = threadIdx.x; e < nz * N; e += kThreads) {
+    const int zl = e / N, x = e - zl * N;
+    const size_t idx = (size_t)x + (size_t)w.kr1 * ((size_t)y + (size_t)w.kr2 * (z0 + zl));
+    rho[idx] += stg[zl * LD + x];
   }
 }
 
-// xvpsi, fused form (same contract as k_xvpsi)
+// xvpsi, fused form (same contract as k_xvpsi): workgroup (z tile, y).  On
+// the FP32 path the next pair's loads are issued after phase kXPre.
+constexpr int kXPre = 1;
 template <typename T, int N>
-__global__ __launch_bounds__(kThreads) void k_xvpsi_f(WParams w, v2<T>* __restrict__ Bg, int npairs,
+__global__ __launch_bounds__(kThreads) F4_WAVES void k_xvpsi_f(WParams w, v2<T>* __restrict__ Bg, int npairs,
                                                       const double* __restrict__ vpot) {
   using V = v2<T>;
-  using X = XFused<T, N>;
+  using X = F4<T, N>;
   constexpr int A = X::A, B = X::B, L = X::L;
   X s;
-  const int y0 = blockIdx.x * L, z = blockIdx.y;
-  s.init(w, y0);
-  // V(r) of this tile, staged x-major (row stride L + 1) through buffer 1 so
-  // that lane (l, g) picks up V(x = g + A k2, y0 + l) for its phase-2 values
+  s.init(w.tw1);
+  const int z0 = blockIdx.x * L, y = blockIdx.y;
+  const int nz = min(L, w.n3 - z0);
+  x_map(s, w, N);
+  // V(r) of this tile, staged x-major (row stride L + 1) through buffer 2 so
+  // that lane (l, g) picks up V(x = g + A k2, z0 + l) for its phase-2 values
   T* vst = reinterpret_cast<T*>(s.buf[1]);
+  static_assert(N * (L + 1) <= 2 * F4Tile<N>::kBuf, "V staging exceeds a buffer");
   for (int e = threadIdx.x; e < L * N; e += kThreads) {
-    const int y = e / N, x = e - y * N;
-    vst[x * (L + 1) + y] =
-        y < s.ny ? (T)vpot[(size_t)x + (size_t)w.kr1 * ((size_t)(y0 + y) + (size_t)w.kr2 * z)] : (T)0;
+    const int zl = e / N, x = e - zl * N;
+    vst[x * (L + 1) + zl] =
+        zl < nz ? (T)vpot[(size_t)x + (size_t)w.kr1 * ((size_t)y + (size_t)w.kr2 * (z0 + zl))] : (T)0;
   }
   __syncthreads();
   T vr[B];
@@ -600,22 +729,21 @@ __global__ __launch_bounds__(kThreads) void k_xvpsi_f(WParams w, v2<T>* __restri
   for (int k = 0; k < B; ++k) vr[k] = s.g < A ? vst[(s.g + A * k) * (L + 1) + s.l] : (T)0;
   if (X::NB == 1) __syncthreads();
   unsigned off[A];
-  s.offsets(w, y0, off);
-  const size_t slab = (size_t)w.ncols * w.n2;   // elements of one (pair, z) slab of B
+  x_offsets(s, w, y, z0, off);
+  const size_t bslab = (size_t)w.ncols * w.n2 * w.n3;
   constexpr bool kPre = sizeof(T) == 4;
   V r[A];
-  if (kPre && npairs > 0) s.fetch(Bg + (size_t)z * slab, off, r);
+  if (kPre && npairs > 0) X::fetch(Bg, off, r);
   for (int pb = 0; pb < npairs; ++pb) {
-    V* Bs = Bg + ((size_t)pb * w.n3 + z) * slab;
+    V* Bs = Bg + (size_t)pb * bslab;
     V v[A];
-    if (kPre) {
-#pragma unroll
-      for (int k = 0; k < A; ++k) v[k] = r[k];
-      if (pb + 1 < npairs) s.fetch(Bs + (size_t)w.n3 * slab, off, r);
+    if constexpr (kPre) {
+      s.template phase1<1>(r, s.buf[0]);
+      if (kXPre == 1 && pb + 1 < npairs) X::fetch(Bs + bslab, off, r);   // in flight through phases 2-4
     } else {
-      s.fetch(Bs, off, v);
+      X::fetch(Bs, off, v);
+      s.template phase1<1>(v, s.buf[0]);
     }
-    s.template phase1<1>(v, s.buf[0]);
     __syncthreads();
     V u[B];
     s.template phase2<1>(u, s.buf[0]);
@@ -623,11 +751,10 @@ __global__ __launch_bounds__(kThreads) void k_xvpsi_f(WParams w, v2<T>* __restri
     for (int k = 0; k < B; ++k) u[k] = cscale(u[k], vr[k]);
     if (X::NB == 1) __syncthreads();
     s.template phase3<-1>(u, s.buf[1]);
+    if (kPre && kXPre == 3 && pb + 1 < npairs) X::fetch(Bs + bslab, off, r);   // in flight through phase 4
     __syncthreads();
     s.template phase4<-1>(v, s.buf[1]);
-#pragma unroll
-    for (int k = 0; k < A; ++k)
-      if (off[k] != X::kNoCol) X::at(Bs, off[k]) = v[k];
+    X::store(Bs, off, v);
     if (X::NB == 1) __syncthreads();
   }
 }
@@ -785,6 +912,27 @@ static Shapes shapes(mc_ctx* ctx) {
   return s;
 }
 
+// y pass of np pairs: the fused kernel (workgroups over columns, z tiles and
+// runs of pairs, >= ~2048 workgroups) for compile-time lengths, else k_ypass
+template <typename T, int DIR>
+static void ypass(mc_ctx* ctx, const WParams& w, const Shapes& s, v2<T>* A, v2<T>* B, int np, size_t shy) {
+  const int kid = DIR > 0 ? MC_KID_YINV : MC_KID_YFWD;
+  with_length(ctx->n2, [&](auto NN) {
+    constexpr int N2 = decltype(NN)::value;
+    if constexpr (f4_ok<T, N2>()) {
+      constexpr int L = F4Tile<N2>::L;
+      const int zt = (ctx->n3 + L - 1) / L;
+      const int runs = std::max(1, std::min(np, (2048 + ctx->ncols * zt - 1) / (ctx->ncols * zt)));
+      const int ppb = (np + runs - 1) / runs;
+      LAUNCH(kid, (k_ypass_f<T, N2, DIR>), dim3(ctx->ncols, zt, (np + ppb - 1) / ppb), dim3(kThreads),
+             (f4_lds<T, N2>()), w, A, B, np, ppb);
+    } else {
+      LAUNCH(kid, (k_ypass<T, N2, DIR>), dim3(ctx->ncols, (ctx->n3 + s.tz - 1) / s.tz, np), dim3(kThreads), shy, w,
+             A, B, s.tz);
+    }
+  });
+}
+
 template <typename T>
 static void ensure_workspace(mc_ctx* ctx, int P) {
   ctx->w_a.ensure(sizeof(v2<T>) * (size_t)P * ctx->n_slots * ctx->n3);
@@ -825,15 +973,12 @@ static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, in
       LAUNCH(MC_KID_ZINV, (k_zinv<T, decltype(NN)::value>), dim3(s.gx, np), dim3(kThreads), shz, w, c0, nstate,
              pair0, A, s.lo, red + (size_t)pair0 * s.gx * 4);
     });
-    with_length(ctx->n2, [&](auto NN) {
-      LAUNCH(MC_KID_YINV, (k_ypass<T, decltype(NN)::value, 1>), dim3(ctx->ncols, (ctx->n3 + s.tz - 1) / s.tz, np),
-             dim3(kThreads), shy, w, A, B, s.tz);
-    });
+    ypass<T, 1>(ctx, w, s, A, B, np, shy);
     with_length(ctx->n1, [&](auto NN) {
       constexpr int N1 = decltype(NN)::value;
-      if constexpr (xfused_ok<T, N1>()) {
-        LAUNCH(MC_KID_XRHO, (k_xrho_f<T, N1>), dim3((ctx->n2 + XTile<N1>::L - 1) / XTile<N1>::L, ctx->n3),
-               dim3(kThreads), (xfused_lds<T, N1>()), w, B, np, pair0, nstate, ctx->w_f.as<double>(), rho);
+      if constexpr (f4_ok<T, N1>()) {
+        LAUNCH(MC_KID_XRHO, (k_xrho_f<T, N1>), dim3((ctx->n3 + F4Tile<N1>::L - 1) / F4Tile<N1>::L, ctx->n2),
+               dim3(kThreads), (f4_lds<T, N1>()), w, B, np, pair0, nstate, ctx->w_f.as<double>(), rho);
       } else {
         LAUNCH(MC_KID_XRHO, (k_xrho<T, N1>), dim3((ctx->n2 + s.ty - 1) / s.ty, ctx->n3), dim3(kThreads), shx, w, B,
                np, pair0, nstate, ctx->w_f.as<double>(), rho, s.ty);
@@ -890,24 +1035,18 @@ static void vpsi_impl(mc_ctx* ctx, const double2* c0, double2* c2, const double*
       LAUNCH(MC_KID_ZINV, (k_zinv<T, decltype(NN)::value>), dim3(s.gx, np), dim3(kThreads), shz, w, c0, nstate,
              pair0, A, s.lo, (double*)nullptr);
     });
-    with_length(ctx->n2, [&](auto NN) {
-      LAUNCH(MC_KID_YINV, (k_ypass<T, decltype(NN)::value, 1>), dim3(ctx->ncols, (ctx->n3 + s.tz - 1) / s.tz, np),
-             dim3(kThreads), shy, w, A, B, s.tz);
-    });
+    ypass<T, 1>(ctx, w, s, A, B, np, shy);
     with_length(ctx->n1, [&](auto NN) {
       constexpr int N1 = decltype(NN)::value;
-      if constexpr (xfused_ok<T, N1>()) {
-        LAUNCH(MC_KID_XVPSI, (k_xvpsi_f<T, N1>), dim3((ctx->n2 + XTile<N1>::L - 1) / XTile<N1>::L, ctx->n3),
-               dim3(kThreads), (xfused_lds<T, N1>()), w, B, np, vpot);
+      if constexpr (f4_ok<T, N1>()) {
+        LAUNCH(MC_KID_XVPSI, (k_xvpsi_f<T, N1>), dim3((ctx->n3 + F4Tile<N1>::L - 1) / F4Tile<N1>::L, ctx->n2),
+               dim3(kThreads), (f4_lds<T, N1>()), w, B, np, vpot);
       } else {
         LAUNCH(MC_KID_XVPSI, (k_xvpsi<T, N1>), dim3((ctx->n2 + s.ty - 1) / s.ty, ctx->n3), dim3(kThreads), shx, w,
                B, np, vpot, s.ty);
       }
     });
-    with_length(ctx->n2, [&](auto NN) {
-      LAUNCH(MC_KID_YFWD, (k_ypass<T, decltype(NN)::value, -1>), dim3(ctx->ncols, (ctx->n3 + s.tz - 1) / s.tz, np),
-             dim3(kThreads), shy, w, A, B, s.tz);
-    });
+    ypass<T, -1>(ctx, w, s, A, B, np, shy);
     with_length(ctx->n3, [&](auto NN) {
       LAUNCH(MC_KID_ZFWD, (k_zfwd<T, decltype(NN)::value>), dim3(s.gx, np), dim3(kThreads), shz, w, A, c0, c2,
              ctx->w_f.as<double>(), nstate, pair0, s.lo, scale);
