@@ -55,6 +55,8 @@ SIGNATURES = {
     "mc_wfn_zero": (C.c_int, [_vp, _vp, C.c_int]),
     "mc_wfn_axpy": (C.c_int, [_vp, C.c_double, _vp, _vp, C.c_int]),
     "mc_wfn_axpby": (C.c_int, [_vp, C.c_double, _vp, C.c_double, _vp, C.c_int]),
+    "mc_cp_posupa": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double]),
+    "mc_cp_velocity": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_double]),
     "mc_wfn_dotp": (C.c_int, [_vp, _vp, _vp, C.c_int, _dp]),
     "mc_wfn_upload": (C.c_int, [_vp, _dp, C.c_int, C.c_int, _vp]),
     "mc_wfn_download": (C.c_int, [_vp, _vp, C.c_int, _dp, C.c_int]),
@@ -81,6 +83,7 @@ SIGNATURES = {
     "mc_set_gk": (C.c_int, [_vp, _dp]),
     "mc_rnlsm2": (C.c_int, [_vp, _dp, C.c_int, C.c_int, _dp]),
     "mc_rnlsm2_dev": (C.c_int, [_vp, _vp, C.c_int, _dp]),
+    "mc_rnlsm_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _dp, _dp]),
     "mc_rnlfor": (C.c_int, [_vp, _dp, C.c_int, _dp, _dp, _dp]),
     "mc_set_density_gset": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp]),
     "mc_set_local_pp": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _dp, _dp]),

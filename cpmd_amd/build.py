@@ -14,7 +14,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libmicpmd.so"
 
-HIP_SOURCES = ["context.hip", "wfft.hip", "gemm.hip", "gemm_x3.hip", "ortho.hip", "vofrho.hip", "capi.hip"]
+HIP_SOURCES = ["context.hip", "wfft.hip", "gemm.hip", "gemm_x3.hip", "gemm_planes.hip", "ortho.hip", "vofrho.hip",
+               "capi.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -223,6 +223,14 @@ int mc_rnlsm2(mc_ctx* ctx, const double* c0, int ldc, int nstate, double* dfnl) 
   MC_API_END
 }
 
+int mc_rnlsm_dev(mc_ctx* ctx, const void* c0_dev, int nstate, int tfor, double* fnl, double* dfnl) {
+  MC_API_BEGIN
+  check_ctx(ctx);
+  MC_CHECK(c0_dev && nstate >= 0, MC_ERR_ARG, "rnlsm: bad argument");
+  mc::rnlsm_dev(ctx, (const double2*)c0_dev, nstate, tfor != 0, fnl, dfnl);
+  MC_API_END
+}
+
 int mc_rnlfor(mc_ctx* ctx, const double* f, int nstate, const double* fnl, const double* dfnl, double* fion) {
   MC_API_BEGIN
   check_ctx(ctx);

@@ -221,6 +221,31 @@ __global__ void k_axpby(double alpha, const double* __restrict__ x, double beta,
     y[i] = beta == 0.0 ? alpha * x[i] : alpha * x[i] + beta * y[i];
 }
 
+// velupa + posupa in one pass over double2 coefficients:
+//   cm += dtm c2 (src/velupa_utils.mod.F90:55-58, dtm = nnlst dt/(2 emass)),
+//   cp = c0 + dt cm (src/posupa_utils.mod.F90:69-73)
+__global__ void k_cp_posupa(const double2* __restrict__ c0, const double2* __restrict__ c2, double2* __restrict__ cm,
+                            double2* __restrict__ cp, double dtm, double dt, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double2 a = c0[i], f = c2[i];
+    double2 v = cm[i];
+    v.x += dtm * f.x;
+    v.y += dtm * f.y;
+    cm[i] = v;
+    cp[i] = make_double2(a.x + dt * v.x, a.y + dt * v.y);
+  }
+}
+
+// velocities after the orthonormality constraint: cm = (cp - c0) / dt, which
+// equals CPMD's cm += c0 gamx / dt (posupa:150-158) for cp = c0 + dt cm + c0 gamx
+__global__ void k_cp_velocity(const double2* __restrict__ c0, const double2* __restrict__ cp,
+                              double2* __restrict__ cm, double odt, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double2 a = c0[i], b = cp[i];
+    cm[i] = make_double2(odt * (b.x - a.x), odt * (b.y - a.y));
+  }
+}
+
 // out[i] = Re a0 Re b0 + 2 sum_{G != 0} Re(conj(a) b) for column i (dotp,
 // src/dotp_utils.mod.F90:58-101); one workgroup per state, fixed order
 __global__ __launch_bounds__(256) void k_dotp_cols(const double2* __restrict__ a, const double2* __restrict__ b,
@@ -481,6 +506,32 @@ int mc_wfn_axpy(mc_ctx* ctx, double alpha, const void* x_dev, void* y_dev, int n
   if (n) {
     hipLaunchKernelGGL(mc::k_axpy, dim3(2048), dim3(256), 0, ctx->stream, alpha, (const double*)x_dev,
                        (double*)y_dev, n);
+    MC_HIP(hipGetLastError());
+  }
+  MC_API_END
+}
+
+int mc_cp_posupa(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, void* cm_dev, void* cp_dev, int nstate,
+                 double dtm, double dt) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c0_dev && c2_dev && cm_dev && cp_dev && nstate >= 0, MC_ERR_ARG, "bad argument");
+  MC_CHECK(cp_dev != c0_dev && cp_dev != c2_dev && cp_dev != cm_dev, MC_ERR_ARG, "cp must not alias c0, c2 or cm");
+  const size_t n = (size_t)ctx->ldg * nstate;
+  if (n) {
+    hipLaunchKernelGGL(mc::k_cp_posupa, dim3(2048), dim3(256), 0, ctx->stream, (const double2*)c0_dev,
+                       (const double2*)c2_dev, (double2*)cm_dev, (double2*)cp_dev, dtm, dt, n);
+    MC_HIP(hipGetLastError());
+  }
+  MC_API_END
+}
+
+int mc_cp_velocity(mc_ctx* ctx, const void* c0_dev, const void* cp_dev, void* cm_dev, int nstate, double dt) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c0_dev && cp_dev && cm_dev && nstate >= 0 && dt != 0.0, MC_ERR_ARG, "bad argument");
+  const size_t n = (size_t)ctx->ldg * nstate;
+  if (n) {
+    hipLaunchKernelGGL(mc::k_cp_velocity, dim3(2048), dim3(256), 0, ctx->stream, (const double2*)c0_dev,
+                       (const double2*)cp_dev, (double2*)cm_dev, 1.0 / dt, n);
     MC_HIP(hipGetLastError());
   }
   MC_API_END

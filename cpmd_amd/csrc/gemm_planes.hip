@@ -1,0 +1,355 @@
+// Split-bf16 A^T B contractions from pre-split operand planes (gfx950).
+//
+// The tall operands (wavefunctions, projectors: K = 2 ngw rows, a few hundred
+// columns) are split once into NP bf16 parts, x = p0 + p1 (+ p2) to 16 (24)
+// bits, and stored as "planes": blocks of 128 columns x 32 rows, the NP parts
+// of a block contiguous (NP x 8 KiB), blocks of one 128-column group
+// consecutive along K.  Inside a part, column c holds its 32 rows in 64 bytes
+// as four 16-byte chunks (k = 8q .. 8q+7), chunk q stored at q ^ ((c >> 2) & 3):
+// the MFMA operand reads (16 lanes = 16 columns, one chunk each) then hit 16
+// distinct 16-byte bank groups.
+//
+// The contraction streams the blocks of a 128 x 128 tile through a three-stage
+// LDS ring with global_load_lds_dwordx4 (each wave copies 1 KiB runs straight
+// into LDS, no registers), so two K slices are in flight while the third is
+// multiplied.  Each 32-row slice is summed by the MFMA from a zero
+// accumulator (the MFMA aligns its products to the accumulator it is given
+// and drops what falls below its last bit) and added to an FP32 accumulator
+// on the VALU; a workgroup covers at most 64 slices and the chunk partials are
+// reduced in FP64.  Products: NP = 3 sums p1 p1, p2 q0,
+// p0 q2, p1 q0, p0 q1, p0 q0 (six MFMAs, error ~2^-16 of |x y| per term and
+// unbiased); NP = 2 sums p1 q0, p0 q1, p0 q0.
+//
+// Diagonal of square overlaps: the products dropped by the six-term split
+// (p1 q2, p2 q1, p2 q2) have random signs also on a diagonal sum_k a_k^2, so
+// the diagonal carries no systematic bias (unlike a two-part split, whose
+// dropped lo*lo is always positive there) and needs no separate FP64 sum.
+//
+// MFMA operand layout (16x16x32 bf16): lane l holds A[i = l%16][k = 8(l/16) ..
+// 8(l/16)+7], B[k = 8(l/16) ..][j = l%16], and D[i = 4(l/16) + r][j = l%16] in
+// accumulator register r.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "mc_internal.h"
+
+namespace mc {
+
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+constexpr int kPT = 128;                  // columns per block / tile edge
+constexpr int kPK = 32;                   // rows per block (one MFMA depth)
+constexpr int kPart = kPT * kPK * 2;      // bytes of one part of one block
+constexpr int kStages = 3;
+constexpr int kXcdP = 8;
+constexpr int kMaxSlices = 64;            // K chunk (slices of 32 rows) per workgroup: FP32 accumulation
+
+__device__ __forceinline__ int swz(int c, int q) { return q ^ ((c >> 2) & 3); }
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// 16 bytes per lane from src straight into LDS at dst + 16 * lane (dst
+// uniform over the wave); outside the kernel template so that the host pass
+// does not instantiate the device-only builtin
+__device__ __forceinline__ void lds_copy16(const void* src, lds_void* dst) {
+  __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+}
+
+__device__ __forceinline__ int xcd_tile_p(int total) {
+  const int id = blockIdx.x;
+  const int per = (total + kXcdP - 1) / kXcdP;
+  return (id % kXcdP) * per + id / kXcdP;
+}
+
+// split 8 FP32 values into NP bf16 parts (round to nearest at each step)
+template <int NP>
+__device__ __forceinline__ void split_parts(float (&v)[8], bf8 (&p)[NP]) {
+#pragma unroll
+  for (int q = 0; q < NP; ++q)
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const __bf16 h = (__bf16)v[u];
+      p[q][u] = h;
+      v[u] -= (float)h;
+    }
+}
+
+template <int NP>
+__device__ __forceinline__ void store_parts(__bf16* out, size_t blk, int c, int q, const bf8 (&p)[NP]) {
+#pragma unroll
+  for (int r = 0; r < NP; ++r)
+    *reinterpret_cast<bf8*>(out + (blk * NP + r) * (kPart / 2) + c * kPK + swz(c, q) * 8) = p[r];
+}
+
+}  // namespace
+
+// X (K x n, column-major, leading dimension ldx doubles; ldx and the column
+// starts 16-byte aligned) -> planes.  One thread per (block, column, chunk),
+// chunk fastest: four lanes read one column's 256 contiguous bytes.
+template <int NP>
+__global__ __launch_bounds__(256) void k_split_planes(const double* __restrict__ X, int ldx, int K, int n, int nkb,
+                                                      size_t nblk, __bf16* __restrict__ out) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const int q = (int)(t & 3), c = (int)((t >> 2) & (kPT - 1));
+  const size_t blk = t >> 9;
+  if (blk >= nblk) return;
+  const int cb = (int)(blk / nkb), kb = (int)(blk - (size_t)cb * nkb);
+  const int col = cb * kPT + c, k0 = kb * kPK + 8 * q;
+  float v[8];
+  if (col < n && k0 + 8 <= K) {
+    const double2* src = reinterpret_cast<const double2*>(X + (size_t)col * ldx + k0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double2 d = src[u];
+      v[2 * u] = (float)d.x;
+      v[2 * u + 1] = (float)d.y;
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = (col < n && k0 + u < K) ? (float)X[(size_t)col * ldx + k0 + u] : 0.f;
+  }
+  bf8 p[NP];
+  split_parts<NP>(v, p);
+  store_parts<NP>(out, blk, c, q, p);
+}
+
+// The projector operand of rnlsm1 / rnlsm2 (as ProjA in gemm.hip) -> planes:
+// row 2g / 2g+1 = Re / Im of (-i)^l tw_g eigr(atom, g), tw_g = twnl_g [gk_g]
+// (halved at G = 0).
+template <int NP>
+__global__ __launch_bounds__(256) void k_proj_planes(ProjA pa, int K, int m, int nkb, size_t nblk,
+                                                     __bf16* __restrict__ out) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const int q = (int)(t & 3), c = (int)((t >> 2) & (kPT - 1));
+  const size_t blk = t >> 9;
+  if (blk >= nblk) return;
+  const int cb = (int)(blk / nkb), kb = (int)(blk - (size_t)cb * nkb);
+  const int col = cb * kPT + c, k0 = kb * kPK + 8 * q;
+  float v[8];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int g = (k0 >> 1) + u;
+    double re = 0.0, im = 0.0;
+    if (col < m && 2 * g + 1 < K) {
+      const double2 e = pa.eigr[(size_t)col * pa.ldg + g];
+      double tw = pa.twnl[g];
+      if (pa.gkk) tw *= pa.gkk[g];
+      if (pa.geq0 && g == 0) tw *= 0.5;
+      switch (pa.lmod4) {   // (-i)^l * tw * e
+        case 0: re = tw * e.x; im = tw * e.y; break;
+        case 1: re = tw * e.y; im = -tw * e.x; break;
+        case 2: re = -tw * e.x; im = -tw * e.y; break;
+        default: re = -tw * e.y; im = tw * e.x; break;
+      }
+    }
+    v[2 * u] = (float)re;
+    v[2 * u + 1] = (float)im;
+  }
+  bf8 p[NP];
+  split_parts<NP>(v, p);
+  store_parts<NP>(out, blk, c, q, p);
+}
+
+// ---------------------------------------------------------------------------
+// A^T B partials from planes: P[s] (m x n, FP64) over the K blocks
+// [s kbper, (s+1) kbper); launch ceil(total/8)*8 workgroups of 256, total =
+// nsplit * tm * tn; upper: skip tiles below the diagonal.
+// ---------------------------------------------------------------------------
+template <int NP>
+__global__ __launch_bounds__(256, 1) void k_tn_planes(const __bf16* __restrict__ Apl, const __bf16* __restrict__ Bpl,
+                                                      int nkb, int m, int n, int kbper, double* __restrict__ P,
+                                                      int upper, int tm, int tn, int total) {
+  constexpr int kStage = 2 * NP * kPart;           // bytes of one ring stage (A block, B block)
+  constexpr int kRuns = 2 * NP * kPart / 1024;     // 1 KiB runs per stage
+  static_assert(kRuns % 4 == 0, "runs per wave");
+  constexpr int kPer = kRuns / 4;                  // copies per wave and stage
+  static_assert(kPer < 64, "vmcnt range");
+  constexpr int kWait1 = (kPer & 15) | 0x70 | 0xF00 | ((kPer >> 4) << 14);
+  constexpr int kWait0 = 0x70 | 0xF00;
+  __shared__ __attribute__((aligned(16))) unsigned char ring[kStages * kStage];
+  const int t = xcd_tile_p(total);
+  if (t >= total) return;
+  const int s = t / (tm * tn);
+  const int tile = t - s * tm * tn;
+  const int ti = tile / tn, tj = tile - ti * tn;
+  if (upper && ti > tj) return;
+  const int kb0 = s * kbper, kb1 = min(nkb, kb0 + kbper);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const unsigned char* Ab = reinterpret_cast<const unsigned char*>(Apl) + (size_t)ti * nkb * NP * kPart;
+  const unsigned char* Bb = reinterpret_cast<const unsigned char*>(Bpl) + (size_t)tj * nkb * NP * kPart;
+  // wave w copies runs w, w + 4, ... of a stage: run j < NP*8 from the A
+  // block, the rest from the B block
+  auto issue = [&](int kb, int stage) {
+#pragma unroll
+    for (int i = 0; i < kRuns / 4; ++i) {
+      const int j = wave + 4 * i;
+      const bool isb = j >= NP * 8;
+      const unsigned char* src =
+          (isb ? Bb : Ab) + (size_t)kb * NP * kPart + (size_t)(isb ? j - NP * 8 : j) * 1024 + lane * 16;
+      lds_copy16(src, (lds_void*)(ring + stage * kStage + j * 1024));
+    }
+  };
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  const int li = lane & 15, lq = lane >> 4;
+  int offa[4], offb[4];   // byte offsets of this lane's chunk in a part
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int r = wr + 16 * a + li, c = wc + 16 * a + li;
+    offa[a] = r * (kPK * 2) + swz(r, lq) * 16;
+    offb[a] = c * (kPK * 2) + swz(c, lq) * 16;
+  }
+  // Accumulation: the six MFMAs of one slice and element start from zero,
+  // small products first, so that none is added to a larger partial sum
+  // (the MFMA aligns its products to the accumulator it is given and drops
+  // what falls below its last bit, a same-signed loss on sums of squares);
+  // the slice sums go to an FP32 accumulator on the VALU.  The host keeps K
+  // chunks to at most kMaxSlices slices and reduces the partials in FP64.
+  f4 acc32[4][4];
+  const f4 z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc32[a][b] = z;
+  const int nsl = kb1 - kb0;
+  if (nsl > 0) issue(kb0, 0);
+  if (nsl > 1) issue(kb0 + 1, 1);
+#pragma unroll 1
+  for (int i = 0; i < nsl; ++i) {
+    // this wave's copies of slice i have landed once at most the next
+    // slice's copies are outstanding (s_waitcnt vmcnt(N), gfx9 encoding:
+    // vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt[5:4] << 14)
+    if (i + 1 < nsl) __builtin_amdgcn_s_waitcnt(kWait1);
+    else __builtin_amdgcn_s_waitcnt(kWait0);
+    // One barrier per slice, without a memory fence (a fence would wait for
+    // every outstanding copy and drain the ring; the "memory" clobber keeps
+    // the compiler from moving LDS accesses across it): past it, every
+    // wave's copies of slice i have landed and every wave is done with slice
+    // i - 1, whose stage is refilled with slice i + 2.
+    asm volatile("s_barrier" ::: "memory");
+    if (i + 2 < nsl) issue(kb0 + i + 2, (i + 2) % kStages);
+    const unsigned char* st = ring + (i % kStages) * kStage;
+    bf8 ap[NP][4];
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+      for (int a = 0; a < 4; ++a) ap[p][a] = *reinterpret_cast<const bf8*>(st + p * kPart + offa[a]);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      bf8 bp[NP];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) bp[p] = *reinterpret_cast<const bf8*>(st + (NP + p) * kPart + offb[b]);
+      f4 u[4];
+      // the four row blocks' chains in lockstep (independent consecutive MFMAs)
+      if (NP == 2) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) u[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[1][a], bp[0], z, 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) u[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[0][a], bp[1], u[a], 0, 0, 0);
+      } else {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) u[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[1][a], bp[1], z, 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+          u[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[NP - 1][a], bp[0], u[a], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+          u[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[0][a], bp[NP - 1], u[a], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) u[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[1][a], bp[0], u[a], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) u[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[0][a], bp[1], u[a], 0, 0, 0);
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) u[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[0][a], bp[0], u[a], 0, 0, 0);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) acc32[a][b] += u[a];
+    }
+  }
+  double* Ps = P + (size_t)s * m * n;
+  const int i0 = ti * kPT, j0 = tj * kPT;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int j = j0 + wc + 16 * b + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + wr + 16 * a + 4 * lq + r;
+        if (i < m && j < n) Ps[(size_t)j * m + i] = (double)acc32[a][b][r];
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+size_t planes_bytes(int K, int n, int np) {
+  const size_t nkb = (K + kPK - 1) / kPK, ncb = (n + kPT - 1) / kPT;
+  return nkb * ncb * (size_t)np * kPart;
+}
+
+void split_planes(mc_ctx* ctx, const double* X, int ldx, int K, int n, int np, DevBuf& out) {
+  MC_CHECK(ldx % 2 == 0 && (reinterpret_cast<uintptr_t>(X) & 15) == 0, MC_ERR_INTERNAL,
+           "split_planes: operand not 16-byte aligned");
+  const int nkb = (K + kPK - 1) / kPK;
+  const size_t nblk = (size_t)nkb * ((n + kPT - 1) / kPT);
+  out.ensure(planes_bytes(K, n, np));
+  const dim3 grid((unsigned)(nblk * 2));   // 512 threads per block
+  timer_begin(ctx, MC_KID_GEMM);
+  if (np == 3)
+    hipLaunchKernelGGL(k_split_planes<3>, grid, dim3(256), 0, ctx->stream, X, ldx, K, n, nkb, nblk, out.as<__bf16>());
+  else
+    hipLaunchKernelGGL(k_split_planes<2>, grid, dim3(256), 0, ctx->stream, X, ldx, K, n, nkb, nblk, out.as<__bf16>());
+  MC_HIP(hipGetLastError());
+  timer_end(ctx, MC_KID_GEMM);
+}
+
+void proj_planes(mc_ctx* ctx, const ProjA& pa, int K, int m, int np, DevBuf& out) {
+  const int nkb = (K + kPK - 1) / kPK;
+  const size_t nblk = (size_t)nkb * ((m + kPT - 1) / kPT);
+  out.ensure(planes_bytes(K, m, np));
+  const dim3 grid((unsigned)(nblk * 2));
+  timer_begin(ctx, MC_KID_GEMM);
+  if (np == 3)
+    hipLaunchKernelGGL(k_proj_planes<3>, grid, dim3(256), 0, ctx->stream, pa, K, m, nkb, nblk, out.as<__bf16>());
+  else
+    hipLaunchKernelGGL(k_proj_planes<2>, grid, dim3(256), 0, ctx->stream, pa, K, m, nkb, nblk, out.as<__bf16>());
+  MC_HIP(hipGetLastError());
+  timer_end(ctx, MC_KID_GEMM);
+}
+
+// out = alpha A^T B [- A(0,:)^T B(0,:)] from planes of A (K x m) and B (K x n);
+// A64 / B64 are the FP64 operands (row 0 for the G = 0 correction);
+// symmetric: upper tiles only, mirrored
+void gemm_tn_planes(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np, int K, int m, int n, double alpha,
+                    const double* A64, int lda, const double* B64, int ldb, bool g0corr, bool symmetric, double* out,
+                    int ldo) {
+  if (m <= 0 || n <= 0) return;
+  const int nkb = (K + kPK - 1) / kPK;
+  const int tm = (m + kPT - 1) / kPT, tn = (n + kPT - 1) / kPT;
+  const int tiles = symmetric ? (tm * (tm + 1)) / 2 : tm * tn;
+  // one workgroup per CU (the ring takes 96-144 KiB of LDS): at least two
+  // rounds of 256 workgroups, K chunks of at most kMaxSlices slices
+  int nsplit = std::max(1, std::min(nkb, std::max((512 + tiles - 1) / tiles, (nkb + kMaxSlices - 1) / kMaxSlices)));
+  const int kbper = (nkb + nsplit - 1) / nsplit;
+  nsplit = (nkb + kbper - 1) / kbper;
+  ctx->w_gemm.ensure(sizeof(double) * (size_t)nsplit * m * n);
+  const int total = nsplit * tm * tn;
+  const int launch = (total + kXcdP - 1) / kXcdP * kXcdP;
+  timer_begin(ctx, MC_KID_GEMM);
+  if (np == 3)
+    hipLaunchKernelGGL(k_tn_planes<3>, dim3(launch), dim3(256), 0, ctx->stream, Apl.as<__bf16>(), Bpl.as<__bf16>(),
+                       nkb, m, n, kbper, ctx->w_gemm.as<double>(), symmetric ? 1 : 0, tm, tn, total);
+  else
+    hipLaunchKernelGGL(k_tn_planes<2>, dim3(launch), dim3(256), 0, ctx->stream, Apl.as<__bf16>(), Bpl.as<__bf16>(),
+                       nkb, m, n, kbper, ctx->w_gemm.as<double>(), symmetric ? 1 : 0, tm, tn, total);
+  MC_HIP(hipGetLastError());
+  timer_end(ctx, MC_KID_GEMM);
+  reduce_tn(ctx, nsplit, m, n, alpha, A64, lda, B64, ldb, g0corr, symmetric, out, ldo);
+}
+
+}  // namespace mc

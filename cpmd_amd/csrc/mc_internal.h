@@ -78,6 +78,18 @@ struct DevBuf {
   T* as() const { return static_cast<T*>(p); }
 };
 
+// bf16 part planes of up to two tall operands (gemm_planes.hip).  Inside a
+// PlaneScope (a library function whose wavefunction operands do not change
+// while it runs) the planes of an operand are kept and reused by the next
+// product with the same operand; outside one they are rebuilt per product.
+struct PlaneCache {
+  bool scoped = false;
+  const void* src[2] = {nullptr, nullptr};
+  int np[2] = {0, 0}, K[2] = {0, 0}, n[2] = {0, 0};
+  DevBuf buf[2];
+  int next = 0;
+};
+
 // Per-kernel timing (HIP events around launches), enabled on request.
 struct KernelTimer {
   bool enabled = false;
@@ -189,6 +201,8 @@ struct mc_ctx {
   bool fft_fp32 = false;   // rhoofr/vpsi transforms and workspaces in FP32
   bool gemm_x3 = false;    // tall wavefunction contractions as split-bf16 MFMA products
   int x3_parts = 3;        // bf16 parts per operand of the split A^T B products (2 or 3)
+  mc::PlaneCache planes;   // bf16 planes of the wavefunction operands of the A^T B products
+  mc::DevBuf w_plp;        // bf16 planes of the projector operand
   mc::DensitySet dens;     // vofrho / ewald data
 
   int batch_pairs_vpsi = 8;
@@ -225,6 +239,19 @@ void reduce_tn(mc_ctx* ctx, int nsplit, int m, int n, double alpha, const double
                bool g0corr, bool symmetric, double* out, int ldo);
 // split-bf16 (3-term, FP32 accumulation) variants used in mixed precision (gemm_x3.hip)
 constexpr int kX3Min = 16384;   // rows of K (tn) or M (nn) from which the split kernels are used
+// bf16 part planes and the contraction that streams them (gemm_planes.hip)
+size_t planes_bytes(int K, int n, int np);
+void split_planes(mc_ctx* ctx, const double* X, int ldx, int K, int n, int np, DevBuf& out);
+void proj_planes(mc_ctx* ctx, const ProjA& pa, int K, int m, int np, DevBuf& out);
+void gemm_tn_planes(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np, int K, int m, int n, double alpha,
+                    const double* A64, int lda, const double* B64, int ldb, bool g0corr, bool symmetric, double* out,
+                    int ldo);
+// keeps operand planes for reuse while alive (see PlaneCache)
+struct PlaneScope {
+  mc_ctx* c;
+  explicit PlaneScope(mc_ctx* ctx);
+  ~PlaneScope();
+};
 void gemm_tn_x3(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, int K, int m, int n, double alpha,
                 bool g0corr, bool symmetric, double* out, int ldo, const ProjA* proj);
 void gemm_nn_x3(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n, double alpha,
@@ -244,6 +271,7 @@ void set_eigr(mc_ctx* ctx, const double* eigr, int lde, int nat);
 void rnlsm1_dev(mc_ctx* ctx, const double2* c0, int n, double* fnl_host);
 void set_gk(mc_ctx* ctx, const double* gk);
 void rnlsm2_dev(mc_ctx* ctx, const double2* c0, int n, double* dfnl_host);
+void rnlsm_dev(mc_ctx* ctx, const double2* c0, int n, bool tfor, double* fnl_host, double* dfnl_host);
 void rnlfor_dev(mc_ctx* ctx, const double* f_host, int n, const double* fnl_host, const double* dfnl_host,
                 double* fion_host);
 void fnonloc_dev(mc_ctx* ctx, double2* c2, const double* f_host, int n, const double* fnl_host);

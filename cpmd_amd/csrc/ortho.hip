@@ -376,6 +376,7 @@ void lagrange_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_h
 void rortog_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n, double epsog,
                 int maxit, int* iters) {
   if (n <= 0) return;
+  PlaneScope keep(ctx);   // c0 and cm are read-only until the final rotation
   double* rho = small(ctx, 0, n);
   double* sig = small(ctx, 1, n);
   double* del = small(ctx, 2, n);
@@ -512,6 +513,14 @@ void rnlsm1_dev(mc_ctx* ctx, const double2* c0, int n, double* fnl_host) {
                           ctx->stream));
     MC_HIP(hipStreamSynchronize(ctx->stream));
   }
+}
+
+// rnlsm (src/rnlsm_utils.mod.F90:23-49): rnlsm1 and, with tfor, rnlsm2 of the
+// same coefficients, sharing the split-bf16 planes of c0 in mixed precision
+void rnlsm_dev(mc_ctx* ctx, const double2* c0, int n, bool tfor, double* fnl_host, double* dfnl_host) {
+  PlaneScope keep(ctx);
+  rnlsm1_dev(ctx, c0, n, fnl_host);
+  if (tfor) rnlsm2_dev(ctx, c0, n, dfnl_host);
 }
 
 void set_gk(mc_ctx* ctx, const double* gk) {

@@ -70,7 +70,7 @@ class KSDevice:
         n = len(f)
         if tau is not None and (self._tau is None or not np.array_equal(tau, self._tau)):
             self.set_ions(tau)
-        fnl = ctx.rnlsm1_dev(d_c0, n, want_fnl=True)
+        fnl, _ = ctx.rnlsm_dev(d_c0, n, tfor=forces, want_fnl=True)
         ekin, _, _ = ctx.rhoofr_dev(d_c0, f, n, self.d_rho)
         en, floc = ctx.vofrho_dev(self.d_rho, self.d_v, forces=forces)
         esr, eself, fesr = ctx.ewald(self.iesr, forces=forces)
@@ -83,6 +83,5 @@ class KSDevice:
             ctx.vpsi_dev(d_c0, gradient, f, self.d_v, n)
             ctx.fnonloc_dev(gradient, f, n)
         if forces:
-            ctx.rnlsm2_dev(d_c0, n)
             out["fion"] = ctx.rnlfor(f, fion=floc + fesr)
         return out

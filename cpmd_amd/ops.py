@@ -139,6 +139,15 @@ class Context:
         """y = alpha x + beta y."""
         check(lib().mc_wfn_axpby(self.h, alpha, x.ptr, beta, y.ptr, nstate))
 
+    def cp_posupa(self, c0: DeviceArray, c2: DeviceArray, cm: DeviceArray, cp: DeviceArray, nstate: int,
+                  dtm: float, dt: float):
+        """cm += dtm c2; cp = c0 + dt cm (velupa + posupa before rortog)."""
+        check(lib().mc_cp_posupa(self.h, c0.ptr, c2.ptr, cm.ptr, cp.ptr, nstate, dtm, dt))
+
+    def cp_velocity(self, c0: DeviceArray, cp: DeviceArray, cm: DeviceArray, nstate: int, dt: float):
+        """cm = (cp - c0) / dt (velocities after rortog)."""
+        check(lib().mc_cp_velocity(self.h, c0.ptr, cp.ptr, cm.ptr, nstate, dt))
+
     def dotp_wfn(self, a: DeviceArray, b: DeviceArray, nstate: int) -> np.ndarray:
         out = np.zeros(nstate)
         check(lib().mc_wfn_dotp(self.h, a.ptr, b.ptr, nstate, _p(out)))
@@ -313,6 +322,16 @@ class Context:
         dfnl = np.zeros((self.nat, self.nhxs, 3, nstate), order="F") if want_dfnl else None
         check(lib().mc_rnlsm2_dev(self.h, c0.ptr, nstate, _p(dfnl) if want_dfnl else None))
         return dfnl
+
+    def rnlsm_dev(self, c0: DeviceArray, nstate: int, tfor: bool, want_fnl: bool = False,
+                  want_dfnl: bool = False):
+        """rnlsm1 (+ rnlsm2 with tfor) of the same coefficients; returns (fnl, dfnl)
+        host copies as requested (None otherwise)."""
+        fnl = np.zeros((self.nat, self.nhxs, nstate), order="F") if want_fnl else None
+        dfnl = np.zeros((self.nat, self.nhxs, 3, nstate), order="F") if (tfor and want_dfnl) else None
+        check(lib().mc_rnlsm_dev(self.h, c0.ptr, nstate, 1 if tfor else 0, _p(fnl) if fnl is not None else None,
+                                 _p(dfnl) if dfnl is not None else None))
+        return fnl, dfnl
 
     def rnlfor(self, f: np.ndarray, fnl: np.ndarray | None = None, dfnl: np.ndarray | None = None,
                fion: np.ndarray | None = None) -> np.ndarray:

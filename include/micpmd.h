@@ -130,6 +130,17 @@ int mc_wfn_axpy(mc_ctx* ctx, double alpha, const void* x_dev, void* y_dev, int n
  * src/posupa_utils.mod.F90:44-187, src/velupa_utils.mod.F90:25-72). */
 int mc_wfn_axpby(mc_ctx* ctx, double alpha, const void* x_dev, double beta, void* y_dev, int nstate);
 
+/* One CP step of the electrons before the orthonormality constraint, in one
+ * pass: cm += dtm c2 (velupa, src/velupa_utils.mod.F90:25-72, dtm = nnlst
+ * dt/(2 emass) in CPMD's notation) and cp = c0 + dt cm (posupa,
+ * src/posupa_utils.mod.F90:69-73).  cp must not alias c0, c2 or cm. */
+int mc_cp_posupa(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, void* cm_dev, void* cp_dev, int nstate,
+                 double dtm, double dt);
+/* Velocities after rortog: cm = (cp - c0) / dt, the same as CPMD's
+ * cm += c0 gamx / dt (src/posupa_utils.mod.F90:150-158) when rortog returned
+ * cp = c0 + dt cm + c0 gamx. */
+int mc_cp_velocity(mc_ctx* ctx, const void* c0_dev, const void* cp_dev, void* cm_dev, int nstate, double dt);
+
 /* out[i] = dotp(a_i, b_i), the Gamma-point inner product of state i
  * (src/dotp_utils.mod.F90:58-101); with b = c2 after vpsi + fnonloc,
  * -sum_i out[i] is the band energy sum_i f_i <c_i|H|c_i>. */
@@ -261,6 +272,13 @@ int mc_set_gk(mc_ctx* ctx, const double* gk);
  * Replaces rnlsm2(c0,nstate,ikpt,ikind), src/rnlsm2_utils.mod.F90:321-482 (Gamma). */
 int mc_rnlsm2(mc_ctx* ctx, const double* c0, int ldc, int nstate, double* dfnl);
 int mc_rnlsm2_dev(mc_ctx* ctx, const void* c0_dev, int nstate, double* dfnl);
+
+/* rnlsm: rnlsm1 and, if tfor != 0, rnlsm2 of the same device coefficients in
+ * one call (in mixed precision the bf16 planes of c0 are built once for all
+ * four contractions); fnl / dfnl as in mc_rnlsm1_dev / mc_rnlsm2_dev, each may
+ * be NULL.  Replaces rnlsm(c0,nstate,ikpt,ikind,tfor),
+ * src/rnlsm_utils.mod.F90:23-49. */
+int mc_rnlsm_dev(mc_ctx* ctx, const void* c0_dev, int nstate, int tfor, double* fnl, double* dfnl);
 
 /* rnlfor: fion(k, isa) -= sum_i 2 f_i sum_{iv,jv} h(iv,jv) dfnl(isa,iv,k,i)
  * fnl(isa,jv,i) (states with |f_i| <= 1e-12 skipped), i.e. adds the nonlocal

@@ -215,3 +215,28 @@ def test_rnlsm2_rnlfor(gpu, kind):
     base = np.ones((3, OP.nat))
     got = ctx.rnlfor(f, fion=base)                                   # device copies, accumulates
     assert rel(got - base, ofion) < 1e-12
+
+
+def test_cp_posupa_velocity(gpu):
+    """velupa + posupa (cm += dtm c2, cp = c0 + dt cm) and cm = (cp - c0)/dt
+    against numpy on the same coefficients (two roundings per element)."""
+    cell, gs, c = small_system("ortho", 5)
+    ctx = ops.Context(cell, gs)
+    rng = np.random.default_rng(17)
+    shape = c.shape
+    c2 = np.asfortranarray(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+    cm = np.asfortranarray(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+    n = shape[1]
+    d0, d2, dm = ctx.upload(c), ctx.upload(c2), ctx.upload(cm)
+    dp = ctx.alloc_wfn(n)
+    dtm, dt = 0.0125, 4.0
+    ctx.cp_posupa(d0, d2, dm, dp, n, dtm, dt)
+    cm_ref = cm + dtm * c2
+    cp_ref = c + dt * cm_ref
+    assert np.abs(ctx.download(dm, n) - cm_ref).max() < 1e-14 * np.abs(cm_ref).max()
+    assert np.abs(ctx.download(dp, n) - cp_ref).max() < 1e-14 * np.abs(cp_ref).max()
+    ctx.cp_velocity(d0, dp, dm, n, dt)
+    v_ref = (cp_ref - c) / dt
+    assert np.abs(ctx.download(dm, n) - v_ref).max() < 1e-13 * np.abs(v_ref).max()
+    with pytest.raises(_lib.McError):
+        ctx.cp_posupa(d0, d2, dm, d0, n, dtm, dt)      # cp aliasing c0 is refused

@@ -62,8 +62,11 @@ def test_mixed_ovlap(gpu, cfg2_130):
         dg = np.abs(np.diag(got - ref))
         off = err - np.diag(dg)
         msg = f"diag err max {dg.max():.3e} off-diagonal err max {off.max():.3e}"
-        assert dg.max() < 1e-13, msg                   # FP64 diagonal
-        assert off.max() < 3e-7, msg                   # near-orthonormal: small entries, random-sign errors
+        # six-term split, FP32 chunk sums: off-diagonal errors at the 1e-8
+        # level; on the diagonal (sums of squares, entries ~1) the MFMA's
+        # alignment of each product to the running sum costs a few 1e-8
+        assert off.max() < 3e-8, msg
+        assert dg.max() < 1e-7, msg
         assert off.max() > 0.0                         # the split kernels ran
     s = ctx.ovlap(cp, cp)
     assert np.array_equal(s, s.T)
@@ -138,3 +141,32 @@ def test_mixed_projectors(gpu):
     for part in (np.real, np.imag):
         err2 = np.abs(part(out) - part(oout))
         assert np.all(err2 <= (U_SPLIT + nat * U32) * bnd + 1e-15 * np.abs(oout).max())
+
+
+def test_mixed_rnlsm_combined(gpu):
+    """rnlsm (fnl + dfnl in one call, shared bf16 planes of c0) equals the
+    separate rnlsm1 / rnlsm2 calls exactly, in FP64 and in mixed precision, and
+    the mixed dfnl stays at FP32-level error against the oracle."""
+    sysm = S.make_system(24.0, 64, 70.0, nstate=64)
+    c = S.initial_coefficients(sysm, seed=43)
+    ngw = sysm.gset.ngw
+    na, nhxs, ngh, lang, hmat, twnl, eigr = S.projector_set(sysm, sysm.gset.miller[:ngw], sysm.gset.hg[:ngw])
+    OP = oracle.Projectors(na=na, nhxs=nhxs, ngh=ngh, lang=lang, hmat=hmat, twnl=twnl, eigr=eigr)
+    gk = (sysm.gset.miller[:ngw] @ sysm.cell.bvec).T
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    ctx.set_projectors(na, nhxs, ngh, lang, hmat, twnl)
+    ctx.set_eigr(eigr)
+    ctx.set_gk(gk)
+    n = c.shape[1]
+    d = ctx.upload(c)
+    for mode in ("fp64", "mixed"):
+        ctx.set_precision(mode)
+        fnl, dfnl = ctx.rnlsm_dev(d, n, tfor=True, want_fnl=True, want_dfnl=True)
+        fnl1 = ctx.rnlsm1_dev(d, n, want_fnl=True)
+        dfnl1 = ctx.rnlsm2_dev(d, n, want_dfnl=True)
+        assert np.array_equal(fnl, fnl1) and np.array_equal(dfnl, dfnl1), mode
+        fnl0, _ = ctx.rnlsm_dev(d, n, tfor=False, want_fnl=True)
+        assert np.array_equal(fnl0, fnl1)
+    odfnl = oracle.rnlsm2(OP, c, gk, sysm.cell.tpiba)
+    rel = np.sqrt(np.mean((dfnl - odfnl) ** 2) / np.mean(odfnl ** 2))
+    assert rel < 5e-7, rel
