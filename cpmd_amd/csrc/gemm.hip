@@ -181,7 +181,9 @@ __global__ __launch_bounds__(256) void k_gemm_tn(const double* __restrict__ A, i
 
 // out(i,j) = alpha * sum_s P[s](i,j) - g0 * a0(i) b0(j); symmetric: only the
 // upper triangle is used and mirrored into the lower one (dmatc).
-__global__ void k_reduce_tn(const double* __restrict__ P, int nsplit, int m, int n, double alpha,
+// out(i, j) = alpha sum_q P[q](i, j) [- A(0, i) B(0, j)], partials P[q] of
+// ldp rows (the first m used), mirrored from the upper triangle if symmetric
+__global__ void k_reduce_tn(const double* __restrict__ P, int nsplit, int ldp, int m, int n, double alpha,
                             const double* __restrict__ a0, int lda, const double* __restrict__ b0, int ldb,
                             int g0, int symmetric, double* __restrict__ out, int ldo) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -190,7 +192,7 @@ __global__ void k_reduce_tn(const double* __restrict__ P, int nsplit, int m, int
   int si = i, sj = j;
   if (symmetric && i > j) { si = j; sj = i; }
   double s = 0.0;
-  for (int q = 0; q < nsplit; ++q) s += P[(size_t)q * m * n + (size_t)sj * m + si];
+  for (int q = 0; q < nsplit; ++q) s += P[(size_t)q * ldp * n + (size_t)sj * ldp + si];
   double v = alpha * s;
   if (g0) v -= a0[(size_t)si * lda] * b0[(size_t)sj * ldb];
   out[(size_t)j * ldo + i] = v;
@@ -340,7 +342,15 @@ void reduce_tn(mc_ctx* ctx, int nsplit, int m, int n, double alpha, const double
                bool g0corr, bool symmetric, double* out, int ldo) {
   timer_begin(ctx, MC_KID_MISC);
   hipLaunchKernelGGL(k_reduce_tn, dim3((m + 255) / 256, n), dim3(256), 0, ctx->stream, ctx->w_gemm.as<double>(),
-                     nsplit, m, n, alpha, A, lda, B, ldb, g0corr ? 1 : 0, symmetric ? 1 : 0, out, ldo);
+                     nsplit, m, m, n, alpha, A, lda, B, ldb, g0corr ? 1 : 0, symmetric ? 1 : 0, out, ldo);
+  MC_HIP(hipGetLastError());
+  timer_end(ctx, MC_KID_MISC);
+}
+
+void reduce_tn_rows(mc_ctx* ctx, int nsplit, int ldp, int r0, int m, int n, double alpha, double* out, int ldo) {
+  timer_begin(ctx, MC_KID_MISC);
+  hipLaunchKernelGGL(k_reduce_tn, dim3((m + 255) / 256, n), dim3(256), 0, ctx->stream, ctx->w_gemm.as<double>() + r0,
+                     nsplit, ldp, m, n, alpha, (const double*)nullptr, 0, (const double*)nullptr, 0, 0, 0, out, ldo);
   MC_HIP(hipGetLastError());
   timer_end(ctx, MC_KID_MISC);
 }

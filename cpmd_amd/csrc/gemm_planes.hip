@@ -154,6 +154,56 @@ __global__ __launch_bounds__(256) void k_proj_planes(ProjA pa, int K, int m, int
   store_parts<NP>(out, blk, c, q, p);
 }
 
+// Stacked projector variants (one thread per column, chunk of the first
+// variant's geometry; eigr, twnl read once for all nv variants): variant v of
+// column c at column block v * ncbv + c / 128; lbase = l mod 4 of the group.
+template <int NP>
+__global__ __launch_bounds__(256) void k_proj_planes_stacked(ProjA pa, const double* __restrict__ gk, int lbase, int v0,
+                                                             int nv, int K, int m, int nkb, int ncbv, size_t nblk,
+                                                             __bf16* __restrict__ out) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const int q = (int)(t & 3), c = (int)((t >> 2) & (kPT - 1));
+  const size_t blk = t >> 9;
+  if (blk >= nblk) return;
+  const int cb = (int)(blk / nkb), kb = (int)(blk - (size_t)cb * nkb);
+  const int col = cb * kPT + c, k0 = kb * kPK + 8 * q;
+  double er[4], ei[4], tw[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int g = (k0 >> 1) + u;
+    er[u] = ei[u] = tw[u] = 0.0;
+    if (col < m && 2 * g + 1 < K) {
+      const double2 e = pa.eigr[(size_t)col * pa.ldg + g];
+      er[u] = e.x;
+      ei[u] = e.y;
+      tw[u] = pa.twnl[g] * ((pa.geq0 && g == 0) ? 0.5 : 1.0);
+    }
+  }
+  for (int vv = 0; vv < nv; ++vv) {
+    const int v = v0 + vv;
+    const int lm = (lbase + (v > 0 ? 1 : 0)) & 3;
+    float x[8];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int g = (k0 >> 1) + u;
+      double w = tw[u];
+      if (v > 0 && w != 0.0) w *= gk[(size_t)(v - 1) * pa.ldg + g];
+      double re, im;
+      switch (lm) {   // (-i)^l * w * e
+        case 0: re = w * er[u]; im = w * ei[u]; break;
+        case 1: re = w * ei[u]; im = -w * er[u]; break;
+        case 2: re = -w * er[u]; im = -w * ei[u]; break;
+        default: re = -w * ei[u]; im = w * er[u]; break;
+      }
+      x[2 * u] = (float)re;
+      x[2 * u + 1] = (float)im;
+    }
+    bf8 p[NP];
+    split_parts<NP>(x, p);
+    store_parts<NP>(out, ((size_t)vv * ncbv + cb) * nkb + kb, c, q, p);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // A^T B partials from planes: P[s] (m x n, FP64) over the K blocks
 // [s kbper, (s+1) kbper); launch ceil(total/8)*8 workgroups of 256, total =
@@ -322,13 +372,25 @@ void proj_planes(mc_ctx* ctx, const ProjA& pa, int K, int m, int np, DevBuf& out
   timer_end(ctx, MC_KID_GEMM);
 }
 
-// out = alpha A^T B [- A(0,:)^T B(0,:)] from planes of A (K x m) and B (K x n);
-// A64 / B64 are the FP64 operands (row 0 for the G = 0 correction);
-// symmetric: upper tiles only, mirrored
-void gemm_tn_planes(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np, int K, int m, int n, double alpha,
-                    const double* A64, int lda, const double* B64, int ldb, bool g0corr, bool symmetric, double* out,
-                    int ldo) {
-  if (m <= 0 || n <= 0) return;
+void proj_planes_stacked(mc_ctx* ctx, const ProjA& base, const double* gk, int v0, int nv, int K, int na, int np,
+                         DevBuf& out) {
+  const int nkb = (K + kPK - 1) / kPK, ncbv = (na + kPT - 1) / kPT;
+  const size_t nblk = (size_t)nkb * ncbv;
+  out.ensure(nblk * nv * (size_t)np * kPart);
+  const dim3 grid((unsigned)(nblk * 2));
+  timer_begin(ctx, MC_KID_GEMM);
+  if (np == 3)
+    hipLaunchKernelGGL(k_proj_planes_stacked<3>, grid, dim3(256), 0, ctx->stream, base, gk, base.lmod4, v0, nv, K, na,
+                       nkb, ncbv, nblk, out.as<__bf16>());
+  else
+    hipLaunchKernelGGL(k_proj_planes_stacked<2>, grid, dim3(256), 0, ctx->stream, base, gk, base.lmod4, v0, nv, K, na,
+                       nkb, ncbv, nblk, out.as<__bf16>());
+  MC_HIP(hipGetLastError());
+  timer_end(ctx, MC_KID_GEMM);
+}
+
+int tn_planes_partials(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np, int K, int m, int n,
+                       bool symmetric) {
   const int nkb = (K + kPK - 1) / kPK;
   const int tm = (m + kPT - 1) / kPT, tn = (n + kPT - 1) / kPT;
   const int tiles = symmetric ? (tm * (tm + 1)) / 2 : tm * tn;
@@ -349,6 +411,17 @@ void gemm_tn_planes(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np, i
                        nkb, m, n, kbper, ctx->w_gemm.as<double>(), symmetric ? 1 : 0, tm, tn, total);
   MC_HIP(hipGetLastError());
   timer_end(ctx, MC_KID_GEMM);
+  return nsplit;
+}
+
+// out = alpha A^T B [- A(0,:)^T B(0,:)] from planes of A (K x m) and B (K x n);
+// A64 / B64 are the FP64 operands (row 0 for the G = 0 correction);
+// symmetric: upper tiles only, mirrored
+void gemm_tn_planes(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np, int K, int m, int n, double alpha,
+                    const double* A64, int lda, const double* B64, int ldb, bool g0corr, bool symmetric, double* out,
+                    int ldo) {
+  if (m <= 0 || n <= 0) return;
+  const int nsplit = tn_planes_partials(ctx, Apl, Bpl, np, K, m, n, symmetric);
   reduce_tn(ctx, nsplit, m, n, alpha, A64, lda, B64, ldb, g0corr, symmetric, out, ldo);
 }
 

@@ -252,6 +252,23 @@ static const DevBuf& planes_for(mc_ctx* ctx, const double* X, int ldx, int K, in
   return C.buf[i];
 }
 
+// fnl / dfnl of one projector group in mixed precision: the nv stacked
+// projector variants [v0, v0 + nv) against the planes of B in one contraction,
+// variant vv reduced into outs[vv] (na x n, leading dimension ldos[vv], scale
+// alphas[vv])
+void proj_contract_x3(mc_ctx* ctx, const ProjA& base, const double* gk, int v0, int nv, const double* B, int ldb,
+                      int K, int na, int n, double* const* outs, const int* ldos, const double* alphas) {
+  if (na <= 0 || n <= 0 || nv <= 0) return;
+  MC_CHECK(ldb % 2 == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0, MC_ERR_INTERNAL,
+           "split-bf16 projector products need an even, 16-byte aligned wavefunction operand");
+  const int np = ctx->x3_parts == 2 ? 2 : 3;
+  const int mv = (na + kT - 1) / kT * kT;   // stacked rows per variant
+  proj_planes_stacked(ctx, base, gk, v0, nv, K, na, np, ctx->w_plp);
+  const DevBuf& Bp = planes_for(ctx, B, ldb, K, n, np, nullptr);
+  const int nsplit = tn_planes_partials(ctx, ctx->w_plp, Bp, np, K, nv * mv, n, false);
+  for (int vv = 0; vv < nv; ++vv) reduce_tn_rows(ctx, nsplit, nv * mv, vv * mv, na, n, alphas[vv], outs[vv], ldos[vv]);
+}
+
 // The A^T B products of the mixed-precision mode (overlaps: orthonormality;
 // fnl, dfnl: E_NL and the nonlocal forces): both operands as bf16 planes
 // (ctx->x3_parts parts each: 3 -> FP32-level products, 2 -> 3-term products)

@@ -237,6 +237,8 @@ void gemm_nn(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, i
 // out = alpha * sum of the nsplit partials in ctx->w_gemm (- G=0 term), mirrored if symmetric
 void reduce_tn(mc_ctx* ctx, int nsplit, int m, int n, double alpha, const double* A, int lda, const double* B, int ldb,
                bool g0corr, bool symmetric, double* out, int ldo);
+// rows [r0, r0 + m) of partials with ldp rows: out = alpha sum of the nsplit partials
+void reduce_tn_rows(mc_ctx* ctx, int nsplit, int ldp, int r0, int m, int n, double alpha, double* out, int ldo);
 // split-bf16 (3-term, FP32 accumulation) variants used in mixed precision (gemm_x3.hip)
 constexpr int kX3Min = 16384;   // rows of K (tn) or M (nn) from which the split kernels are used
 // bf16 part planes and the contraction that streams them (gemm_planes.hip)
@@ -246,12 +248,22 @@ void proj_planes(mc_ctx* ctx, const ProjA& pa, int K, int m, int np, DevBuf& out
 void gemm_tn_planes(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np, int K, int m, int n, double alpha,
                     const double* A64, int lda, const double* B64, int ldb, bool g0corr, bool symmetric, double* out,
                     int ldo);
+// partials of A^T B from planes into ctx->w_gemm (m x n each); returns their number
+int tn_planes_partials(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np, int K, int m, int n, bool symmetric);
+// planes of the projector variants of one (species, projector) group, stacked
+// along the columns in blocks of ceil(na/128)*128: variant v = 0 (fnl:
+// (-i)^l twnl eigr) and v = 1..3 (dfnl: (-i)^(l+1) twnl gk_v eigr), for
+// v in [v0, v0 + nv)
+void proj_planes_stacked(mc_ctx* ctx, const ProjA& base, const double* gk, int v0, int nv, int K, int na, int np,
+                         DevBuf& out);
 // keeps operand planes for reuse while alive (see PlaneCache)
 struct PlaneScope {
   mc_ctx* c;
   explicit PlaneScope(mc_ctx* ctx);
   ~PlaneScope();
 };
+void proj_contract_x3(mc_ctx* ctx, const ProjA& base, const double* gk, int v0, int nv, const double* B, int ldb,
+                      int K, int na, int n, double* const* outs, const int* ldos, const double* alphas);
 void gemm_tn_x3(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, int K, int m, int n, double alpha,
                 bool g0corr, bool symmetric, double* out, int ldo, const ProjA* proj);
 void gemm_nn_x3(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n, double alpha,

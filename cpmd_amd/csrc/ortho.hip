@@ -484,15 +484,36 @@ void set_eigr(mc_ctx* ctx, const double* eigr, int lde, int nat) {
   MC_HIP(hipStreamSynchronize(ctx->stream));
 }
 
-void rnlsm1_dev(mc_ctx* ctx, const double2* c0, int n, double* fnl_host) {
+// fnl(isa, iv, i) = 2 Re sum_G conj((-i)^l twnl eigr) c (rnlsm1,
+// src/rnlsm1_utils.mod.F90:42-193) and dfnl(isa, iv, k, i) = 2 tpiba Re sum_G
+// conj(gk_k twnl (-i)^(l+1) eigr) c (rnlsm2, src/rnlsm2_utils.mod.F90:321-482),
+// G = 0 halved, per (species, projector) group.  In mixed precision the
+// requested variants of a group are one stacked split-bf16 contraction over
+// the planes of c0.
+static void rnlsm_groups(mc_ctx* ctx, const double2* c0, int n, bool do_fnl, bool do_dfnl, double* fnl_host,
+                         double* dfnl_host) {
   auto& P = ctx->proj;
-  MC_CHECK(P.ready, MC_ERR_ARG, "rnlsm1: projectors not set (mc_set_projectors)");
-  MC_CHECK(P.eigr_nat == P.nat, MC_ERR_ARG, "rnlsm1: structure factors not set for all atoms (mc_set_eigr)");
+  MC_CHECK(P.ready, MC_ERR_ARG, "rnlsm: projectors not set (mc_set_projectors)");
+  MC_CHECK(P.eigr_nat == P.nat, MC_ERR_ARG, "rnlsm: structure factors not set for all atoms (mc_set_eigr)");
+  MC_CHECK(!do_dfnl || P.gk_ready, MC_ERR_ARG, "rnlsm2: G vectors not set (mc_set_gk)");
   const size_t nf = (size_t)P.nat * P.nhxs * std::max(n, 1);
-  P.d_fnl.ensure(sizeof(double) * nf);
-  P.fnl_nstate = n;
-  hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, ctx->stream, P.d_fnl.as<double>(), nf);
-  MC_HIP(hipGetLastError());
+  if (do_fnl) {
+    P.d_fnl.ensure(sizeof(double) * nf);
+    P.fnl_nstate = n;
+    hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, ctx->stream, P.d_fnl.as<double>(), nf);
+    MC_HIP(hipGetLastError());
+  }
+  if (do_dfnl) {
+    P.d_dfnl.ensure(sizeof(double) * nf * 3);
+    P.dfnl_nstate = n;
+    hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, ctx->stream, P.d_dfnl.as<double>(), nf * 3);
+    MC_HIP(hipGetLastError());
+  }
+  const double tpiba = std::sqrt(ctx->tpiba2);
+  const int K = wfn_rows(ctx), ldb = wfn_ld(ctx);
+  const double* B = (const double*)c0;
+  const bool x3 = ctx->gemm_x3 && K >= kX3Min;
+  const int ldf = P.nat * P.nhxs, ldd = P.nat * P.nhxs * 3;
   int isa0 = 0;
   for (int is = 0; is < P.nsp; ++is) {
     for (int iv = 0; iv < P.ngh[is]; ++iv) {
@@ -503,24 +524,57 @@ void rnlsm1_dev(mc_ctx* ctx, const double2* c0, int n, double* fnl_host) {
       pa.ldg = ctx->ldg;
       pa.lmod4 = ((P.lang[iv + (size_t)P.nhxs * is] % 4) + 4) % 4;
       pa.geq0 = ctx->geq0;
-      gemm_tn(ctx, nullptr, 0, (const double*)c0, wfn_ld(ctx), wfn_rows(ctx), P.na[is], n, 2.0, false, false,
-              P.d_fnl.as<double>() + isa0 + (size_t)P.nat * iv, P.nat * P.nhxs, &pa);
+      double* fout = P.d_fnl.as<double>() + isa0 + (size_t)P.nat * iv;
+      double* dout[3];
+      for (int k = 0; k < 3; ++k)
+        dout[k] = do_dfnl ? P.d_dfnl.as<double>() + isa0 + (size_t)P.nat * (iv + (size_t)P.nhxs * k) : nullptr;
+      if (x3) {
+        double* outs[4];
+        int ldos[4];
+        double alphas[4];
+        int nv = 0;
+        if (do_fnl) { outs[nv] = fout; ldos[nv] = ldf; alphas[nv] = 2.0; ++nv; }
+        if (do_dfnl)
+          for (int k = 0; k < 3; ++k) { outs[nv] = dout[k]; ldos[nv] = ldd; alphas[nv] = 2.0 * tpiba; ++nv; }
+        proj_contract_x3(ctx, pa, do_dfnl ? P.d_gk.as<double>() : nullptr, do_fnl ? 0 : 1, nv, B, ldb, K, P.na[is],
+                         n, outs, ldos, alphas);
+      } else {
+        if (do_fnl) gemm_tn(ctx, nullptr, 0, B, ldb, K, P.na[is], n, 2.0, false, false, fout, ldf, &pa);
+        if (do_dfnl) {
+          ProjA pd = pa;
+          pd.lmod4 = (pa.lmod4 + 1) % 4;
+          for (int k = 0; k < 3; ++k) {
+            pd.gkk = P.d_gk.as<double>() + (size_t)k * ctx->ldg;
+            gemm_tn(ctx, nullptr, 0, B, ldb, K, P.na[is], n, 2.0 * tpiba, false, false, dout[k], ldd, &pd);
+          }
+        }
+      }
     }
     isa0 += P.na[is];
   }
-  if (fnl_host) {
+  if (fnl_host && do_fnl) {
     MC_HIP(hipMemcpyAsync(fnl_host, P.d_fnl.p, sizeof(double) * (size_t)P.nat * P.nhxs * n, hipMemcpyDeviceToHost,
                           ctx->stream));
-    MC_HIP(hipStreamSynchronize(ctx->stream));
   }
+  if (dfnl_host && do_dfnl) {
+    MC_HIP(hipMemcpyAsync(dfnl_host, P.d_dfnl.p, sizeof(double) * (size_t)P.nat * P.nhxs * 3 * n,
+                          hipMemcpyDeviceToHost, ctx->stream));
+  }
+  if ((fnl_host && do_fnl) || (dfnl_host && do_dfnl)) MC_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void rnlsm1_dev(mc_ctx* ctx, const double2* c0, int n, double* fnl_host) {
+  rnlsm_groups(ctx, c0, n, true, false, fnl_host, nullptr);
+}
+
+void rnlsm2_dev(mc_ctx* ctx, const double2* c0, int n, double* dfnl_host) {
+  rnlsm_groups(ctx, c0, n, false, true, nullptr, dfnl_host);
 }
 
 // rnlsm (src/rnlsm_utils.mod.F90:23-49): rnlsm1 and, with tfor, rnlsm2 of the
-// same coefficients, sharing the split-bf16 planes of c0 in mixed precision
+// same coefficients, in one pass over the projector groups
 void rnlsm_dev(mc_ctx* ctx, const double2* c0, int n, bool tfor, double* fnl_host, double* dfnl_host) {
-  PlaneScope keep(ctx);
-  rnlsm1_dev(ctx, c0, n, fnl_host);
-  if (tfor) rnlsm2_dev(ctx, c0, n, dfnl_host);
+  rnlsm_groups(ctx, c0, n, true, tfor, fnl_host, dfnl_host);
 }
 
 void set_gk(mc_ctx* ctx, const double* gk) {
@@ -537,45 +591,6 @@ void set_gk(mc_ctx* ctx, const double* gk) {
   MC_HIP(hipGetLastError());
   MC_HIP(hipStreamSynchronize(ctx->stream));
   P.gk_ready = true;
-}
-
-// dfnl(isa, iv, k, i) = 2 tpiba Re sum_G conj(gk_k twnl (-i)^(l+1) eigr) c (G=0
-// halved): the rnlsm1 contraction with the operand scaled by one G component.
-void rnlsm2_dev(mc_ctx* ctx, const double2* c0, int n, double* dfnl_host) {
-  auto& P = ctx->proj;
-  MC_CHECK(P.ready, MC_ERR_ARG, "rnlsm2: projectors not set (mc_set_projectors)");
-  MC_CHECK(P.eigr_nat == P.nat, MC_ERR_ARG, "rnlsm2: structure factors not set for all atoms (mc_set_eigr)");
-  MC_CHECK(P.gk_ready, MC_ERR_ARG, "rnlsm2: G vectors not set (mc_set_gk)");
-  const size_t nf = (size_t)P.nat * P.nhxs * 3 * std::max(n, 1);
-  P.d_dfnl.ensure(sizeof(double) * nf);
-  P.dfnl_nstate = n;
-  hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, ctx->stream, P.d_dfnl.as<double>(), nf);
-  MC_HIP(hipGetLastError());
-  const double tpiba = std::sqrt(ctx->tpiba2);
-  for (int k = 0; k < 3; ++k) {
-    int isa0 = 0;
-    for (int is = 0; is < P.nsp; ++is) {
-      for (int iv = 0; iv < P.ngh[is]; ++iv) {
-        if (P.na[is] == 0 || n == 0) continue;
-        ProjA pa;
-        pa.eigr = P.d_eigr.as<double2>() + (size_t)isa0 * ctx->ldg;
-        pa.twnl = P.d_twnl.as<double>() + (size_t)(is * P.nhxs + iv) * ctx->ldg;
-        pa.ldg = ctx->ldg;
-        pa.lmod4 = ((P.lang[iv + (size_t)P.nhxs * is] + 1) % 4 + 4) % 4;
-        pa.geq0 = ctx->geq0;
-        pa.gkk = P.d_gk.as<double>() + (size_t)k * ctx->ldg;
-        gemm_tn(ctx, nullptr, 0, (const double*)c0, wfn_ld(ctx), wfn_rows(ctx), P.na[is], n, 2.0 * tpiba, false,
-                false, P.d_dfnl.as<double>() + isa0 + (size_t)P.nat * (iv + (size_t)P.nhxs * k),
-                P.nat * P.nhxs * 3, &pa);
-      }
-      isa0 += P.na[is];
-    }
-  }
-  if (dfnl_host) {
-    MC_HIP(hipMemcpyAsync(dfnl_host, P.d_dfnl.p, sizeof(double) * (size_t)P.nat * P.nhxs * 3 * n,
-                          hipMemcpyDeviceToHost, ctx->stream));
-    MC_HIP(hipStreamSynchronize(ctx->stream));
-  }
 }
 
 void rnlfor_dev(mc_ctx* ctx, const double* f_host, int n, const double* fnl_host, const double* dfnl_host,

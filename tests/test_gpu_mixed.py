@@ -144,9 +144,11 @@ def test_mixed_projectors(gpu):
 
 
 def test_mixed_rnlsm_combined(gpu):
-    """rnlsm (fnl + dfnl in one call, shared bf16 planes of c0) equals the
-    separate rnlsm1 / rnlsm2 calls exactly, in FP64 and in mixed precision, and
-    the mixed dfnl stays at FP32-level error against the oracle."""
+    """rnlsm (fnl + dfnl in one call: one stacked contraction of the four
+    projector variants over the bf16 planes of c0) equals the separate rnlsm1 /
+    rnlsm2 calls exactly in FP64 and to rounding in mixed precision (the
+    stacking changes the K chunking of the FP32 sums), and the mixed fnl and
+    dfnl stay at FP32-level error against the oracle."""
     sysm = S.make_system(24.0, 64, 70.0, nstate=64)
     c = S.initial_coefficients(sysm, seed=43)
     ngw = sysm.gset.ngw
@@ -164,9 +166,15 @@ def test_mixed_rnlsm_combined(gpu):
         fnl, dfnl = ctx.rnlsm_dev(d, n, tfor=True, want_fnl=True, want_dfnl=True)
         fnl1 = ctx.rnlsm1_dev(d, n, want_fnl=True)
         dfnl1 = ctx.rnlsm2_dev(d, n, want_dfnl=True)
-        assert np.array_equal(fnl, fnl1) and np.array_equal(dfnl, dfnl1), mode
+        if mode == "fp64":
+            assert np.array_equal(fnl, fnl1) and np.array_equal(dfnl, dfnl1)
+        else:
+            for a, b in ((fnl, fnl1), (dfnl, dfnl1)):
+                assert np.sqrt(np.mean((a - b) ** 2) / np.mean(b ** 2)) < 3e-7
         fnl0, _ = ctx.rnlsm_dev(d, n, tfor=False, want_fnl=True)
         assert np.array_equal(fnl0, fnl1)
+    ofnl = oracle.rnlsm1(OP, c)
+    assert np.sqrt(np.mean((fnl - ofnl) ** 2) / np.mean(ofnl ** 2)) < 5e-7
     odfnl = oracle.rnlsm2(OP, c, gk, sysm.cell.tpiba)
     rel = np.sqrt(np.mean((dfnl - odfnl) ** 2) / np.mean(odfnl ** 2))
     assert rel < 5e-7, rel
