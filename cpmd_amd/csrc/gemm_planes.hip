@@ -335,6 +335,157 @@ __global__ __launch_bounds__(256, 1) void k_tn_planes(const __bf16* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// D = alpha A G + beta D with the small G (k x n) as two-part bf16 planes
+// (split once per product) and A (M x k, column-major, the tall wavefunction
+// or projector operand) split per workgroup: 128 x 128 tiles of D, 4 waves of
+// 64 x 64, K slices of 32.  A slices are fetched as FP64 into registers one
+// slice ahead, split into hi/lo and staged in LDS as [row][k] (stride 40);
+// G blocks are copied into a two-stage LDS ring with global_load_lds.  Each
+// slice product hi*hi + hi*lo + lo*hi starts from a zero accumulator and is
+// added to the FP32 tile sum on the VALU.  Epilogue as k_x3_nn (plain
+// alpha/beta, or the fnonloc twnl * ctm factors).  upper_g: G upper
+// triangular (the k loop stops after the diagonal block).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256, 3) void k_nn_planes(const double* __restrict__ A, int lda,
+                                                      const __bf16* __restrict__ Gpl, int gnkb, int M, int k, int n,
+                                                      double alpha, double beta, double* __restrict__ D, int ldd,
+                                                      int upper_g, FnlEpi epi, int tn, int total) {
+  constexpr int kLdA = 40;
+  constexpr int kGStage = 2 * kPart;   // two parts of one G block
+  __shared__ __attribute__((aligned(16))) __bf16 As[2 * kPT * kLdA];
+  __shared__ __attribute__((aligned(16))) unsigned char Gs[2 * kGStage];
+  const int t = xcd_tile_p(total);
+  if (t >= total) return;
+  const int rt = t / tn, ct = t - rt * tn;
+  const int i0 = rt * kPT, j0 = ct * kPT;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  const int li = lane & 15, lq = lane >> 4;
+  const int kmax = upper_g ? min(k, j0 + kPT) : k;
+  const int nsl = (kmax + kPK - 1) / kPK;
+  // A slice (128 rows x 32 k): wave w loads k = 8w .. 8w+7 of rows i0+2l, i0+2l+1
+  const int ar = 2 * lane, ak = 8 * wave;
+  double2 ra[8];
+  auto fetch_a = [&](int kb) {
+    const int row = i0 + ar;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int kk = kb * kPK + ak + u;
+      ra[u] = (kk < kmax && row < M) ? *reinterpret_cast<const double2*>(A + (size_t)kk * lda + row)
+                                     : double2{0.0, 0.0};
+    }
+  };
+  const unsigned char* Gb = reinterpret_cast<const unsigned char*>(Gpl) + (size_t)ct * gnkb * kGStage;
+  auto issue_g = [&](int kb, int stage) {
+#pragma unroll
+    for (int i = 0; i < kGStage / 1024 / 4; ++i) {
+      const int j = wave + 4 * i;
+      lds_copy16(Gb + (size_t)kb * kGStage + j * 1024 + lane * 16, (lds_void*)(Gs + stage * kGStage + j * 1024));
+    }
+  };
+  int offb[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int c = wc + 16 * b + li;
+    offb[b] = c * (kPK * 2) + swz(c, lq) * 16;
+  }
+  const int offa = li * kLdA + 8 * lq;
+  f4 acc[4][4];
+  const f4 z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = z;
+  if (nsl > 0) {
+    fetch_a(0);
+    issue_g(0, 0);
+  }
+  for (int s = 0; s < nsl; ++s) {
+    {   // A slice -> hi / lo rows ar, ar + 1, k = ak .. ak + 7
+      float v0[8], v1[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        v0[u] = (float)ra[u].x;
+        v1[u] = (float)ra[u].y;
+      }
+      bf8 p0[2], p1[2];
+      split_parts<2>(v0, p0);
+      split_parts<2>(v1, p1);
+      *reinterpret_cast<bf8*>(As + ar * kLdA + ak) = p0[0];
+      *reinterpret_cast<bf8*>(As + kPT * kLdA + ar * kLdA + ak) = p0[1];
+      *reinterpret_cast<bf8*>(As + (ar + 1) * kLdA + ak) = p1[0];
+      *reinterpret_cast<bf8*>(As + kPT * kLdA + (ar + 1) * kLdA + ak) = p1[1];
+    }
+    __syncthreads();   // A slice staged and G block s landed (the fence waits for the copies)
+    if (s + 1 < nsl) {
+      fetch_a(s + 1);
+      issue_g(s + 1, (s + 1) & 1);
+    }
+    const unsigned char* gs = Gs + (s & 1) * kGStage;
+    bf8 ah[4], al[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      ah[a] = *reinterpret_cast<const bf8*>(As + (wr + 16 * a) * kLdA + offa);
+      al[a] = *reinterpret_cast<const bf8*>(As + kPT * kLdA + (wr + 16 * a) * kLdA + offa);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const bf8 bh = *reinterpret_cast<const bf8*>(gs + offb[b]);
+      const bf8 bl = *reinterpret_cast<const bf8*>(gs + kPart + offb[b]);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        f4 u = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh, z, 0, 0, 0);
+        u = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl, u, 0, 0, 0);
+        u = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh, u, 0, 0, 0);
+        acc[a][b] += u;
+      }
+    }
+    // every wave is done reading A slice s and G stage s & 1 (their reads
+    // completed before the MFMAs); no fence: the next slice's copies stay in flight
+    asm volatile("s_barrier" ::: "memory");
+  }
+  // epilogue: lane holds rows i .. i+3 (i = 4 lq + 16a + wr + i0) of column j
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const int j = j0 + wc + 16 * b + li;
+    if (j >= n) continue;
+    double f = 1.0;
+    if (epi.twnl) {
+      f = epi.f[j];
+      if (f < 1e-5) f = 1.0;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int i = i0 + wr + 16 * a + 4 * lq;
+      if (i >= M) continue;
+      double* d = D + (size_t)j * ldd + i;
+      if (epi.twnl) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          const double x0 = acc[a][b][2 * p], x1 = acc[a][b][2 * p + 1];
+          const double tw = epi.twnl[(i >> 1) + p];
+          double yr, yi;
+          switch (epi.lmod4) {   // ctm = -f (-i)^l ; y = tw * ctm * x
+            case 0: yr = -f * tw * x0; yi = -f * tw * x1; break;
+            case 1: yr = -f * tw * x1; yi = f * tw * x0; break;
+            case 2: yr = f * tw * x0; yi = f * tw * x1; break;
+            default: yr = f * tw * x1; yi = -f * tw * x0; break;
+          }
+          if (i + 2 * p < M) {
+            d[2 * p] += yr;
+            d[2 * p + 1] += yi;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (i + r < M) d[r] = (beta == 0.0 ? 0.0 : beta * d[r]) + alpha * (double)acc[a][b][r];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 size_t planes_bytes(int K, int n, int np) {
@@ -385,6 +536,20 @@ void proj_planes_stacked(mc_ctx* ctx, const ProjA& base, const double* gk, int v
   else
     hipLaunchKernelGGL(k_proj_planes_stacked<2>, grid, dim3(256), 0, ctx->stream, base, gk, base.lmod4, v0, nv, K, na,
                        nkb, ncbv, nblk, out.as<__bf16>());
+  MC_HIP(hipGetLastError());
+  timer_end(ctx, MC_KID_GEMM);
+}
+
+void gemm_nn_planes(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n,
+                    double alpha, double beta, double* D, int ldd, bool upper_g, const FnlEpi& epi) {
+  split_planes(ctx, G, ldgm, k, n, 2, ctx->w_plg);
+  const int gnkb = (k + kPK - 1) / kPK;
+  const int tm = (M + kPT - 1) / kPT, tn = (n + kPT - 1) / kPT;
+  const int total = tm * tn;
+  const int launch = (total + kXcdP - 1) / kXcdP * kXcdP;
+  timer_begin(ctx, MC_KID_GEMM);
+  hipLaunchKernelGGL(k_nn_planes, dim3(launch), dim3(256), 0, ctx->stream, A, lda, ctx->w_plg.as<__bf16>(), gnkb, M,
+                     k, n, alpha, beta, D, ldd, upper_g ? 1 : 0, epi, tn, total);
   MC_HIP(hipGetLastError());
   timer_end(ctx, MC_KID_GEMM);
 }

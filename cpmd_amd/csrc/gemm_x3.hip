@@ -298,6 +298,10 @@ void gemm_nn_x3(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm
            "split-bf16 gemm_nn needs even M, lda, ldd and a 16-byte aligned A");
   FnlEpi e{};
   if (epi) e = *epi;
+  if (ldgm % 2 == 0 && (reinterpret_cast<uintptr_t>(G) & 15) == 0) {
+    gemm_nn_planes(ctx, A, lda, G, ldgm, M, k, n, alpha, beta, D, ldd, upper_g, e);
+    return;
+  }
   const int tm = (M + kT - 1) / kT, tn = (n + kT - 1) / kT;
   const int total = tm * tn;
   const int launch = (total + kXcd - 1) / kXcd * kXcd;

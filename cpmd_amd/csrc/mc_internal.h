@@ -203,6 +203,7 @@ struct mc_ctx {
   int x3_parts = 3;        // bf16 parts per operand of the split A^T B products (2 or 3)
   mc::PlaneCache planes;   // bf16 planes of the wavefunction operands of the A^T B products
   mc::DevBuf w_plp;        // bf16 planes of the projector operand
+  mc::DevBuf w_plg;        // bf16 planes of the small right operand of A G products
   mc::DensitySet dens;     // vofrho / ewald data
 
   int batch_pairs_vpsi = 8;
@@ -248,6 +249,9 @@ void proj_planes(mc_ctx* ctx, const ProjA& pa, int K, int m, int np, DevBuf& out
 void gemm_tn_planes(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np, int K, int m, int n, double alpha,
                     const double* A64, int lda, const double* B64, int ldb, bool g0corr, bool symmetric, double* out,
                     int ldo);
+// D = alpha A G + beta D (epilogue as gemm_nn) with G as bf16 planes (gemm_planes.hip)
+void gemm_nn_planes(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n,
+                    double alpha, double beta, double* D, int ldd, bool upper_g, const FnlEpi& epi);
 // partials of A^T B from planes into ctx->w_gemm (m x n each); returns their number
 int tn_planes_partials(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np, int K, int m, int n, bool symmetric);
 // planes of the projector variants of one (species, projector) group, stacked
