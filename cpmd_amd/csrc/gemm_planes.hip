@@ -47,6 +47,7 @@ constexpr int kPart = kPT * kPK * 2;      // bytes of one part of one block
 constexpr int kStages = 3;
 constexpr int kXcdP = 8;
 constexpr int kMaxSlices = 64;            // K chunk (slices of 32 rows) per workgroup: FP32 accumulation
+constexpr int kTnWaves = 8;               // waves per k_tn_planes workgroup (two per SIMD)
 
 __device__ __forceinline__ int swz(int c, int q) { return q ^ ((c >> 2) & 3); }
 
@@ -210,13 +211,16 @@ __global__ __launch_bounds__(256) void k_proj_planes_stacked(ProjA pa, const dou
 // nsplit * tm * tn; upper: skip tiles below the diagonal.
 // ---------------------------------------------------------------------------
 template <int NP>
-__global__ __launch_bounds__(256, 1) void k_tn_planes(const __bf16* __restrict__ Apl, const __bf16* __restrict__ Bpl,
-                                                      int nkb, int m, int n, int kbper, double* __restrict__ P,
-                                                      int upper, int tm, int tn, int total) {
+__global__ __launch_bounds__(64 * kTnWaves, 1) void k_tn_planes(const __bf16* __restrict__ Apl,
+                                                                const __bf16* __restrict__ Bpl, int nkb, int m, int n,
+                                                                int kbper, double* __restrict__ P, int upper, int tm,
+                                                                int tn, int total) {
+  constexpr int NW = kTnWaves;                     // waves: 2 x NW/2 blocks of 64 x (128 / (NW/2)) of the tile
+  constexpr int NBC = 8 / NW * 2;                  // 16-column MFMA tiles per wave (4 with 4 waves, 2 with 8)
   constexpr int kStage = 2 * NP * kPart;           // bytes of one ring stage (A block, B block)
   constexpr int kRuns = 2 * NP * kPart / 1024;     // 1 KiB runs per stage
-  static_assert(kRuns % 4 == 0, "runs per wave");
-  constexpr int kPer = kRuns / 4;                  // copies per wave and stage
+  static_assert(kRuns % NW == 0, "runs per wave");
+  constexpr int kPer = kRuns / NW;                 // copies per wave and stage
   static_assert(kPer < 64, "vmcnt range");
   constexpr int kWait1 = (kPer & 15) | 0x70 | 0xF00 | ((kPer >> 4) << 14);
   constexpr int kWait0 = 0x70 | 0xF00;
@@ -231,26 +235,30 @@ __global__ __launch_bounds__(256, 1) void k_tn_planes(const __bf16* __restrict__
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const unsigned char* Ab = reinterpret_cast<const unsigned char*>(Apl) + (size_t)ti * nkb * NP * kPart;
   const unsigned char* Bb = reinterpret_cast<const unsigned char*>(Bpl) + (size_t)tj * nkb * NP * kPart;
-  // wave w copies runs w, w + 4, ... of a stage: run j < NP*8 from the A
+  // wave w copies runs w, w + NW, ... of a stage: run j < NP*8 from the A
   // block, the rest from the B block
   auto issue = [&](int kb, int stage) {
 #pragma unroll
-    for (int i = 0; i < kRuns / 4; ++i) {
-      const int j = wave + 4 * i;
+    for (int i = 0; i < kPer; ++i) {
+      const int j = wave + NW * i;
       const bool isb = j >= NP * 8;
       const unsigned char* src =
           (isb ? Bb : Ab) + (size_t)kb * NP * kPart + (size_t)(isb ? j - NP * 8 : j) * 1024 + lane * 16;
       lds_copy16(src, (lds_void*)(ring + stage * kStage + j * 1024));
     }
   };
-  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+  const int wr = (wave / (NW / 2)) * 64, wc = (wave % (NW / 2)) * (16 * NBC);
   const int li = lane & 15, lq = lane >> 4;
-  int offa[4], offb[4];   // byte offsets of this lane's chunk in a part
+  int offa[4], offb[NBC];   // byte offsets of this lane's chunk in a part
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const int r = wr + 16 * a + li, c = wc + 16 * a + li;
+    const int r = wr + 16 * a + li;
     offa[a] = r * (kPK * 2) + swz(r, lq) * 16;
-    offb[a] = c * (kPK * 2) + swz(c, lq) * 16;
+  }
+#pragma unroll
+  for (int b = 0; b < NBC; ++b) {
+    const int c = wc + 16 * b + li;
+    offb[b] = c * (kPK * 2) + swz(c, lq) * 16;
   }
   // Accumulation: the six MFMAs of one slice and element start from zero,
   // small products first, so that none is added to a larger partial sum
@@ -258,12 +266,12 @@ __global__ __launch_bounds__(256, 1) void k_tn_planes(const __bf16* __restrict__
   // what falls below its last bit, a same-signed loss on sums of squares);
   // the slice sums go to an FP32 accumulator on the VALU.  The host keeps K
   // chunks to at most kMaxSlices slices and reduces the partials in FP64.
-  f4 acc32[4][4];
+  f4 acc32[4][NBC];
   const f4 z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc32[a][b] = z;
+    for (int b = 0; b < NBC; ++b) acc32[a][b] = z;
   const int nsl = kb1 - kb0;
   if (nsl > 0) issue(kb0, 0);
   if (nsl > 1) issue(kb0 + 1, 1);
@@ -288,7 +296,7 @@ __global__ __launch_bounds__(256, 1) void k_tn_planes(const __bf16* __restrict__
 #pragma unroll
       for (int a = 0; a < 4; ++a) ap[p][a] = *reinterpret_cast<const bf8*>(st + p * kPart + offa[a]);
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
+    for (int b = 0; b < NBC; ++b) {
       bf8 bp[NP];
 #pragma unroll
       for (int p = 0; p < NP; ++p) bp[p] = *reinterpret_cast<const bf8*>(st + (NP + p) * kPart + offb[b]);
@@ -324,7 +332,7 @@ __global__ __launch_bounds__(256, 1) void k_tn_planes(const __bf16* __restrict__
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
+    for (int b = 0; b < NBC; ++b) {
       const int j = j0 + wc + 16 * b + li;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -569,10 +577,10 @@ int tn_planes_partials(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np
   const int launch = (total + kXcdP - 1) / kXcdP * kXcdP;
   timer_begin(ctx, MC_KID_GEMM);
   if (np == 3)
-    hipLaunchKernelGGL(k_tn_planes<3>, dim3(launch), dim3(256), 0, ctx->stream, Apl.as<__bf16>(), Bpl.as<__bf16>(),
+    hipLaunchKernelGGL(k_tn_planes<3>, dim3(launch), dim3(64 * kTnWaves), 0, ctx->stream, Apl.as<__bf16>(), Bpl.as<__bf16>(),
                        nkb, m, n, kbper, ctx->w_gemm.as<double>(), symmetric ? 1 : 0, tm, tn, total);
   else
-    hipLaunchKernelGGL(k_tn_planes<2>, dim3(launch), dim3(256), 0, ctx->stream, Apl.as<__bf16>(), Bpl.as<__bf16>(),
+    hipLaunchKernelGGL(k_tn_planes<2>, dim3(launch), dim3(64 * kTnWaves), 0, ctx->stream, Apl.as<__bf16>(), Bpl.as<__bf16>(),
                        nkb, m, n, kbper, ctx->w_gemm.as<double>(), symmetric ? 1 : 0, tm, tn, total);
   MC_HIP(hipGetLastError());
   timer_end(ctx, MC_KID_GEMM);
