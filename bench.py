@@ -224,11 +224,10 @@ def main():
             r = ks.energy(buf["c0"], f, tau=tau, gradient=buf["c2"], forces=True)
             vion = vion + dt / ION_MASS * r["fion"].T
             tau = tau + dt * vion
-            # velupa + posupa: cm += dt/emass c2, cp = c0 + dt cm (one pass)
-            ctx.cp_posupa(buf["c0"], buf["c2"], buf["cm"], buf["cp"], nstate, dt / args.emass, dt)
-            _, it = ctx.rortog_dev(buf["c0"], buf["cp"], nstate, epsog=1e-8, maxit=60)
+            # velupa + posupa: cm += dt/emass c2, cp = c0 + dt cm, rortog, cm = (cp - c0)/dt
+            _, it = ctx.posupa_dev(buf["c0"], buf["c2"], buf["cm"], buf["cp"], nstate, dt / args.emass, dt,
+                                   epsog=1e-8, maxit=60)
             iters.append(it)
-            ctx.cp_velocity(buf["c0"], buf["cp"], buf["cm"], nstate, dt)         # cm = (cp - c0) / dt
             buf["c0"], buf["cp"] = buf["cp"], buf["c0"]
 
         def step_wfopt():

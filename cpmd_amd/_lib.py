@@ -57,6 +57,8 @@ SIGNATURES = {
     "mc_wfn_axpby": (C.c_int, [_vp, C.c_double, _vp, C.c_double, _vp, C.c_int]),
     "mc_cp_posupa": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double]),
     "mc_cp_velocity": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_double]),
+    "mc_posupa_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _dp,
+                                C.POINTER(C.c_int)]),
     "mc_wfn_dotp": (C.c_int, [_vp, _vp, _vp, C.c_int, _dp]),
     "mc_wfn_upload": (C.c_int, [_vp, _dp, C.c_int, C.c_int, _vp]),
     "mc_wfn_download": (C.c_int, [_vp, _vp, C.c_int, _dp, C.c_int]),

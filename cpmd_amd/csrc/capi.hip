@@ -231,6 +231,20 @@ int mc_rnlsm_dev(mc_ctx* ctx, const void* c0_dev, int nstate, int tfor, double* 
   MC_API_END
 }
 
+int mc_posupa_dev(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, void* cm_dev, void* cp_dev, int nstate,
+                  double dtm, double dt, double epsog, int maxit, double* gam, int* iters) {
+  MC_API_BEGIN
+  check_ctx(ctx);
+  MC_CHECK(c0_dev && cm_dev && cp_dev && nstate >= 0 && dt != 0.0 && maxit > 0, MC_ERR_ARG, "posupa: bad argument");
+  MC_CHECK(cp_dev != c0_dev && cp_dev != cm_dev && cp_dev != c2_dev, MC_ERR_ARG,
+           "posupa: cp must not alias c0, c2 or cm");
+  int it = 0;
+  mc::posupa_dev(ctx, (const double2*)c0_dev, (const double2*)c2_dev, (double2*)cm_dev, (double2*)cp_dev, nstate, dtm,
+                 dt, epsog, maxit, gam, &it);
+  if (iters) *iters = it;
+  MC_API_END
+}
+
 int mc_rnlfor(mc_ctx* ctx, const double* f, int nstate, const double* fnl, const double* dfnl, double* fion) {
   MC_API_BEGIN
   check_ctx(ctx);

@@ -38,6 +38,23 @@ void DevBuf::ensure(size_t nbytes) {
   MC_HIP(hipMalloc(&p, nbytes));
   bytes = nbytes;
 }
+void upload_async(mc_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return;
+  if (ctx->h_stage_done) MC_HIP(hipEventSynchronize(ctx->h_stage_done));
+  if (bytes > ctx->h_stage_bytes) {
+    if (ctx->h_stage) MC_HIP(hipHostFree(ctx->h_stage));
+    ctx->h_stage = nullptr;
+    ctx->h_stage_bytes = 0;
+    const size_t nb = std::max(bytes, (size_t)65536);
+    MC_HIP(hipHostMalloc(&ctx->h_stage, nb, hipHostMallocDefault));
+    ctx->h_stage_bytes = nb;
+  }
+  if (!ctx->h_stage_done) MC_HIP(hipEventCreateWithFlags(&ctx->h_stage_done, hipEventDisableTiming));
+  std::memcpy(ctx->h_stage, src, bytes);
+  MC_HIP(hipMemcpyAsync(dst, ctx->h_stage, bytes, hipMemcpyHostToDevice, ctx->stream));
+  MC_HIP(hipEventRecord(ctx->h_stage_done, ctx->stream));
+}
+
 void DevBuf::release() {
   if (p) (void)hipFree(p);
   p = nullptr;
@@ -227,7 +244,7 @@ __global__ void k_axpby(double alpha, const double* __restrict__ x, double beta,
 __global__ void k_cp_posupa(const double2* __restrict__ c0, const double2* __restrict__ c2, double2* __restrict__ cm,
                             double2* __restrict__ cp, double dtm, double dt, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const double2 a = c0[i], f = c2[i];
+    const double2 a = c0[i], f = c2 ? c2[i] : make_double2(0.0, 0.0);
     double2 v = cm[i];
     v.x += dtm * f.x;
     v.y += dtm * f.y;
@@ -244,6 +261,25 @@ __global__ void k_cp_velocity(const double2* __restrict__ c0, const double2* __r
     const double2 a = c0[i], b = cp[i];
     cm[i] = make_double2(odt * (b.x - a.x), odt * (b.y - a.y));
   }
+}
+
+void cp_posupa_dev(mc_ctx* ctx, const double2* c0, const double2* c2, double2* cm, double2* cp, int nstate, double dtm,
+                   double dt) {
+  const size_t n = (size_t)ctx->ldg * nstate;
+  if (!n) return;
+  timer_begin(ctx, MC_KID_MISC);
+  hipLaunchKernelGGL(k_cp_posupa, dim3(2048), dim3(256), 0, ctx->stream, c0, c2, cm, cp, c2 ? dtm : 0.0, dt, n);
+  MC_HIP(hipGetLastError());
+  timer_end(ctx, MC_KID_MISC);
+}
+
+void cp_velocity_dev(mc_ctx* ctx, const double2* c0, const double2* cp, double2* cm, int nstate, double dt) {
+  const size_t n = (size_t)ctx->ldg * nstate;
+  if (!n) return;
+  timer_begin(ctx, MC_KID_MISC);
+  hipLaunchKernelGGL(k_cp_velocity, dim3(2048), dim3(256), 0, ctx->stream, c0, cp, cm, 1.0 / dt, n);
+  MC_HIP(hipGetLastError());
+  timer_end(ctx, MC_KID_MISC);
 }
 
 // out[i] = Re a0 Re b0 + 2 sum_{G != 0} Re(conj(a) b) for column i (dotp,
@@ -409,6 +445,8 @@ int mc_ctx_destroy(mc_ctx* ctx) {
   if (!ctx) return MC_OK;
   (void)hipStreamSynchronize(ctx->stream);
   for (auto e : ctx->timer.ev) (void)hipEventDestroy(e);
+  if (ctx->h_stage_done) (void)hipEventDestroy(ctx->h_stage_done);
+  if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   MC_API_END
@@ -511,29 +549,21 @@ int mc_wfn_axpy(mc_ctx* ctx, double alpha, const void* x_dev, void* y_dev, int n
   MC_API_END
 }
 
+
 int mc_cp_posupa(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, void* cm_dev, void* cp_dev, int nstate,
                  double dtm, double dt) {
   MC_API_BEGIN
   MC_CHECK(ctx && c0_dev && c2_dev && cm_dev && cp_dev && nstate >= 0, MC_ERR_ARG, "bad argument");
   MC_CHECK(cp_dev != c0_dev && cp_dev != c2_dev && cp_dev != cm_dev, MC_ERR_ARG, "cp must not alias c0, c2 or cm");
-  const size_t n = (size_t)ctx->ldg * nstate;
-  if (n) {
-    hipLaunchKernelGGL(mc::k_cp_posupa, dim3(2048), dim3(256), 0, ctx->stream, (const double2*)c0_dev,
-                       (const double2*)c2_dev, (double2*)cm_dev, (double2*)cp_dev, dtm, dt, n);
-    MC_HIP(hipGetLastError());
-  }
+  mc::cp_posupa_dev(ctx, (const double2*)c0_dev, (const double2*)c2_dev, (double2*)cm_dev, (double2*)cp_dev, nstate,
+                    dtm, dt);
   MC_API_END
 }
 
 int mc_cp_velocity(mc_ctx* ctx, const void* c0_dev, const void* cp_dev, void* cm_dev, int nstate, double dt) {
   MC_API_BEGIN
   MC_CHECK(ctx && c0_dev && cp_dev && cm_dev && nstate >= 0 && dt != 0.0, MC_ERR_ARG, "bad argument");
-  const size_t n = (size_t)ctx->ldg * nstate;
-  if (n) {
-    hipLaunchKernelGGL(mc::k_cp_velocity, dim3(2048), dim3(256), 0, ctx->stream, (const double2*)c0_dev,
-                       (const double2*)cp_dev, (double2*)cm_dev, 1.0 / dt, n);
-    MC_HIP(hipGetLastError());
-  }
+  mc::cp_velocity_dev(ctx, (const double2*)c0_dev, (const double2*)cp_dev, (double2*)cm_dev, nstate, dt);
   MC_API_END
 }
 

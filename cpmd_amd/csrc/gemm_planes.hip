@@ -118,6 +118,72 @@ __global__ __launch_bounds__(256) void k_split_planes(const double* __restrict__
   store_parts<NP>(out, blk, c, q, p);
 }
 
+// velupa + posupa with the planes of both coefficient sets as a by-product:
+// cm += dtm c2 (c2 may be null), cp = c0 + dt cm over all ldx stored rows, and
+// the bf16 planes of c0 and of cp for the nkb = ceil(K / 32) row blocks of the
+// contractions (layout of k_split_planes; rows >= K and columns >= n zero).
+// One wave per (column, 512-row segment): lane l moves rows 2l + 128u (u < 4)
+// as 16-byte runs, so every load and store of c0, c2, cm, cp covers 1 KiB of a
+// column; the fp32 values pass through LDS so that lane l then holds rows
+// 8l..8l+7, the plane chunk it splits and stores.
+template <int NP>
+__global__ __launch_bounds__(256) void k_posupa_planes(const double* __restrict__ c0, const double* __restrict__ c2,
+                                                       double* __restrict__ cm, double* __restrict__ cp, int ldx, int K,
+                                                       int n, int ncolp, int nseg, int nkb, double dtm, double dt,
+                                                       __bf16* __restrict__ pl0, __bf16* __restrict__ plp) {
+  __shared__ float4 lds[4][2][128];   // [wave][c0 | cp][512 rows]
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const size_t unit = (size_t)blockIdx.x * 4 + wv;
+  const bool live = unit < (size_t)ncolp * nseg;
+  const int col = live ? (int)(unit / nseg) : 0;
+  const int seg = live ? (int)(unit - (size_t)col * nseg) : 0;
+  float2* s0 = reinterpret_cast<float2*>(lds[wv][0]);
+  float2* sp = reinterpret_cast<float2*>(lds[wv][1]);
+  const int r0 = seg * 512 + 2 * lane;
+  const size_t o = (size_t)col * ldx + r0;
+  const bool any = live && col < n;
+  double2 x[4], w[4], f[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const bool ok = any && r0 + 128 * u < ldx;
+    x[u] = ok ? *reinterpret_cast<const double2*>(c0 + o + 128 * u) : make_double2(0.0, 0.0);
+    w[u] = ok ? *reinterpret_cast<const double2*>(cm + o + 128 * u) : make_double2(0.0, 0.0);
+    f[u] = ok && c2 ? *reinterpret_cast<const double2*>(c2 + o + 128 * u) : make_double2(0.0, 0.0);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    w[u].x += dtm * f[u].x;
+    w[u].y += dtm * f[u].y;
+    f[u] = make_double2(x[u].x + dt * w[u].x, x[u].y + dt * w[u].y);   // f now holds cp
+    const bool in = r0 + 128 * u < K;                                  // K even: whole complex rows
+    s0[lane + 64 * u] = in ? make_float2((float)x[u].x, (float)x[u].y) : make_float2(0.f, 0.f);
+    sp[lane + 64 * u] = in ? make_float2((float)f[u].x, (float)f[u].y) : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const int kb = seg * 16 + (lane >> 2);
+  if (live && kb < nkb) {
+    const size_t blk = (size_t)(col / kPT) * nkb + kb;
+    const int c = col % kPT, q = lane & 3;
+    float v[8];
+    bf8 p[NP];
+    const float4 a0 = lds[wv][0][2 * lane], a1 = lds[wv][0][2 * lane + 1];
+    v[0] = a0.x; v[1] = a0.y; v[2] = a0.z; v[3] = a0.w; v[4] = a1.x; v[5] = a1.y; v[6] = a1.z; v[7] = a1.w;
+    split_parts<NP>(v, p);
+    store_parts<NP>(pl0, blk, c, q, p);
+    const float4 b0 = lds[wv][1][2 * lane], b1 = lds[wv][1][2 * lane + 1];
+    v[0] = b0.x; v[1] = b0.y; v[2] = b0.z; v[3] = b0.w; v[4] = b1.x; v[5] = b1.y; v[6] = b1.z; v[7] = b1.w;
+    split_parts<NP>(v, p);
+    store_parts<NP>(plp, blk, c, q, p);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (any && r0 + 128 * u < ldx) {
+      if (c2) *reinterpret_cast<double2*>(cm + o + 128 * u) = w[u];
+      *reinterpret_cast<double2*>(cp + o + 128 * u) = f[u];
+    }
+  }
+}
+
 // The projector operand of rnlsm1 / rnlsm2 (as ProjA in gemm.hip) -> planes:
 // row 2g / 2g+1 = Re / Im of (-i)^l tw_g eigr(atom, g), tw_g = twnl_g [gk_g]
 // (halved at G = 0).
@@ -515,6 +581,38 @@ void split_planes(mc_ctx* ctx, const double* X, int ldx, int K, int n, int np, D
     hipLaunchKernelGGL(k_split_planes<2>, grid, dim3(256), 0, ctx->stream, X, ldx, K, n, nkb, nblk, out.as<__bf16>());
   MC_HIP(hipGetLastError());
   timer_end(ctx, MC_KID_GEMM);
+}
+
+void posupa_planes(mc_ctx* ctx, const double* c0, const double* c2, double* cm, double* cp, int ldx, int K, int n,
+                   double dtm, double dt) {
+  auto& C = ctx->planes;
+  MC_CHECK(C.scoped, MC_ERR_INTERNAL, "posupa_planes outside a plane scope");
+  MC_CHECK(ldx % kPK == 0 && K % 2 == 0 && K <= ldx && (reinterpret_cast<uintptr_t>(c0) & 15) == 0 &&
+               (reinterpret_cast<uintptr_t>(cm) & 15) == 0 && (reinterpret_cast<uintptr_t>(cp) & 15) == 0 &&
+               (!c2 || (reinterpret_cast<uintptr_t>(c2) & 15) == 0),
+           MC_ERR_INTERNAL, "posupa_planes: operands not 16-byte aligned");
+  const int np = ctx->x3_parts == 2 ? 2 : 3;
+  const int nkb = (K + kPK - 1) / kPK, ncolp = (n + kPT - 1) / kPT * kPT, nseg = (ldx + 511) / 512;
+  C.buf[0].ensure(planes_bytes(K, n, np));
+  C.buf[1].ensure(planes_bytes(K, n, np));
+  const dim3 grid((unsigned)(((size_t)ncolp * nseg + 3) / 4));
+  timer_begin(ctx, MC_KID_MISC);
+  if (np == 3)
+    hipLaunchKernelGGL(k_posupa_planes<3>, grid, dim3(256), 0, ctx->stream, c0, c2, cm, cp, ldx, K, n, ncolp, nseg,
+                       nkb, c2 ? dtm : 0.0, dt, C.buf[0].as<__bf16>(), C.buf[1].as<__bf16>());
+  else
+    hipLaunchKernelGGL(k_posupa_planes<2>, grid, dim3(256), 0, ctx->stream, c0, c2, cm, cp, ldx, K, n, ncolp, nseg,
+                       nkb, c2 ? dtm : 0.0, dt, C.buf[0].as<__bf16>(), C.buf[1].as<__bf16>());
+  MC_HIP(hipGetLastError());
+  timer_end(ctx, MC_KID_MISC);
+  const void* src[2] = {c0, cp};
+  for (int i = 0; i < 2; ++i) {
+    C.src[i] = src[i];
+    C.np[i] = np;
+    C.K[i] = K;
+    C.n[i] = n;
+  }
+  C.next = 0;
 }
 
 void proj_planes(mc_ctx* ctx, const ProjA& pa, int K, int m, int np, DevBuf& out) {

@@ -143,7 +143,7 @@ struct DensitySet {
   double a[9] = {}, inv[9] = {};   // lattice vectors a[3i+q]; fractional f_i = sum_q d_q inv[3q+i]
   DevBuf d_gm, d_gidx, d_scg, d_gk;
   DevBuf d_sp_start, d_sp_of, d_zv, d_raggio, d_vps, d_rhops;
-  DevBuf d_tau, d_frac, d_ph1, d_ph2, d_ph3;
+  DevBuf d_tau, d_ph1, d_ph2, d_ph3;   // d_tau: [tau (3, nat) | fractional coordinates (3, nat)]
   DevBuf w_W, w_g, w_part;
 };
 
@@ -161,6 +161,10 @@ struct mc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;   // borrowed or owned
   bool own_stream = false;
+  // pinned staging buffer of small asynchronous uploads (mc::upload_async)
+  void* h_stage = nullptr;
+  size_t h_stage_bytes = 0;
+  hipEvent_t h_stage_done = nullptr;   // recorded after the last copy out of h_stage
 
   // stick layout (see context.hip for the definitions)
   int n_own = 0;        // owner sticks (hold half-sphere G's); owner 0 is (0,0)
@@ -246,6 +250,15 @@ constexpr int kX3Min = 16384;   // rows of K (tn) or M (nn) from which the split
 size_t planes_bytes(int K, int n, int np);
 void split_planes(mc_ctx* ctx, const double* X, int ldx, int K, int n, int np, DevBuf& out);
 void proj_planes(mc_ctx* ctx, const ProjA& pa, int K, int m, int np, DevBuf& out);
+// cm += dtm c2 (c2 may be null), cp = c0 + dt cm, writing the planes of c0 and
+// cp into the plane cache of the current PlaneScope
+void posupa_planes(mc_ctx* ctx, const double* c0, const double* c2, double* cm, double* cp, int ldx, int K, int n,
+                   double dtm, double dt);
+void cp_posupa_dev(mc_ctx* ctx, const double2* c0, const double2* c2, double2* cm, double2* cp, int nstate, double dtm,
+                   double dt);
+void cp_velocity_dev(mc_ctx* ctx, const double2* c0, const double2* cp, double2* cm, int nstate, double dt);
+void posupa_dev(mc_ctx* ctx, const double2* c0, const double2* c2, double2* cm, double2* cp, int n, double dtm,
+                double dt, double epsog, int maxit, double* gam_host, int* iters);
 void gemm_tn_planes(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np, int K, int m, int n, double alpha,
                     const double* A64, int lda, const double* B64, int ldb, bool g0corr, bool symmetric, double* out,
                     int ldo);
@@ -268,6 +281,10 @@ struct PlaneScope {
 };
 void proj_contract_x3(mc_ctx* ctx, const ProjA& base, const double* gk, int v0, int nv, const double* B, int ldb,
                       int K, int na, int n, double* const* outs, const int* ldos, const double* alphas);
+// copy bytes from host memory to the device on the context stream without
+// waiting for the stream: through the pinned staging buffer, which is reused
+// once the previous copy out of it has completed
+void upload_async(mc_ctx* ctx, void* dst, const void* src, size_t bytes);
 void gemm_tn_x3(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, int K, int m, int n, double alpha,
                 bool g0corr, bool symmetric, double* out, int ldo, const ProjA* proj);
 void gemm_nn_x3(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n, double alpha,

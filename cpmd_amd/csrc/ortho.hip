@@ -373,10 +373,35 @@ void lagrange_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_h
 }
 
 // CP orthonormality constraint, rortho (src/rortog_utils.mod.F90:315-498).
+static void rortog_core(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n,
+                        double epsog, int maxit, int* iters);
+
 void rortog_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n, double epsog,
                 int maxit, int* iters) {
   if (n <= 0) return;
   PlaneScope keep(ctx);   // c0 and cm are read-only until the final rotation
+  rortog_core(ctx, c0, cm, gam_host, tnon, n, epsog, maxit, iters);
+}
+
+// posupa (src/posupa_utils.mod.F90:41-160) with velupa's kick folded in:
+// cm += dtm c2 (c2 may be null), cp = c0 + dt cm, rortog(c0, cp) (cp += c0
+// gamx), cm = (cp - c0) / dt.  In mixed precision the first pass also builds
+// the bf16 planes of c0 and cp for the two overlaps of rortog.
+void posupa_dev(mc_ctx* ctx, const double2* c0, const double2* c2, double2* cm, double2* cp, int n, double dtm,
+                double dt, double epsog, int maxit, double* gam_host, int* iters) {
+  if (n <= 0) return;
+  PlaneScope keep(ctx);
+  const int K = wfn_rows(ctx);
+  if (ctx->gemm_x3 && K >= kX3Min)
+    posupa_planes(ctx, (const double*)c0, (const double*)c2, (double*)cm, (double*)cp, wfn_ld(ctx), K, n, dtm, dt);
+  else
+    cp_posupa_dev(ctx, c0, c2, cm, cp, n, dtm, dt);
+  rortog_core(ctx, c0, cp, gam_host, 0, n, epsog, maxit, iters);
+  cp_velocity_dev(ctx, c0, cp, cm, n, dt);
+}
+
+static void rortog_core(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n,
+                        double epsog, int maxit, int* iters) {
   double* rho = small(ctx, 0, n);
   double* sig = small(ctx, 1, n);
   double* del = small(ctx, 2, n);

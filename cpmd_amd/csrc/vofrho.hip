@@ -529,20 +529,23 @@ void set_ions(mc_ctx* ctx, const double* tau, const double* ht) {
     D.inv[k] = t[k];
   }
   D.tau.assign(tau, tau + 3 * (size_t)nat);
-  D.d_tau.ensure(sizeof(double) * frac.size());
-  D.d_frac.ensure(sizeof(double) * frac.size());
-  MC_HIP(hipMemcpy(D.d_tau.p, tau, sizeof(double) * 3 * (size_t)nat, hipMemcpyHostToDevice));
-  MC_HIP(hipMemcpy(D.d_frac.p, frac.data(), sizeof(double) * frac.size(), hipMemcpyHostToDevice));
+  // tau and the fractional coordinates in one asynchronous upload: d_tau
+  // holds [tau | frac], d_frac points at its second half
+  std::vector<double> both(6 * (size_t)std::max(nat, 1), 0.0);
+  std::copy(tau, tau + 3 * (size_t)nat, both.begin());
+  std::copy(frac.begin(), frac.end(), both.begin() + 3 * (size_t)std::max(nat, 1));
+  D.d_tau.ensure(sizeof(double) * both.size());
+  upload_async(ctx, D.d_tau.p, both.data(), sizeof(double) * both.size());
+  const double* d_frac = D.d_tau.as<double>() + 3 * (size_t)std::max(nat, 1);
   D.d_ph1.ensure(sizeof(double2) * (size_t)std::max(nat, 1) * (ctx->n1 + 1));
   D.d_ph2.ensure(sizeof(double2) * (size_t)std::max(nat, 1) * (ctx->n2 + 1));
   D.d_ph3.ensure(sizeof(double2) * (size_t)std::max(nat, 1) * (ctx->n3 + 1));
   if (nat > 0) {
-    hipLaunchKernelGGL(k_phase_tables, dim3(1, nat, 3), dim3(256), 0, ctx->stream, D.d_frac.as<double>(), nat,
+    hipLaunchKernelGGL(k_phase_tables, dim3(1, nat, 3), dim3(256), 0, ctx->stream, d_frac, nat,
                        ctx->n1, ctx->n2, ctx->n3, D.d_ph1.as<double2>(), D.d_ph2.as<double2>(),
                        D.d_ph3.as<double2>());
     MC_HIP(hipGetLastError());
   }
-  MC_HIP(hipStreamSynchronize(ctx->stream));
   D.ions_ready = true;
 }
 
@@ -736,7 +739,6 @@ void phfac(mc_ctx* ctx) {
   LAUNCH(MC_KID_VOFRHO, k_eigr, dim3((ctx->ldg + 255) / 256, D.nat), dim3(256), 0, ctx->d_gmw.as<int>(), ctx->ngw,
          ctx->ldg, ctx->n1, ctx->n2, ctx->n3, D.d_ph1.as<double2>(), D.d_ph2.as<double2>(), D.d_ph3.as<double2>(),
          P.d_eigr.as<double2>());
-  MC_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 }  // namespace mc

@@ -144,6 +144,16 @@ class Context:
         """cm += dtm c2; cp = c0 + dt cm (velupa + posupa before rortog)."""
         check(lib().mc_cp_posupa(self.h, c0.ptr, c2.ptr, cm.ptr, cp.ptr, nstate, dtm, dt))
 
+    def posupa_dev(self, c0: DeviceArray, c2: DeviceArray | None, cm: DeviceArray, cp: DeviceArray, nstate: int,
+                   dtm: float, dt: float, epsog: float = 1e-8, maxit: int = 60, want_gam: bool = False):
+        """cm += dtm c2; cp = c0 + dt cm; rortog(c0, cp); cm = (cp - c0)/dt.
+        Returns (gam or None, rortog iterations)."""
+        gam = np.zeros((nstate, nstate), order="F") if want_gam else None
+        it = C.c_int(0)
+        check(lib().mc_posupa_dev(self.h, c0.ptr, c2.ptr if c2 is not None else None, cm.ptr, cp.ptr, nstate, dtm,
+                                  dt, epsog, maxit, _p(gam) if want_gam else None, C.byref(it)))
+        return gam, it.value
+
     def cp_velocity(self, c0: DeviceArray, cp: DeviceArray, cm: DeviceArray, nstate: int, dt: float):
         """cm = (cp - c0) / dt (velocities after rortog)."""
         check(lib().mc_cp_velocity(self.h, c0.ptr, cp.ptr, cm.ptr, nstate, dt))

@@ -136,6 +136,18 @@ int mc_wfn_axpby(mc_ctx* ctx, double alpha, const void* x_dev, double beta, void
  * src/posupa_utils.mod.F90:69-73).  cp must not alias c0, c2 or cm. */
 int mc_cp_posupa(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, void* cm_dev, void* cp_dev, int nstate,
                  double dtm, double dt);
+/* posupa with velupa's kick folded in, on device wavefunctions: cm += dtm c2
+ * (skipped when c2 is NULL), cp = c0 + dt cm, the orthonormality constraint
+ * rortog(c0, cp) (cp += c0 gamx, epsog / maxit as in mc_rortog_dev), and the
+ * velocities cm = (cp - c0) / dt.  c0 keeps the old coefficients; the caller
+ * swaps c0 and cp (CPMD's final dcopy).  gam: the (nstate, nstate) gamx on
+ * the host, or NULL; iters: the rortog iterations, or NULL.  In mixed
+ * precision the first pass also writes the split-bf16 planes of c0 and cp
+ * used by the two overlaps of rortog.
+ * Replaces velupa + posupa(c0,cm,c2,gamx,nstate), src/velupa_utils.mod.F90:25-72,
+ * src/posupa_utils.mod.F90:41-160 (cntl%tharm, tmass off). */
+int mc_posupa_dev(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, void* cm_dev, void* cp_dev, int nstate,
+                  double dtm, double dt, double epsog, int maxit, double* gam, int* iters);
 /* Velocities after rortog: cm = (cp - c0) / dt, the same as CPMD's
  * cm += c0 gamx / dt (src/posupa_utils.mod.F90:150-158) when rortog returned
  * cp = c0 + dt cm + c0 gamx. */

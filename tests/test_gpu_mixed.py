@@ -178,3 +178,29 @@ def test_mixed_rnlsm_combined(gpu):
     odfnl = oracle.rnlsm2(OP, c, gk, sysm.cell.tpiba)
     rel = np.sqrt(np.mean((dfnl - odfnl) ** 2) / np.mean(odfnl ** 2))
     assert rel < 5e-7, rel
+
+
+def test_posupa_combined(gpu, cfg2_130):
+    """mc_posupa_dev (velupa kick + posupa + rortog + velocities in one call,
+    the bf16 planes of c0 and cp built by its first pass in mixed precision)
+    equals the separate velupa/posupa, rortog and velocity calls exactly in
+    FP64 and in mixed precision."""
+    sysm, c, cp = cfg2_130
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    n = c.shape[1]
+    rng = np.random.default_rng(61)
+    c2 = np.asfortranarray(rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape)) * 1e-3
+    c2[0] = c2[0].real
+    cm = np.asfortranarray((cp - c) / 0.5)
+    for mode in ("fp64", "mixed"):
+        ctx.set_precision(mode)
+        d0, d2 = ctx.upload(c), ctx.upload(c2)
+        dm_a, dm_b = ctx.upload(cm), ctx.upload(cm)
+        dp_a, dp_b = ctx.alloc_wfn(n), ctx.alloc_wfn(n)
+        gam_a, it_a = ctx.posupa_dev(d0, d2, dm_a, dp_a, n, 0.01, 0.5, want_gam=True)
+        ctx.cp_posupa(d0, d2, dm_b, dp_b, n, 0.01, 0.5)
+        gam_b, it_b = ctx.rortog_dev(d0, dp_b, n, epsog=1e-8, maxit=60)
+        ctx.cp_velocity(d0, dp_b, dm_b, n, 0.5)
+        assert it_a == it_b and np.array_equal(gam_a, gam_b), mode
+        pa, pb = ctx.download(dp_a, n), ctx.download(dp_b, n)
+        assert np.array_equal(pa, pb) and np.array_equal(ctx.download(dm_a, n), ctx.download(dm_b, n)), mode
