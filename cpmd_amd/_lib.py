@@ -96,6 +96,7 @@ SIGNATURES = {
     "mc_phfac": (C.c_int, [_vp]),
     "mc_fnonloc": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp]),
     "mc_fnonloc_dev": (C.c_int, [_vp, _vp, _dp, C.c_int, _dp]),
+    "mc_forces_c2_dev": (C.c_int, [_vp, _vp, _vp, _dp, _vp, C.c_int]),
     "mc_timer_enable": (C.c_int, [_vp, C.c_int]),
     "mc_timer_reset": (C.c_int, [_vp]),
     "mc_timer_get": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_longlong)]),

@@ -641,6 +641,15 @@ int mc_vpsi_dev(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const double* f, 
   MC_API_END
 }
 
+int mc_forces_c2_dev(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const double* f, const void* vpot_dev,
+                     int nstate) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c0_dev && c2_dev && f && vpot_dev, MC_ERR_ARG, "null argument");
+  MC_CHECK(nstate >= 0, MC_ERR_ARG, "nstate < 0");
+  mc::forces_c2_dev(ctx, (const double2*)c0_dev, (double2*)c2_dev, f, (const double*)vpot_dev, nstate);
+  MC_API_END
+}
+
 int mc_vpsi(mc_ctx* ctx, const double* c0, int ldc, double* c2, const double* f, const double* vpot,
             int nstate) {
   MC_API_BEGIN

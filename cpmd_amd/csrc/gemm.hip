@@ -290,8 +290,14 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn(const double* __restrict__ A
           d[0] += yr;
           if (i + 1 < M) d[1] += yi;
         } else {
-          d[0] = (beta == 0.0 ? 0.0 : beta * d[0]) + alpha * x0;
-          if (i + 1 < M) d[1] = (beta == 0.0 ? 0.0 : beta * d[1]) + alpha * x1;
+          const double y0 = (beta == 0.0 ? 0.0 : beta * d[0]) + alpha * x0;
+          d[0] = y0;
+          if (epi.vcm) epi.vcm[(size_t)j * ldd + i] = epi.vodt * (y0 - A[(size_t)j * lda + i]);
+          if (i + 1 < M) {
+            const double y1 = (beta == 0.0 ? 0.0 : beta * d[1]) + alpha * x1;
+            d[1] = y1;
+            if (epi.vcm) epi.vcm[(size_t)j * ldd + i + 1] = epi.vodt * (y1 - A[(size_t)j * lda + i + 1]);
+          }
         }
       }
     }

@@ -518,44 +518,59 @@ __global__ __launch_bounds__(256, 3) void k_nn_planes(const double* __restrict__
     // completed before the MFMAs); no fence: the next slice's copies stay in flight
     asm volatile("s_barrier" ::: "memory");
   }
-  // epilogue: lane holds rows i .. i+3 (i = 4 lq + 16a + wr + i0) of column j
+  // epilogue: lane holds rows 4 lq + 16 a + wr + i0 .. +3 of column j0 + wc + 16 b + li.
+  // Each wave stages its 64 x 16 block of columns wc + 16 b .. through its own
+  // 4 KiB of Gs (fp32, 4-row groups XOR-swizzled by column), then reads it back
+  // as rows 2 pr, 2 pr + 1 of two columns per instruction: the global reads and
+  // writes of D (and of A and vcm) are 256-byte column runs.  The last loop
+  // iteration ended with a barrier and no copy into Gs is in flight.
+  float* st = reinterpret_cast<float*>(Gs) + wave * (64 * 16);
+  const int hl = lane >> 5, pr = lane & 31;
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
-    const int j = j0 + wc + 16 * b + li;
-    if (j >= n) continue;
-    double f = 1.0;
-    if (epi.twnl) {
-      f = epi.f[j];
-      if (f < 1e-5) f = 1.0;
-    }
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int i = i0 + wr + 16 * a + 4 * lq;
-      if (i >= M) continue;
+    for (int a = 0; a < 4; ++a)
+      *reinterpret_cast<f4*>(st + li * 64 + (((4 * a + lq) ^ li) * 4)) = acc[a][b];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll 2
+    for (int it = 0; it < 8; ++it) {
+      const int c = 2 * it + hl, row = 2 * pr;
+      const float2 x = *reinterpret_cast<const float2*>(st + c * 64 + (((row >> 2) ^ c) * 4) + (row & 3));
+      const int j = j0 + wc + 16 * b + c, i = i0 + wr + row;   // i even, M even: rows i, i + 1
+      if (j >= n || i >= M) continue;
       double* d = D + (size_t)j * ldd + i;
+      const double x0 = (double)x.x, x1 = (double)x.y;
       if (epi.twnl) {
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-          const double x0 = acc[a][b][2 * p], x1 = acc[a][b][2 * p + 1];
-          const double tw = epi.twnl[(i >> 1) + p];
-          double yr, yi;
-          switch (epi.lmod4) {   // ctm = -f (-i)^l ; y = tw * ctm * x
-            case 0: yr = -f * tw * x0; yi = -f * tw * x1; break;
-            case 1: yr = -f * tw * x1; yi = f * tw * x0; break;
-            case 2: yr = f * tw * x0; yi = f * tw * x1; break;
-            default: yr = f * tw * x1; yi = -f * tw * x0; break;
-          }
-          if (i + 2 * p < M) {
-            d[2 * p] += yr;
-            d[2 * p + 1] += yi;
-          }
+        double f = epi.f[j];
+        if (f < 1e-5) f = 1.0;
+        const double tw = epi.twnl[i >> 1];
+        double yr, yi;
+        switch (epi.lmod4) {   // ctm = -f (-i)^l ; y = tw * ctm * x
+          case 0: yr = -f * tw * x0; yi = -f * tw * x1; break;
+          case 1: yr = -f * tw * x1; yi = f * tw * x0; break;
+          case 2: yr = f * tw * x0; yi = f * tw * x1; break;
+          default: yr = f * tw * x1; yi = -f * tw * x0; break;
         }
+        d[0] += yr;
+        d[1] += yi;
       } else {
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (i + r < M) d[r] = (beta == 0.0 ? 0.0 : beta * d[r]) + alpha * (double)acc[a][b][r];
+        const double y0 = (beta == 0.0 ? 0.0 : beta * d[0]) + alpha * x0;
+        const double y1 = (beta == 0.0 ? 0.0 : beta * d[1]) + alpha * x1;
+        d[0] = y0;
+        d[1] = y1;
+        if (epi.vcm) {
+          const double* av = A + (size_t)j * lda + i;
+          double* v = epi.vcm + (size_t)j * ldd + i;
+          v[0] = epi.vodt * (y0 - av[0]);
+          v[1] = epi.vodt * (y1 - av[1]);
+        }
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 

@@ -218,7 +218,11 @@ __global__ __launch_bounds__(256, 2) void k_x3_nn(const double* __restrict__ A, 
       } else {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (i + r < M) d[r] = (beta == 0.0 ? 0.0 : beta * d[r]) + alpha * (double)acc[a][b][r];
+          if (i + r < M) {
+            const double y = (beta == 0.0 ? 0.0 : beta * d[r]) + alpha * (double)acc[a][b][r];
+            d[r] = y;
+            if (epi.vcm) epi.vcm[(size_t)j * ldd + i + r] = epi.vodt * (y - A[(size_t)j * lda + i + r]);
+          }
       }
     }
   }

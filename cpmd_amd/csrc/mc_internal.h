@@ -116,6 +116,10 @@ struct FnlEpi {
   const double* twnl;   // (ldg) of this projector; nullptr for a plain GEMM
   const double* f;      // (n) occupations on the device
   int lmod4;
+  // plain GEMM only: also vcm(i, j) = vodt (D_new(i, j) - A(i, j)) with vcm
+  // at the leading dimension of D (A and D of one layout); nullptr: off
+  double* vcm = nullptr;
+  double vodt = 0.0;
 };
 
 // Nonlocal pseudopotential data of the current geometry.
@@ -315,5 +319,7 @@ void wfn_download(mc_ctx* ctx, const double2* dev, int nstate, double* host, int
 void rhoofr_dev(mc_ctx* ctx, const double2* c0, const double* f_host, int nstate,
                 double* rho, double* ekin, double* csumg, double* csumr);
 void vpsi_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host,
-              const double* vpot, int nstate);
+              const double* vpot, int nstate, bool acc = true);
+void forces_c2_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host,
+                   const double* vpot, int nstate);
 }  // namespace mc

@@ -374,7 +374,7 @@ void lagrange_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_h
 
 // CP orthonormality constraint, rortho (src/rortog_utils.mod.F90:315-498).
 static void rortog_core(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n,
-                        double epsog, int maxit, int* iters);
+                        double epsog, int maxit, int* iters, double2* vel = nullptr, double vdt = 0.0);
 
 void rortog_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n, double epsog,
                 int maxit, int* iters) {
@@ -386,7 +386,8 @@ void rortog_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, i
 // posupa (src/posupa_utils.mod.F90:41-160) with velupa's kick folded in:
 // cm += dtm c2 (c2 may be null), cp = c0 + dt cm, rortog(c0, cp) (cp += c0
 // gamx), cm = (cp - c0) / dt.  In mixed precision the first pass also builds
-// the bf16 planes of c0 and cp for the two overlaps of rortog.
+// the bf16 planes of c0 and cp for the two overlaps of rortog; the velocity is
+// written by the epilogue of the final rotation.
 void posupa_dev(mc_ctx* ctx, const double2* c0, const double2* c2, double2* cm, double2* cp, int n, double dtm,
                 double dt, double epsog, int maxit, double* gam_host, int* iters) {
   if (n <= 0) return;
@@ -396,12 +397,11 @@ void posupa_dev(mc_ctx* ctx, const double2* c0, const double2* c2, double2* cm, 
     posupa_planes(ctx, (const double*)c0, (const double*)c2, (double*)cm, (double*)cp, wfn_ld(ctx), K, n, dtm, dt);
   else
     cp_posupa_dev(ctx, c0, c2, cm, cp, n, dtm, dt);
-  rortog_core(ctx, c0, cp, gam_host, 0, n, epsog, maxit, iters);
-  cp_velocity_dev(ctx, c0, cp, cm, n, dt);
+  rortog_core(ctx, c0, cp, gam_host, 0, n, epsog, maxit, iters, cm, dt);
 }
 
 static void rortog_core(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n,
-                        double epsog, int maxit, int* iters) {
+                        double epsog, int maxit, int* iters, double2* vel, double vdt) {
   double* rho = small(ctx, 0, n);
   double* sig = small(ctx, 1, n);
   double* del = small(ctx, 2, n);
@@ -449,7 +449,16 @@ static void rortog_core(mc_ctx* ctx, const double2* c0, double2* cm, double* gam
            "RORTHO: DIFGAM=" + std::to_string(difgam) + " ITER=" + std::to_string(it) + " > " + std::to_string(maxit));
   hipLaunchKernelGGL(k_symma, g2, dim3(256), 0, ctx->stream, gam, n);
   MC_HIP(hipGetLastError());
-  rotate_dev(ctx, 1.0, c0, 1.0, cm, gam, n);
+  if (vel) {
+    // cm += c0 gam with the velocity vel = (cm - c0) / vdt of the result in the same pass
+    FnlEpi e{};
+    e.vcm = (double*)vel;
+    e.vodt = 1.0 / vdt;
+    gemm_nn(ctx, (const double*)c0, wfn_ld(ctx), gam, n, wfn_ld(ctx), n, n, 1.0, 1.0, (double*)cm, wfn_ld(ctx), false,
+            &e);
+  } else {
+    rotate_dev(ctx, 1.0, c0, 1.0, cm, gam, n);
+  }
   if (gam_host) {
     MC_HIP(hipMemcpyAsync(gam_host, gam, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->stream));
     MC_HIP(hipStreamSynchronize(ctx->stream));

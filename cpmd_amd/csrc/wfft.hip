@@ -770,7 +770,7 @@ template <typename T, int N>
 __global__ __launch_bounds__(kThreads) void k_zfwd(WParams w, const v2<T>* __restrict__ A,
                                                    const double2* __restrict__ c0, double2* __restrict__ c2,
                                                    const double* __restrict__ f, int nstate, int pair0, int lo,
-                                                   double scale) {
+                                                   double scale, int acc) {
   using V = v2<T>;
   const int n3 = N > 0 ? N : w.n3;
   V* tw = lds_base<T>();
@@ -817,13 +817,13 @@ __global__ __launch_bounds__(kThreads) void k_zfwd(WParams w, const v2<T>* __res
     const double2 fm = csub(psin, psii);
     const double g2 = w.tpiba2 * w.hg[j];
     const double2 x1 = ca[j];
-    double2 r1 = da[j];
+    double2 r1 = acc ? da[j] : make_double2(0.0, 0.0);
     r1.x += -fi * (g2 * x1.x + fp.x);
     r1.y += -fi * (g2 * x1.y + fm.y);
     da[j] = r1;
     if (has2) {
       const double2 x2 = cb[j];
-      double2 r2 = db[j];
+      double2 r2 = acc ? db[j] : make_double2(0.0, 0.0);
       r2.x += -fip1 * (g2 * x2.x + fp.y);
       r2.y += -fip1 * (g2 * x2.y - fm.x);
       db[j] = r2;
@@ -1014,7 +1014,7 @@ static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, in
 
 template <typename T>
 static void vpsi_impl(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, const double* vpot,
-                      int nstate) {
+                      int nstate, bool acc) {
   using V = v2<T>;
   const WParams w = make_params(ctx);
   const Shapes s = shapes<T>(ctx);
@@ -1049,7 +1049,7 @@ static void vpsi_impl(mc_ctx* ctx, const double2* c0, double2* c2, const double*
     ypass<T, -1>(ctx, w, s, A, B, np, shy);
     with_length(ctx->n3, [&](auto NN) {
       LAUNCH(MC_KID_ZFWD, (k_zfwd<T, decltype(NN)::value>), dim3(s.gx, np), dim3(kThreads), shz, w, A, c0, c2,
-             ctx->w_f.as<double>(), nstate, pair0, s.lo, scale);
+             ctx->w_f.as<double>(), nstate, pair0, s.lo, scale, acc ? 1 : 0);
     });
   }
 }
@@ -1062,11 +1062,33 @@ void rhoofr_dev(mc_ctx* ctx, const double2* c0, const double* f_host, int nstate
     rhoofr_impl<double>(ctx, c0, f_host, nstate, rho, ekin, csumg, csumr);
 }
 
-void vpsi_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, const double* vpot, int nstate) {
+void vpsi_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, const double* vpot, int nstate,
+              bool acc) {
   if (ctx->fft_fp32)
-    vpsi_impl<float>(ctx, c0, c2, f_host, vpot, nstate);
+    vpsi_impl<float>(ctx, c0, c2, f_host, vpot, nstate, acc);
   else
-    vpsi_impl<double>(ctx, c0, c2, f_host, vpot, nstate);
+    vpsi_impl<double>(ctx, c0, c2, f_host, vpot, nstate, acc);
+}
+
+// rows ngw .. ldg-1 of the first n states of c (ld ldg complex rows) <- 0
+__global__ void k_zero_pad_rows(double2* __restrict__ c, int ngw, int ldg, int n) {
+  const int np = ldg - ngw;
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= (size_t)np * n) return;
+  const int col = (int)(t / np), r = ngw + (int)(t - (size_t)col * np);
+  c[(size_t)col * ldg + r] = make_double2(0.0, 0.0);
+}
+
+void forces_c2_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, const double* vpot,
+                   int nstate) {
+  if (nstate <= 0) return;
+  if (ctx->ldg > ctx->ngw) {
+    const size_t nt = (size_t)(ctx->ldg - ctx->ngw) * nstate;
+    LAUNCH(MC_KID_MISC, k_zero_pad_rows, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c2, ctx->ngw, ctx->ldg,
+           nstate);
+  }
+  vpsi_dev(ctx, c0, c2, f_host, vpot, nstate, false);
+  fnonloc_dev(ctx, c2, f_host, nstate, nullptr);
 }
 
 }  // namespace mc

@@ -57,7 +57,11 @@ class KSDevice:
         for is_, na in enumerate(self.na):
             h = self.hmat[:, :, is_]
             blk = fnl[isa0:isa0 + na]                                   # (na, nhxs, n)
-            e += float(np.einsum("i,aji,jk,aki->", f, blk, h, blk))
+            for j in range(blk.shape[1]):
+                for k in range(blk.shape[1]):
+                    if h[j, k] != 0.0:
+                        # per-state column dots, then the occupation-weighted sum
+                        e += float(h[j, k] * (np.einsum("ai,ai->i", blk[:, j, :], blk[:, k, :]) @ f))
             isa0 += na
         return e
 
@@ -79,9 +83,7 @@ class KSDevice:
         out["etot"] = (out["ekin"] + out["ehep"] + out["epseu"] + out["exc"] + out["esr"] - out["eself"]
                        + out["enl"])
         if gradient is not None:
-            ctx.zero_wfn(gradient, n)
-            ctx.vpsi_dev(d_c0, gradient, f, self.d_v, n)
-            ctx.fnonloc_dev(gradient, f, n)
+            ctx.forces_c2_dev(d_c0, gradient, f, self.d_v, n)
         if forces:
             out["fion"] = ctx.rnlfor(f, fion=floc + fesr)
         return out

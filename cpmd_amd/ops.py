@@ -421,6 +421,12 @@ class Context:
         f = np.ascontiguousarray(f, dtype=np.float64)
         check(lib().mc_fnonloc_dev(self.h, c2.ptr, _p(f), nstate, None))
 
+    def forces_c2_dev(self, c0: DeviceArray, c2: DeviceArray, f: np.ndarray, vpot: DeviceArray, nstate: int):
+        """c2 <- vpsi + fnonloc gradient of c0 (c2 overwritten, not accumulated);
+        fnonloc uses the device fnl of the last rnlsm1."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        check(lib().mc_forces_c2_dev(self.h, c0.ptr, c2.ptr, _p(f), vpot.ptr, nstate))
+
     # ------------------------------------------------------------ timing
     def timer_enable(self, on: bool = True):
         check(lib().mc_timer_enable(self.h, 1 if on else 0))

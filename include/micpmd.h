@@ -111,7 +111,9 @@ size_t mc_wfn_ld(const mc_ctx* ctx);
 enum { MC_PREC_FP64 = 0, MC_PREC_FP32 = 1, MC_PREC_MIXED = 2, MC_PREC_MIXED_BF16X3 = 3 };
 int mc_set_precision(mc_ctx* ctx, int fft_precision);
 
-/* Device memory helpers (hipMalloc / hipFree / hipMemcpy). */
+/* Device memory helpers (hipMalloc / hipFree / hipMemcpy).  The copies run on
+ * the null stream and do not wait for work queued on a context's stream: call
+ * mc_ctx_synchronize first when the buffer was written by a context call. */
 int mc_dev_alloc(size_t bytes, void** ptr);
 int mc_dev_free(void* ptr);
 int mc_memcpy_h2d(void* dst, const void* src, size_t bytes);
@@ -271,6 +273,15 @@ int mc_rnlsm1_dev(mc_ctx* ctx, const void* c0_dev, int nstate, double* fnl);
  * src/fnonloc_utils.mod.F90:55-397 (Gamma). */
 int mc_fnonloc(mc_ctx* ctx, double* c2, int ldc, const double* f, int nstate, const double* fnl);
 int mc_fnonloc_dev(mc_ctx* ctx, void* c2_dev, const double* f, int nstate, const double* fnl);
+
+/* Electronic gradient of forces: c2 <- 0, vpsi, fnonloc (with the device fnl
+ * of the last rnlsm1) in one call.  The vpsi pass writes c2 instead of adding
+ * to it, so c2 is neither cleared nor read first; rows ngw .. ldg-1 are zeroed.
+ * Replaces the zeroing(c2) / vpsi / fnonloc sequence of
+ * forces(c0,c2,tau0,fion,rhoe,psi,nstate,nkpoint,lproj,tfor),
+ * src/forces_driver.mod.F90:175-282 (Gamma, one k point). */
+int mc_forces_c2_dev(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const double* f,
+                     const void* vpot_dev, int nstate);
 
 /* G vectors gk(3, ngw) in units of 2 pi / alat, CPMD G order (cppt::gk, the
  * first ngw columns; set once per cell).  Needed by mc_rnlsm2. */

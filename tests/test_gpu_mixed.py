@@ -204,3 +204,37 @@ def test_posupa_combined(gpu, cfg2_130):
         assert it_a == it_b and np.array_equal(gam_a, gam_b), mode
         pa, pb = ctx.download(dp_a, n), ctx.download(dp_b, n)
         assert np.array_equal(pa, pb) and np.array_equal(ctx.download(dm_a, n), ctx.download(dm_b, n)), mode
+
+
+def test_forces_c2_combined(gpu):
+    """forces_c2 (vpsi writing c2 instead of adding to it, then fnonloc with the
+    device fnl) equals c2 <- 0, vpsi, fnonloc bit for bit in both precisions,
+    whatever c2 held before, padding rows included."""
+    sysm = S.make_system(24.0, 64, 70.0, nstate=64)
+    c = S.initial_coefficients(sysm, seed=47)
+    ngw = sysm.gset.ngw
+    na, nhxs, ngh, lang, hmat, twnl, eigr = S.projector_set(sysm, sysm.gset.miller[:ngw], sysm.gset.hg[:ngw])
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    ctx.set_projectors(na, nhxs, ngh, lang, hmat, twnl)
+    ctx.set_eigr(eigr)
+    n = c.shape[1]
+    f = np.full(n, 2.0)
+    f[5] = 0.0
+    d = ctx.upload(c)
+    v = S.real_space_potential(sysm)
+    vd = ops.DeviceArray(8 * ctx.nnr1).from_host(v)
+    nd = 2 * ctx.ld * n
+    rng = np.random.default_rng(11)
+    for mode in ("fp64", "mixed"):
+        ctx.set_precision(mode)
+        ctx.rnlsm_dev(d, n, tfor=False)
+        ref = ctx.alloc_wfn(n)
+        ctx.zero_wfn(ref, n)
+        ctx.vpsi_dev(d, ref, f, vd, n)
+        ctx.fnonloc_dev(ref, f, n)
+        out = ctx.alloc_wfn(n).from_host(rng.standard_normal(nd))
+        ctx.forces_c2_dev(d, out, f, vd, n)
+        ctx.synchronize()          # to_host copies on the null stream
+        a, b = out.to_host(nd), ref.to_host(nd)
+        assert np.array_equal(a, b)
+        assert np.abs(b).max() > 0.0
