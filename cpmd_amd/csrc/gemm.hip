@@ -369,6 +369,12 @@ void gemm_nn(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, i
     gemm_nn_x3(ctx, A, lda, G, ldgm, M, k, n, alpha, beta, D, ldd, upper_g, epi);
     return;
   }
+  gemm_nn_f64(ctx, A, lda, G, ldgm, M, k, n, alpha, beta, D, ldd, upper_g, epi, MC_KID_GEMM);
+}
+
+void gemm_nn_f64(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n, double alpha,
+                 double beta, double* D, int ldd, bool upper_g, const FnlEpi* epi, int kid) {
+  if (M <= 0 || n <= 0) return;
   FnlEpi e{};
   if (epi) e = *epi;
   // 128-tiles when they already give >= 1024 workgroups, 64-tiles otherwise
@@ -377,7 +383,7 @@ void gemm_nn(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, i
   const int tm = (M + BT - 1) / BT, tn = (n + BT - 1) / BT;
   const int total = tm * tn;
   const int launch = (total + kXcd - 1) / kXcd * kXcd;
-  timer_begin(ctx, MC_KID_GEMM);
+  timer_begin(ctx, kid);
   if (BT == 128)
     hipLaunchKernelGGL(k_gemm_nn<128>, dim3(launch), dim3(256), 0, ctx->stream, A, lda, G, ldgm, M, k, n, alpha, beta,
                        D, ldd, upper_g ? 1 : 0, e, tn, total);
@@ -385,7 +391,7 @@ void gemm_nn(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, i
     hipLaunchKernelGGL(k_gemm_nn<64>, dim3(launch), dim3(256), 0, ctx->stream, A, lda, G, ldgm, M, k, n, alpha, beta,
                        D, ldd, upper_g ? 1 : 0, e, tn, total);
   MC_HIP(hipGetLastError());
-  timer_end(ctx, MC_KID_GEMM);
+  timer_end(ctx, kid);
 }
 
 }  // namespace mc

@@ -149,6 +149,11 @@ struct DensitySet {
   DevBuf d_sp_start, d_sp_of, d_zv, d_raggio, d_vps, d_rhops;
   DevBuf d_tau, d_ph1, d_ph2, d_ph3;   // d_tau: [tau (3, nat) | fractional coordinates (3, nat)]
   DevBuf w_W, w_g, w_part;
+  // z-sticks of the density set: the distinct (m1, m2) (d_stk, sorted) and
+  // the stick of each G (d_sidx); operands of the stick GEMMs of vofrho
+  int nstick = 0;
+  DevBuf d_stk, d_sidx;
+  DevBuf w_q, w_bs, w_bf, w_sf, w_y, w_t;
 };
 
 }  // namespace mc
@@ -241,6 +246,9 @@ void phfac(mc_ctx* ctx);
 // contractions (gemm.hip)
 void gemm_tn(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, int K, int m, int n, double alpha,
              bool g0corr, bool symmetric, double* out, int ldo, const ProjA* proj);
+// the FP64 kernel of gemm_nn whatever the context precision, timed under kid
+void gemm_nn_f64(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n, double alpha,
+                 double beta, double* D, int ldd, bool upper_g, const FnlEpi* epi, int kid);
 void gemm_nn(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n, double alpha,
              double beta, double* D, int ldd, bool upper_g, const FnlEpi* epi);
 // out = alpha * sum of the nsplit partials in ctx->w_gemm (- G=0 term), mirrored if symmetric

@@ -22,6 +22,7 @@
 // potfor); the tables are built on the device from the fractional positions.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -32,7 +33,6 @@ namespace mc {
 
 namespace {
 
-constexpr int kAtomsPerBlock = 16;   // atoms per potfor workgroup
 constexpr int kRedBlocks = 1024;     // workgroups of the per-G and x passes
 
 struct DParams {
@@ -230,18 +230,14 @@ struct GParams {
   const double2* ph1;
   const double2* ph2;
   const double2* ph3;
+  const int* sidx;           // (nhg) stick of each G
+  int nstick, L3;            // sticks; L3 = n3 + 1 positions m3 + h3 per stick
+  const double* sf;          // (nsp) blocks of (nstick x 2 L3): Re | Im of the structure factor
 };
 
-__device__ __forceinline__ double2 phase_of(const GParams& g, int isa, int m1, int m2, int m3, int n1, int n2,
-                                            int n3) {
-  const double2 a = g.ph1[(size_t)isa * (n1 + 1) + m1 + g.h1];
-  const double2 b = g.ph2[(size_t)isa * (n2 + 1) + m2 + g.h2];
-  const double2 c = g.ph3[(size_t)isa * (n3 + 1) + m3 + g.h3];
-  return cmul(cmul(a, b), c);
-}
-
 // eicalc + ppener: rho_e(G) = W(+G)/N, eivps, eirop, vtemp; block partials of
-// (eh, ei, ee, eps) over G != 0; the G = 0 terms are added on the host
+// (eh, ei, ee, eps) over G != 0; the G = 0 terms are added on the host.  The
+// species structure factors sum_{a in sp} e^{-iG.R_a} come from the stick GEMM.
 __global__ __launch_bounds__(kThreads) void k_eicalc_ppener(GParams g, int n1, int n2, int n3,
                                                             const double2* __restrict__ W, double scale,
                                                             double2* __restrict__ rhog, double2* __restrict__ eirop,
@@ -249,17 +245,13 @@ __global__ __launch_bounds__(kThreads) void k_eicalc_ppener(GParams g, int n1, i
                                                             double* __restrict__ part) {
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int ig = blockIdx.x * blockDim.x + threadIdx.x; ig < g.nhg; ig += gridDim.x * blockDim.x) {
-    const int m1 = g.gm[3 * ig], m2 = g.gm[3 * ig + 1], m3 = g.gm[3 * ig + 2];
     const double2 w = W[g.gidx[2 * ig]];
     const double2 re = make_double2(w.x * scale, w.y * scale);
     double2 ev = make_double2(0.0, 0.0), er = make_double2(0.0, 0.0);
+    const size_t ps = (size_t)(g.gm[3 * ig + 2] + g.h3) * g.nstick + g.sidx[ig];
     for (int is = 0; is < g.nsp; ++is) {
-      double2 sf = make_double2(0.0, 0.0);
-      for (int isa = g.sp_start[is]; isa < g.sp_start[is + 1]; ++isa) {
-        const double2 e = phase_of(g, isa, m1, m2, m3, n1, n2, n3);
-        sf.x += e.x;
-        sf.y += e.y;
-      }
+      const double* S = g.sf + (size_t)is * g.nstick * 2 * g.L3;
+      const double2 sf = make_double2(S[ps], S[ps + (size_t)g.L3 * g.nstick]);
       const double vp = g.vps[is + (size_t)g.nsp * ig], rp = g.rhops[is + (size_t)g.nsp * ig];
       ev.x += vp * sf.x; ev.y += vp * sf.y;
       er.x += rp * sf.x; er.y += rp * sf.y;
@@ -295,54 +287,98 @@ __global__ void k_scatter_vtemp(GParams g, const double2* __restrict__ vtemp, do
   }
 }
 
-// potfor partials: block (gb, ab) sums over its G range for atoms
-// [ab*16, ab*16+16): part[(gb * nat + isa) * 3 + k] = sum_G -Im(e^{-iG.R} X) gk_k,
-// X = rhops scg conj(rho_tot) + vps conj(rho_e), G != 0
-__global__ __launch_bounds__(kThreads) void k_potfor(GParams g, int n1, int n2, int n3, const double* __restrict__ gk,
-                                                     const double2* __restrict__ rhog,
-                                                     const double2* __restrict__ eirop, const int* __restrict__ sp_of,
-                                                     int gchunk, double* __restrict__ part) {
-  const int a0 = blockIdx.y * kAtomsPerBlock;
-  const int na = min(kAtomsPerBlock, g.nat - a0);
-  const int gbeg = blockIdx.x * gchunk;
-  const int gend = min(g.nhg, gbeg + gchunk);
-  double acc[kAtomsPerBlock * 3];
-#pragma unroll
-  for (int q = 0; q < kAtomsPerBlock * 3; ++q) acc[q] = 0.0;
-  for (int ig = gbeg + threadIdx.x; ig < gend; ig += kThreads) {
+// ---------------------------------------------------------------------------
+// Structure factors and local-potential ion forces as FP64 GEMMs over the
+// z-sticks (m1, m2) of the density set.  With e^{-iG.R_a} = p1_a(m1) p2_a(m2)
+// p3_a(m3) (phase tables) and Q(s, a) = p1_a(m1_s) p2_a(m2_s):
+//   S_sp(s, m3) = sum_{a in sp} Q(s, a) p3_a(m3)                        (eicalc)
+//   T_k(s, a)   = sum_m3 p3_a(m3) X_sp(G) G_k,  G = (m1_s, m2_s, m3)     (potfor)
+//   F_k(a)      = -sum_s Im(Q(s, a) T_k(s, a))
+// Complex products run as real GEMMs: one operand holds (re, im) pairs along
+// the contraction index, the other the real 2 x 2 expansion of p3.
+// ---------------------------------------------------------------------------
+
+// Q (nstick x 2 nat, ld nstick): columns 2a, 2a + 1 = Re, Im of Q(s, a)
+__global__ void k_stick_q(const int* __restrict__ stk, int nstick, int n1, int n2, const double2* __restrict__ ph1,
+                          const double2* __restrict__ ph2, double* __restrict__ Q) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x, a = blockIdx.y;
+  if (s >= nstick) return;
+  const double2 p = cmul(ph1[(size_t)a * (n1 + 1) + stk[2 * s] + n1 / 2], ph2[(size_t)a * (n2 + 1) + stk[2 * s + 1] + n2 / 2]);
+  Q[(size_t)(2 * a) * nstick + s] = p.x;
+  Q[(size_t)(2 * a + 1) * nstick + s] = p.y;
+}
+
+// Bs (2 nat x 2 L3, ld 2 nat), S = Q Bs: rows 2a, 2a + 1 of column t (Re S)
+// hold (Re p3, -Im p3), of column L3 + t (Im S) (Im p3, Re p3).
+// Bf (2 L3 x 2 nat, ld 2 L3), T = Y Bf: per species a block of columns
+// [Re | Im] (atom a of a species with first atom a0 and na atoms: Re column
+// 2 a0 + (a - a0), Im column 2 a0 + na + (a - a0)); rows 2t, 2t + 1 hold
+// (Re p3, -Im p3) in the Re column and (Im p3, Re p3) in the Im column.
+__global__ void k_stick_p3(const double2* __restrict__ ph3, int nat, int n3, const int* __restrict__ sp_of,
+                           const int* __restrict__ sp_start, double* __restrict__ Bs, double* __restrict__ Bf) {
+  const int L3 = n3 + 1;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x, a = blockIdx.y;
+  if (t >= L3) return;
+  const double2 p = ph3[(size_t)a * L3 + t];
+  const size_t lds = 2 * (size_t)nat;
+  Bs[(size_t)t * lds + 2 * a] = p.x;
+  Bs[(size_t)t * lds + 2 * a + 1] = -p.y;
+  Bs[(size_t)(L3 + t) * lds + 2 * a] = p.y;
+  Bs[(size_t)(L3 + t) * lds + 2 * a + 1] = p.x;
+  const int sp = sp_of[a], a0 = sp_start[sp], na = sp_start[sp + 1] - a0;
+  const size_t ldf = 2 * (size_t)L3;
+  double* re = Bf + (size_t)(2 * a0 + (a - a0)) * ldf;
+  double* im = Bf + (size_t)(2 * a0 + na + (a - a0)) * ldf;
+  re[2 * t] = p.x;
+  re[2 * t + 1] = -p.y;
+  im[2 * t] = p.y;
+  im[2 * t + 1] = p.x;
+}
+
+// Y (nsp blocks of 3 nstick x 2 L3, ld 3 nstick, zeroed before): row
+// k nstick + s, columns 2t, 2t + 1 = Re, Im of X_sp(G) G_k for G != 0 on stick
+// s at t = m3 + h3, X_sp = rhops_sp scg conj(rho_tot) + vps_sp conj(rho_e)
+__global__ void k_stick_y(GParams g, const double* __restrict__ gk, const double2* __restrict__ rhog,
+                          const double2* __restrict__ eirop, double* __restrict__ Y) {
+  const size_t ldy = 3 * (size_t)g.nstick, blk = ldy * 2 * g.L3;
+  for (int ig = blockIdx.x * blockDim.x + threadIdx.x; ig < g.nhg; ig += gridDim.x * blockDim.x) {
     if (g.geq0 && ig == 0) continue;
-    const int m1 = g.gm[3 * ig], m2 = g.gm[3 * ig + 1], m3 = g.gm[3 * ig + 2];
+    const int s = g.sidx[ig], t = g.gm[3 * ig + 2] + g.h3;
     const double2 re = rhog[ig], er = eirop[ig];
-    const double s = g.scg[ig];
-    const double2 A = make_double2(s * (re.x + er.x), -s * (re.y + er.y));   // scg conj(rho_tot)
-    const double2 B = make_double2(re.x, -re.y);                            // conj(rho_e)
-    const double gx = gk[3 * ig], gy = gk[3 * ig + 1], gz = gk[3 * ig + 2];
+    const double sc = g.scg[ig];
+    const double2 A = make_double2(sc * (re.x + er.x), -sc * (re.y + er.y));   // scg conj(rho_tot)
+    const double2 B = make_double2(re.x, -re.y);                              // conj(rho_e)
+    const double gv[3] = {gk[3 * ig], gk[3 * ig + 1], gk[3 * ig + 2]};
+    for (int is = 0; is < g.nsp; ++is) {
+      const double rp = g.rhops[is + (size_t)g.nsp * ig], vp = g.vps[is + (size_t)g.nsp * ig];
+      const double2 X = make_double2(rp * A.x + vp * B.x, rp * A.y + vp * B.y);
+      double* y = Y + is * blk + s;
 #pragma unroll
-    for (int a = 0; a < kAtomsPerBlock; ++a) {
-      if (a < na) {
-        const int isa = a0 + a;
-        const int is = sp_of[isa];
-        const double rp = g.rhops[is + (size_t)g.nsp * ig], vp = g.vps[is + (size_t)g.nsp * ig];
-        const double2 X = make_double2(rp * A.x + vp * B.x, rp * A.y + vp * B.y);
-        const double2 e = phase_of(g, isa, m1, m2, m3, n1, n2, n3);
-        const double im = e.x * X.y + e.y * X.x;
-        acc[3 * a] -= im * gx;
-        acc[3 * a + 1] -= im * gy;
-        acc[3 * a + 2] -= im * gz;
+      for (int k = 0; k < 3; ++k) {
+        y[(size_t)(2 * t) * ldy + (size_t)k * g.nstick] = X.x * gv[k];
+        y[(size_t)(2 * t + 1) * ldy + (size_t)k * g.nstick] = X.y * gv[k];
       }
     }
   }
-  block_sum_store<kAtomsPerBlock * 3>(acc, part + ((size_t)blockIdx.x * g.nat + a0) * 3, na * 3);
 }
 
-// out[q] = scale * sum_b part[b * stride + q], b = 0 .. nb-1 in order
-__global__ void k_reduce_cols(const double* __restrict__ part, int nb, int stride, int nq, double scale,
-                              double* __restrict__ out) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nq) return;
-  double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[(size_t)b * stride + q];
-  out[q] = scale * s;
+// fout[3a + k] = -scale sum_s Im(Q(s, a) T_k(s, a)); T (3 nstick x 2 nat, ld
+// 3 nstick) in the column order of Bf.  One workgroup per (a, k), fixed order.
+__global__ __launch_bounds__(kThreads) void k_stick_force(const double* __restrict__ Q, const double* __restrict__ T,
+                                                          int nstick, const int* __restrict__ sp_of,
+                                                          const int* __restrict__ sp_start, double scale,
+                                                          double* __restrict__ fout) {
+  const int a = blockIdx.x, k = blockIdx.y;
+  const int sp = sp_of[a], a0 = sp_start[sp], na = sp_start[sp + 1] - a0;
+  const size_t ldt = 3 * (size_t)nstick;
+  const double* qr = Q + (size_t)(2 * a) * nstick;
+  const double* qi = qr + nstick;
+  const double* tr = T + (size_t)(2 * a0 + (a - a0)) * ldt + (size_t)k * nstick;
+  const double* ti = T + (size_t)(2 * a0 + na + (a - a0)) * ldt + (size_t)k * nstick;
+  double acc[1] = {0.0};
+  for (int s = threadIdx.x; s < nstick; s += kThreads) acc[0] += qr[s] * ti[s] + qi[s] * tr[s];
+  acc[0] *= -scale;
+  block_sum_store<1>(acc, fout + 3 * (size_t)a + k, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -447,6 +483,26 @@ void set_density_gset(mc_ctx* ctx, int nhg, const int* inyh, const double* hg, c
       scg[ig] = 4.0 * M_PI / (ctx->tpiba2 * hg[ig]);
     }
   }
+  // z-sticks: the distinct (m1, m2) in increasing order and the stick of each G
+  const long long w2 = nr[1] + 1;
+  std::vector<long long> key(nhg);
+  for (int ig = 0; ig < nhg; ++ig)
+    key[ig] = (long long)(gm[3 * (size_t)ig] + nr[0] / 2) * w2 + (gm[3 * (size_t)ig + 1] + nr[1] / 2);
+  std::vector<long long> uk(key);
+  std::sort(uk.begin(), uk.end());
+  uk.erase(std::unique(uk.begin(), uk.end()), uk.end());
+  const int nstick = (int)uk.size();
+  std::vector<int> sidx(nhg), stk(2 * (size_t)nstick);
+  for (int ig = 0; ig < nhg; ++ig) sidx[ig] = (int)(std::lower_bound(uk.begin(), uk.end(), key[ig]) - uk.begin());
+  for (int st = 0; st < nstick; ++st) {
+    stk[2 * (size_t)st] = (int)(uk[st] / w2) - nr[0] / 2;
+    stk[2 * (size_t)st + 1] = (int)(uk[st] % w2) - nr[1] / 2;
+  }
+  D.nstick = nstick;
+  D.d_sidx.ensure(sizeof(int) * sidx.size());
+  D.d_stk.ensure(sizeof(int) * stk.size());
+  MC_HIP(hipMemcpy(D.d_sidx.p, sidx.data(), sizeof(int) * sidx.size(), hipMemcpyHostToDevice));
+  MC_HIP(hipMemcpy(D.d_stk.p, stk.data(), sizeof(int) * stk.size(), hipMemcpyHostToDevice));
   D.nhg = nhg;
   D.d_gm.ensure(sizeof(int) * gm.size());
   D.d_gidx.ensure(sizeof(int) * gidx.size());
@@ -568,6 +624,10 @@ static GParams make_gparams(mc_ctx* ctx) {
   g.ph1 = D.d_ph1.as<double2>();
   g.ph2 = D.d_ph2.as<double2>();
   g.ph3 = D.d_ph3.as<double2>();
+  g.sidx = D.d_sidx.as<int>();
+  g.nstick = D.nstick;
+  g.L3 = ctx->n3 + 1;
+  g.sf = nullptr;
   return g;
 }
 
@@ -595,17 +655,41 @@ void vofrho_dev(mc_ctx* ctx, const double* rho, double* v, double* energies, dou
   const size_t sh3 = sizeof(double2) * ((size_t)ctx->n3 + (size_t)(ctx->n3 + 1) * t3);
   const int gx_blocks = (ctx->n2 + tx - 1) / tx;
   const int n_xblk = gx_blocks * ctx->n3;
-  // partial-sum area: ppener (4 per block), xc (2 per x block), potfor
-  const int gchunk = std::max(kThreads, (D.nhg + 255) / 256);
-  const int gb = (D.nhg + gchunk - 1) / gchunk;
-  const int ab = (D.nat + kAtomsPerBlock - 1) / kAtomsPerBlock;
-  const size_t npart = 4 * (size_t)kRedBlocks + 2 * (size_t)n_xblk + (size_t)gb * std::max(D.nat, 1) * 3 +
-                       3 * (size_t)std::max(D.nat, 1);
+  // partial-sum area: ppener (4 per block), xc (2 per x block), ion forces (3 per atom)
+  const size_t npart = 4 * (size_t)kRedBlocks + 2 * (size_t)n_xblk + 3 * (size_t)std::max(D.nat, 1);
   D.w_part.ensure(sizeof(double) * npart);
   double* p_pp = D.w_part.as<double>();
   double* p_xc = p_pp + 4 * kRedBlocks;
-  double* p_pf = p_xc + 2 * (size_t)n_xblk;
-  double* f_out = p_pf + (size_t)gb * std::max(D.nat, 1) * 3;
+  double* f_out = p_xc + 2 * (size_t)n_xblk;
+  // stick operands: Q, the p3 expansions, the species structure factors
+  const int ns = D.nstick, L3 = ctx->n3 + 1, nat = D.nat;
+  const size_t sfblk = (size_t)ns * 2 * L3;
+  D.w_q.ensure(sizeof(double) * (size_t)ns * 2 * std::max(nat, 1));
+  D.w_bs.ensure(sizeof(double) * 4 * (size_t)std::max(nat, 1) * L3);
+  D.w_bf.ensure(sizeof(double) * 4 * (size_t)std::max(nat, 1) * L3);
+  D.w_sf.ensure(sizeof(double) * sfblk * D.nsp);
+  double* Q = D.w_q.as<double>();
+  double* Sf = D.w_sf.as<double>();
+  if (nat > 0) {
+    LAUNCH(MC_KID_VOFRHO, k_stick_q, dim3((ns + 255) / 256, nat), dim3(256), 0, D.d_stk.as<int>(), ns, ctx->n1,
+           ctx->n2, g.ph1, g.ph2, Q);
+    LAUNCH(MC_KID_VOFRHO, k_stick_p3, dim3((L3 + 255) / 256, nat), dim3(256), 0, g.ph3, nat, ctx->n3,
+           D.d_sp_of.as<int>(), D.d_sp_start.as<int>(), D.w_bs.as<double>(), D.w_bf.as<double>());
+  }
+  {
+    int a0 = 0;
+    for (int is = 0; is < D.nsp; ++is) {
+      const int na = D.na[is];
+      if (na > 0)
+        gemm_nn_f64(ctx, Q + (size_t)2 * a0 * ns, ns, D.w_bs.as<double>() + 2 * a0, 2 * nat, ns, 2 * na, 2 * L3,
+                    1.0, 0.0, Sf + is * sfblk, ns, false, nullptr, MC_KID_VOFRHO);
+      else
+        LAUNCH(MC_KID_VOFRHO, k_fill, dim3(256), dim3(256), 0, Sf + is * sfblk, sfblk, 0.0);
+      a0 += na;
+    }
+  }
+  GParams gs = g;
+  gs.sf = Sf;
 
   // forward transform of rho
   with_length(ctx->n1, [&](auto NN) {
@@ -621,13 +705,27 @@ void vofrho_dev(mc_ctx* ctx, const double* rho, double* v, double* energies, dou
            dim3(kThreads), sh3, d, W, t3);
   });
   // per-G energies and potential
-  LAUNCH(MC_KID_VOFRHO, k_eicalc_ppener, dim3(kRedBlocks), dim3(kThreads), 0, g, ctx->n1, ctx->n2, ctx->n3, W,
+  LAUNCH(MC_KID_VOFRHO, k_eicalc_ppener, dim3(kRedBlocks), dim3(kThreads), 0, gs, ctx->n1, ctx->n2, ctx->n3, W,
          1.0 / nn, rhog, eirop, eivps, vtemp, p_pp);
-  if (fion_host && D.nat > 0) {
-    LAUNCH(MC_KID_VOFRHO, k_potfor, dim3(gb, ab), dim3(kThreads), 0, g, ctx->n1, ctx->n2, ctx->n3,
-           D.d_gk.as<double>(), rhog, eirop, D.d_sp_of.as<int>(), gchunk, p_pf);
-    LAUNCH(MC_KID_VOFRHO, k_reduce_cols, dim3((3 * D.nat + 255) / 256), dim3(256), 0, p_pf, gb, 3 * D.nat,
-           3 * D.nat, 2.0 * ctx->omega * std::sqrt(ctx->tpiba2), f_out);
+  if (fion_host && nat > 0) {
+    // potfor: Y = X G_k on the sticks, T = Y Bf per species, F = -Im sum_s Q T
+    const size_t yblk = 3 * sfblk;
+    D.w_y.ensure(sizeof(double) * yblk * D.nsp);
+    D.w_t.ensure(sizeof(double) * 3 * (size_t)ns * 2 * nat);
+    double* Y = D.w_y.as<double>();
+    double* T = D.w_t.as<double>();
+    LAUNCH(MC_KID_VOFRHO, k_fill, dim3(1024), dim3(256), 0, Y, yblk * D.nsp, 0.0);
+    LAUNCH(MC_KID_VOFRHO, k_stick_y, dim3(kRedBlocks), dim3(256), 0, gs, D.d_gk.as<double>(), rhog, eirop, Y);
+    int a0 = 0;
+    for (int is = 0; is < D.nsp; ++is) {
+      const int na = D.na[is];
+      if (na > 0)
+        gemm_nn_f64(ctx, Y + is * yblk, 3 * ns, D.w_bf.as<double>() + (size_t)2 * a0 * 2 * L3, 2 * L3, 3 * ns,
+                    2 * L3, 2 * na, 1.0, 0.0, T + (size_t)2 * a0 * 3 * ns, 3 * ns, false, nullptr, MC_KID_VOFRHO);
+      a0 += na;
+    }
+    LAUNCH(MC_KID_VOFRHO, k_stick_force, dim3(nat, 3), dim3(kThreads), 0, Q, T, ns, D.d_sp_of.as<int>(),
+           D.d_sp_start.as<int>(), 2.0 * ctx->omega * std::sqrt(ctx->tpiba2), f_out);
   }
   LAUNCH(MC_KID_VOFRHO, k_fill, dim3(1024), dim3(256), 0, (double*)W, 2 * nw, 0.0);
   LAUNCH(MC_KID_VOFRHO, k_scatter_vtemp, dim3(kRedBlocks), dim3(kThreads), 0, g, vtemp, W);

@@ -61,6 +61,29 @@ def test_vofrho_parity(gpu, cfg):
     assert en2 == en
 
 
+@pytest.mark.parametrize("na", [(5, 3), (5, 0, 3)])
+def test_vofrho_parity_species(gpu, na):
+    """Several species (one of them without atoms): the species blocks of the
+    stick structure factors and of the potfor contraction."""
+    sysm, ctx, L, rho, inyh, gk, _, _, _, _, frac = _case(12.0, 8, 40.0, 16, seed=5)
+    gs = sysm.gset
+    sc = (1.0, 0.6, 1.3)
+    spp = [S.local_pp_set(sysm, gs.hg, raggio=r) for r in (1.2, 0.9, 1.05)[:len(na)]]
+    zv = np.concatenate([p[0] * sc[i] for i, p in enumerate(spp)])
+    rg = np.concatenate([p[1] for p in spp])
+    vps = np.asfortranarray(np.vstack([p[2] * sc[i] for i, p in enumerate(spp)]))
+    rhops = np.asfortranarray(np.vstack([p[3] * sc[i] for i, p in enumerate(spp)]))
+    ctx.set_local_pp(list(na), zv, rg, vps, rhops)
+    ctx.set_ions(sysm.tau, sysm.cell.avec)
+    v, en, f = ctx.vofrho(rho)
+    ov, oen, of, _ = oracle.vofrho(L, inyh, gs.hg, gk, list(na), vps, rhops, frac, rho)
+    assert rel(v, ov) < 1e-12
+    for k in ("ehep", "ehii", "ehee", "epseu", "exc", "vxc"):
+        assert en[k] == pytest.approx(oen[k], rel=1e-11), k
+    assert rel(f, of) < 1e-10
+    assert np.abs(of).max() > 0.0
+
+
 def test_ewald_parity(gpu):
     sysm, ctx, L, rho, inyh, gk, zv, rg, vps, rhops, frac = _case(12.0, 8, 40.0, 16, seed=3)
     esr, eself, f = ctx.ewald(iesr=2)
