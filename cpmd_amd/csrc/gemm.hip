@@ -198,6 +198,46 @@ __global__ void k_reduce_tn(const double* __restrict__ P, int nsplit, int ldp, i
   out[(size_t)j * ldo + i] = v;
 }
 
+// The symmetric case of k_reduce_tn (m == n, only tiles on or above the
+// diagonal computed): out(i, j) = out(j, i) = the value of k_reduce_tn at
+// i <= j.  One workgroup per 32 x 32 block (bi <= bj): coalesced sums down the
+// columns of P, the mirrored block written through LDS.
+__global__ __launch_bounds__(256) void k_reduce_tn_sym(const double* __restrict__ P, int nsplit, int ldp, int n,
+                                                       double alpha, const double* __restrict__ a0, int lda,
+                                                       const double* __restrict__ b0, int ldb, int g0,
+                                                       double* __restrict__ out, int ldo) {
+  __shared__ double tile[32][33];
+  const int nb = (n + 31) / 32;
+  int bi = 0, bj = blockIdx.x;   // blockIdx.x -> (bi, bj), bi <= bj, row-major over the upper triangle
+  while (bj >= nb - bi) {
+    bj -= nb - bi;
+    ++bi;
+  }
+  bj += bi;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int i = 32 * bi + tx;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int jl = ty + 8 * u, j = 32 * bj + jl;
+    double v = 0.0;
+    if (i < n && j < n && i <= j) {
+      double s = 0.0;
+      for (int q = 0; q < nsplit; ++q) s += P[(size_t)q * ldp * n + (size_t)j * ldp + i];
+      v = alpha * s;
+      if (g0) v -= a0[(size_t)i * lda] * b0[(size_t)j * ldb];
+      out[(size_t)j * ldo + i] = v;
+    }
+    tile[tx][jl] = v;
+  }
+  __syncthreads();
+  // mirror: out(j, i) = value(i, j) for i < j; thread tx runs along j
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int il = ty + 8 * u, ii = 32 * bi + il, jj = 32 * bj + tx;
+    if (ii < n && jj < n && ii < jj) out[(size_t)ii * ldo + jj] = tile[il][tx];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // D = alpha A G + beta D.  A (M x k) column-major lda; G (k x n) column-major
 // ldgm; D (M x n) column-major ldd.  upper_g: G upper triangular (the k loop
@@ -347,6 +387,14 @@ void gemm_tn(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, in
 void reduce_tn(mc_ctx* ctx, int nsplit, int m, int n, double alpha, const double* A, int lda, const double* B, int ldb,
                bool g0corr, bool symmetric, double* out, int ldo) {
   timer_begin(ctx, MC_KID_MISC);
+  if (symmetric && m == n) {
+    const int nb = (n + 31) / 32;
+    hipLaunchKernelGGL(k_reduce_tn_sym, dim3(nb * (nb + 1) / 2), dim3(256), 0, ctx->stream, ctx->w_gemm.as<double>(),
+                       nsplit, m, n, alpha, A, lda, B, ldb, g0corr ? 1 : 0, out, ldo);
+    MC_HIP(hipGetLastError());
+    timer_end(ctx, MC_KID_MISC);
+    return;
+  }
   hipLaunchKernelGGL(k_reduce_tn, dim3((m + 255) / 256, n), dim3(256), 0, ctx->stream, ctx->w_gemm.as<double>(),
                      nsplit, m, m, n, alpha, A, lda, B, ldb, g0corr ? 1 : 0, symmetric ? 1 : 0, out, ldo);
   MC_HIP(hipGetLastError());

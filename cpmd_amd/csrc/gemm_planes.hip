@@ -119,16 +119,18 @@ __global__ __launch_bounds__(256) void k_split_planes(const double* __restrict__
 }
 
 // velupa + posupa with the planes of both coefficient sets as a by-product:
-// cm += dtm c2 (c2 may be null), cp = c0 + dt cm over all ldx stored rows, and
+// cp = c0 + dt (cm + dtm c2) (c2 may be null) over all ldx stored rows, and
 // the bf16 planes of c0 and of cp for the nkb = ceil(K / 32) row blocks of the
 // contractions (layout of k_split_planes; rows >= K and columns >= n zero).
 // One wave per (column, 512-row segment): lane l moves rows 2l + 128u (u < 4)
 // as 16-byte runs, so every load and store of c0, c2, cm, cp covers 1 KiB of a
 // column; the fp32 values pass through LDS so that lane l then holds rows
-// 8l..8l+7, the plane chunk it splits and stores.
+// 8l..8l+7, the plane chunk it splits and stores.  cm is read, not written: the
+// caller's final rotation sets the velocity (cp_final - c0) / dt.
 template <int NP>
 __global__ __launch_bounds__(256) void k_posupa_planes(const double* __restrict__ c0, const double* __restrict__ c2,
-                                                       double* __restrict__ cm, double* __restrict__ cp, int ldx, int K,
+                                                       const double* __restrict__ cm, double* __restrict__ cp, int ldx,
+                                                       int K,
                                                        int n, int ncolp, int nseg, int nkb, double dtm, double dt,
                                                        __bf16* __restrict__ pl0, __bf16* __restrict__ plp) {
   __shared__ float4 lds[4][2][128];   // [wave][c0 | cp][512 rows]
@@ -177,10 +179,7 @@ __global__ __launch_bounds__(256) void k_posupa_planes(const double* __restrict_
   }
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    if (any && r0 + 128 * u < ldx) {
-      if (c2) *reinterpret_cast<double2*>(cm + o + 128 * u) = w[u];
-      *reinterpret_cast<double2*>(cp + o + 128 * u) = f[u];
-    }
+    if (any && r0 + 128 * u < ldx) *reinterpret_cast<double2*>(cp + o + 128 * u) = f[u];
   }
 }
 
@@ -598,8 +597,8 @@ void split_planes(mc_ctx* ctx, const double* X, int ldx, int K, int n, int np, D
   timer_end(ctx, MC_KID_GEMM);
 }
 
-void posupa_planes(mc_ctx* ctx, const double* c0, const double* c2, double* cm, double* cp, int ldx, int K, int n,
-                   double dtm, double dt) {
+void posupa_planes(mc_ctx* ctx, const double* c0, const double* c2, const double* cm, double* cp, int ldx, int K,
+                   int n, double dtm, double dt) {
   auto& C = ctx->planes;
   MC_CHECK(C.scoped, MC_ERR_INTERNAL, "posupa_planes outside a plane scope");
   MC_CHECK(ldx % kPK == 0 && K % 2 == 0 && K <= ldx && (reinterpret_cast<uintptr_t>(c0) & 15) == 0 &&

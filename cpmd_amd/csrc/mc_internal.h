@@ -262,10 +262,10 @@ constexpr int kX3Min = 16384;   // rows of K (tn) or M (nn) from which the split
 size_t planes_bytes(int K, int n, int np);
 void split_planes(mc_ctx* ctx, const double* X, int ldx, int K, int n, int np, DevBuf& out);
 void proj_planes(mc_ctx* ctx, const ProjA& pa, int K, int m, int np, DevBuf& out);
-// cm += dtm c2 (c2 may be null), cp = c0 + dt cm, writing the planes of c0 and
-// cp into the plane cache of the current PlaneScope
-void posupa_planes(mc_ctx* ctx, const double* c0, const double* c2, double* cm, double* cp, int ldx, int K, int n,
-                   double dtm, double dt);
+// cp = c0 + dt (cm + dtm c2) (c2 may be null; cm is not written), writing the
+// planes of c0 and cp into the plane cache of the current PlaneScope
+void posupa_planes(mc_ctx* ctx, const double* c0, const double* c2, const double* cm, double* cp, int ldx, int K,
+                   int n, double dtm, double dt);
 void cp_posupa_dev(mc_ctx* ctx, const double2* c0, const double2* c2, double2* cm, double2* cp, int nstate, double dtm,
                    double dt);
 void cp_velocity_dev(mc_ctx* ctx, const double2* c0, const double2* cp, double2* cm, int nstate, double dt);

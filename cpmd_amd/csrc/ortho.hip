@@ -210,17 +210,18 @@ static double max_abs_diff(mc_ctx* ctx, const double* a, const double* b, size_t
 // rnlfor (src/rnlfor_utils.mod.F90:201-280, Gamma): one thread per (k, isa);
 // fion(k, isa) = -sum_i 2 f_i sum_{iv,jv} h(iv,jv) dfnl(isa,iv,k,i) fnl(isa,jv,i),
 // states with |f_i| <= 1e-12 skipped.  out (3, nat).
-__global__ void k_rnlfor(const double* __restrict__ fnl, const double* __restrict__ dfnl, const double* __restrict__ f,
-                         int n, int nat, int nhxs, const int* __restrict__ sp_of, const int* __restrict__ ngh,
-                         const double* __restrict__ h, double* __restrict__ out) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= 3 * nat) return;
-  const int k = e % 3, isa = e / 3;
+// one workgroup per (atom, direction): threads over states, fixed-order sum
+__global__ __launch_bounds__(256) void k_rnlfor(const double* __restrict__ fnl, const double* __restrict__ dfnl,
+                                                const double* __restrict__ f, int n, int nat, int nhxs,
+                                                const int* __restrict__ sp_of, const int* __restrict__ ngh,
+                                                const double* __restrict__ h, double* __restrict__ out) {
+  __shared__ double red[256];
+  const int isa = blockIdx.x, k = blockIdx.y;
   const int is = sp_of[isa];
   const int nh = ngh[is];
   const double* hs = h + (size_t)nhxs * nhxs * is;
   double acc = 0.0;
-  for (int i = 0; i < n; ++i) {
+  for (int i = threadIdx.x; i < n; i += 256) {
     const double fi = f[i];
     if (fabs(fi) <= 1e-12) continue;
     double s = 0.0;
@@ -232,7 +233,13 @@ __global__ void k_rnlfor(const double* __restrict__ fnl, const double* __restric
     }
     acc += 2.0 * fi * s;
   }
-  out[k + 3 * (size_t)isa] = -acc;
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[k + 3 * (size_t)isa] = -red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -386,15 +393,17 @@ void rortog_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, i
 // posupa (src/posupa_utils.mod.F90:41-160) with velupa's kick folded in:
 // cm += dtm c2 (c2 may be null), cp = c0 + dt cm, rortog(c0, cp) (cp += c0
 // gamx), cm = (cp - c0) / dt.  In mixed precision the first pass also builds
-// the bf16 planes of c0 and cp for the two overlaps of rortog; the velocity is
-// written by the epilogue of the final rotation.
+// the bf16 planes of c0 and cp for the two overlaps of rortog and leaves cm
+// alone (the kicked velocity is only an intermediate); the velocity is written
+// by the epilogue of the final rotation.
 void posupa_dev(mc_ctx* ctx, const double2* c0, const double2* c2, double2* cm, double2* cp, int n, double dtm,
                 double dt, double epsog, int maxit, double* gam_host, int* iters) {
   if (n <= 0) return;
   PlaneScope keep(ctx);
   const int K = wfn_rows(ctx);
   if (ctx->gemm_x3 && K >= kX3Min)
-    posupa_planes(ctx, (const double*)c0, (const double*)c2, (double*)cm, (double*)cp, wfn_ld(ctx), K, n, dtm, dt);
+    posupa_planes(ctx, (const double*)c0, (const double*)c2, (const double*)cm, (double*)cp, wfn_ld(ctx), K, n, dtm,
+                  dt);
   else
     cp_posupa_dev(ctx, c0, c2, cm, cp, n, dtm, dt);
   rortog_core(ctx, c0, cp, gam_host, 0, n, epsog, maxit, iters, cm, dt);
@@ -656,7 +665,7 @@ void rnlfor_dev(mc_ctx* ctx, const double* f_host, int n, const double* fnl_host
   MC_HIP(hipMemcpyAsync(base, sp_of.data(), sizeof(int) * P.nat, hipMemcpyHostToDevice, ctx->stream));
   MC_HIP(hipMemcpyAsync(base + off_ngh, P.ngh.data(), sizeof(int) * P.nsp, hipMemcpyHostToDevice, ctx->stream));
   MC_HIP(hipMemcpyAsync(base + off_f, f_host, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_rnlfor, dim3((3 * P.nat + 127) / 128), dim3(128), 0, ctx->stream, P.d_fnl.as<double>(),
+  hipLaunchKernelGGL(k_rnlfor, dim3(P.nat, 3), dim3(256), 0, ctx->stream, P.d_fnl.as<double>(),
                      P.d_dfnl.as<double>(), (const double*)(base + off_f), n, P.nat, P.nhxs, (const int*)base,
                      (const int*)(base + off_ngh), P.d_h.as<double>(), (double*)(base + off_out));
   MC_HIP(hipGetLastError());
