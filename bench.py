@@ -123,8 +123,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="cfg4")
     ap.add_argument("--nstate", type=int, default=0)
-    ap.add_argument("--batch-rho", type=int, default=16)
-    ap.add_argument("--batch-vpsi", type=int, default=8)
+    ap.add_argument("--batch-rho", type=int, default=128)
+    ap.add_argument("--batch-vpsi", type=int, default=64)
     ap.add_argument("--step", choices=("cp", "wfopt"), default="cp")
     ap.add_argument("--dt", type=float, default=0.0, help="time step (a.u.; cp default 1, wfopt default 0.02)")
     ap.add_argument("--emass", type=float, default=400.0)

@@ -219,8 +219,10 @@ struct mc_ctx {
   mc::DevBuf w_plg;        // bf16 planes of the small right operand of A G products
   mc::DensitySet dens;     // vofrho / ewald data
 
-  int batch_pairs_vpsi = 8;
-  int batch_pairs_rho = 16;
+  // pairs per transform batch: larger batches amortise the per-launch tails
+  // (512 states on 192^3: 128 / 64 pairs, FP64 workspace 6.9 / 3.4 GB)
+  int batch_pairs_vpsi = 64;
+  int batch_pairs_rho = 128;
 
   mc::KernelTimer timer;
 };
