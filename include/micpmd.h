@@ -147,7 +147,9 @@ int mc_cp_posupa(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, void* cm_d
  * swaps c0 and cp (CPMD's final dcopy).  gam: the (nstate, nstate) gamx on
  * the host, or NULL; iters: the rortog iterations, or NULL.  In mixed
  * precision the first pass also writes the split-bf16 planes of c0 and cp
- * used by the two overlaps of rortog.
+ * used by the two overlaps of rortog.  cm is written once, by the final
+ * rotation: if rortog fails (MC_ERR_NOCONV) cm may still hold the old
+ * velocities in mixed precision and the kicked ones in FP64.
  * Replaces velupa + posupa(c0,cm,c2,gamx,nstate), src/velupa_utils.mod.F90:25-72,
  * src/posupa_utils.mod.F90:41-160 (cntl%tharm, tmass off). */
 int mc_posupa_dev(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, void* cm_dev, void* cp_dev, int nstate,
