@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_split_planes(const double* __restrict__
 // cp = c0 + dt (cm + dtm c2) (c2 may be null) over all ldx stored rows, and
 // the bf16 planes of c0 and of cp for the nkb = ceil(K / 32) row blocks of the
 // contractions (layout of k_split_planes; rows >= K and columns >= n zero).
-// One wave per (column, 512-row segment): lane l moves rows 2l + 128u (u < 4)
+// One wave per (512-row segment, column): lane l moves rows 2l + 128u (u < 4)
 // as 16-byte runs, so every load and store of c0, c2, cm, cp covers 1 KiB of a
 // column; the fp32 values pass through LDS so that lane l then holds rows
 // 8l..8l+7, the plane chunk it splits and stores.  cm is read, not written: the
@@ -137,8 +137,10 @@ __global__ __launch_bounds__(256) void k_posupa_planes(const double* __restrict_
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const size_t unit = (size_t)blockIdx.x * 4 + wv;
   const bool live = unit < (size_t)ncolp * nseg;
-  const int col = live ? (int)(unit / nseg) : 0;
-  const int seg = live ? (int)(unit - (size_t)col * nseg) : 0;
+  // columns fastest: the waves of a workgroup write neighbouring columns of a
+  // plane block, whose 64-byte column chunks share 128-byte lines
+  const int seg = live ? (int)(unit / ncolp) : 0;
+  const int col = live ? (int)(unit - (size_t)seg * ncolp) : 0;
   float2* s0 = reinterpret_cast<float2*>(lds[wv][0]);
   float2* sp = reinterpret_cast<float2*>(lds[wv][1]);
   const int r0 = seg * 512 + 2 * lane;
