@@ -183,6 +183,22 @@ __global__ __launch_bounds__(256) void k_gemm_tn(const double* __restrict__ A, i
 // upper triangle is used and mirrored into the lower one (dmatc).
 // out(i, j) = alpha sum_q P[q](i, j) [- A(0, i) B(0, j)], partials P[q] of
 // ldp rows (the first m used), mirrored from the upper triangle if symmetric
+// sum_q P[q * stride + o], q = 0 .. nsplit-1 in order; eight loads in flight
+// per step (the adds keep the sequential order, so the sum is the same)
+__device__ __forceinline__ double sum_partials(const double* __restrict__ P, int nsplit, size_t stride, size_t o) {
+  double s = 0.0;
+  int q = 0;
+  for (; q + 8 <= nsplit; q += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = P[(size_t)(q + u) * stride + o];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; q < nsplit; ++q) s += P[(size_t)q * stride + o];
+  return s;
+}
+
 __global__ void k_reduce_tn(const double* __restrict__ P, int nsplit, int ldp, int m, int n, double alpha,
                             const double* __restrict__ a0, int lda, const double* __restrict__ b0, int ldb,
                             int g0, int symmetric, double* __restrict__ out, int ldo) {
@@ -191,8 +207,7 @@ __global__ void k_reduce_tn(const double* __restrict__ P, int nsplit, int ldp, i
   if (i >= m || j >= n) return;
   int si = i, sj = j;
   if (symmetric && i > j) { si = j; sj = i; }
-  double s = 0.0;
-  for (int q = 0; q < nsplit; ++q) s += P[(size_t)q * ldp * n + (size_t)sj * ldp + si];
+  const double s = sum_partials(P, nsplit, (size_t)ldp * n, (size_t)sj * ldp + si);
   double v = alpha * s;
   if (g0) v -= a0[(size_t)si * lda] * b0[(size_t)sj * ldb];
   out[(size_t)j * ldo + i] = v;
@@ -221,8 +236,7 @@ __global__ __launch_bounds__(256) void k_reduce_tn_sym(const double* __restrict_
     const int jl = ty + 8 * u, j = 32 * bj + jl;
     double v = 0.0;
     if (i < n && j < n && i <= j) {
-      double s = 0.0;
-      for (int q = 0; q < nsplit; ++q) s += P[(size_t)q * ldp * n + (size_t)j * ldp + i];
+      const double s = sum_partials(P, nsplit, (size_t)ldp * n, (size_t)j * ldp + i);
       v = alpha * s;
       if (g0) v -= a0[(size_t)i * lda] * b0[(size_t)j * ldb];
       out[(size_t)j * ldo + i] = v;

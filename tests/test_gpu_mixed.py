@@ -206,11 +206,35 @@ def test_posupa_combined(gpu, cfg2_130):
         assert np.array_equal(pa, pb) and np.array_equal(ctx.download(dm_a, n), ctx.download(dm_b, n)), mode
 
 
-def test_forces_c2_combined(gpu):
+def test_posupa_combined_odd_no_gradient(gpu, cfg2_130):
+    """An odd number of states (the last column block part empty) and no
+    gradient (c2 = NULL: no velupa kick) against the separate calls with a
+    zero gradient, in both precisions."""
+    sysm, c, cp = cfg2_130
+    c, cp = np.asfortranarray(c[:, :129]), np.asfortranarray(cp[:, :129])
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    n = c.shape[1]
+    cm = np.asfortranarray((cp - c) / 0.5)
+    for mode in ("fp64", "mixed"):
+        ctx.set_precision(mode)
+        d0, dz = ctx.upload(c), ctx.alloc_wfn(n)
+        dm_a, dm_b = ctx.upload(cm), ctx.upload(cm)
+        dp_a, dp_b = ctx.alloc_wfn(n), ctx.alloc_wfn(n)
+        gam_a, it_a = ctx.posupa_dev(d0, None, dm_a, dp_a, n, 0.01, 0.5, want_gam=True)
+        ctx.cp_posupa(d0, dz, dm_b, dp_b, n, 0.01, 0.5)
+        gam_b, it_b = ctx.rortog_dev(d0, dp_b, n, epsog=1e-8, maxit=60)
+        ctx.cp_velocity(d0, dp_b, dm_b, n, 0.5)
+        assert it_a == it_b and np.array_equal(gam_a, gam_b), mode
+        assert np.array_equal(ctx.download(dp_a, n), ctx.download(dp_b, n)), mode
+        assert np.array_equal(ctx.download(dm_a, n), ctx.download(dm_b, n)), mode
+
+
+@pytest.mark.parametrize("nstate", [64, 63])
+def test_forces_c2_combined(gpu, nstate):
     """forces_c2 (vpsi writing c2 instead of adding to it, then fnonloc with the
     device fnl) equals c2 <- 0, vpsi, fnonloc bit for bit in both precisions,
     whatever c2 held before, padding rows included."""
-    sysm = S.make_system(24.0, 64, 70.0, nstate=64)
+    sysm = S.make_system(24.0, 64, 70.0, nstate=nstate)
     c = S.initial_coefficients(sysm, seed=47)
     ngw = sysm.gset.ngw
     na, nhxs, ngh, lang, hmat, twnl, eigr = S.projector_set(sysm, sysm.gset.miller[:ngw], sysm.gset.hg[:ngw])
