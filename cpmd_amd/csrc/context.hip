@@ -55,6 +55,34 @@ void upload_async(mc_ctx* ctx, void* dst, const void* src, size_t bytes) {
   MC_HIP(hipEventRecord(ctx->h_stage_done, ctx->stream));
 }
 
+void dl_finish(mc_ctx* ctx) {
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  for (const auto& c : ctx->h_down_pending)
+    std::memcpy(c.dst, static_cast<const char*>(ctx->h_down) + c.off, c.bytes);
+  ctx->h_down_pending.clear();
+  ctx->h_down_used = 0;
+}
+
+void dl_add(mc_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return;
+  size_t off = (ctx->h_down_used + 255) & ~(size_t)255;
+  if (off + bytes > ctx->h_down_bytes) {
+    if (!ctx->h_down_pending.empty()) dl_finish(ctx);
+    off = 0;
+    if (bytes > ctx->h_down_bytes) {
+      if (ctx->h_down) MC_HIP(hipHostFree(ctx->h_down));
+      ctx->h_down = nullptr;
+      ctx->h_down_bytes = 0;
+      const size_t nb = std::max(bytes, (size_t)1 << 20);
+      MC_HIP(hipHostMalloc(&ctx->h_down, nb, hipHostMallocDefault));
+      ctx->h_down_bytes = nb;
+    }
+  }
+  MC_HIP(hipMemcpyAsync(static_cast<char*>(ctx->h_down) + off, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  ctx->h_down_pending.push_back({dst, off, bytes});
+  ctx->h_down_used = off + bytes;
+}
+
 void DevBuf::release() {
   if (p) (void)hipFree(p);
   p = nullptr;
@@ -447,6 +475,7 @@ int mc_ctx_destroy(mc_ctx* ctx) {
   for (auto e : ctx->timer.ev) (void)hipEventDestroy(e);
   if (ctx->h_stage_done) (void)hipEventDestroy(ctx->h_stage_done);
   if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+  if (ctx->h_down) (void)hipHostFree(ctx->h_down);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   MC_API_END

@@ -174,6 +174,14 @@ struct mc_ctx {
   void* h_stage = nullptr;
   size_t h_stage_bytes = 0;
   hipEvent_t h_stage_done = nullptr;   // recorded after the last copy out of h_stage
+  // pinned staging buffer of device-to-host result copies (mc::dl_add / dl_finish)
+  void* h_down = nullptr;
+  size_t h_down_bytes = 0, h_down_used = 0;
+  struct DlCopy {
+    void* dst;
+    size_t off, bytes;
+  };
+  std::vector<DlCopy> h_down_pending;
 
   // stick layout (see context.hip for the definitions)
   int n_own = 0;        // owner sticks (hold half-sphere G's); owner 0 is (0,0)
@@ -299,6 +307,12 @@ void proj_contract_x3(mc_ctx* ctx, const ProjA& base, const double* gk, int v0, 
 // waiting for the stream: through the pinned staging buffer, which is reused
 // once the previous copy out of it has completed
 void upload_async(mc_ctx* ctx, void* dst, const void* src, size_t bytes);
+// Device-to-host copies of results through the pinned download stage: dl_add
+// queues a copy of bytes from src (device) on the context stream, dl_finish
+// waits for the stream and moves every queued copy to its host destination
+// (pageable destinations would make the runtime stage each copy itself).
+void dl_add(mc_ctx* ctx, void* dst, const void* src, size_t bytes);
+void dl_finish(mc_ctx* ctx);
 void gemm_tn_x3(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, int K, int m, int n, double alpha,
                 bool g0corr, bool symmetric, double* out, int ldo, const ProjA* proj);
 void gemm_nn_x3(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n, double alpha,

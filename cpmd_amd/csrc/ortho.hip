@@ -200,8 +200,8 @@ static double max_abs_diff(mc_ctx* ctx, const double* a, const double* b, size_t
   hipLaunchKernelGGL(k_maxdiff, dim3(nb), dim3(256), 0, ctx->stream, a, b, nn, ctx->w_red2.as<double>());
   MC_HIP(hipGetLastError());
   double h[nb];
-  MC_HIP(hipMemcpyAsync(h, ctx->w_red2.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-  MC_HIP(hipStreamSynchronize(ctx->stream));
+  dl_add(ctx, h, ctx->w_red2.p, sizeof(h));
+  dl_finish(ctx);
   double m = 0.0;
   for (int q = 0; q < nb; ++q) m = std::max(m, h[q]);
   return m;
@@ -595,15 +595,9 @@ static void rnlsm_groups(mc_ctx* ctx, const double2* c0, int n, bool do_fnl, boo
     }
     isa0 += P.na[is];
   }
-  if (fnl_host && do_fnl) {
-    MC_HIP(hipMemcpyAsync(fnl_host, P.d_fnl.p, sizeof(double) * (size_t)P.nat * P.nhxs * n, hipMemcpyDeviceToHost,
-                          ctx->stream));
-  }
-  if (dfnl_host && do_dfnl) {
-    MC_HIP(hipMemcpyAsync(dfnl_host, P.d_dfnl.p, sizeof(double) * (size_t)P.nat * P.nhxs * 3 * n,
-                          hipMemcpyDeviceToHost, ctx->stream));
-  }
-  if ((fnl_host && do_fnl) || (dfnl_host && do_dfnl)) MC_HIP(hipStreamSynchronize(ctx->stream));
+  if (fnl_host && do_fnl) dl_add(ctx, fnl_host, P.d_fnl.p, sizeof(double) * (size_t)P.nat * P.nhxs * n);
+  if (dfnl_host && do_dfnl) dl_add(ctx, dfnl_host, P.d_dfnl.p, sizeof(double) * (size_t)P.nat * P.nhxs * 3 * n);
+  if ((fnl_host && do_fnl) || (dfnl_host && do_dfnl)) dl_finish(ctx);
 }
 
 void rnlsm1_dev(mc_ctx* ctx, const double2* c0, int n, double* fnl_host) {
@@ -670,8 +664,8 @@ void rnlfor_dev(mc_ctx* ctx, const double* f_host, int n, const double* fnl_host
                      (const int*)(base + off_ngh), P.d_h.as<double>(), (double*)(base + off_out));
   MC_HIP(hipGetLastError());
   std::vector<double> out(3 * (size_t)P.nat);
-  MC_HIP(hipMemcpyAsync(out.data(), base + off_out, sizeof(double) * out.size(), hipMemcpyDeviceToHost, ctx->stream));
-  MC_HIP(hipStreamSynchronize(ctx->stream));
+  dl_add(ctx, out.data(), base + off_out, sizeof(double) * out.size());
+  dl_finish(ctx);
   for (size_t e = 0; e < out.size(); ++e) fion_host[e] += out[e];
 }
 

@@ -745,17 +745,17 @@ void vofrho_dev(mc_ctx* ctx, const double* rho, double* v, double* energies, dou
     LAUNCH(MC_KID_VOFRHO, k_pad_zero, dim3(1024), dim3(256), 0, v, ctx->n2, ctx->n3, ctx->kr1, ctx->kr2, ctx->kr3);
   // host: fixed-order sums of the partials and the G = 0 terms
   std::vector<double> hp(4 * (size_t)kRedBlocks + 2 * (size_t)n_xblk);
-  MC_HIP(hipMemcpyAsync(hp.data(), p_pp, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, ctx->stream));
+  dl_add(ctx, hp.data(), p_pp, sizeof(double) * hp.size());
   std::vector<double> hf(3 * (size_t)D.nat);
   if (fion_host && D.nat > 0)
-    MC_HIP(hipMemcpyAsync(hf.data(), f_out, sizeof(double) * hf.size(), hipMemcpyDeviceToHost, ctx->stream));
+    dl_add(ctx, hf.data(), f_out, sizeof(double) * hf.size());
   double2 g0[3] = {};
   if (ctx->geq0) {
-    MC_HIP(hipMemcpyAsync(&g0[0], rhog, sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
-    MC_HIP(hipMemcpyAsync(&g0[1], eirop, sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
-    MC_HIP(hipMemcpyAsync(&g0[2], eivps, sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
+    dl_add(ctx, &g0[0], rhog, sizeof(double2));
+    dl_add(ctx, &g0[1], eirop, sizeof(double2));
+    dl_add(ctx, &g0[2], eivps, sizeof(double2));
   }
-  MC_HIP(hipStreamSynchronize(ctx->stream));
+  dl_finish(ctx);
   double s[4] = {0, 0, 0, 0};
   for (int b = 0; b < kRedBlocks; ++b)
     for (int q = 0; q < 4; ++q) s[q] += hp[4 * (size_t)b + q];
@@ -809,8 +809,8 @@ void ewald(mc_ctx* ctx, int iesr, double* esr, double* eself, double* fion_host)
   D.w_part.ensure(sizeof(double) * 4 * (size_t)nat);
   LAUNCH(MC_KID_VOFRHO, k_ewald, dim3(nat), dim3(kThreads), 0, p, D.w_part.as<double>());
   std::vector<double> h(4 * (size_t)nat);
-  MC_HIP(hipMemcpyAsync(h.data(), D.w_part.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
-  MC_HIP(hipStreamSynchronize(ctx->stream));
+  dl_add(ctx, h.data(), D.w_part.p, sizeof(double) * h.size());
+  dl_finish(ctx);
   double e = 0.0;
   for (int l = 0; l < nat; ++l) {
     e += h[4 * (size_t)l];

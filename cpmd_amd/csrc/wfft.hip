@@ -988,8 +988,8 @@ static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, in
   if (npairs > 0) LAUNCH(MC_KID_MISC, k_reduce_pairs, dim3(npairs), dim3(64), 0, red, s.gx, pair_sums);
   LAUNCH(MC_KID_MISC, k_sum_partial, dim3(nblk_sum), dim3(kThreads), 0, rho, nnr1, rsum_part);
   std::vector<double> hs((size_t)npairs * 4 + nblk_sum);
-  MC_HIP(hipMemcpyAsync(hs.data(), pair_sums, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, ctx->stream));
-  MC_HIP(hipStreamSynchronize(ctx->stream));
+  dl_add(ctx, hs.data(), pair_sums, sizeof(double) * hs.size());
+  dl_finish(ctx);
   double sg = 0.0, sk = 0.0, sr = 0.0;
   for (int p = 0; p < npairs; ++p) {
     const int is1 = 2 * p, is2 = is1 + 1;
