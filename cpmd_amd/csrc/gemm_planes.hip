@@ -49,7 +49,11 @@ constexpr int kXcdP = 8;
 constexpr int kMaxSlices = 64;            // K chunk (slices of 32 rows) per workgroup: FP32 accumulation
 constexpr int kTnWaves = 8;               // waves per k_tn_planes workgroup (two per SIMD)
 
-__device__ __forceinline__ int swz(int c, int q) { return q ^ ((c >> 2) & 3); }
+// chunk q (16 bytes) of column c of a part sits at slot swz(c, q): columns are
+// 64 bytes apart, so in a 128-byte bank row columns c and c + 2 share the
+// slots; the XOR with (c >> 1) & 3 gives any 8 consecutive columns (the lanes
+// of one LDS read phase) 8 distinct slots
+__device__ __forceinline__ int swz(int c, int q) { return q ^ ((c >> 1) & 3); }
 
 typedef __attribute__((address_space(3))) void lds_void;
 
