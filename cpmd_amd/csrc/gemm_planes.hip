@@ -539,37 +539,57 @@ __global__ __launch_bounds__(256, 3) void k_nn_planes(const double* __restrict__
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll 2
-    for (int it = 0; it < 8; ++it) {
-      const int c = 2 * it + hl, row = 2 * pr;
-      const float2 x = *reinterpret_cast<const float2*>(st + c * 64 + (((row >> 2) ^ c) * 4) + (row & 3));
-      const int j = j0 + wc + 16 * b + c, i = i0 + wr + row;   // i even, M even: rows i, i + 1
-      if (j >= n || i >= M) continue;
-      double* d = D + (size_t)j * ldd + i;
-      const double x0 = (double)x.x, x1 = (double)x.y;
-      if (epi.twnl) {
-        double f = epi.f[j];
-        if (f < 1e-5) f = 1.0;
-        const double tw = epi.twnl[i >> 1];
-        double yr, yi;
-        switch (epi.lmod4) {   // ctm = -f (-i)^l ; y = tw * ctm * x
-          case 0: yr = -f * tw * x0; yi = -f * tw * x1; break;
-          case 1: yr = -f * tw * x1; yi = f * tw * x0; break;
-          case 2: yr = f * tw * x0; yi = f * tw * x1; break;
-          default: yr = f * tw * x1; yi = -f * tw * x0; break;
+    // four column pairs at a time: their loads (D, and c0 for the velocity)
+    // first, then the arithmetic and the stores
+#pragma unroll
+    for (int h = 0; h < 8; h += 4) {
+      double2 dv[4], av[4];
+      float2 xs[4];
+#pragma unroll
+      for (int it = 0; it < 4; ++it) {
+        const int c = 2 * (h + it) + hl, row = 2 * pr;
+        xs[it] = *reinterpret_cast<const float2*>(st + c * 64 + (((row >> 2) ^ c) * 4) + (row & 3));
+        const int j = j0 + wc + 16 * b + c, i = i0 + wr + row;   // i even, M even: rows i, i + 1
+        dv[it] = av[it] = make_double2(0.0, 0.0);
+        if (j < n && i < M) {
+          const double* d = D + (size_t)j * ldd + i;
+          if (epi.twnl || beta != 0.0) dv[it] = make_double2(d[0], d[1]);
+          if (!epi.twnl && epi.vcm) {
+            const double* a = A + (size_t)j * lda + i;
+            av[it] = make_double2(a[0], a[1]);
+          }
         }
-        d[0] += yr;
-        d[1] += yi;
-      } else {
-        const double y0 = (beta == 0.0 ? 0.0 : beta * d[0]) + alpha * x0;
-        const double y1 = (beta == 0.0 ? 0.0 : beta * d[1]) + alpha * x1;
-        d[0] = y0;
-        d[1] = y1;
-        if (epi.vcm) {
-          const double* av = A + (size_t)j * lda + i;
-          double* v = epi.vcm + (size_t)j * ldd + i;
-          v[0] = epi.vodt * (y0 - av[0]);
-          v[1] = epi.vodt * (y1 - av[1]);
+      }
+#pragma unroll
+      for (int it = 0; it < 4; ++it) {
+        const int c = 2 * (h + it) + hl, row = 2 * pr;
+        const int j = j0 + wc + 16 * b + c, i = i0 + wr + row;
+        if (j >= n || i >= M) continue;
+        double* d = D + (size_t)j * ldd + i;
+        const double x0 = (double)xs[it].x, x1 = (double)xs[it].y;
+        if (epi.twnl) {
+          double f = epi.f[j];
+          if (f < 1e-5) f = 1.0;
+          const double tw = epi.twnl[i >> 1];
+          double yr, yi;
+          switch (epi.lmod4) {   // ctm = -f (-i)^l ; y = tw * ctm * x
+            case 0: yr = -f * tw * x0; yi = -f * tw * x1; break;
+            case 1: yr = -f * tw * x1; yi = f * tw * x0; break;
+            case 2: yr = f * tw * x0; yi = f * tw * x1; break;
+            default: yr = f * tw * x1; yi = -f * tw * x0; break;
+          }
+          d[0] = dv[it].x + yr;
+          d[1] = dv[it].y + yi;
+        } else {
+          const double y0 = (beta == 0.0 ? 0.0 : beta * dv[it].x) + alpha * x0;
+          const double y1 = (beta == 0.0 ? 0.0 : beta * dv[it].y) + alpha * x1;
+          d[0] = y0;
+          d[1] = y1;
+          if (epi.vcm) {
+            double* v = epi.vcm + (size_t)j * ldd + i;
+            v[0] = epi.vodt * (y0 - av[it].x);
+            v[1] = epi.vodt * (y1 - av[it].y);
+          }
         }
       }
     }
