@@ -400,6 +400,8 @@ struct F4Tile {
 };
 
 // LDS bytes of the fused kernels: the two twiddle tables, nb buffers, one
+// dynamic LDS the fused engine may take per workgroup (one buffer above 64 KB)
+constexpr size_t kF4MaxLds = 98304;
 // int map of length N
 template <typename T, int N>
 constexpr size_t f4_lds(int nb) {
@@ -412,8 +414,17 @@ constexpr size_t f4_lds() { return f4_lds<T, N>(f4_bufs<T, N>()); }
 // lengths with a fused kernel that fits the 64 KiB of LDS of one workgroup
 template <typename T, int N>
 constexpr bool f4_ok() {
-  if constexpr (N > 0) return f4_lds<T, N>() <= 65536;
+  if constexpr (N > 0) return f4_lds<T, N>() <= kF4MaxLds;
   else return false;
+}
+// Launches of the fused engine above 64 KB of dynamic LDS (a single FP64
+// buffer at N = 240 takes 70 KB; gfx950 has 160 KB per CU) raise the
+// kernel's dynamic LDS limit first.
+template <typename K>
+static void f4_allow_lds(K* kern, size_t bytes) {
+  if (bytes > 65536)
+    MC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes));
 }
 
 // Buffer addressing.  Logical row r of row group q (groups of P rows) is
@@ -924,6 +935,7 @@ static void ypass(mc_ctx* ctx, const WParams& w, const Shapes& s, v2<T>* A, v2<T
       const int zt = (ctx->n3 + L - 1) / L;
       const int runs = std::max(1, std::min(np, (2048 + ctx->ncols * zt - 1) / (ctx->ncols * zt)));
       const int ppb = (np + runs - 1) / runs;
+      f4_allow_lds(&k_ypass_f<T, N2, DIR>, f4_lds<T, N2>());
       LAUNCH(kid, (k_ypass_f<T, N2, DIR>), dim3(ctx->ncols, zt, (np + ppb - 1) / ppb), dim3(kThreads),
              (f4_lds<T, N2>()), w, A, B, np, ppb);
     } else {
@@ -977,6 +989,7 @@ static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, in
     with_length(ctx->n1, [&](auto NN) {
       constexpr int N1 = decltype(NN)::value;
       if constexpr (f4_ok<T, N1>()) {
+        f4_allow_lds(&k_xrho_f<T, N1>, f4_lds<T, N1>());
         LAUNCH(MC_KID_XRHO, (k_xrho_f<T, N1>), dim3((ctx->n3 + F4Tile<N1>::L - 1) / F4Tile<N1>::L, ctx->n2),
                dim3(kThreads), (f4_lds<T, N1>()), w, B, np, pair0, nstate, ctx->w_f.as<double>(), rho);
       } else {
@@ -1039,6 +1052,7 @@ static void vpsi_impl(mc_ctx* ctx, const double2* c0, double2* c2, const double*
     with_length(ctx->n1, [&](auto NN) {
       constexpr int N1 = decltype(NN)::value;
       if constexpr (f4_ok<T, N1>()) {
+        f4_allow_lds(&k_xvpsi_f<T, N1>, f4_lds<T, N1>());
         LAUNCH(MC_KID_XVPSI, (k_xvpsi_f<T, N1>), dim3((ctx->n3 + F4Tile<N1>::L - 1) / F4Tile<N1>::L, ctx->n2),
                dim3(kThreads), (f4_lds<T, N1>()), w, B, np, vpot);
       } else {

@@ -82,6 +82,31 @@ def test_cfg1_parity(gpu, batch):
     _check_vpsi(ctx, L, c, sysm.f, v)
 
 
+def test_cfg5_pair(gpu):
+    """One state pair on the 240^3 mesh of config 5 (the FP64 tile of the fused
+    x / y engine is above 64 KB of LDS there): rhoofr and vpsi in FP64 to the
+    parity tolerances and with FP32 transforms to 2e-6, against the oracle."""
+    sysm, c = config_system("cfg5", nstate=2)
+    assert tuple(sysm.cell.nr) == (240, 240, 240)
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    L = oracle.Layout(sysm.cell, sysm.gset)
+    f = sysm.f[:2]
+    v = S.real_space_potential(sysm)
+    rho_o, ek_o, cg_o, cr_o = oracle.rhoofr(L, c, f)
+    c2_o = oracle.vpsi(L, c, np.zeros_like(c), f, v)
+    for mode, rtol in (("fp64", None), ("fp32", 2e-6)):
+        ctx.set_precision(mode)
+        rho_g, ek_g, cg_g, cr_g = ctx.rhoofr(c, f)
+        c2_g = ctx.vpsi(c, np.zeros_like(c), f, v)
+        if rtol is None:
+            assert np.abs(rho_g - rho_o).max() < RHO_TOL
+            assert ek_g == pytest.approx(ek_o, rel=1e-12) and cg_g == pytest.approx(cg_o, rel=1e-12)
+            assert np.abs(c2_g - c2_o).max() < C2_TOL * np.abs(c2_o).max()
+        else:
+            assert np.abs(rho_g - rho_o).max() < rtol * np.abs(rho_o).max(), mode
+            assert np.abs(c2_g - c2_o).max() < rtol * np.abs(c2_o).max(), mode
+
+
 def test_cfg2_parity_subset(gpu):
     sysm, c = config_system("cfg2", nstate=6)
     ctx = ops.Context(sysm.cell, sysm.gset)
