@@ -1102,7 +1102,7 @@ void forces_c2_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_
            nstate);
   }
   vpsi_dev(ctx, c0, c2, f_host, vpot, nstate, false);
-  fnonloc_dev(ctx, c2, f_host, nstate, nullptr);
+  if (ctx->proj.ready) fnonloc_dev(ctx, c2, f_host, nstate, nullptr);   // no projectors: local potential only
 }
 
 }  // namespace mc

@@ -279,7 +279,7 @@ int mc_fnonloc(mc_ctx* ctx, double* c2, int ldc, const double* f, int nstate, co
 int mc_fnonloc_dev(mc_ctx* ctx, void* c2_dev, const double* f, int nstate, const double* fnl);
 
 /* Electronic gradient of forces: c2 <- 0, vpsi, fnonloc (with the device fnl
- * of the last rnlsm1) in one call.  The vpsi pass writes c2 instead of adding
+ * of the last rnlsm1; skipped when no projectors are set) in one call.  The vpsi pass writes c2 instead of adding
  * to it, so c2 is neither cleared nor read first; rows ngw .. ldg-1 are zeroed.
  * Replaces the zeroing(c2) / vpsi / fnonloc sequence of
  * forces(c0,c2,tau0,fion,rhoe,psi,nstate,nkpoint,lproj,tfor),

@@ -205,3 +205,23 @@ def test_fp32_transforms_small(gpu, kind, nstate):
     c2 = ctx.vpsi(c, np.zeros_like(c), f, v)
     c2_o = oracle.vpsi(L, c, np.zeros_like(c), f, v)
     assert np.abs(c2 - c2_o).max() < 2e-6 * np.abs(c2_o).max()
+
+
+def test_forces_c2_local_only(gpu):
+    """Without projectors the electronic gradient is vpsi alone: equal to vpsi
+    into a cleared c2, whatever c2 held before, padding rows included."""
+    cell, gs, c = small_system("cubic", 5)
+    n = c.shape[1]
+    ctx = ops.Context(cell, gs)
+    sysm = S.SyntheticSystem(cell=cell, gset=gs, tau=np.zeros((1, 3)), nstate=n, f=np.full(n, 2.0))
+    v = S.real_space_potential(sysm, seed=13)
+    f = np.array([2.0, 0.0, 1.0, 2.0, 0.5])
+    d = ctx.upload(c)
+    vd = ops.DeviceArray(8 * ctx.nnr1).from_host(v)
+    ref = ctx.alloc_wfn(n)
+    ctx.vpsi_dev(d, ref, f, vd, n)
+    nd = 2 * ctx.ld * n
+    out = ctx.alloc_wfn(n).from_host(np.random.default_rng(2).standard_normal(nd))
+    ctx.forces_c2_dev(d, out, f, vd, n)
+    ctx.synchronize()
+    assert np.array_equal(out.to_host(nd), ref.to_host(nd))
