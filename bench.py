@@ -29,7 +29,8 @@ Reported errors: E_KS and F_ion of the headline path against FP64 on the same
 coefficients and positions, the same after the warmup + timed steps from the
 same start (both evaluated in FP64), and ||C^T C - I||.
 
-Prints one JSON line (see the measurement contract in DESIGN.md).
+Prints one JSON line: metric, value (ms per CP step of the mixed path), roofline of the dominant
+transform kernel, MFMA and transform summaries, error fields and the CPU baseline.
 """
 from __future__ import annotations
 
@@ -76,7 +77,7 @@ def kernel_bytes(ctx_geom, kid: str, np_pairs: float, cw: float = 16.0) -> float
 
 
 def step_model_bytes(ctx_geom, nstate: int, cw: float = 16.0) -> float:
-    """Per-step algorithmic bytes of rhoofr + vpsi (the survey's model with the
+    """Per-step algorithmic bytes of rhoofr + vpsi (the stick-sparse transform model with the
     transform workspaces at cw bytes per complex): per state pair
     vpsi = cw(4 S N + 4 N^2 Zw) + 16*6 NGW + 8 N^3, rhoofr = cw(2 S N + 2 N^2 Zw) + 16*2 NGW,
     plus rho written once (8 N^3)."""
@@ -137,7 +138,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: the one-GPU build runs the whole workload on a "
-              "single MI355X; multi-GPU sharding is described in DESIGN.md, not launched.",
+              "single MI355X; multi-GPU sharding is not launched by this benchmark.",
               file=sys.stderr)
         sys.exit(2)
 

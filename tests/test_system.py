@@ -1,8 +1,8 @@
 """Layout and G-vector set against CPMD's own printed values.
 
 Known answers (mesh, NGW, NHG, x-ray sticks, z extent) are those CPMD 4.3
-prints / builds for these cells (serial run, default FFT table); see the
-geometry table of the survey of the reference.  The stick geometry is also
+prints / builds for these cells (serial run, default FFT table).  The stick
+geometry is also
 pinned by brute force (counting distinct (m1,m2) projections of the full
 sphere)."""
 import numpy as np
