@@ -392,6 +392,103 @@ void timer_flush(mc_ctx* ctx) {
   t.kid.clear();
 }
 
+// hesele (src/hesele_utils.mod.F90:32-142): the diagonal preconditioner of the
+// electronic gradient on the device rows, with prec
+//   vpp = 1 / max(0.5 tpiba2 hg + vploc + sum_is sum_iv,jv (h na) tw_iv tw_jv, hthrs)
+// (hw = h na per species, zero outside the (l, m) blocks), else vpp = svar2;
+// padding rows get 0 so that updates leave them zero.
+__global__ void k_hesele(const double* __restrict__ hg, const double* __restrict__ twnl, int ldg, int ngw,
+                         int nhxs, int nsp, const double* __restrict__ hw, double vploc, double ftpi, double hthrs,
+                         int prec, double svar2, double* __restrict__ vpp) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ldg) return;
+  if (j >= ngw) {
+    vpp[j] = 0.0;
+    return;
+  }
+  if (!prec) {
+    vpp[j] = svar2;
+    return;
+  }
+  double v = vploc;
+  for (int is = 0; is < nsp; ++is)
+    for (int iv = 0; iv < nhxs; ++iv) {
+      const double ti = twnl[(size_t)ldg * (iv + nhxs * is) + j];
+      for (int jv = 0; jv < nhxs; ++jv) {
+        const double dd = hw[iv + nhxs * (jv + nhxs * is)];
+        if (dd != 0.0) v += dd * ti * twnl[(size_t)ldg * (jv + nhxs * is) + j];
+      }
+    }
+  v = ftpi * hg[j] + v;
+  if (v < hthrs) v = hthrs;
+  vpp[j] = 1.0 / v;
+}
+
+// csize (src/csize_utils.mod.F90:100-131) per state i: out[4i] = dotp(c2_i, c2_i)
+// in the order of k_dotp_cols; out[4i+1..3] = the izamax pick over the state:
+// largest |Re| + |Im| (BLAS dcabs1), ties to the smallest CPMD index
+// i ngw + ig, then that index and the modulus of the element.
+__global__ __launch_bounds__(256) void k_csize_cols(const double2* __restrict__ c2, const int* __restrict__ perm,
+                                                    int ngw, int ldg, int geq0, double* __restrict__ out) {
+  const int i = blockIdx.x;
+  const double2* x = c2 + (size_t)ldg * i;
+  double s = 0.0, key = -1.0, idx = 0.0, mod = 0.0;
+  for (int g = threadIdx.x; g < ngw; g += 256) {
+    const double2 v = x[g];
+    if (g >= geq0) s += v.x * v.x + v.y * v.y;
+    const double k = fabs(v.x) + fabs(v.y);
+    const double id = (double)i * ngw + perm[g];
+    if (k > key || (k == key && id < idx)) {
+      key = k;
+      idx = id;
+      mod = hypot(v.x, v.y);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_down(s, off, 64);
+    const double k = __shfl_down(key, off, 64), id = __shfl_down(idx, off, 64), m = __shfl_down(mod, off, 64);
+    if (k > key || (k == key && id < idx)) {
+      key = k;
+      idx = id;
+      mod = m;
+    }
+  }
+  __shared__ double part[4][4];
+  if ((threadIdx.x & 63) == 0) {
+    const int w = threadIdx.x >> 6;
+    part[w][0] = s;
+    part[w][1] = key;
+    part[w][2] = idx;
+    part[w][3] = mod;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t = part[0][0] + part[1][0] + part[2][0] + part[3][0];
+    int b = 0;
+    for (int w = 1; w < 4; ++w)
+      if (part[w][1] > part[b][1] || (part[w][1] == part[b][1] && part[w][2] < part[b][2])) b = w;
+    out[4 * i] = 2.0 * t + (geq0 ? x[0].x * x[0].x : 0.0);
+    out[4 * i + 1] = part[b][1];
+    out[4 * i + 2] = part[b][2];
+    out[4 * i + 3] = part[b][3];
+  }
+}
+
+// Preconditioned steepest descent of updwf (src/updwf_utils.mod.F90:118-125):
+// c0(g, i) += (vpp(g) pf1) c2(g, i), one complex element per thread.
+__global__ void k_sd_update(double2* __restrict__ c0, const double2* __restrict__ c2, const double* __restrict__ vpp,
+                            double pf1, int ldg) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ldg) return;
+  const size_t o = (size_t)ldg * blockIdx.y + j;
+  const double w = vpp[j] * pf1;
+  const double2 g = c2[o];
+  double2 a = c0[o];
+  a.x += w * g.x;
+  a.y += w * g.y;
+  c0[o] = a;
+}
+
 }  // namespace mc
 
 // ===========================================================================
@@ -605,6 +702,75 @@ int mc_wfn_axpby(mc_ctx* ctx, double alpha, const void* x_dev, double beta, void
                        (double*)y_dev, n);
     MC_HIP(hipGetLastError());
   }
+  MC_API_END
+}
+
+int mc_hesele(mc_ctx* ctx, int prec, double svar2, double vploc, double hthrs) {
+  MC_API_BEGIN
+  MC_CHECK(ctx, MC_ERR_ARG, "null context");
+  MC_CHECK(!prec || hthrs > 0.0, MC_ERR_ARG, "hesele: hthrs must be positive");
+  const auto& P = ctx->proj;
+  const int nsp = P.ready ? P.nsp : 0, nhxs = P.ready ? P.nhxs : 0;
+  // h na per species, the dd of hesele's Goedecker / wsg branches
+  std::vector<double> hw((size_t)nhxs * nhxs * nsp + 1, 0.0);
+  for (int is = 0; is < nsp; ++is)
+    for (int jv = 0; jv < P.ngh[is]; ++jv)
+      for (int iv = 0; iv < P.ngh[is]; ++iv) {
+        const size_t o = iv + (size_t)nhxs * (jv + (size_t)nhxs * is);
+        hw[o] = P.h_h[o] * P.na[is];
+      }
+  ctx->w_hw.ensure(sizeof(double) * hw.size());
+  MC_HIP(hipMemcpyAsync(ctx->w_hw.p, hw.data(), sizeof(double) * hw.size(), hipMemcpyHostToDevice, ctx->stream));
+  ctx->d_vpp.ensure(sizeof(double) * ctx->ldg);
+  hipLaunchKernelGGL(mc::k_hesele, dim3((ctx->ldg + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_hg.as<double>(),
+                     nsp ? P.d_twnl.as<double>() : nullptr, ctx->ldg, ctx->ngw, nhxs, nsp, ctx->w_hw.as<double>(),
+                     vploc, 0.5 * ctx->tpiba2, hthrs, prec, svar2, ctx->d_vpp.as<double>());
+  MC_HIP(hipGetLastError());
+  // hw is a host temporary
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->vpp_ready = true;
+  MC_API_END
+}
+
+int mc_csize_dev(mc_ctx* ctx, const void* c2_dev, const double* f, int nstate, double* gemax, double* cnorm) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c2_dev && f && gemax && cnorm && nstate >= 0, MC_ERR_ARG, "bad argument");
+  int nocc = 0;
+  for (int i = 0; i < nstate; ++i)
+    if (f[i] > 1e-5) ++nocc;
+  *gemax = 0.0;
+  *cnorm = 0.0;
+  if (nocc == 0) return MC_OK;
+  ctx->w_red2.ensure(sizeof(double) * 4 * nocc);
+  hipLaunchKernelGGL(mc::k_csize_cols, dim3(nocc), dim3(256), 0, ctx->stream, (const double2*)c2_dev,
+                     ctx->d_perm.as<int>(), ctx->ngw, ctx->ldg, ctx->geq0, ctx->w_red2.as<double>());
+  MC_HIP(hipGetLastError());
+  std::vector<double> r((size_t)4 * nocc);
+  mc::dl_add(ctx, r.data(), ctx->w_red2.p, sizeof(double) * r.size());
+  mc::dl_finish(ctx);
+  double s = 0.0;
+  int b = 0;
+  for (int i = 0; i < nocc; ++i) {
+    s += r[4 * i];
+    // states in order: on equal keys the earlier state has the smaller index
+    if (r[4 * i + 1] > r[4 * b + 1]) b = i;
+  }
+  *gemax = r[4 * b + 3];
+  *cnorm = std::sqrt(s / ((double)nocc * ctx->ngw));
+  MC_API_END
+}
+
+int mc_sd_update_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, int nstate, double pf1) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c0_dev && c2_dev && nstate >= 0, MC_ERR_ARG, "bad argument");
+  MC_CHECK(ctx->vpp_ready, MC_ERR_ARG, "sd_update: call mc_hesele first");
+  MC_CHECK(nstate <= 65535, MC_ERR_ARG, "sd_update: nstate > 65535");
+  if (nstate == 0) return MC_OK;
+  mc::timer_begin(ctx, MC_KID_MISC);
+  hipLaunchKernelGGL(mc::k_sd_update, dim3((ctx->ldg + 255) / 256, nstate), dim3(256), 0, ctx->stream,
+                     (double2*)c0_dev, (const double2*)c2_dev, ctx->d_vpp.as<double>(), pf1, ctx->ldg);
+  MC_HIP(hipGetLastError());
+  mc::timer_end(ctx, MC_KID_MISC);
   MC_API_END
 }
 

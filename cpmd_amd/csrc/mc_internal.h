@@ -217,6 +217,9 @@ struct mc_ctx {
   mc::DevBuf w_gemm;       // split-K partials
   mc::DevBuf w_small[8];    // n x n matrices
   mc::DevBuf w_red2, w_info, w_wfn, w_f2, w_dd;
+  // preconditioner of the electronic gradient (mc_hesele), device rows
+  mc::DevBuf d_vpp, w_hw;
+  bool vpp_ready = false;
   mc::Projectors proj;
 
   bool fft_fp32 = false;   // rhoofr/vpsi transforms and workspaces in FP32

@@ -8,6 +8,8 @@ for the Gamma, closed-shell, LDA, norm-conserving path
 
 E_KS = ekin + ehep + epseu + exc + esr - eself + enl (ener_com%etot).  The
 wavefunctions stay on the device; only energies, fnl and forces come back.
+KSDevice.optimize is the wavefunction optimisation at fixed ions with the
+mixed-then-FP64 precision policy.
 """
 from __future__ import annotations
 
@@ -35,6 +37,8 @@ class KSDevice:
         nh = np.array([n // 2 + 1 for n in sysm.cell.nr])
         ctx.set_density_gset(gs.miller + nh, gs.hg, gs.miller @ sysm.cell.bvec)
         ctx.set_local_pp(na, zv, rg, vps, rhops)
+        # G=0 local potential (ppener's vploc = eivps(G=0), src/ppener_utils.mod.F90:58-60)
+        self.vploc = float(np.dot(np.asarray(na, dtype=np.float64), vps[:, 0])) if gs.hg[0] == 0.0 else 0.0
         self.ctx = ctx
         self.hmat = np.asarray(hmat, dtype=np.float64).reshape(nhxs, nhxs, -1, order="F")
         self.na = np.asarray(na)
@@ -87,3 +91,49 @@ class KSDevice:
         if forces:
             out["fion"] = ctx.rnlfor(f, fion=floc + fesr)
         return out
+
+    def optimize(self, d_c0: ops.DeviceArray, f: np.ndarray, d_c2: ops.DeviceArray, tol: float = 1e-5,
+                 switch: float = 1e-4, mixed: bool = True, dt: float = 20.0, emass: float = 400.0,
+                 hthrs: float = 0.5, maxiter: int = 2000) -> dict:
+        """Wavefunction optimisation at fixed ions: rwfopt -> updwf with the
+        preconditioned steepest descent (src/updwf_utils.mod.F90:64-160).  Each
+        iteration runs the forces driver (E_KS and c2 = -f H c), the Lagrange
+        projection of c2 (lproj), csize, c0 += vpp pf1 c2 (hesele's vpp,
+        pf1 = dt^2 hthrs / emass) and Gram-Schmidt (ortho -> rgs).
+
+        Precision policy: with mixed, the transforms run in FP32 and the
+        contractions in split bf16 while the gradient size gemax is >= switch;
+        from the first iteration below it everything runs in FP64 until
+        gemax < tol (the convergence test of tol_chk_cnvgrad; 1e-5 is CPMD's
+        default, src/control_test_utils.mod.F90:595-601).  mixed=False is
+        the plain FP64 optimisation.  The returned etot is the energy of the
+        final (converged) coefficients, evaluated in FP64."""
+        ctx = self.ctx
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        n = len(f)
+        ctx.hesele(True, 0.0, self.vploc, hthrs)
+        pf1 = dt * dt * hthrs / emass
+        mode = "mixed" if mixed and switch > tol else "fp64"
+        ctx.set_precision(mode)
+        hist = []
+        iters = {"mixed": 0, "fp64": 0}
+        converged = False
+        e, gemax, cnorm = None, float("inf"), float("inf")
+        for _ in range(maxiter):
+            e = self.energy(d_c0, f, gradient=d_c2)
+            ctx.lagrange_dev(d_c0, d_c2, f, n)
+            gemax, cnorm = ctx.csize_dev(d_c2, f, n)
+            hist.append((mode, e["etot"], gemax, cnorm))
+            iters[mode] += 1
+            if not np.isfinite(gemax):
+                break
+            if mode == "fp64" and gemax < tol:
+                converged = True
+                break
+            ctx.sd_update_dev(d_c0, d_c2, n, pf1)
+            ctx.rgs_dev(d_c0, n)
+            if mode == "mixed" and gemax < switch:
+                mode = "fp64"
+                ctx.set_precision(mode)
+        return dict(etot=e["etot"] if e else float("nan"), terms=e, gemax=gemax, cnorm=cnorm, converged=converged,
+                    iters_mixed=iters["mixed"], iters_fp64=iters["fp64"], history=hist)

@@ -158,6 +158,21 @@ class Context:
         """cm = (cp - c0) / dt (velocities after rortog)."""
         check(lib().mc_cp_velocity(self.h, c0.ptr, cp.ptr, cm.ptr, nstate, dt))
 
+    def hesele(self, prec: bool, svar2: float = 0.0, vploc: float = 0.0, hthrs: float = 0.5):
+        """Set the diagonal preconditioner vpp(G) of the electronic gradient."""
+        check(lib().mc_hesele(self.h, int(prec), svar2, vploc, hthrs))
+
+    def csize_dev(self, c2: DeviceArray, f: np.ndarray, nstate: int) -> tuple[float, float]:
+        """(gemax, cnorm) of the gradient c2 over the occupied states."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        ge, cn = C.c_double(), C.c_double()
+        check(lib().mc_csize_dev(self.h, c2.ptr, _p(f), nstate, C.byref(ge), C.byref(cn)))
+        return ge.value, cn.value
+
+    def sd_update_dev(self, c0: DeviceArray, c2: DeviceArray, nstate: int, pf1: float):
+        """c0 += vpp pf1 c2 (preconditioned steepest descent)."""
+        check(lib().mc_sd_update_dev(self.h, c0.ptr, c2.ptr, nstate, pf1))
+
     def dotp_wfn(self, a: DeviceArray, b: DeviceArray, nstate: int) -> np.ndarray:
         out = np.zeros(nstate)
         check(lib().mc_wfn_dotp(self.h, a.ptr, b.ptr, nstate, _p(out)))

@@ -305,12 +305,15 @@ def gth_local_potential_g(sys: SyntheticSystem, hg: np.ndarray) -> np.ndarray:
 
 
 def gth_projector_g(sys: SyntheticSystem, hg: np.ndarray) -> np.ndarray:
-    """Analytic s-projector form factor twnl(G) of the GTH pseudopotential,
-    normalised with 1/sqrt(Omega) (the role of twnl in src/rnlsm1_utils.mod.F90:92-110)."""
+    """Analytic s-projector twnl(G) of the GTH pseudopotential: the radial
+    form factor times Y_00 = 1/sqrt(4 pi) times 1/sqrt(Omega), as CPMD builds
+    twnl = ylmb tw vol (src/rinforce_utils.mod.F90:564-576), so that the
+    projector has unit norm (sum over the G sphere of twnl^2 -> 1) and the
+    nonlocal operator of one atom has eigenvalue h11."""
     pp = sys.pp
     g2 = hg * sys.cell.tpiba2
     r0 = pp.r0
-    return (4.0 * math.sqrt(2.0) * math.pi ** 1.25 * r0 ** 1.5
+    return (4.0 * math.sqrt(2.0) * math.pi ** 1.25 * r0 ** 1.5 / math.sqrt(4.0 * math.pi)
             * np.exp(-0.5 * g2 * r0 * r0) / math.sqrt(sys.cell.omega))
 
 

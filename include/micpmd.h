@@ -159,6 +159,29 @@ int mc_posupa_dev(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, void* cm_
  * cp = c0 + dt cm + c0 gamx. */
 int mc_cp_velocity(mc_ctx* ctx, const void* c0_dev, const void* cp_dev, void* cm_dev, int nstate, double dt);
 
+/* Wavefunction optimisation by preconditioned steepest descent.
+ * hesele: the diagonal preconditioner vpp(G), kept in the context.  With
+ * prec != 0, vpp = 1 / max(0.5 tpiba2 hg + vploc + sum_is na_is
+ * sum_iv,jv hmat(iv,jv,is) twnl(G,iv,is) twnl(G,jv,is), hthrs) over the
+ * projectors of mc_set_projectors (none set: no nonlocal term); vploc is
+ * the G=0 local potential sum_is na_is vps(is, G=0) (ppener's vploc);
+ * prec = 0: vpp = svar2.  Replaces hesele(svar2,vpp),
+ * src/hesele_utils.mod.F90:32-142 (Goedecker and wsg branches). */
+int mc_hesele(mc_ctx* ctx, int prec, double svar2, double vploc, double hthrs);
+/* csize: size of the electronic gradient over the nocc states with
+ * f > 1e-5 (taken as the first nocc): gemax = |c2| of the element that
+ * izamax picks (largest |Re| + |Im|, first in CPMD order on ties),
+ * cnorm = sqrt(sum_i dotp(c2_i, c2_i) / (nocc ngw)).  Synchronous.
+ * Replaces csize(c2,nstate,gemax,cnorm), src/csize_utils.mod.F90:100-131
+ * (closed shell, Gamma point). */
+int mc_csize_dev(mc_ctx* ctx, const void* c2_dev, const double* f, int nstate, double* gemax, double* cnorm);
+/* c0(G, i) += vpp(G) pf1 c2(G, i) for all nstate states, vpp from mc_hesele
+ * (MC_ERR_ARG before the first mc_hesele).  With prec, pf1 =
+ * dt^2 hthrs / emass; without, mc_hesele(ctx, 0, dt^2/emass, ...) and
+ * pf1 = 1 give CPMD's daxpy(dt2bye).  Replaces the steepest-descent branch
+ * of updwf, src/updwf_utils.mod.F90:117-128. */
+int mc_sd_update_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, int nstate, double pf1);
+
 /* out[i] = dotp(a_i, b_i), the Gamma-point inner product of state i
  * (src/dotp_utils.mod.F90:58-101); with b = c2 after vpsi + fnonloc,
  * -sum_i out[i] is the band energy sum_i f_i <c_i|H|c_i>. */

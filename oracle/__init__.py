@@ -115,6 +115,10 @@ def _setup_dense(L):
     L.or_rotate.argtypes = [_ci, C.c_double, _dp, C.c_double, _dp, _ci, _dp, _ci]
     L.or_lagrange.argtypes = [_ci, _ci, _dp, _dp, _ci, _dp, _ci]
     L.or_rortho.argtypes = [_ci, _ci, _dp, _dp, _ci, _dp, _ci, _ci, C.c_double, _ci]
+    L.or_hesele.argtypes = [_ci, _ci, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _ci, _ip, _ci, _ip, _dp,
+                            _dp, _ci, _dp]
+    L.or_csize.argtypes = [_ci, _ci, _dp, _ci, _dp, _ci, _dp, _dp]
+    L.or_sd_update.argtypes = [_ci, _dp, _dp, _ci, _ci, _dp, C.c_double]
     L.or_rortho.restype = _ci
     L.or_rnlsm1.argtypes = [_ci, _ci, _ci, _ip, _ci, _ip, _ip, _dp, _ci, _dp, _ci, _dp, _ci, _ci, _dp]
     L.or_fnonloc.argtypes = [_ci, _ci, _ip, _ci, _ip, _ip, _dp, _dp, _ci, _dp, _ci, _dp, _dp, _dp, _ci, _ci]
@@ -203,6 +207,40 @@ def lagrange(c0, c2, f, geq0=True):
     out = _F(c2)
     f = np.ascontiguousarray(f, dtype=np.float64)
     _L().or_lagrange(c0.shape[0], int(geq0), _p(c0), _p(out), c0.shape[0], _p(f), c0.shape[1])
+    return out
+
+
+def hesele(prec, svar2, vploc, hthrs, tpiba2, hg, P=None):
+    """vpp(G) in CPMD G order; P: the projector set (None: no projectors)."""
+    hg = np.ascontiguousarray(hg, dtype=np.float64)
+    ngw = len(hg)
+    vpp = np.zeros(ngw)
+    if P is None:
+        na = np.zeros(1, np.int32); ngh = np.zeros(1, np.int32)
+        hm = np.zeros(1); tw = np.zeros(1); nsp, nhxs, ldtw = 0, 1, 1
+    else:
+        na = np.ascontiguousarray(P.na, np.int32); ngh = np.ascontiguousarray(P.ngh, np.int32)
+        hm = np.asfortranarray(P.hmat, np.float64); tw = np.asfortranarray(P.twnl, np.float64)
+        nsp, nhxs, ldtw = len(na), P.nhxs, tw.shape[0]
+    _L().or_hesele(ngw, int(prec), svar2, vploc, hthrs, tpiba2, _p(hg), nsp, _p(na, _ip), nhxs, _p(ngh, _ip),
+                   _p(hm), _p(tw), ldtw, _p(vpp))
+    return vpp
+
+
+def csize(c2, f, geq0=True):
+    """(gemax, cnorm) of the electronic gradient c2 (ngw, n)."""
+    c2 = np.asfortranarray(c2, dtype=np.complex128)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    ge, cn = C.c_double(), C.c_double()
+    _L().or_csize(c2.shape[0], int(geq0), _p(c2), c2.shape[0], _p(f), c2.shape[1], C.byref(ge), C.byref(cn))
+    return ge.value, cn.value
+
+
+def sd_update(c0, c2, vpp, pf1):
+    out = _F(c0)
+    c2 = np.asfortranarray(c2, dtype=np.complex128)
+    vpp = np.ascontiguousarray(vpp, dtype=np.float64)
+    _L().or_sd_update(out.shape[0], _p(out), _p(c2), out.shape[0], out.shape[1], _p(vpp), pf1)
     return out
 
 

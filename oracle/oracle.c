@@ -404,6 +404,77 @@ void or_lagrange(int ngw, int geq0, const double* c0, double* c2, int ldc, const
   free(gam);
 }
 
+/* hesele, src/hesele_utils.mod.F90:32-142: vpp(ig) over ngw G vectors in CPMD
+ * order.  prec: vp = vploc, then per species, projector pair (iv, jv) with
+ * hmat(iv,jv,is) != 0 (Goedecker: equal l and m; wsg: diagonal)
+ * vp += (hmat na) twnl(ig,iv) twnl(ig,jv) (:76-104), vp = ftpi hg + vp,
+ * vp = max(vp, hthrs), vpp = 1/vp (:109-131); otherwise vpp = svar2. */
+void or_hesele(int ngw, int prec, double svar2, double vploc, double hthrs, double tpiba2, const double* hg, int nsp,
+               const int* na, int nhxs, const int* ngh, const double* hmat, const double* twnl, int ldtw,
+               double* vpp) {
+  for (int ig = 0; ig < ngw; ++ig) {
+    if (!prec) {
+      vpp[ig] = svar2;
+      continue;
+    }
+    double v = vploc;
+    for (int is = 0; is < nsp; ++is)
+      for (int iv = 0; iv < ngh[is]; ++iv)
+        for (int jv = 0; jv < ngh[is]; ++jv) {
+          const double h = hmat[iv + (size_t)nhxs * (jv + (size_t)nhxs * is)];
+          if (h == 0.0) continue;
+          const double dd = h * na[is];
+          v += dd * twnl[ig + (size_t)ldtw * (iv + (size_t)nhxs * is)] * twnl[ig + (size_t)ldtw * (jv + (size_t)nhxs * is)];
+        }
+    v = 0.5 * tpiba2 * hg[ig] + v;
+    if (v < hthrs) v = hthrs;
+    vpp[ig] = 1.0 / v;
+  }
+}
+
+/* dotp of src/dotp_utils.mod.F90:58-101 on raw columns */
+static double dotp_raw(int ngw, int geq0, const cplx* a, const cplx* b) {
+  or_layout L;
+  memset(&L, 0, sizeof L);
+  L.ngw = ngw;
+  L.geq0 = geq0;
+  return dotp(&L, a, b);
+}
+
+/* csize, src/csize_utils.mod.F90:100-131 (closed shell, Gamma): over the first
+ * nocc states (nocc = number with f > 1e-5), izamax (first largest
+ * |Re| + |Im| in column-major order), gemax = its modulus;
+ * cnorm = sqrt(sum_i dotp(c2_i, c2_i) / (nocc ngw)). */
+void or_csize(int ngw, int geq0, const double* c2d, int ldc, const double* f, int n, double* gemax, double* cnorm) {
+  const cplx* c2 = (const cplx*)c2d;
+  int nocc = 0;
+  for (int i = 0; i < n; ++i)
+    if (f[i] > 1e-5) ++nocc;
+  double best = -1.0, s = 0.0;
+  *gemax = 0.0;
+  for (int i = 0; i < nocc; ++i) {
+    for (int ig = 0; ig < ngw; ++ig) {
+      const cplx z = c2[ig + (size_t)ldc * i];
+      const double k = fabs(creal(z)) + fabs(cimag(z));
+      if (k > best) {
+        best = k;
+        *gemax = cabs(z);
+      }
+    }
+    s += dotp_raw(ngw, geq0, c2 + (size_t)ldc * i, c2 + (size_t)ldc * i);
+  }
+  *cnorm = nocc ? sqrt(s / ((double)nocc * ngw)) : 0.0;
+}
+
+/* updwf steepest descent, src/updwf_utils.mod.F90:117-128:
+ * c0(ig,i) = c0(ig,i) + vpp(ig) pf1 c2(ig,i) */
+void or_sd_update(int ngw, double* c0d, const double* c2d, int ldc, int n, const double* vpp, double pf1) {
+  cplx* c0 = (cplx*)c0d;
+  const cplx* c2 = (const cplx*)c2d;
+  for (int i = 0; i < n; ++i)
+    for (int ig = 0; ig < ngw; ++ig) c0[ig + (size_t)ldc * i] += vpp[ig] * pf1 * c2[ig + (size_t)ldc * i];
+}
+
 /* rortho, src/rortog_utils.mod.F90:315-498; returns iterations, -1 if > maxit */
 int or_rortho(int ngw, int geq0, const double* c0, double* cm, int ldc, double* gam, int tnon, int n, double epsog,
               int maxit) {

@@ -1,0 +1,111 @@
+"""The oracle's wavefunction-optimisation pieces pinned by what fixes them
+independently: hesele against closed forms (no projectors; one s projector
+whose diagonal is h na twnl^2; two identical projectors that reduce to one),
+csize against BLAS izamax semantics and the Gamma-point norm written with
+complex arithmetic, and the preconditioned steepest-descent step as a descent
+direction of E_KS on a small system."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from cpmd_amd import system as S
+
+from test_oracle_energy import ks_system
+
+
+@pytest.fixture(scope="module")
+def small():
+    return ks_system(a=9.0, nat=3, ecut=25.0, nstate=6)
+
+
+def _kin(sysm):
+    gs = sysm.gset
+    return 0.5 * sysm.cell.tpiba2 * gs.hg[:gs.ngw]
+
+
+def test_hesele_no_preconditioner(small):
+    sysm, K, _ = small
+    ngw = sysm.gset.ngw
+    vpp = oracle.hesele(0, 0.0625, -0.3, 0.5, sysm.cell.tpiba2, sysm.gset.hg[:ngw], K.projectors(sysm.tau))
+    assert np.all(vpp == 0.0625)
+
+
+def test_hesele_local_only_closed_form(small):
+    sysm, _, _ = small
+    ngw = sysm.gset.ngw
+    vploc, hthrs = -0.3, 0.5
+    vpp = oracle.hesele(1, 0.0, vploc, hthrs, sysm.cell.tpiba2, sysm.gset.hg[:ngw], None)
+    want = 1.0 / np.maximum(_kin(sysm) + vploc, hthrs)
+    np.testing.assert_allclose(vpp, want, rtol=1e-15)
+    assert vpp.max() == 1.0 / hthrs                      # the clamp is reached at small G
+
+
+def test_hesele_s_projector_diagonal(small):
+    sysm, K, _ = small
+    ngw = sysm.gset.ngw
+    P = K.projectors(sysm.tau)
+    vploc, hthrs = 0.1, 0.5
+    vpp = oracle.hesele(1, 0.0, vploc, hthrs, sysm.cell.tpiba2, sysm.gset.hg[:ngw], P)
+    # <G|V_nl|G> of na atoms with one s projector: na h11 twnl(G)^2, the phases cancel
+    tw = S.gth_projector_g(sysm, sysm.gset.hg[:ngw])
+    d = _kin(sysm) + vploc + sysm.nat * sysm.pp.h11 * tw * tw
+    np.testing.assert_allclose(vpp, 1.0 / np.maximum(d, hthrs), rtol=1e-14)
+    # the projector has unit norm once the G sphere holds it (Gamma: +-G, G=0 once)
+    big = S.make_system(9.0, 3, 120.0)
+    tb = S.gth_projector_g(big, big.gset.hg[:big.gset.ngw])
+    assert 2.0 * np.sum(tb * tb) - tb[0] ** 2 == pytest.approx(1.0, abs=1e-6)
+
+
+def test_hesele_two_projectors_reduce_to_one(small):
+    sysm, K, _ = small
+    ngw = sysm.gset.ngw
+    hg = sysm.gset.hg[:ngw]
+    P1 = K.projectors(sysm.tau)
+    a, b, c = 3.0, -0.7, 1.5
+    tw = np.asfortranarray(np.repeat(P1.twnl, 2, axis=1))                   # (ngw, 2, 1)
+    h2 = np.array([[a, b], [b, c]]).reshape(2, 2, 1)
+    P2 = oracle.Projectors(P1.na, 2, [2], np.zeros((2, 1), np.int32), h2, tw, P1.eigr)
+    P1.hmat = np.asfortranarray(np.full((1, 1, 1), a + 2.0 * b + c))
+    v1 = oracle.hesele(1, 0.0, 0.0, 1e-3, sysm.cell.tpiba2, hg, P1)
+    v2 = oracle.hesele(1, 0.0, 0.0, 1e-3, sysm.cell.tpiba2, hg, P2)
+    np.testing.assert_allclose(v2, v1, rtol=1e-13)
+
+
+def test_csize_izamax_semantics():
+    ngw, n = 40, 4
+    rng = np.random.default_rng(3)
+    c2 = np.asfortranarray(0.1 * (rng.standard_normal((ngw, n)) + 1j * rng.standard_normal((ngw, n))))
+    c2[0] = c2[0].real
+    c2[5, 1] = 3.0 + 3.0j        # largest |Re| + |Im| (6), modulus 4.243
+    c2[7, 2] = 5.0               # largest modulus, |Re| + |Im| = 5
+    c2[3, 3] = 100.0             # state 3 is empty
+    f = np.array([2.0, 2.0, 1.0, 0.0])
+    ge, cn = oracle.csize(c2, f)
+    assert ge == pytest.approx(math.sqrt(18.0), rel=1e-15)
+    # ties go to the first element in column-major order
+    t = c2.copy()
+    t[2, 0] = 0.0 + 6.0j         # |.|_1 = 6 first, modulus 6
+    assert oracle.csize(t, f)[0] == 6.0
+    # cnorm with complex arithmetic: dotp(c, c) = |c0|^2 + 2 sum_{G != 0} |c|^2 (c0 real)
+    occ = c2[:, :3]
+    s = sum(abs(occ[0, i]) ** 2 + 2.0 * np.vdot(occ[1:, i], occ[1:, i]).real for i in range(3))
+    assert cn == pytest.approx(math.sqrt(s / (3 * ngw)), rel=1e-14)
+
+
+def test_sd_step_descends(small):
+    sysm, K, c = small
+    f = sysm.f
+    ngw = sysm.gset.ngw
+    r = oracle.ks_energy(K, c, f, sysm.tau, gradient=True)
+    c2 = oracle.lagrange(c, r["c2"], f)
+    vpp = oracle.hesele(1, 0.0, 0.0, 0.5, sysm.cell.tpiba2, sysm.gset.hg[:ngw], K.projectors(sysm.tau))
+    e = []
+    for pf1 in (1e-4, 2e-4):
+        cn, _ = oracle.rgs(oracle.sd_update(c, c2, vpp, pf1))
+        e.append(oracle.ks_energy(K, cn, f, sysm.tau)["etot"])
+    # a descent direction, and the decrease is first order in the step
+    de = r["etot"] - np.array(e)
+    assert de[0] > 0.0 and de[1] > de[0]
+    assert de[1] / de[0] == pytest.approx(2.0, rel=2e-2)
