@@ -131,8 +131,20 @@ def main():
     ap.add_argument("--emass", type=float, default=400.0)
     ap.add_argument("--prep-steps", type=int, default=30,
                     help="untimed FP64 steepest-descent steps before the CP paths (the CP start state)")
+    ap.add_argument("--prep", choices=("sd", "opt"), default="sd",
+                    help="CP start: sd = --prep-steps plain steepest-descent steps; opt = the wavefunction "
+                         "optimisation (mixed then FP64) to gemax < --prep-tol")
+    ap.add_argument("--prep-tol", type=float, default=1e-5)
     ap.add_argument("--paths", default="mixed,fp64", help="comma list of paths to time (first = headline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--opt-policy", action="store_true",
+                    help="also optimise the wavefunctions from the seeded start (preconditioned steepest "
+                         "descent) mixed-then-FP64 and FP64 only, and report iterations, times and energies")
+    ap.add_argument("--opt-tol", type=float, default=1e-5)
+    ap.add_argument("--opt-dt", type=float, default=10.0,
+                    help="optimisation time step (CPMD TIMESTEP; hthrs 2); its bound from the Hartree "
+                         "curvature at the smallest G falls as 1/a: 10 for a = 24 bohr, 7 for a = 32")
+    ap.add_argument("--opt-maxiter", type=int, default=4000)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -183,6 +195,11 @@ def main():
         tn_gemm_flops = 2 * nat * n * K + 2 * n * n * K + n * n * K
         step_desc = "rnlsm1 + rhoofr + vofrho + vpsi + fnonloc + Lagrange projection + SD update + Loewdin"
 
+    def opt_progress(it, mode, etot, gemax):
+        if it % 250 == 0:
+            log(f"[bench]   optimisation iteration {it} ({mode}): E_KS {etot:.10f} gemax {gemax:.3e}")
+
+    start_desc = "the seeded coefficients"
     if args.step == "cp" and args.prep_steps > 0:
         # CP dynamics starts from a (partially) optimised state: a few FP64
         # steepest-descent steps (Lagrange projection, dt 0.02, Loewdin) bring
@@ -201,6 +218,23 @@ def main():
         c0 = ctx.download(buf["c0"], nstate)
         log(f"[bench] {args.prep_steps} FP64 optimisation steps ({time.perf_counter() - t0:.1f} s): "
             f"E_KS {e_prep:.8f} Ha")
+        start_desc = f"{args.prep_steps} FP64 steepest-descent steps from the seeded coefficients"
+
+    c_sd = c0
+    if args.step == "cp" and args.prep == "opt":
+        ks.set_ions(sysm.tau)
+        ctx.upload(c0, buf["c0"])
+        t0 = time.perf_counter()
+        r = ks.optimize(buf["c0"], f, buf["c2"], tol=args.prep_tol, dt=args.opt_dt, hthrs=2.0, emass=args.emass,
+                        maxiter=args.opt_maxiter, callback=opt_progress)
+        if not r["converged"]:
+            log(f"[bench] the start optimisation did not reach gemax < {args.prep_tol} (gemax {r['gemax']:.2e})")
+            sys.exit(1)
+        c0 = ctx.download(buf["c0"], nstate)
+        start_desc = (f"{start_desc}, then optimised to gemax < {args.prep_tol} "
+                      f"({r['iters_mixed']} mixed + {r['iters_fp64']} FP64 iterations)")
+        log(f"[bench] start optimisation: {r['iters_mixed']} mixed + {r['iters_fp64']} fp64 iterations "
+            f"({time.perf_counter() - t0:.1f} s): E_KS {r['etot']:.8f} Ha")
 
     TERMS = ("ekin", "ehep", "epseu", "exc", "enl", "esr", "eself")
 
@@ -212,11 +246,12 @@ def main():
     modes = {"mixed": "mixed", "fp64": "fp64", "fp32fft": "fp32", "bf16x3": "mixed_bf16x3"}
 
     def run_path(path):
-        ctx.set_precision(modes[path])
         ctx.upload(c0, buf["c0"])                 # every path starts from the same coefficients
         ctx.axpby_wfn(0.0, buf["c0"], 0.0, buf["cm"], nstate)
         tau = sysm.tau.copy()
         vion = np.zeros_like(tau)
+        e0_fp64 = fp64_eval(tau)["etot"]          # the start in FP64: E_cons = E_ham at rest
+        ctx.set_precision(modes[path])
         start = ks.energy(buf["c0"], f, tau=tau, gradient=buf["c2"], forces=True)
         iters = []
 
@@ -262,8 +297,28 @@ def main():
         orth = float(np.abs(ctx.ovlap_dev(buf["c0"], buf["c0"], nstate) - np.eye(nstate)).max())
         log(f"[bench] {path}: {args.steps} steps: {ms:.3f} ms/step, E_KS(start) {start['etot']:.10f}, "
             f"E_KS(end, fp64) {end['etot']:.10f}, |C^T C - I| {orth:.2e}, iters {iters[-args.steps:]}")
+        energy = None
+        if args.step == "cp":
+            # CPMD's constants of motion (md_driver): E_cons = E_KS + ekinp (ions),
+            # E_ham = E_cons + ekinc, ekinc = emass sum_i dotp(cm_i, cm_i) (rekine),
+            # with full-step velocities: the leapfrog half-step ones plus the half
+            # kick of the end forces (for the electrons the gradient after the
+            # Lagrange projection, the velocity constraint to first order)
+            ctx.lagrange_dev(buf["c0"], buf["c2"], f, nstate)
+            ctx.axpy_wfn(0.5 * dt / args.emass, buf["c2"], buf["cm"], nstate)
+            ekinc = args.emass * float(ctx.dotp_wfn(buf["cm"], buf["cm"], nstate).sum())
+            vfull = vion + 0.5 * dt / ION_MASS * end["fion"].T
+            ekinp = 0.5 * ION_MASS * float(np.sum(vfull * vfull))
+            econs = end["etot"] + ekinp
+            energy = {"E_ham_start_Ha": e0_fp64, "E_KS_end_Ha": end["etot"], "ekinp_end_Ha": ekinp,
+                      "ekinc_end_Ha": ekinc, "E_cons_end_Ha": econs, "E_ham_end_Ha": econs + ekinc,
+                      "E_ham_drift_per_atom_Ha": (econs + ekinc - e0_fp64) / nat,
+                      "E_cons_drift_per_atom_Ha": (econs - e0_fp64) / nat,
+                      "steps": args.warmup + args.steps}
+            log(f"[bench] {path}: E_ham drift {energy['E_ham_drift_per_atom_Ha']:.3e} Ha/atom over "
+                f"{energy['steps']} steps (ekinc {ekinc:.3e}, ekinp {ekinp:.3e})")
         return {"ms": ms, "kernels": kernels, "start": start, "end": end, "orth": orth,
-                "iters": iters[-args.steps:]}
+                "iters": iters[-args.steps:], "energy": energy}
 
     paths = [p.strip() for p in args.paths.split(",") if p.strip()]
     res = {p: run_path(p) for p in paths}
@@ -318,6 +373,33 @@ def main():
         cpu = cpu_baseline(sysm, c0, np.ascontiguousarray(ks.d_v.to_host(ks.nnr1)), max_pairs=8)
         log(f"[bench] cpu baseline {cpu['value']:.1f} ms/step")
 
+    opt = None
+    if args.opt_policy:
+        # the precision policy of the wavefunction optimisation: FP32 transforms /
+        # split-bf16 contractions while gemax >= 1e-4, FP64 after
+        opt = {"tol": args.opt_tol, "switch": 1e-4, "dt": args.opt_dt, "emass": args.emass, "hthrs": 2.0,
+               "start": "the CP start before optimisation (the steepest-descent prepared state)"}
+        ks.set_ions(sysm.tau)
+        for label, mixed in (("mixed_then_fp64", True), ("fp64", False)):
+            ctx.upload(c_sd, buf["c0"])
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            r = ks.optimize(buf["c0"], f, buf["c2"], tol=args.opt_tol, switch=1e-4, mixed=mixed, dt=args.opt_dt,
+                            hthrs=2.0, emass=args.emass, maxiter=args.opt_maxiter, callback=opt_progress)
+            ctx.synchronize()
+            el = time.perf_counter() - t0
+            h = r["history"]
+            k = next((i for i, x in enumerate(h) if x[2] < 1e-4), None)
+            opt[label] = {"converged": r["converged"], "iters_mixed": r["iters_mixed"],
+                          "iters_fp64": r["iters_fp64"], "seconds": round(el, 3),
+                          "ms_per_iter": round(el * 1e3 / max(len(h), 1), 3),
+                          "first_iter_below_1e-4": k, "gemax": r["gemax"], "E_KS_Ha": r["etot"]}
+            log(f"[bench] optimise {label}: converged {r['converged']} after {r['iters_mixed']} mixed + "
+                f"{r['iters_fp64']} fp64 iterations, {el:.2f} s, E_KS {r['etot']:.10f}, gemax {r['gemax']:.2e}")
+        if opt["fp64"]["converged"] and opt["mixed_then_fp64"]["converged"]:
+            opt["dE_KS_per_atom_Ha"] = abs(opt["mixed_then_fp64"]["E_KS_Ha"] - opt["fp64"]["E_KS_Ha"]) / nat
+        ctx.set_precision(modes[head])
+
     compare = {}
     if "fp64" in res and head != "fp64":
         F = res["fp64"]
@@ -335,6 +417,10 @@ def main():
             "E_KS_fp64_Ha": F["start"]["etot"],
             "dE_terms_per_atom_Ha": {k: (R["start"][k] - F["start"][k]) / nat for k in TERMS},
         }
+        if R["energy"] and F["energy"]:
+            compare["E_ham_drift_per_atom_Ha"] = R["energy"]["E_ham_drift_per_atom_Ha"]
+            compare["E_ham_drift_per_atom_Ha_fp64"] = F["energy"]["E_ham_drift_per_atom_Ha"]
+            compare["dE_ham_end_per_atom_Ha"] = abs(R["energy"]["E_ham_end_Ha"] - F["energy"]["E_ham_end_Ha"]) / nat
         for p in paths:
             if p not in ("fp64", head):
                 compare[f"dE_KS_per_atom_Ha_{p}"] = abs(res[p]["start"]["etot"] - F["start"]["etot"]) / nat
@@ -358,8 +444,7 @@ def main():
                         f"{sysm.gset.nhg} density G, {nat} atoms, a={cfg['a']} bohr, ecut {cfg['ecut']} Ry",
             "step": step_desc,
             "step_kind": args.step,
-            "start": (f"{args.prep_steps} FP64 steepest-descent steps from the seeded coefficients"
-                      if args.step == "cp" else "the seeded coefficients"),
+            "start": start_desc,
             "dt": dt,
             "emass": args.emass,
             "precision_path": head,
@@ -383,11 +468,13 @@ def main():
         "mfma_roofline": mfma_roofline(head, R),
         "paths": {p: {"ms_per_step": round(res[p]["ms"], 4), "E_KS_start_Ha": res[p]["start"]["etot"],
                       "E_KS_end_fp64_Ha": res[p]["end"]["etot"], "orthonormality_err": res[p]["orth"],
-                      "constraint_iters": res[p]["iters"],
+                      "constraint_iters": res[p]["iters"], "cp_energy": res[p]["energy"],
                       "mfma_roofline": mfma_roofline(p, res[p]),
                       "kernels": res[p]["kernels"]} for p in paths},
         "cpu_baseline": cpu,
     }
+    if opt is not None:
+        out["optimize"] = opt
     print(json.dumps(out), flush=True)
 
 

@@ -93,8 +93,8 @@ class KSDevice:
         return out
 
     def optimize(self, d_c0: ops.DeviceArray, f: np.ndarray, d_c2: ops.DeviceArray, tol: float = 1e-5,
-                 switch: float = 1e-4, mixed: bool = True, dt: float = 20.0, emass: float = 400.0,
-                 hthrs: float = 0.5, maxiter: int = 2000) -> dict:
+                 switch: float = 1e-4, mixed: bool = True, dt: float = 10.0, emass: float = 400.0,
+                 hthrs: float = 2.0, maxiter: int = 4000, callback=None) -> dict:
         """Wavefunction optimisation at fixed ions: rwfopt -> updwf with the
         preconditioned steepest descent (src/updwf_utils.mod.F90:64-160).  Each
         iteration runs the forces driver (E_KS and c2 = -f H c), the Lagrange
@@ -107,7 +107,15 @@ class KSDevice:
         gemax < tol (the convergence test of tol_chk_cnvgrad; 1e-5 is CPMD's
         default, src/control_test_utils.mod.F90:595-601).  mixed=False is
         the plain FP64 optimisation.  The returned etot is the energy of the
-        final (converged) coefficients, evaluated in FP64."""
+        final (converged) coefficients, evaluated in FP64.
+
+        dt and hthrs are CPMD's TIMESTEP and HTHRS inputs.  The step on a
+        component with preconditioned curvature 1 is f pf1 (1 for f = 2,
+        dt = 10, hthrs = 2: Newton-like at large G); below the clamp it is
+        f dt^2 / emass * curvature, which bounds dt by the charge-sloshing
+        (Hartree) curvature at the smallest G: dt = 20, hthrs = 0.5 diverges
+        for a 24 bohr cell where dt = 10, hthrs = 2 converges.  callback, if
+        given, is called after each gradient as callback(it, mode, etot, gemax)."""
         ctx = self.ctx
         f = np.ascontiguousarray(f, dtype=np.float64)
         n = len(f)
@@ -125,6 +133,8 @@ class KSDevice:
             gemax, cnorm = ctx.csize_dev(d_c2, f, n)
             hist.append((mode, e["etot"], gemax, cnorm))
             iters[mode] += 1
+            if callback is not None:
+                callback(len(hist) - 1, mode, e["etot"], gemax)
             if not np.isfinite(gemax):
                 break
             if mode == "fp64" and gemax < tol:
