@@ -141,6 +141,8 @@ def main():
                     help="also optimise the wavefunctions from the seeded start (preconditioned steepest "
                          "descent) mixed-then-FP64 and FP64 only, and report iterations, times and energies")
     ap.add_argument("--opt-tol", type=float, default=1e-5)
+    ap.add_argument("--opt-method", choices=("sd", "diis"), default="sd",
+                    help="optimiser of --prep opt and --opt-policy (diis: CPMD's ODIIS, hthrs 0.5)")
     ap.add_argument("--opt-dt", type=float, default=10.0,
                     help="optimisation time step (CPMD TIMESTEP; hthrs 2); its bound from the Hartree "
                          "curvature at the smallest G falls as 1/a: 10 for a = 24 bohr, 7 for a = 32")
@@ -225,8 +227,8 @@ def main():
         ks.set_ions(sysm.tau)
         ctx.upload(c0, buf["c0"])
         t0 = time.perf_counter()
-        r = ks.optimize(buf["c0"], f, buf["c2"], tol=args.prep_tol, dt=args.opt_dt, hthrs=2.0, emass=args.emass,
-                        maxiter=args.opt_maxiter, callback=opt_progress)
+        r = ks.optimize(buf["c0"], f, buf["c2"], tol=args.prep_tol, dt=args.opt_dt, emass=args.emass,
+                        maxiter=args.opt_maxiter, callback=opt_progress, method=args.opt_method)
         if not r["converged"]:
             log(f"[bench] the start optimisation did not reach gemax < {args.prep_tol} (gemax {r['gemax']:.2e})")
             sys.exit(1)
@@ -377,7 +379,8 @@ def main():
     if args.opt_policy:
         # the precision policy of the wavefunction optimisation: FP32 transforms /
         # split-bf16 contractions while gemax >= 1e-4, FP64 after
-        opt = {"tol": args.opt_tol, "switch": 1e-4, "dt": args.opt_dt, "emass": args.emass, "hthrs": 2.0,
+        opt = {"tol": args.opt_tol, "switch": 1e-4, "dt": args.opt_dt, "emass": args.emass,
+               "method": args.opt_method, "hthrs": 2.0 if args.opt_method == "sd" else 0.5,
                "start": "the CP start before optimisation (the steepest-descent prepared state)"}
         ks.set_ions(sysm.tau)
         for label, mixed in (("mixed_then_fp64", True), ("fp64", False)):
@@ -385,7 +388,8 @@ def main():
             ctx.synchronize()
             t0 = time.perf_counter()
             r = ks.optimize(buf["c0"], f, buf["c2"], tol=args.opt_tol, switch=1e-4, mixed=mixed, dt=args.opt_dt,
-                            hthrs=2.0, emass=args.emass, maxiter=args.opt_maxiter, callback=opt_progress)
+                            emass=args.emass, maxiter=args.opt_maxiter, callback=opt_progress,
+                            method=args.opt_method)
             ctx.synchronize()
             el = time.perf_counter() - t0
             h = r["history"]

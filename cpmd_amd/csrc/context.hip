@@ -489,6 +489,139 @@ __global__ void k_sd_update(double2* __restrict__ c0, const double2* __restrict_
   c0[o] = a;
 }
 
+// odiis (src/odiis_utils.mod.F90:492-740) kernels.  Ring slot w of pme/gde
+// holds the occupied coefficients and the negated gradient of one iteration.
+__global__ void k_odiis_store(const double2* __restrict__ c0, const double2* __restrict__ c2,
+                              double2* __restrict__ pme, double2* __restrict__ gde, int ldg) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ldg) return;
+  const size_t o = (size_t)ldg * blockIdx.y + j;
+  const double2 g = c2[o];
+  pme[o] = c0[o];
+  gde[o] = make_double2(-g.x, -g.y);
+}
+
+// part[i nocc + k] = dotp(gde_i(:, k), 2 ff_k^2 v^2 gde_w(:, k)), v = vpp / (1 + vpp gamma);
+// one workgroup per (state, slot), fixed order as k_dotp_cols
+__global__ __launch_bounds__(256) void k_odiis_bdot(const double2* __restrict__ gde, size_t slot, int w,
+                                                    const double* __restrict__ vpp, double gamma,
+                                                    const double* __restrict__ ff, int ngw, int ldg, int nocc,
+                                                    int geq0, double* __restrict__ part) {
+  const int k = blockIdx.x, i = blockIdx.y;
+  const double2* a = gde + slot * i + (size_t)ldg * k;
+  const double2* b = gde + slot * w + (size_t)ldg * k;
+  const double s2 = 2.0 * ff[k] * ff[k];
+  double s = 0.0;
+  for (int g = threadIdx.x + (geq0 ? 1 : 0); g < ngw; g += 256) {
+    const double v = gamma != 0.0 ? vpp[g] / (1.0 + vpp[g] * gamma) : vpp[g];
+    const double wgt = s2 * v * v;
+    s += a[g].x * (b[g].x * wgt) + a[g].y * (b[g].y * wgt);
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  __shared__ double pt[4];
+  if ((threadIdx.x & 63) == 0) pt[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 2.0 * (pt[0] + pt[1] + pt[2] + pt[3]);
+    if (geq0) {
+      const double v = gamma != 0.0 ? vpp[0] / (1.0 + vpp[0] * gamma) : vpp[0];
+      t += a[0].x * (b[0].x * (s2 * v * v));
+    }
+    part[(size_t)i * nocc + k] = t;
+  }
+}
+
+struct DiisCoef {
+  double vc[20];
+  int nv;
+};
+
+// c0(:, k < nocc) = sum_i vc_i pme_i - sum_i vc_i ff_k v gde_i;  c0(:, k >= nocc) += svar2 c2
+__global__ void k_odiis_combine(double2* __restrict__ c0, const double2* __restrict__ c2,
+                                const double2* __restrict__ pme, const double2* __restrict__ gde, size_t slot,
+                                const double* __restrict__ vpp, double gamma, const double* __restrict__ ff,
+                                DiisCoef dc, int ldg, int nocc, double svar2) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ldg) return;
+  const int k = blockIdx.y;
+  const size_t o = (size_t)ldg * k + j;
+  if (k >= nocc) {
+    const double2 g = c2[o];
+    double2 a = c0[o];
+    a.x += svar2 * g.x;
+    a.y += svar2 * g.y;
+    c0[o] = a;
+    return;
+  }
+  const double v = gamma != 0.0 ? vpp[j] / (1.0 + vpp[j] * gamma) : vpp[j];
+  double2 s = make_double2(0.0, 0.0);
+  for (int i = 0; i < dc.nv; ++i) {
+    const double2 p = pme[slot * i + o];
+    s.x += dc.vc[i] * p.x;
+    s.y += dc.vc[i] * p.y;
+  }
+  for (int i = 0; i < dc.nv; ++i) {
+    const double2 g = gde[slot * i + o];
+    const double cw = dc.vc[i] * ff[k] * v;
+    s.x -= cw * g.x;
+    s.y -= cw * g.y;
+  }
+  c0[o] = s;
+}
+
+// Minimum-norm least-squares solution of the symmetric system b x = v (n <= 21)
+// through a cyclic Jacobi eigendecomposition, dropping eigenvalues below
+// rcond max|lambda|: the result of LAPACK dgelss for a symmetric matrix.
+static void sym_lstsq(std::vector<double> b, int n, std::vector<double>& v, double rcond) {
+  std::vector<double> z((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) z[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int p = 0; p < n; ++p)
+      for (int q = 0; q < n; ++q) {
+        tot += b[(size_t)p * n + q] * b[(size_t)p * n + q];
+        if (p != q) off += b[(size_t)p * n + q] * b[(size_t)p * n + q];
+      }
+    if (off <= 1e-30 * tot) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = b[(size_t)p * n + q];
+        if (apq == 0.0) continue;
+        const double app = b[(size_t)p * n + p], aqq = b[(size_t)q * n + q];
+        const double th = 0.5 * (aqq - app) / apq;
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int r = 0; r < n; ++r) {   // columns p, q
+          const double brp = b[(size_t)r * n + p], brq = b[(size_t)r * n + q];
+          b[(size_t)r * n + p] = c * brp - sn * brq;
+          b[(size_t)r * n + q] = sn * brp + c * brq;
+        }
+        for (int r = 0; r < n; ++r) {   // rows p, q
+          const double bpr = b[(size_t)p * n + r], bqr = b[(size_t)q * n + r];
+          b[(size_t)p * n + r] = c * bpr - sn * bqr;
+          b[(size_t)q * n + r] = sn * bpr + c * bqr;
+        }
+        for (int r = 0; r < n; ++r) {
+          const double zrp = z[(size_t)r * n + p], zrq = z[(size_t)r * n + q];
+          z[(size_t)r * n + p] = c * zrp - sn * zrq;
+          z[(size_t)r * n + q] = sn * zrp + c * zrq;
+        }
+      }
+  }
+  double lmax = 0.0;
+  for (int i = 0; i < n; ++i) lmax = std::max(lmax, std::fabs(b[(size_t)i * n + i]));
+  std::vector<double> x(n, 0.0);
+  for (int e = 0; e < n; ++e) {
+    const double l = b[(size_t)e * n + e];
+    if (std::fabs(l) <= rcond * lmax || l == 0.0) continue;
+    double d = 0.0;
+    for (int r = 0; r < n; ++r) d += z[(size_t)r * n + e] * v[r];
+    d /= l;
+    for (int r = 0; r < n; ++r) x[r] += d * z[(size_t)r * n + e];
+  }
+  v = x;
+}
+
 }  // namespace mc
 
 // ===========================================================================
@@ -726,9 +859,11 @@ int mc_hesele(mc_ctx* ctx, int prec, double svar2, double vploc, double hthrs) {
                      nsp ? P.d_twnl.as<double>() : nullptr, ctx->ldg, ctx->ngw, nhxs, nsp, ctx->w_hw.as<double>(),
                      vploc, 0.5 * ctx->tpiba2, hthrs, prec, svar2, ctx->d_vpp.as<double>());
   MC_HIP(hipGetLastError());
-  // hw is a host temporary
+  // hw is a host temporary; vpp(G=0) for odiis
+  MC_HIP(hipMemcpyAsync(&ctx->vpp0, ctx->d_vpp.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   MC_HIP(hipStreamSynchronize(ctx->stream));
   ctx->vpp_ready = true;
+  ctx->vpp_prec = prec != 0;
   MC_API_END
 }
 
@@ -771,6 +906,88 @@ int mc_sd_update_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, int nstate, 
                      (double2*)c0_dev, (const double2*)c2_dev, ctx->d_vpp.as<double>(), pf1, ctx->ldg);
   MC_HIP(hipGetLastError());
   mc::timer_end(ctx, MC_KID_MISC);
+  MC_API_END
+}
+
+int mc_odiis_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, const double* f, int nstate, double etot,
+                 double svar2, int mdiis, int* reinit) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c0_dev && c2_dev && f && reinit && nstate > 0, MC_ERR_ARG, "bad argument");
+  MC_CHECK(mdiis >= 1 && mdiis <= 20, MC_ERR_ARG, "odiis: mdiis must be in [1, 20]");
+  MC_CHECK(nstate <= 65535, MC_ERR_ARG, "odiis: nstate > 65535");
+  MC_CHECK(ctx->vpp_ready, MC_ERR_ARG, "odiis: call mc_hesele first");
+  auto& D = ctx->odiis;
+  const size_t slot = (size_t)ctx->ldg * nstate;   // one ring slot (occupied columns used)
+  if (*reinit || D.istate == 0 || D.mdiis != mdiis || D.nstate != nstate) {
+    D.istate = 0;
+    D.eold = 9999.0;
+    D.gamma = 0.0;
+    *reinit = 0;
+    D.nocc = 0;
+    for (int i = 0; i < nstate; ++i)
+      if (f[i] > 1e-5) ++D.nocc;
+    D.ndiis = 0;
+    D.mdiis = mdiis;
+    D.nstate = nstate;
+    D.pme.ensure(sizeof(double2) * slot * mdiis);
+    D.gde.ensure(sizeof(double2) * slot * mdiis);
+    std::vector<double> ff(nstate, 1.0);
+    for (int k = 0; k < nstate; ++k)
+      if (ctx->vpp_prec && f[k] > 0.1) ff[k] = 1.0 / f[k];
+    D.d_ff.ensure(sizeof(double) * nstate);
+    MC_HIP(hipMemcpy(D.d_ff.p, ff.data(), sizeof(double) * nstate, hipMemcpyHostToDevice));
+  } else {
+    // the damping of the low-G steps grows while the energy rises (:541-551)
+    if (ctx->geq0) D.gamma = etot - D.eold > 0.0 ? D.gamma + 1.0 / ctx->vpp0 : 0.75 * D.gamma;
+  }
+  const int nocc = D.nocc;
+  if (D.istate == 0) D.ndiis = D.nowv = D.nsize = 0;   // updis(.., 0)
+  D.istate = 2;
+  D.nowv = D.ndiis % mdiis + 1;                         // updis(.., 1)
+  D.ndiis += 1;
+  D.nsize = std::min(D.ndiis + 1, mdiis + 1);
+  const int w = D.nowv - 1, nv = D.nsize - 1;
+  if (nocc > 0) {
+    hipLaunchKernelGGL(mc::k_odiis_store, dim3((ctx->ldg + 255) / 256, nocc), dim3(256), 0, ctx->stream,
+                       (const double2*)c0_dev, (const double2*)c2_dev, D.pme.as<double2>() + slot * w,
+                       D.gde.as<double2>() + slot * w, ctx->ldg);
+    MC_HIP(hipGetLastError());
+    D.w_part.ensure(sizeof(double) * (size_t)nv * nocc);
+    hipLaunchKernelGGL(mc::k_odiis_bdot, dim3(nocc, nv), dim3(256), 0, ctx->stream, D.gde.as<double2>(), slot, w,
+                       ctx->d_vpp.as<double>(), D.gamma, D.d_ff.as<double>(), ctx->ngw, ctx->ldg, nocc,
+                       ctx->geq0, D.w_part.as<double>());
+    MC_HIP(hipGetLastError());
+    std::vector<double> part((size_t)nv * nocc);
+    mc::dl_add(ctx, part.data(), D.w_part.p, sizeof(double) * part.size());
+    mc::dl_finish(ctx);
+    for (int i = 0; i < nv; ++i) {
+      double t = 0.0;
+      for (int k = 0; k < nocc; ++k) t += part[(size_t)i * nocc + k];
+      D.diism[i + 20 * w] = t;
+      D.diism[w + 20 * i] = t;
+    }
+  }
+  // the DIIS equations with the Lagrange row (:612-631), solved like solve/dgelss
+  const int ns = D.nsize;
+  std::vector<double> bc((size_t)ns * ns, 0.0), vc(ns, 0.0);
+  for (int i = 0; i < nv; ++i)
+    for (int j = 0; j < nv; ++j) bc[(size_t)i * ns + j] = D.diism[i + 20 * j];
+  for (int i = 0; i < nv; ++i) {
+    bc[(size_t)i * ns + nv] = -1.0;
+    bc[(size_t)nv * ns + i] = -1.0;
+  }
+  vc[nv] = -1.0;
+  mc::sym_lstsq(bc, ns, vc, 0.2e-15);
+  mc::DiisCoef dc{};
+  dc.nv = nv;
+  for (int i = 0; i < nv; ++i) dc.vc[i] = vc[i];
+  mc::timer_begin(ctx, MC_KID_MISC);
+  hipLaunchKernelGGL(mc::k_odiis_combine, dim3((ctx->ldg + 255) / 256, nstate), dim3(256), 0, ctx->stream,
+                     (double2*)c0_dev, (const double2*)c2_dev, D.pme.as<double2>(), D.gde.as<double2>(), slot,
+                     ctx->d_vpp.as<double>(), D.gamma, D.d_ff.as<double>(), dc, ctx->ldg, nocc, svar2);
+  MC_HIP(hipGetLastError());
+  mc::timer_end(ctx, MC_KID_MISC);
+  D.eold = etot;
   MC_API_END
 }
 

@@ -220,6 +220,15 @@ struct mc_ctx {
   // preconditioner of the electronic gradient (mc_hesele), device rows
   mc::DevBuf d_vpp, w_hw;
   bool vpp_ready = false;
+  double vpp0 = 0.0;   // vpp at G = 0 (odiis' damping step)
+  bool vpp_prec = false;
+  // ODIIS history (mc_odiis_dev): ring of mdiis coefficient / gradient sets
+  struct Odiis {
+    int istate = 0, ndiis = 0, nowv = 0, nsize = 0, nocc = 0, mdiis = 0, nstate = 0;
+    double eold = 0.0, gamma = 0.0;
+    double diism[20 * 20] = {};
+    mc::DevBuf pme, gde, d_ff, w_part;
+  } odiis;
   mc::Projectors proj;
 
   bool fft_fp32 = false;   // rhoofr/vpsi transforms and workspaces in FP32

@@ -94,12 +94,19 @@ class KSDevice:
 
     def optimize(self, d_c0: ops.DeviceArray, f: np.ndarray, d_c2: ops.DeviceArray, tol: float = 1e-5,
                  switch: float = 1e-4, mixed: bool = True, dt: float = 10.0, emass: float = 400.0,
-                 hthrs: float = 2.0, maxiter: int = 4000, callback=None) -> dict:
-        """Wavefunction optimisation at fixed ions: rwfopt -> updwf with the
-        preconditioned steepest descent (src/updwf_utils.mod.F90:64-160).  Each
-        iteration runs the forces driver (E_KS and c2 = -f H c), the Lagrange
-        projection of c2 (lproj), csize, c0 += vpp pf1 c2 (hesele's vpp,
-        pf1 = dt^2 hthrs / emass) and Gram-Schmidt (ortho -> rgs).
+                 hthrs: float | None = None, maxiter: int = 4000, callback=None, method: str = "sd",
+                 mdiis: int = 10) -> dict:
+        """Wavefunction optimisation at fixed ions: rwfopt -> updwf
+        (src/updwf_utils.mod.F90:64-160).  Each iteration runs the forces
+        driver (E_KS and c2 = -f H c), the Lagrange projection of c2 (lproj),
+        csize, the update and Gram-Schmidt (ortho -> rgs).  The update is
+        method "sd", the preconditioned steepest descent c0 += vpp pf1 c2
+        (hesele's vpp, pf1 = dt^2 hthrs / emass), or "diis", CPMD's default
+        ODIIS over mdiis iterations (mc_odiis_dev, hesele's vpp, empty
+        states by dt^2 / emass).  hthrs defaults to 2 for "sd" and to CPMD's
+        0.5 for "diis".  The DIIS history is kept across the precision switch
+        (its mixed-precision gradients are accurate far below the switch
+        threshold).
 
         Precision policy: with mixed, the transforms run in FP32 and the
         contractions in split bf16 while the gradient size gemax is >= switch;
@@ -119,8 +126,13 @@ class KSDevice:
         ctx = self.ctx
         f = np.ascontiguousarray(f, dtype=np.float64)
         n = len(f)
+        if method not in ("sd", "diis"):
+            raise ValueError("method must be 'sd' or 'diis'")
+        if hthrs is None:
+            hthrs = 2.0 if method == "sd" else 0.5      # 0.5: CPMD's default HTHRS
         ctx.hesele(True, 0.0, self.vploc, hthrs)
         pf1 = dt * dt * hthrs / emass
+        reinit = True
         mode = "mixed" if mixed and switch > tol else "fp64"
         ctx.set_precision(mode)
         hist = []
@@ -140,7 +152,11 @@ class KSDevice:
             if mode == "fp64" and gemax < tol:
                 converged = True
                 break
-            ctx.sd_update_dev(d_c0, d_c2, n, pf1)
+            if method == "sd":
+                ctx.sd_update_dev(d_c0, d_c2, n, pf1)
+            else:
+                ctx.odiis_dev(d_c0, d_c2, f, n, e["etot"], dt * dt / emass, mdiis=mdiis, reinit=reinit)
+                reinit = False
             ctx.rgs_dev(d_c0, n)
             if mode == "mixed" and gemax < switch:
                 mode = "fp64"

@@ -173,6 +173,13 @@ class Context:
         """c0 += vpp pf1 c2 (preconditioned steepest descent)."""
         check(lib().mc_sd_update_dev(self.h, c0.ptr, c2.ptr, nstate, pf1))
 
+    def odiis_dev(self, c0: DeviceArray, c2: DeviceArray, f: np.ndarray, nstate: int, etot: float, svar2: float,
+                  mdiis: int = 10, reinit: bool = False):
+        """One DIIS step of the occupied states (CPMD's odiis); the history stays in the context."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        r = C.c_int(int(reinit))
+        check(lib().mc_odiis_dev(self.h, c0.ptr, c2.ptr, _p(f), nstate, etot, svar2, mdiis, C.byref(r)))
+
     def dotp_wfn(self, a: DeviceArray, b: DeviceArray, nstate: int) -> np.ndarray:
         out = np.zeros(nstate)
         check(lib().mc_wfn_dotp(self.h, a.ptr, b.ptr, nstate, _p(out)))

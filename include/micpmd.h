@@ -181,6 +181,21 @@ int mc_csize_dev(mc_ctx* ctx, const void* c2_dev, const double* f, int nstate, d
  * pf1 = 1 give CPMD's daxpy(dt2bye).  Replaces the steepest-descent branch
  * of updwf, src/updwf_utils.mod.F90:117-128. */
 int mc_sd_update_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, int nstate, double pf1);
+/* odiis: one DIIS step on the occupied states (f > 1e-5, the first nocc)
+ * with the preconditioner of mc_hesele.  The context keeps the history of
+ * mdiis (1..20) coefficient and gradient sets (2 mdiis nstate ldg complex
+ * doubles of device memory).  c0 <- sum_i vc_i (c0_i + ff vpp' c2_i) with vc
+ * from the DIIS equations over <c2_i| 2 ff^2 vpp'^2 |c2_j> (solved as LAPACK
+ * dgelss, rcond 0.2e-15), ff = 1/f with prec (f > 0.1), vpp' = vpp / (1 +
+ * vpp gamma), gamma += 1/vpp(G=0) when etot rose since the last call, else
+ * gamma *= 0.75; empty states: c0 += svar2 c2.  *reinit != 0 (or a new
+ * nstate / mdiis) restarts the history; it is cleared on return.  The
+ * orthonormalisation (rgs) follows as in updwf.  The progress reset of
+ * nreset >= 3 is not offered (CPMD's default nreset is 0).
+ * Replaces odiis(c0,c2,vpp,nstate,pme,gde,svar2,reinit),
+ * src/odiis_utils.mod.F90:492-740 (closed shell, uncompressed history). */
+int mc_odiis_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, const double* f, int nstate, double etot,
+                 double svar2, int mdiis, int* reinit);
 
 /* out[i] = dotp(a_i, b_i), the Gamma-point inner product of state i
  * (src/dotp_utils.mod.F90:58-101); with b = c2 after vpsi + fnonloc,

@@ -244,6 +244,78 @@ def sd_update(c0, c2, vpp, pf1):
     return out
 
 
+def _dotp_cols(a, b, geq0=True):
+    """dotp per column (src/dotp_utils.mod.F90:58-101)."""
+    s = 2.0 * (a.real * b.real + a.imag * b.imag).sum(axis=0)
+    if geq0:
+        s -= a[0].real * b[0].real
+    return s
+class ODIIS:
+    """odiis, src/odiis_utils.mod.F90:492-740 (closed shell, uncompressed
+    history, no progress reset): DIIS over the occupied states with the
+    preconditioner vpp.  step() returns the new coefficients before the
+    orthonormalisation that updwf applies after it."""
+
+    def __init__(self, mdiis=10, nreset=0, prec=True):
+        self.mdiis, self.nreset, self.prec = mdiis, nreset, prec
+        self.istate = 0
+    def _updis(self, iact):
+        # updis, src/odiis_utils.mod.F90:36-58 (1-based nowv)
+        if iact == 1:
+            self.nowv = self.ndiis % self.mdiis + 1
+            self.ndiis += 1
+            self.nsize = min(self.ndiis + 1, self.mdiis + 1)
+        else:
+            self.ndiis = self.nowv = self.nsize = 0
+    def step(self, c0, c2, vpp, f, etot, svar2, reinit=False, geq0=True):
+        n = c0.shape[1]
+        if reinit or self.istate == 0:
+            self.istate, self.eold, self.gamma = 0, 9999.0, 0.0
+            self.nocc = int(np.count_nonzero(np.asarray(f) > 1e-5))
+            self.ndiis = 0
+            m = self.mdiis
+            self.diism = np.zeros((m, m))
+            self.pme, self.gde = [None] * m, [None] * m
+        nocc = self.nocc
+        if self.istate != 0:
+            de1 = etot - self.eold
+            if geq0:      # :541-551
+                self.gamma = self.gamma + 1.0 / vpp[0] if de1 > 0.0 else 0.75 * self.gamma
+        if self.istate == 0:
+            self._updis(0)
+        self.istate = 2
+        self._updis(1)
+        w = self.nowv - 1
+        v = vpp / (1.0 + vpp * self.gamma) if self.gamma != 0.0 else vpp
+        g = -c2[:, :nocc]
+        self.pme[w] = c0[:, :nocc].copy()
+        self.gde[w] = g.copy()
+        ff = np.where(self.prec & (np.asarray(f[:nocc]) > 0.1), 1.0 / np.asarray(f[:nocc]), 1.0)
+        sg = g * (2.0 * ff * ff)[None, :] * (v * v)[:, None]
+        ns = self.nsize
+        for i in range(ns - 1):
+            self.diism[i, w] = _dotp_cols(self.gde[i], sg, geq0).sum()
+        for i in range(ns - 1):
+            self.diism[w, i] = self.diism[i, w]
+        bc = np.zeros((ns, ns))
+        bc[:ns - 1, :ns - 1] = self.diism[:ns - 1, :ns - 1]
+        bc[:ns - 1, ns - 1] = bc[ns - 1, :ns - 1] = -1.0
+        vc = np.zeros(ns); vc[ns - 1] = -1.0
+        # solve: dgelss, minimum-norm least squares, rcond 0.2e-15 (:60-89)
+        vc = np.linalg.lstsq(bc, vc, rcond=0.2e-15)[0]
+        out = c0.copy()
+        cn = np.zeros_like(c0[:, :nocc])
+        for i in range(ns - 1):
+            cn += vc[i] * self.pme[i]
+        for i in range(ns - 1):
+            cn -= vc[i] * ff[None, :] * v[:, None] * self.gde[i]
+        out[:, :nocc] = cn
+        if nocc < n:
+            out[:, nocc:] += svar2 * c2[:, nocc:]
+        self.eold = etot
+        return out
+
+
 def rortho(c0, cm, tnon=False, epsog=1e-8, maxit=30, geq0=True):
     c0 = np.asfortranarray(c0, dtype=np.complex128)
     out = _F(cm)

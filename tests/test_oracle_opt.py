@@ -109,3 +109,51 @@ def test_sd_step_descends(small):
     de = r["etot"] - np.array(e)
     assert de[0] > 0.0 and de[1] > de[0]
     assert de[1] / de[0] == pytest.approx(2.0, rel=2e-2)
+
+
+def _rand(shape, seed):
+    rng = np.random.default_rng(seed)
+    c = np.asfortranarray(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+    c[0] = c[0].real
+    return c
+
+
+def test_odiis_first_step_is_preconditioned_descent():
+    ngw, n = 30, 4
+    c0, c2 = _rand((ngw, n), 1), _rand((ngw, n), 2)
+    vpp = np.linspace(0.1, 2.0, ngw)
+    f = np.array([2.0, 2.0, 1.0, 0.0])
+    out = oracle.ODIIS(mdiis=4).step(c0, c2, vpp, f, -1.0, 0.25)
+    ff = np.array([0.5, 0.5, 1.0])
+    np.testing.assert_allclose(out[:, :3], c0[:, :3] + ff[None, :] * vpp[:, None] * c2[:, :3], rtol=1e-14)
+    np.testing.assert_allclose(out[:, 3], c0[:, 3] + 0.25 * c2[:, 3], rtol=1e-14)
+
+
+def test_odiis_cancels_collinear_gradients():
+    # two stored gradients g and 2g: the residual sum vc_i g_i vanishes for vc = (2, -1)
+    ngw, n = 30, 2
+    g = _rand((ngw, n), 3)
+    c1, c2 = _rand((ngw, n), 4), _rand((ngw, n), 5)
+    vpp = np.linspace(0.2, 1.5, ngw)
+    f = np.array([2.0, 2.0])
+    D = oracle.ODIIS(mdiis=4)
+    D.step(c1, g, vpp, f, 0.0, 0.0)
+    out = D.step(c2, 2.0 * g, vpp, f, -1.0, 0.0)       # energy fell: no damping
+    assert D.gamma == 0.0
+    s = 0.5 * vpp[:, None]
+    want = 2.0 * (c1 + s * g) - (c2 + s * 2.0 * g)
+    np.testing.assert_allclose(out, want, rtol=1e-9, atol=1e-9)
+
+
+def test_odiis_damping_follows_energy():
+    ngw, n = 20, 2
+    vpp = np.linspace(0.2, 1.5, ngw)
+    f = np.array([2.0, 2.0])
+    D = oracle.ODIIS(mdiis=3)
+    c = _rand((ngw, n), 6)
+    for k, e in enumerate([5.0, 6.0, 4.0]):
+        c = D.step(c, _rand((ngw, n), 10 + k), vpp, f, e, 0.0)
+        if k == 1:
+            assert D.gamma == pytest.approx(1.0 / vpp[0], rel=1e-15)       # energy rose
+        if k == 2:
+            assert D.gamma == pytest.approx(0.75 / vpp[0], rel=1e-15)      # energy fell
