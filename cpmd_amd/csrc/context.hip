@@ -991,6 +991,19 @@ int mc_odiis_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, const double* f,
   MC_API_END
 }
 
+int mc_odiis_release(mc_ctx* ctx) {
+  MC_API_BEGIN
+  MC_CHECK(ctx, MC_ERR_ARG, "null context");
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  auto& D = ctx->odiis;
+  D.pme.release();
+  D.gde.release();
+  D.w_part.release();
+  D.istate = 0;
+  D.mdiis = 0;
+  MC_API_END
+}
+
 int mc_wfn_dotp(mc_ctx* ctx, const void* a_dev, const void* b_dev, int nstate, double* out) {
   MC_API_BEGIN
   MC_CHECK(ctx && a_dev && b_dev && out && nstate >= 0, MC_ERR_ARG, "bad argument");

@@ -161,5 +161,7 @@ class KSDevice:
             if mode == "mixed" and gemax < switch:
                 mode = "fp64"
                 ctx.set_precision(mode)
+        if method == "diis":
+            ctx.odiis_release()      # 2 mdiis n ldg complex doubles of history
         return dict(etot=e["etot"] if e else float("nan"), terms=e, gemax=gemax, cnorm=cnorm, converged=converged,
                     iters_mixed=iters["mixed"], iters_fp64=iters["fp64"], history=hist)

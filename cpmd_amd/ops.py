@@ -180,6 +180,10 @@ class Context:
         r = C.c_int(int(reinit))
         check(lib().mc_odiis_dev(self.h, c0.ptr, c2.ptr, _p(f), nstate, etot, svar2, mdiis, C.byref(r)))
 
+    def odiis_release(self):
+        """Free the ODIIS history held by the context."""
+        check(lib().mc_odiis_release(self.h))
+
     def dotp_wfn(self, a: DeviceArray, b: DeviceArray, nstate: int) -> np.ndarray:
         out = np.zeros(nstate)
         check(lib().mc_wfn_dotp(self.h, a.ptr, b.ptr, nstate, _p(out)))

@@ -196,6 +196,8 @@ int mc_sd_update_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, int nstate, 
  * src/odiis_utils.mod.F90:492-740 (closed shell, uncompressed history). */
 int mc_odiis_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, const double* f, int nstate, double etot,
                  double svar2, int mdiis, int* reinit);
+/* Frees the ODIIS history (the next mc_odiis_dev starts a new one). */
+int mc_odiis_release(mc_ctx* ctx);
 
 /* out[i] = dotp(a_i, b_i), the Gamma-point inner product of state i
  * (src/dotp_utils.mod.F90:58-101); with b = c2 after vpsi + fnonloc,

@@ -1,5 +1,5 @@
-"""Wavefunction optimisation on the device: hesele, csize and the
-preconditioned steepest-descent update against the oracle, and the precision
+"""Wavefunction optimisation on the device: hesele, csize, the
+preconditioned steepest-descent update and ODIIS against the oracle, and the precision
 policy (FP32 transforms / split-bf16 contractions while the gradient size
 gemax >= 1e-4, FP64 after) against a plain FP64 optimisation from the same
 start: the same converged E_KS (|dE| < 1e-6 Ha/atom) at CPMD's default
