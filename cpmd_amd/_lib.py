@@ -62,7 +62,8 @@ SIGNATURES = {
     "mc_hesele": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_double]),
     "mc_csize_dev": (C.c_int, [_vp, _vp, _dp, C.c_int, _dp, _dp]),
     "mc_sd_update_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double]),
-    "mc_odiis_dev": (C.c_int, [_vp, _vp, _vp, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int)]),
+    "mc_odiis_dev": (C.c_int, [_vp, _vp, _vp, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                               C.POINTER(C.c_int)]),
     "mc_odiis_release": (C.c_int, [_vp]),
     "mc_wfn_dotp": (C.c_int, [_vp, _vp, _vp, C.c_int, _dp]),
     "mc_wfn_upload": (C.c_int, [_vp, _dp, C.c_int, C.c_int, _vp]),

@@ -424,18 +424,21 @@ __global__ void k_hesele(const double* __restrict__ hg, const double* __restrict
   vpp[j] = 1.0 / v;
 }
 
-// csize (src/csize_utils.mod.F90:100-131) per state i: out[4i] = dotp(c2_i, c2_i)
-// in the order of k_dotp_cols; out[4i+1..3] = the izamax pick over the state:
+// csize (src/csize_utils.mod.F90:100-131) per state i: out[6i] = dotp(c2_i, c2_i)
+// in the order of k_dotp_cols; out[6i+1..3] = the izamax pick over the state:
 // largest |Re| + |Im| (BLAS dcabs1), ties to the smallest CPMD index
-// i ngw + ig, then that index and the modulus of the element.
+// i ngw + ig, then that index and the modulus of the element; out[6i+4..5] =
+// max |Re| and max |Im| (grimax, src/odiis_utils.mod.F90:408-450).
 __global__ __launch_bounds__(256) void k_csize_cols(const double2* __restrict__ c2, const int* __restrict__ perm,
                                                     int ngw, int ldg, int geq0, double* __restrict__ out) {
   const int i = blockIdx.x;
   const double2* x = c2 + (size_t)ldg * i;
-  double s = 0.0, key = -1.0, idx = 0.0, mod = 0.0;
+  double s = 0.0, key = -1.0, idx = 0.0, mod = 0.0, mre = 0.0, mim = 0.0;
   for (int g = threadIdx.x; g < ngw; g += 256) {
     const double2 v = x[g];
     if (g >= geq0) s += v.x * v.x + v.y * v.y;
+    mre = fmax(mre, fabs(v.x));
+    mim = fmax(mim, fabs(v.y));
     const double k = fabs(v.x) + fabs(v.y);
     const double id = (double)i * ngw + perm[g];
     if (k > key || (k == key && id < idx)) {
@@ -446,6 +449,8 @@ __global__ __launch_bounds__(256) void k_csize_cols(const double2* __restrict__ 
   }
   for (int off = 32; off > 0; off >>= 1) {
     s += __shfl_down(s, off, 64);
+    mre = fmax(mre, __shfl_down(mre, off, 64));
+    mim = fmax(mim, __shfl_down(mim, off, 64));
     const double k = __shfl_down(key, off, 64), id = __shfl_down(idx, off, 64), m = __shfl_down(mod, off, 64);
     if (k > key || (k == key && id < idx)) {
       key = k;
@@ -453,13 +458,15 @@ __global__ __launch_bounds__(256) void k_csize_cols(const double2* __restrict__ 
       mod = m;
     }
   }
-  __shared__ double part[4][4];
+  __shared__ double part[4][6];
   if ((threadIdx.x & 63) == 0) {
     const int w = threadIdx.x >> 6;
     part[w][0] = s;
     part[w][1] = key;
     part[w][2] = idx;
     part[w][3] = mod;
+    part[w][4] = mre;
+    part[w][5] = mim;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -467,10 +474,12 @@ __global__ __launch_bounds__(256) void k_csize_cols(const double2* __restrict__ 
     int b = 0;
     for (int w = 1; w < 4; ++w)
       if (part[w][1] > part[b][1] || (part[w][1] == part[b][1] && part[w][2] < part[b][2])) b = w;
-    out[4 * i] = 2.0 * t + (geq0 ? x[0].x * x[0].x : 0.0);
-    out[4 * i + 1] = part[b][1];
-    out[4 * i + 2] = part[b][2];
-    out[4 * i + 3] = part[b][3];
+    out[6 * i] = 2.0 * t + (geq0 ? x[0].x * x[0].x : 0.0);
+    out[6 * i + 1] = part[b][1];
+    out[6 * i + 2] = part[b][2];
+    out[6 * i + 3] = part[b][3];
+    out[6 * i + 4] = fmax(fmax(part[0][4], part[1][4]), fmax(part[2][4], part[3][4]));
+    out[6 * i + 5] = fmax(fmax(part[0][5], part[1][5]), fmax(part[2][5], part[3][5]));
   }
 }
 
@@ -876,21 +885,21 @@ int mc_csize_dev(mc_ctx* ctx, const void* c2_dev, const double* f, int nstate, d
   *gemax = 0.0;
   *cnorm = 0.0;
   if (nocc == 0) return MC_OK;
-  ctx->w_red2.ensure(sizeof(double) * 4 * nocc);
+  ctx->w_red2.ensure(sizeof(double) * 6 * nocc);
   hipLaunchKernelGGL(mc::k_csize_cols, dim3(nocc), dim3(256), 0, ctx->stream, (const double2*)c2_dev,
                      ctx->d_perm.as<int>(), ctx->ngw, ctx->ldg, ctx->geq0, ctx->w_red2.as<double>());
   MC_HIP(hipGetLastError());
-  std::vector<double> r((size_t)4 * nocc);
+  std::vector<double> r((size_t)6 * nocc);
   mc::dl_add(ctx, r.data(), ctx->w_red2.p, sizeof(double) * r.size());
   mc::dl_finish(ctx);
   double s = 0.0;
   int b = 0;
   for (int i = 0; i < nocc; ++i) {
-    s += r[4 * i];
+    s += r[6 * i];
     // states in order: on equal keys the earlier state has the smaller index
-    if (r[4 * i + 1] > r[4 * b + 1]) b = i;
+    if (r[6 * i + 1] > r[6 * b + 1]) b = i;
   }
-  *gemax = r[4 * b + 3];
+  *gemax = r[6 * b + 3];
   *cnorm = std::sqrt(s / ((double)nocc * ctx->ngw));
   MC_API_END
 }
@@ -910,7 +919,7 @@ int mc_sd_update_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, int nstate, 
 }
 
 int mc_odiis_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, const double* f, int nstate, double etot,
-                 double svar2, int mdiis, int* reinit) {
+                 double svar2, int mdiis, int nreset, int* reinit) {
   MC_API_BEGIN
   MC_CHECK(ctx && c0_dev && c2_dev && f && reinit && nstate > 0, MC_ERR_ARG, "bad argument");
   MC_CHECK(mdiis >= 1 && mdiis <= 20, MC_ERR_ARG, "odiis: mdiis must be in [1, 20]");
@@ -939,6 +948,17 @@ int mc_odiis_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, const double* f,
   } else {
     // the damping of the low-G steps grows while the energy rises (:541-551)
     if (ctx->geq0) D.gamma = etot - D.eold > 0.0 ? D.gamma + 1.0 / ctx->vpp0 : 0.75 * D.gamma;
+    // insufficient progress over the history: start again (:553-569)
+    if (D.ndiis > nreset && nreset >= 3) {
+      const int nowm = D.nowv == mdiis ? 1 : D.nowv + 1;
+      const double g1 = std::sqrt(D.grmax[D.nowv - 1] * D.grmax[D.nowv - 1] + D.gimax[D.nowv - 1] * D.gimax[D.nowv - 1]);
+      const double g2 = std::sqrt(D.grmax[nowm - 1] * D.grmax[nowm - 1] + D.gimax[nowm - 1] * D.gimax[nowm - 1]);
+      const double raim = 1.0 - (double)mdiis / 100.0;
+      if (g1 / g2 > raim && D.gnorm[D.nowv - 1] / D.gnorm[nowm - 1] > raim) {
+        D.istate = 0;
+        D.gamma = 0.0;
+      }
+    }
   }
   const int nocc = D.nocc;
   if (D.istate == 0) D.ndiis = D.nowv = D.nsize = 0;   // updis(.., 0)
@@ -952,14 +972,28 @@ int mc_odiis_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, const double* f,
                        (const double2*)c0_dev, (const double2*)c2_dev, D.pme.as<double2>() + slot * w,
                        D.gde.as<double2>() + slot * w, ctx->ldg);
     MC_HIP(hipGetLastError());
-    D.w_part.ensure(sizeof(double) * (size_t)nv * nocc);
+    D.w_part.ensure(sizeof(double) * ((size_t)nv * nocc + 6 * (size_t)nocc));
+    // grimax of the new gradient (|-c2| = |c2|): max |Re|, max |Im|, gnorm
+    hipLaunchKernelGGL(mc::k_csize_cols, dim3(nocc), dim3(256), 0, ctx->stream, (const double2*)c2_dev,
+                       ctx->d_perm.as<int>(), ctx->ngw, ctx->ldg, ctx->geq0, D.w_part.as<double>() + (size_t)nv * nocc);
+    MC_HIP(hipGetLastError());
     hipLaunchKernelGGL(mc::k_odiis_bdot, dim3(nocc, nv), dim3(256), 0, ctx->stream, D.gde.as<double2>(), slot, w,
                        ctx->d_vpp.as<double>(), D.gamma, D.d_ff.as<double>(), ctx->ngw, ctx->ldg, nocc,
                        ctx->geq0, D.w_part.as<double>());
     MC_HIP(hipGetLastError());
-    std::vector<double> part((size_t)nv * nocc);
+    std::vector<double> part((size_t)nv * nocc + 6 * (size_t)nocc);
     mc::dl_add(ctx, part.data(), D.w_part.p, sizeof(double) * part.size());
     mc::dl_finish(ctx);
+    const double* gi = part.data() + (size_t)nv * nocc;
+    double sn = 0.0, mr = 0.0, mi = 0.0;
+    for (int k = 0; k < nocc; ++k) {
+      sn += gi[6 * k];
+      mr = std::max(mr, gi[6 * k + 4]);
+      mi = std::max(mi, gi[6 * k + 5]);
+    }
+    D.grmax[w] = mr;
+    D.gimax[w] = mi == 0.0 ? 2.220446049250313e-16 : mi;
+    D.gnorm[w] = std::sqrt(sn / ((double)nocc * ctx->ngw));
     for (int i = 0; i < nv; ++i) {
       double t = 0.0;
       for (int k = 0; k < nocc; ++k) t += part[(size_t)i * nocc + k];

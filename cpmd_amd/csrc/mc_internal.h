@@ -227,6 +227,7 @@ struct mc_ctx {
     int istate = 0, ndiis = 0, nowv = 0, nsize = 0, nocc = 0, mdiis = 0, nstate = 0;
     double eold = 0.0, gamma = 0.0;
     double diism[20 * 20] = {};
+    double grmax[20] = {}, gimax[20] = {}, gnorm[20] = {};   // grimax of each slot
     mc::DevBuf pme, gde, d_ff, w_part;
   } odiis;
   mc::Projectors proj;

@@ -174,11 +174,13 @@ class Context:
         check(lib().mc_sd_update_dev(self.h, c0.ptr, c2.ptr, nstate, pf1))
 
     def odiis_dev(self, c0: DeviceArray, c2: DeviceArray, f: np.ndarray, nstate: int, etot: float, svar2: float,
-                  mdiis: int = 10, reinit: bool = False):
-        """One DIIS step of the occupied states (CPMD's odiis); the history stays in the context."""
+                  mdiis: int = 10, reinit: bool = False, nreset: int | None = None):
+        """One DIIS step of the occupied states (CPMD's odiis); the history stays in the context.
+        nreset defaults to mdiis, CPMD's setting for wavefunction optimisation."""
         f = np.ascontiguousarray(f, dtype=np.float64)
         r = C.c_int(int(reinit))
-        check(lib().mc_odiis_dev(self.h, c0.ptr, c2.ptr, _p(f), nstate, etot, svar2, mdiis, C.byref(r)))
+        nr = mdiis if nreset is None else nreset
+        check(lib().mc_odiis_dev(self.h, c0.ptr, c2.ptr, _p(f), nstate, etot, svar2, mdiis, nr, C.byref(r)))
 
     def odiis_release(self):
         """Free the ODIIS history held by the context."""

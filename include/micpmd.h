@@ -189,13 +189,17 @@ int mc_sd_update_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, int nstate, 
  * dgelss, rcond 0.2e-15), ff = 1/f with prec (f > 0.1), vpp' = vpp / (1 +
  * vpp gamma), gamma += 1/vpp(G=0) when etot rose since the last call, else
  * gamma *= 0.75; empty states: c0 += svar2 c2.  *reinit != 0 (or a new
- * nstate / mdiis) restarts the history; it is cleared on return.  The
- * orthonormalisation (rgs) follows as in updwf.  The progress reset of
- * nreset >= 3 is not offered (CPMD's default nreset is 0).
+ * nstate / mdiis) restarts the history; it is cleared on return.  With
+ * nreset >= 3 the history also restarts once more than nreset steps are
+ * stored and neither the largest gradient element (grimax) nor the gradient
+ * norm fell below (1 - mdiis/100) of the oldest stored one (CPMD sets
+ * nreset = mdiis for wavefunction optimisation,
+ * src/control_test_utils.mod.F90:452-465).  The orthonormalisation (rgs)
+ * follows as in updwf.
  * Replaces odiis(c0,c2,vpp,nstate,pme,gde,svar2,reinit),
  * src/odiis_utils.mod.F90:492-740 (closed shell, uncompressed history). */
 int mc_odiis_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, const double* f, int nstate, double etot,
-                 double svar2, int mdiis, int* reinit);
+                 double svar2, int mdiis, int nreset, int* reinit);
 /* Frees the ODIIS history (the next mc_odiis_dev starts a new one). */
 int mc_odiis_release(mc_ctx* ctx);
 

@@ -252,12 +252,15 @@ def _dotp_cols(a, b, geq0=True):
     return s
 class ODIIS:
     """odiis, src/odiis_utils.mod.F90:492-740 (closed shell, uncompressed
-    history, no progress reset): DIIS over the occupied states with the
+    history): DIIS over the occupied states with the
     preconditioner vpp.  step() returns the new coefficients before the
     orthonormalisation that updwf applies after it."""
 
-    def __init__(self, mdiis=10, nreset=0, prec=True):
-        self.mdiis, self.nreset, self.prec = mdiis, nreset, prec
+    def __init__(self, mdiis=10, nreset=None, prec=True):
+        # nreset: CPMD sets it to mdiis for wavefunction optimisation
+        # (src/control_test_utils.mod.F90:452-465)
+        self.mdiis, self.prec = mdiis, prec
+        self.nreset = mdiis if nreset is None else nreset
         self.istate = 0
     def _updis(self, iact):
         # updis, src/odiis_utils.mod.F90:36-58 (1-based nowv)
@@ -275,12 +278,20 @@ class ODIIS:
             self.ndiis = 0
             m = self.mdiis
             self.diism = np.zeros((m, m))
+            self.grmax, self.gimax, self.gnorm = np.zeros(m), np.zeros(m), np.zeros(m)
             self.pme, self.gde = [None] * m, [None] * m
         nocc = self.nocc
         if self.istate != 0:
             de1 = etot - self.eold
             if geq0:      # :541-551
                 self.gamma = self.gamma + 1.0 / vpp[0] if de1 > 0.0 else 0.75 * self.gamma
+            if self.ndiis > self.nreset and self.nreset >= 3:      # :553-569
+                nowm = 1 if self.nowv == self.mdiis else self.nowv + 1
+                g1 = np.sqrt(self.grmax[self.nowv - 1] ** 2 + self.gimax[self.nowv - 1] ** 2)
+                g2 = np.sqrt(self.grmax[nowm - 1] ** 2 + self.gimax[nowm - 1] ** 2)
+                raim = 1.0 - self.mdiis / 100.0
+                if g1 / g2 > raim and self.gnorm[self.nowv - 1] / self.gnorm[nowm - 1] > raim:
+                    self.istate, self.gamma = 0, 0.0
         if self.istate == 0:
             self._updis(0)
         self.istate = 2
@@ -290,6 +301,11 @@ class ODIIS:
         g = -c2[:, :nocc]
         self.pme[w] = c0[:, :nocc].copy()
         self.gde[w] = g.copy()
+        # grimax, src/odiis_utils.mod.F90:408-450
+        self.grmax[w] = np.abs(g.real).max()
+        gi = np.abs(g.imag).max()
+        self.gimax[w] = gi if gi != 0.0 else np.finfo(np.float64).eps
+        self.gnorm[w] = np.sqrt(_dotp_cols(g, g, geq0).sum() / (nocc * c0.shape[0]))
         ff = np.where(self.prec & (np.asarray(f[:nocc]) > 0.1), 1.0 / np.asarray(f[:nocc]), 1.0)
         sg = g * (2.0 * ff * ff)[None, :] * (v * v)[:, None]
         ns = self.nsize

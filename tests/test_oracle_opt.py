@@ -157,3 +157,19 @@ def test_odiis_damping_follows_energy():
             assert D.gamma == pytest.approx(1.0 / vpp[0], rel=1e-15)       # energy rose
         if k == 2:
             assert D.gamma == pytest.approx(0.75 / vpp[0], rel=1e-15)      # energy fell
+
+
+@pytest.mark.parametrize("shrink,restarted", [(1.0, True), (0.5, False)])
+def test_odiis_progress_reset(shrink, restarted):
+    # history of 4 (nreset = mdiis = 4): once more than 4 steps are stored, a
+    # gradient that has not fallen below 96% of the oldest one restarts DIIS
+    ngw, n = 20, 2
+    vpp = np.linspace(0.2, 1.5, ngw)
+    f = np.array([2.0, 2.0])
+    D = oracle.ODIIS(mdiis=4)
+    c = _rand((ngw, n), 7)
+    g = _rand((ngw, n), 8)
+    for k in range(6):
+        c = D.step(c, g * shrink ** k, vpp, f, -float(k), 0.0)
+    assert (D.ndiis == 1) == restarted
+    assert D.ndiis == (1 if restarted else 6)
