@@ -141,7 +141,7 @@ def main():
                     help="also optimise the wavefunctions from the seeded start (preconditioned steepest "
                          "descent) mixed-then-FP64 and FP64 only, and report iterations, times and energies")
     ap.add_argument("--opt-tol", type=float, default=1e-5)
-    ap.add_argument("--opt-method", choices=("sd", "diis"), default="sd",
+    ap.add_argument("--opt-method", choices=("sd", "diis"), default="diis",
                     help="optimiser of --prep opt and --opt-policy (diis: CPMD's ODIIS, hthrs 0.5)")
     ap.add_argument("--opt-dt", type=float, default=10.0,
                     help="optimisation time step (CPMD TIMESTEP; hthrs 2); its bound from the Hartree "

@@ -94,19 +94,19 @@ class KSDevice:
 
     def optimize(self, d_c0: ops.DeviceArray, f: np.ndarray, d_c2: ops.DeviceArray, tol: float = 1e-5,
                  switch: float = 1e-4, mixed: bool = True, dt: float = 10.0, emass: float = 400.0,
-                 hthrs: float | None = None, maxiter: int = 4000, callback=None, method: str = "sd",
+                 hthrs: float | None = None, maxiter: int = 4000, callback=None, method: str = "diis",
                  mdiis: int = 10) -> dict:
         """Wavefunction optimisation at fixed ions: rwfopt -> updwf
         (src/updwf_utils.mod.F90:64-160).  Each iteration runs the forces
         driver (E_KS and c2 = -f H c), the Lagrange projection of c2 (lproj),
         csize, the update and Gram-Schmidt (ortho -> rgs).  The update is
-        method "sd", the preconditioned steepest descent c0 += vpp pf1 c2
-        (hesele's vpp, pf1 = dt^2 hthrs / emass), or "diis", CPMD's default
-        ODIIS over mdiis iterations (mc_odiis_dev, hesele's vpp, empty
-        states by dt^2 / emass).  hthrs defaults to 2 for "sd" and to CPMD's
-        0.5 for "diis".  The DIIS history is kept across the precision switch
-        (its mixed-precision gradients are accurate far below the switch
-        threshold).
+        method "diis" (default), CPMD's default ODIIS over mdiis iterations
+        with its progress reset at nreset = mdiis (mc_odiis_dev; hesele's
+        vpp; empty states move by dt^2 / emass), or "sd", the preconditioned
+        steepest descent c0 += vpp pf1 c2 (pf1 = dt^2 hthrs / emass).  hthrs
+        defaults to CPMD's 0.5 for "diis" and to 2 for "sd".  The DIIS history
+        is kept across the precision switch (its mixed-precision gradients
+        are accurate far below the switch threshold).
 
         Precision policy: with mixed, the transforms run in FP32 and the
         contractions in split bf16 while the gradient size gemax is >= switch;
@@ -116,8 +116,8 @@ class KSDevice:
         the plain FP64 optimisation.  The returned etot is the energy of the
         final (converged) coefficients, evaluated in FP64.
 
-        dt and hthrs are CPMD's TIMESTEP and HTHRS inputs.  The step on a
-        component with preconditioned curvature 1 is f pf1 (1 for f = 2,
+        dt and hthrs are CPMD's TIMESTEP and HTHRS inputs.  For "sd" the step
+        on a component with preconditioned curvature 1 is f pf1 (1 for f = 2,
         dt = 10, hthrs = 2: Newton-like at large G); below the clamp it is
         f dt^2 / emass * curvature, which bounds dt by the charge-sloshing
         (Hartree) curvature at the smallest G: dt = 20, hthrs = 0.5 diverges

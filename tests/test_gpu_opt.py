@@ -74,7 +74,7 @@ def _optimize(ks, c_start, d, f, mixed, tol):
     n = len(f)
     ctx.upload(c_start, d["c0"])
     r = ks.optimize(d["c0"], f, d["c2"], tol=tol, switch=1e-4, mixed=mixed, dt=20.0, hthrs=0.5,
-                    maxiter=2500)
+                    maxiter=2500, method="sd")
     ctx.set_precision("fp64")
     c = ctx.download(d["c0"], n)
     return r, c
