@@ -32,48 +32,120 @@ namespace mc {
 // small kernels
 // ---------------------------------------------------------------------------
 
-// Cholesky S = U^T U of the upper triangle of S (n x n, column-major lds), in
-// place; lower triangle set to zero.  info = 0 or the order of the first
-// non-positive leading minor (dpotrf).
-__global__ __launch_bounds__(1024) void k_potrf_upper(double* __restrict__ S, int n, int lds, int* info) {
-  __shared__ double col[4096];   // column j of U (rows < j); n <= 4096
-  __shared__ double diag;
+// Cholesky S = U^T U of the upper triangle of S (n x n, column-major lds) in
+// place, blocked right-looking with 64-wide blocks; the lower triangle is set
+// to zero.  info (device) = 0 or the order of the first non-positive leading
+// minor (dpotrf); once set, the later launches return at once.
+constexpr int kPb = 64;
+
+// factor the diagonal block S(kb:kb+nb, kb:kb+nb) in LDS, right-looking
+__global__ __launch_bounds__(256) void k_potrf_diag(double* __restrict__ S, int lds, int kb, int nb, int* info) {
+  __shared__ double T[kPb][kPb + 1];
   __shared__ int bad;
-  if (threadIdx.x == 0) bad = 0;
-  for (int j = 0; j < n; ++j) {
-    // U(0:j, j) is final; stage it
-    for (int k = threadIdx.x; k < j; k += blockDim.x) col[k] = S[k + (size_t)lds * j];
-    __syncthreads();
-    if (threadIdx.x < 64) {
-      double s = 0.0;
-      for (int k = threadIdx.x; k < j; k += 64) s += col[k] * col[k];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-      if (threadIdx.x == 0) {
-        const double d = S[j + (size_t)lds * j] - s;
-        if (!(d > 0.0)) { bad = j + 1; diag = 1.0; }
-        else diag = sqrt(d);
-        S[j + (size_t)lds * j] = diag;
-      }
+  if (kb > 0 && *info) return;
+  const int t = threadIdx.x;
+  for (int e = t; e < nb * nb; e += 256) {
+    const int r = e % nb, c = e / nb;
+    T[r][c] = S[(kb + r) + (size_t)lds * (kb + c)];
+  }
+  if (t == 0) bad = 0;
+  __syncthreads();
+  for (int j = 0; j < nb; ++j) {
+    if (t == 0) {
+      const double d = T[j][j];
+      if (!(d > 0.0)) bad = kb + j + 1;
+      else T[j][j] = sqrt(d);
     }
     __syncthreads();
     if (bad) break;
-    const double dj = diag;
-    // row j of U: U(j, i) = (S(j,i) - sum_k U(k,j) U(k,i)) / U(j,j), i > j
-    for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) {
-      const double* ci = S + (size_t)lds * i;
-      double s = 0.0;
-      for (int k = 0; k < j; ++k) s += col[k] * ci[k];
-      S[j + (size_t)lds * i] = (ci[j] - s) / dj;
+    const double dj = T[j][j];
+    for (int c = j + 1 + t; c < nb; c += 256) T[j][c] /= dj;
+    __syncthreads();
+    const int m = nb - j - 1;   // trailing (r, c), j < r <= c < nb
+    for (int e = t; e < m * m; e += 256) {
+      const int r = j + 1 + e % m, c = j + 1 + e / m;
+      if (r <= c) T[r][c] -= T[j][r] * T[j][c];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) *info = bad;
-  __syncthreads();
-  for (size_t e = threadIdx.x; e < (size_t)n * n; e += blockDim.x) {
-    const int i = (int)(e % n), j = (int)(e / n);
-    if (i > j) S[i + (size_t)lds * j] = 0.0;
+  for (int e = t; e < nb * nb; e += 256) {
+    const int r = e % nb, c = e / nb;
+    if (r <= c) S[(kb + r) + (size_t)lds * (kb + c)] = T[r][c];
   }
+  if (t == 0 && (bad || kb == 0)) *info = bad;
+}
+
+// block row: U(kb:kb+nb, j) = U_kk^-T S(kb:kb+nb, j) for kb+nb <= j < n; one
+// thread per column, forward substitution with the column staged in LDS
+__global__ __launch_bounds__(64) void k_potrf_trsm(double* __restrict__ S, int lds, int n, int kb, int nb,
+                                                   const int* info) {
+  __shared__ double U[kPb][kPb + 1];
+  __shared__ double y[kPb][kPb + 1];
+  if (*info) return;
+  const int t = threadIdx.x;
+  for (int e = t; e < nb * nb; e += 64) {
+    const int r = e % nb, c = e / nb;
+    U[r][c] = S[(kb + r) + (size_t)lds * (kb + c)];
+  }
+  const int j = kb + nb + blockIdx.x * 64 + t;
+  if (j < n)
+    for (int r = 0; r < nb; ++r) y[r][t] = S[(kb + r) + (size_t)lds * j];
+  __syncthreads();
+  if (j >= n) return;
+  for (int r = 0; r < nb; ++r) {
+    double v = y[r][t];
+    for (int k = 0; k < r; ++k) v -= U[k][r] * y[k][t];
+    y[r][t] = v / U[r][r];
+  }
+  for (int r = 0; r < nb; ++r) S[(kb + r) + (size_t)lds * j] = y[r][t];
+}
+
+// trailing update S(i, j) -= sum_k U(k, i) U(k, j), k in the block row,
+// kb+nb <= i <= j < n, 32 x 32 tiles (upper tiles only)
+__global__ __launch_bounds__(256) void k_potrf_update(double* __restrict__ S, int lds, int n, int kb, int nb,
+                                                      const int* info) {
+  __shared__ double A[kPb][33];
+  __shared__ double B[kPb][33];
+  if (*info) return;
+  const int ti = blockIdx.x, tj = blockIdx.y;
+  if (ti > tj) return;
+  const int i0 = kb + nb + 32 * ti, j0 = kb + nb + 32 * tj;
+  const int t = threadIdx.x;
+  for (int e = t; e < nb * 32; e += 256) {
+    const int k = e % nb, c = e / nb;
+    A[k][c] = i0 + c < n ? S[(kb + k) + (size_t)lds * (i0 + c)] : 0.0;
+    B[k][c] = j0 + c < n ? S[(kb + k) + (size_t)lds * (j0 + c)] : 0.0;
+  }
+  __syncthreads();
+  for (int e = t; e < 32 * 32; e += 256) {
+    const int r = e % 32, c = e / 32;
+    const int i = i0 + r, j = j0 + c;
+    if (i >= n || j >= n || i > j) continue;
+    double s = 0.0;
+    for (int k = 0; k < nb; ++k) s += A[k][r] * B[k][c];
+    S[i + (size_t)lds * j] -= s;
+  }
+}
+
+__global__ void k_zero_lower(double* __restrict__ S, int n, int lds) {
+  const int j = blockIdx.x;
+  for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) S[i + (size_t)lds * j] = 0.0;
+}
+
+static void potrf_upper_dev(mc_ctx* ctx, double* S, int n, int lds, int* info) {
+  for (int kb = 0; kb < n; kb += kPb) {
+    const int nb = std::min(kPb, n - kb), rest = n - kb - nb;
+    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, ctx->stream, S, lds, kb, nb, info);
+    MC_HIP(hipGetLastError());
+    if (rest <= 0) break;
+    hipLaunchKernelGGL(k_potrf_trsm, dim3((rest + 63) / 64), dim3(64), 0, ctx->stream, S, lds, n, kb, nb, info);
+    MC_HIP(hipGetLastError());
+    const int nt = (rest + 31) / 32;
+    hipLaunchKernelGGL(k_potrf_update, dim3(nt, nt), dim3(256), 0, ctx->stream, S, lds, n, kb, nb, info);
+    MC_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_zero_lower, dim3(n), dim3(256), 0, ctx->stream, S, n, lds);
+  MC_HIP(hipGetLastError());
 }
 
 // X = U^{-1} (upper): one 64-thread workgroup per column j solves U x = e_j
@@ -283,8 +355,7 @@ void rgs_dev(mc_ctx* ctx, double2* c, int n, double* smat_host) {
   double* X = small(ctx, 1, n);
   ovlap_dev(ctx, n, S, n, c, c);
   ctx->w_info.ensure(sizeof(int));
-  hipLaunchKernelGGL(k_potrf_upper, dim3(1), dim3(1024), 0, ctx->stream, S, n, n, ctx->w_info.as<int>());
-  MC_HIP(hipGetLastError());
+  potrf_upper_dev(ctx, S, n, n, ctx->w_info.as<int>());
   int info = 0;
   MC_HIP(hipMemcpyAsync(&info, ctx->w_info.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   MC_HIP(hipStreamSynchronize(ctx->stream));
