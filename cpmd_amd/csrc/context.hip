@@ -944,7 +944,8 @@ int mc_odiis_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, const double* f,
     for (int k = 0; k < nstate; ++k)
       if (ctx->vpp_prec && f[k] > 0.1) ff[k] = 1.0 / f[k];
     D.d_ff.ensure(sizeof(double) * nstate);
-    MC_HIP(hipMemcpy(D.d_ff.p, ff.data(), sizeof(double) * nstate, hipMemcpyHostToDevice));
+    // stream-ordered: kernels of an earlier call may still read d_ff
+    mc::upload_async(ctx, D.d_ff.p, ff.data(), sizeof(double) * nstate);
   } else {
     // the damping of the low-G steps grows while the energy rises (:541-551)
     if (ctx->geq0) D.gamma = etot - D.eold > 0.0 ? D.gamma + 1.0 / ctx->vpp0 : 0.75 * D.gamma;
