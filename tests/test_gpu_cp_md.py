@@ -74,3 +74,56 @@ def test_cp_steps_mixed_vs_fp64(gpu):
     ctx.upload(cmx, buf["c0"])
     ov = ctx.ovlap_dev(buf["c0"], buf["c0"], n)
     assert np.abs(ov - np.eye(n)).max() < 1e-6
+
+
+def test_cp_energy_conservation_from_ground_state(gpu):
+    """CP dynamics as production runs it: ODIIS ground state (mixed then
+    FP64), then 10 CP steps at dt 4 a.u., emass 400 a.u. in each precision.
+    CPMD's constant of motion E_ham = E_KS + ionic kinetic energy + emass
+    sum_i dotp(cm_i, cm_i) (md_driver, rekine) with full-step velocities
+    stays within 1e-5 Ha/atom, and the two precisions end within 1e-6 Ha/atom."""
+    sysm = S.make_system(12.0, 8, 70.0, nstate=16, seed=5, with_density=True)
+    ks = KSDevice(sysm)
+    ctx = ks.ctx
+    f = sysm.f
+    n = len(f)
+    nat = sysm.nat
+    dt = 4.0
+    buf = {k: ctx.alloc_wfn(n) for k in ("c0", "cm", "cp", "c2")}
+    ks.set_ions(sysm.tau)
+    ctx.upload(S.initial_coefficients(sysm, seed=5), buf["c0"])
+    r = ks.optimize(buf["c0"], f, buf["c2"], tol=1e-6, method="diis", maxiter=400)
+    assert r["converged"]
+    ctx.set_precision("fp64")
+    c_gs = ctx.download(buf["c0"], n)
+
+    def e_ham(tau, vion):
+        ctx.set_precision("fp64")
+        e = ks.energy(buf["c0"], f, tau=tau, gradient=buf["c2"], forces=True)
+        ctx.lagrange_dev(buf["c0"], buf["c2"], f, n)
+        ctx.axpy_wfn(0.5 * dt / EMASS, buf["c2"], buf["cm"], n)
+        ekinc = EMASS * float(ctx.dotp_wfn(buf["cm"], buf["cm"], n).sum())
+        v = vion + 0.5 * dt / ION_MASS * e["fion"].T
+        return e["etot"] + 0.5 * ION_MASS * float(np.sum(v * v)) + ekinc
+
+    out = {}
+    for mode in ("fp64", "mixed"):
+        ctx.set_precision("fp64")
+        ctx.upload(c_gs, buf["c0"])
+        ctx.axpby_wfn(0.0, buf["c0"], 0.0, buf["cm"], n)
+        tau = sysm.tau.copy()
+        vion = np.zeros_like(tau)
+        e0 = ks.energy(buf["c0"], f, tau=tau)["etot"]
+        ctx.set_precision(mode)
+        for _ in range(10):
+            rr = ks.energy(buf["c0"], f, tau=tau, gradient=buf["c2"], forces=True)
+            vion = vion + dt / ION_MASS * rr["fion"].T
+            tau = tau + dt * vion
+            ctx.posupa_dev(buf["c0"], buf["c2"], buf["cm"], buf["cp"], n, dt / EMASS, dt, epsog=1e-8, maxit=60)
+            buf["c0"], buf["cp"] = buf["cp"], buf["c0"]
+        out[mode] = (e0, e_ham(tau, vion), float(np.abs(tau - sysm.tau).max()))
+    for mode, (e0, e1, moved) in out.items():
+        print(f"{mode}: E_ham drift {(e1 - e0) / nat:.3e} Ha/atom, ions moved up to {moved:.2e} bohr", flush=True)
+        assert moved > 1e-4                          # the ions do move
+        assert abs(e1 - e0) / nat < 1e-5
+    assert abs(out["mixed"][1] - out["fp64"][1]) / nat < 1e-6
