@@ -425,16 +425,20 @@ __global__ void k_hesele(const double* __restrict__ hg, const double* __restrict
 }
 
 // csize (src/csize_utils.mod.F90:100-131) per state i: out[6i] = dotp(c2_i, c2_i)
-// in the order of k_dotp_cols; out[6i+1..3] = the izamax pick over the state:
+// in a fixed order (lanes strided by the block, then per wave, then the waves in
+// order); out[6i+1..3] = the izamax pick over the state:
 // largest |Re| + |Im| (BLAS dcabs1), ties to the smallest CPMD index
 // i ngw + ig, then that index and the modulus of the element; out[6i+4..5] =
 // max |Re| and max |Im| (grimax, src/odiis_utils.mod.F90:408-450).
-__global__ __launch_bounds__(256) void k_csize_cols(const double2* __restrict__ c2, const int* __restrict__ perm,
-                                                    int ngw, int ldg, int geq0, double* __restrict__ out) {
+constexpr int kCsizeThreads = 1024;   // 16 waves per state: enough loads in flight per CU
+
+__global__ __launch_bounds__(kCsizeThreads) void k_csize_cols(const double2* __restrict__ c2,
+                                                              const int* __restrict__ perm, int ngw, int ldg,
+                                                              int geq0, double* __restrict__ out) {
   const int i = blockIdx.x;
   const double2* x = c2 + (size_t)ldg * i;
   double s = 0.0, key = -1.0, idx = 0.0, mod = 0.0, mre = 0.0, mim = 0.0;
-  for (int g = threadIdx.x; g < ngw; g += 256) {
+  for (int g = threadIdx.x; g < ngw; g += kCsizeThreads) {
     const double2 v = x[g];
     if (g >= geq0) s += v.x * v.x + v.y * v.y;
     mre = fmax(mre, fabs(v.x));
@@ -458,7 +462,8 @@ __global__ __launch_bounds__(256) void k_csize_cols(const double2* __restrict__ 
       mod = m;
     }
   }
-  __shared__ double part[4][6];
+  constexpr int nw = kCsizeThreads / 64;
+  __shared__ double part[nw][6];
   if ((threadIdx.x & 63) == 0) {
     const int w = threadIdx.x >> 6;
     part[w][0] = s;
@@ -470,16 +475,20 @@ __global__ __launch_bounds__(256) void k_csize_cols(const double2* __restrict__ 
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const double t = part[0][0] + part[1][0] + part[2][0] + part[3][0];
+    double t = 0.0, mr = 0.0, mi = 0.0;
     int b = 0;
-    for (int w = 1; w < 4; ++w)
+    for (int w = 0; w < nw; ++w) {
+      t += part[w][0];
+      mr = fmax(mr, part[w][4]);
+      mi = fmax(mi, part[w][5]);
       if (part[w][1] > part[b][1] || (part[w][1] == part[b][1] && part[w][2] < part[b][2])) b = w;
+    }
     out[6 * i] = 2.0 * t + (geq0 ? x[0].x * x[0].x : 0.0);
     out[6 * i + 1] = part[b][1];
     out[6 * i + 2] = part[b][2];
     out[6 * i + 3] = part[b][3];
-    out[6 * i + 4] = fmax(fmax(part[0][4], part[1][4]), fmax(part[2][4], part[3][4]));
-    out[6 * i + 5] = fmax(fmax(part[0][5], part[1][5]), fmax(part[2][5], part[3][5]));
+    out[6 * i + 4] = mr;
+    out[6 * i + 5] = mi;
   }
 }
 
@@ -886,7 +895,7 @@ int mc_csize_dev(mc_ctx* ctx, const void* c2_dev, const double* f, int nstate, d
   *cnorm = 0.0;
   if (nocc == 0) return MC_OK;
   ctx->w_red2.ensure(sizeof(double) * 6 * nocc);
-  hipLaunchKernelGGL(mc::k_csize_cols, dim3(nocc), dim3(256), 0, ctx->stream, (const double2*)c2_dev,
+  hipLaunchKernelGGL(mc::k_csize_cols, dim3(nocc), dim3(mc::kCsizeThreads), 0, ctx->stream, (const double2*)c2_dev,
                      ctx->d_perm.as<int>(), ctx->ngw, ctx->ldg, ctx->geq0, ctx->w_red2.as<double>());
   MC_HIP(hipGetLastError());
   std::vector<double> r((size_t)6 * nocc);
@@ -975,7 +984,7 @@ int mc_odiis_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, const double* f,
     MC_HIP(hipGetLastError());
     D.w_part.ensure(sizeof(double) * ((size_t)nv * nocc + 6 * (size_t)nocc));
     // grimax of the new gradient (|-c2| = |c2|): max |Re|, max |Im|, gnorm
-    hipLaunchKernelGGL(mc::k_csize_cols, dim3(nocc), dim3(256), 0, ctx->stream, (const double2*)c2_dev,
+    hipLaunchKernelGGL(mc::k_csize_cols, dim3(nocc), dim3(mc::kCsizeThreads), 0, ctx->stream, (const double2*)c2_dev,
                        ctx->d_perm.as<int>(), ctx->ngw, ctx->ldg, ctx->geq0, D.w_part.as<double>() + (size_t)nv * nocc);
     MC_HIP(hipGetLastError());
     hipLaunchKernelGGL(mc::k_odiis_bdot, dim3(nocc, nv), dim3(256), 0, ctx->stream, D.gde.as<double2>(), slot, w,
