@@ -189,14 +189,17 @@ def main():
     K = 2.0 * ngw
     n = float(nstate)
     if args.step == "cp":
-        gemm_flops = (4 * 2 * nat * n * K        # rnlsm1 + rnlsm2 (3 components)
+        tn_gemm_flops = (4 * 2 * nat * n * K      # rnlsm1 + rnlsm2 (3 components): the A^T B products
+                         + 2 * n * n * K          # rortog: <c0|cp>
+                         + n * n * K              # rortog: <cp|cp> (symmetric)
+                         + 2 * n * n * K)         # rortv: <cm|c0>
+        gemm_flops = (tn_gemm_flops
                       + 2 * K * n * nat          # fnonloc
-                      + 2 * n * n * K            # rortog: <c0|cp>
-                      + n * n * K                # rortog: <cp|cp> (symmetric)
-                      + 2 * K * n * n)           # rortog: cp += c0 X
-        tn_gemm_flops = 4 * 2 * nat * n * K + 2 * n * n * K + n * n * K   # the A^T B products
-        step_desc = ("phfac + rnlsm1 + rnlsm2 + rhoofr + vofrho(+potfor) + Ewald + vpsi + fnonloc + rnlfor "
-                     "+ velocity Verlet (ions, electrons) + rortog")
+                      + 2 * K * n * n            # rortog: cp += c0 X
+                      + 2 * K * n * n)           # rortv: cm += c0 gamy
+        step_desc = ("CPMD's velocity Verlet CP step: velupi + velupa + posupi + phfac + posupa (rortog), "
+                     "forces (rnlsm1 + rnlsm2 + rhoofr + vofrho(+potfor) + Ewald + vpsi + fnonloc + rnlfor), "
+                     "velupi + velupa + rortv + rekine")
     else:
         gemm_flops = (2 * nat * n * K + 2 * K * n * nat + 2 * n * n * K + 2 * K * n * n + n * n * K + 2 * K * n * n)
         tn_gemm_flops = 2 * nat * n * K + 2 * n * n * K + n * n * K
@@ -256,25 +259,28 @@ def main():
     modes = {"mixed": "mixed", "fp64": "fp64", "fp32fft": "fp32", "bf16x3": "mixed_bf16x3"}
 
     def run_path(path):
-        ctx.upload(c0, buf["c0"])                 # every path starts from the same coefficients
+        ctx.upload(c0, buf["c0"])                 # every path starts from the same coefficients, at rest
         ctx.axpby_wfn(0.0, buf["c0"], 0.0, buf["cm"], nstate)
-        tau = sysm.tau.copy()
-        vion = np.zeros_like(tau)
-        e0_fp64 = fp64_eval(tau)["etot"]          # the start in FP64: E_cons = E_ham at rest
-        ctx.set_precision(modes[path])
-        start = ks.energy(buf["c0"], f, tau=tau, gradient=buf["c2"], forces=True)
         iters = []
+        eham = []
+        if args.step == "cp":
+            # the start in FP64: E_ham = E_KS at rest
+            ctx.set_precision("fp64")
+            e0_fp64 = ks.cp_init(buf["c0"], buf["cm"], buf["c2"], f, sysm.tau, emass=args.emass,
+                                 ion_mass=ION_MASS)["eham"]
+            ctx.set_precision(modes[path])
+            st = ks.cp_init(buf["c0"], buf["cm"], buf["c2"], f, sysm.tau, emass=args.emass, ion_mass=ION_MASS)
+            start = st["terms"]
+        else:
+            tau = sysm.tau.copy()
+            ctx.set_precision(modes[path])
+            start = ks.energy(buf["c0"], f, tau=tau, gradient=buf["c2"], forces=True)
 
         def step_cp():
-            nonlocal tau, vion
-            r = ks.energy(buf["c0"], f, tau=tau, gradient=buf["c2"], forces=True)
-            vion = vion + dt / ION_MASS * r["fion"].T
-            tau = tau + dt * vion
-            # velupa + posupa: cm += dt/emass c2, cp = c0 + dt cm, rortog, cm = (cp - c0)/dt
-            _, it = ctx.posupa_dev(buf["c0"], buf["c2"], buf["cm"], buf["cp"], nstate, dt / args.emass, dt,
-                                   epsog=1e-8, maxit=60)
-            iters.append(it)
-            buf["c0"], buf["cp"] = buf["cp"], buf["c0"]
+            # CPMD's velocity Verlet CP step (src/mdmain_utils.mod.F90:710-966)
+            ks.cp_step(st, buf, f, dt, epsog=1e-8, maxit=60)
+            iters.append(st["iters"])
+            eham.append(st["eham"])
 
         def step_wfopt():
             ks.energy(buf["c0"], f, tau=tau, gradient=buf["c2"])
@@ -298,35 +304,36 @@ def main():
         ctx.timer_enable(False)
         ms = (t1 - t0) * 1e3 / args.steps
         kernels = {}
-        for kid in FFT_KIDS + ("misc", "gemm", "vofrho"):
+        for kid in FFT_KIDS + ("misc", "gemm", "gemm_tn", "vofrho"):
             tot, cnt = ctx.timer_get(kid)
             if cnt:
                 kernels[kid] = {"total_ms_per_step": tot / args.steps, "launches": cnt, "avg_ms": tot / cnt}
         # the state after warmup + steps, evaluated in FP64
-        end = fp64_eval(tau)
+        ctx.set_precision("fp64")
+        energy = None
+        if args.step == "cp":
+            # CPMD's constants of motion (md_driver): E_cons = E_KS + ekinp,
+            # E_ham = E_cons + ekinc, with the full-step velocities of the
+            # velocity Verlet step (ekinc from rekine after rortv), all in FP64
+            st_end = ks.cp_init(buf["c0"], buf["cm"], buf["c2"], f, st["tau"], vion=st["vion"], emass=args.emass,
+                                ion_mass=ION_MASS)
+            end = st_end["terms"]
+            energy = {"E_ham_start_Ha": e0_fp64, "E_KS_end_Ha": end["etot"], "ekinp_end_Ha": st_end["ekinp"],
+                      "ekinc_end_Ha": st_end["ekinc"], "E_cons_end_Ha": st_end["econs"],
+                      "E_ham_end_Ha": st_end["eham"],
+                      "E_ham_drift_per_atom_Ha": (st_end["eham"] - e0_fp64) / nat,
+                      "E_cons_drift_per_atom_Ha": (st_end["econs"] - e0_fp64) / nat,
+                      "E_ham_path_per_step_Ha": eham,
+                      "E_ham_max_excursion_per_atom_Ha": max(abs(e - e0_fp64) for e in eham) / nat,
+                      "steps": args.warmup + args.steps}
+            log(f"[bench] {path}: E_ham drift {energy['E_ham_drift_per_atom_Ha']:.3e} Ha/atom over "
+                f"{energy['steps']} steps (ekinc {st_end['ekinc']:.3e}, ekinp {st_end['ekinp']:.3e}), largest "
+                f"excursion {energy['E_ham_max_excursion_per_atom_Ha']:.3e} Ha/atom")
+        else:
+            end = fp64_eval(tau)
         orth = float(np.abs(ctx.ovlap_dev(buf["c0"], buf["c0"], nstate) - np.eye(nstate)).max())
         log(f"[bench] {path}: {args.steps} steps: {ms:.3f} ms/step, E_KS(start) {start['etot']:.10f}, "
             f"E_KS(end, fp64) {end['etot']:.10f}, |C^T C - I| {orth:.2e}, iters {iters[-args.steps:]}")
-        energy = None
-        if args.step == "cp":
-            # CPMD's constants of motion (md_driver): E_cons = E_KS + ekinp (ions),
-            # E_ham = E_cons + ekinc, ekinc = emass sum_i dotp(cm_i, cm_i) (rekine),
-            # with full-step velocities: the leapfrog half-step ones plus the half
-            # kick of the end forces (for the electrons the gradient after the
-            # Lagrange projection, the velocity constraint to first order)
-            ctx.lagrange_dev(buf["c0"], buf["c2"], f, nstate)
-            ctx.axpy_wfn(0.5 * dt / args.emass, buf["c2"], buf["cm"], nstate)
-            ekinc = args.emass * float(ctx.dotp_wfn(buf["cm"], buf["cm"], nstate).sum())
-            vfull = vion + 0.5 * dt / ION_MASS * end["fion"].T
-            ekinp = 0.5 * ION_MASS * float(np.sum(vfull * vfull))
-            econs = end["etot"] + ekinp
-            energy = {"E_ham_start_Ha": e0_fp64, "E_KS_end_Ha": end["etot"], "ekinp_end_Ha": ekinp,
-                      "ekinc_end_Ha": ekinc, "E_cons_end_Ha": econs, "E_ham_end_Ha": econs + ekinc,
-                      "E_ham_drift_per_atom_Ha": (econs + ekinc - e0_fp64) / nat,
-                      "E_cons_drift_per_atom_Ha": (econs - e0_fp64) / nat,
-                      "steps": args.warmup + args.steps}
-            log(f"[bench] {path}: E_ham drift {energy['E_ham_drift_per_atom_Ha']:.3e} Ha/atom over "
-                f"{energy['steps']} steps (ekinc {ekinc:.3e}, ekinp {ekinp:.3e})")
         return {"ms": ms, "kernels": kernels, "start": start, "end": end, "orth": orth,
                 "iters": iters[-args.steps:], "energy": energy}
 
@@ -334,19 +341,46 @@ def main():
     res = {p: run_path(p) for p in paths}
     head = paths[0]
     R = res[head]
-    cw = 8.0 if head == "mixed" else 16.0
+    cw = 8.0 if head in ("mixed", "fp32fft", "bf16x3") else 16.0
     kernels = R["kernels"]
-    dom = max((k for k in kernels if k in FFT_KIDS), key=lambda k: kernels[k]["total_ms_per_step"])
     lr = -(-npairs // min(args.batch_rho, npairs))
     lv = -(-npairs // min(args.batch_vpsi, npairs))
-    if dom == "xrho":
-        ppl = npairs / lr
-    elif dom in ("xvpsi", "yfwd", "zfwd"):
-        ppl = npairs / lv
-    else:   # zinv / yinv run in both rhoofr and vpsi batches
-        ppl = 2 * npairs / (lr + lv)
-    b_launch = kernel_bytes(geom, dom, 0, cw) + (kernel_bytes(geom, dom, 1, cw) - kernel_bytes(geom, dom, 0, cw)) * ppl
-    achieved = b_launch / (kernels[dom]["avg_ms"] * 1e-3) / 1e9
+
+    def fft_kernel_roofline(k):
+        if k == "xrho":
+            ppl = npairs / lr
+        elif k in ("xvpsi", "yfwd", "zfwd"):
+            ppl = npairs / lv
+        else:   # zinv / yinv run in both rhoofr and vpsi batches
+            ppl = 2 * npairs / (lr + lv)
+        b = kernel_bytes(geom, k, 0, cw) + (kernel_bytes(geom, k, 1, cw) - kernel_bytes(geom, k, 0, cw)) * ppl
+        a = b / (kernels[k]["avg_ms"] * 1e-3) / 1e9
+        return {"bound": "hbm", "kernel": k, "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(a / HBM_PEAK_GBS, 4), "bytes_per_launch": b, "avg_launch_ms": kernels[k]["avg_ms"]}
+
+    def tn_kernel_roofline():
+        """The A^T B contraction kernel (k_tn_planes in the split-bf16 paths,
+        k_gemm_tn in FP64): the algorithm's MFMA flops per step (6 bf16
+        products per FP64-equivalent product for the 3-part split, symmetric
+        overlaps counted once) over its time per step."""
+        k = kernels["gemm_tn"]
+        split = head in ("mixed", "bf16x3")
+        mult = (6.0 if head == "mixed" else 3.0) if split else 1.0
+        flops = mult * tn_gemm_flops
+        a = flops / (k["total_ms_per_step"] * 1e-3) / 1e12
+        peak = 2500.0 if split else 78.6
+        return {"bound": "mfma", "kernel": "tn_planes" if split else "gemm_tn", "achieved": round(a, 2),
+                "peak": peak, "unit": "TFLOP/s", "frac": round(a / peak, 4),
+                "flops_per_launch": flops * args.steps / k["launches"], "avg_launch_ms": k["avg_ms"],
+                "dtype": "bf16 (3-part split)" if head == "mixed" else ("bf16 (2-part split)" if split else "f64"),
+                "useful_fp64_equiv_tflops": round(tn_gemm_flops / (k["total_ms_per_step"] * 1e-3) / 1e12, 2)}
+
+    # the dominant kernel: the largest time per step among the transform
+    # kernels and the A^T B contraction kernel
+    cands = [k for k in FFT_KIDS if k in kernels] + (["gemm_tn"] if "gemm_tn" in kernels else [])
+    dom = max(cands, key=lambda k: kernels[k]["total_ms_per_step"])
+    roof = tn_kernel_roofline() if dom == "gemm_tn" else fft_kernel_roofline(dom)
+    dom_fft = max((k for k in FFT_KIDS if k in kernels), key=lambda k: kernels[k]["total_ms_per_step"])
     step_bytes = step_model_bytes(geom, nstate, cw)
     fft_ms = sum(kernels[k]["total_ms_per_step"] for k in FFT_KIDS if k in kernels)
 
@@ -355,7 +389,7 @@ def main():
         (6 products per FP64-equivalent product for the 3-part A^T B products,
         3 for the 2-part ones) against the 2.5 PF dense bf16 peak; 'useful'
         counts each product once."""
-        ms = r["kernels"].get("gemm", {}).get("total_ms_per_step", float("nan"))
+        ms = sum(r["kernels"].get(k, {}).get("total_ms_per_step", 0.0) for k in ("gemm", "gemm_tn")) or float("nan")
         useful = gemm_flops / (ms * 1e-3) / 1e12
         if path in ("mixed", "bf16x3"):
             # 3-part A^T B products issue 6 bf16 products per FP64 product, 2-part 3
@@ -368,13 +402,19 @@ def main():
         return {"bound": "mfma", "dtype": "f64", "flops_per_step": gemm_flops, "gemm_ms_per_step": ms,
                 "achieved": round(useful, 2), "peak": 78.6, "unit": "TFLOP/s", "frac": round(useful / 78.6, 4)}
 
+    # HBM bytes per launch of the dominant kernel from the rocprofv3 --pmc
+    # summary committed under profiles/ (FETCH_SIZE, WRITE_SIZE of the same
+    # command, scaled by a copy kernel); a profile constant, not measured here
     traffic = None
     tf = ROOT / "profiles" / "traffic.json"
     if tf.exists():
         try:
-            traffic = json.loads(tf.read_text()).get(f"{head}:{dom}")
+            traffic = json.loads(tf.read_text()).get(f"{head}:{roof['kernel']}")
         except Exception:
             traffic = None
+    roof["traffic"] = traffic
+    roof["traffic_source"] = ("profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE per dispatch, "
+                              "copy-kernel scaled)") if traffic is not None else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -463,17 +503,8 @@ def main():
             "batch_pairs": {"rhoofr": args.batch_rho, "vpsi": args.batch_vpsi},
         },
         **compare,
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom,
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "bytes_per_launch": b_launch,
-            "avg_launch_ms": kernels[dom]["avg_ms"],
-        },
+        "roofline": roof,
+        "fft_kernel_roofline": fft_kernel_roofline(dom_fft),
         "fft_roofline": {"model_bytes": step_bytes, "ms_per_step": round(fft_ms, 4),
                          "achieved": round(step_bytes / (fft_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(step_bytes / (fft_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},

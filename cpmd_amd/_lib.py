@@ -18,7 +18,8 @@ MC_OK = 0
 ERRORS = {1: "MC_ERR_ARG", 2: "MC_ERR_HIP", 3: "MC_ERR_NOTPOSDEF", 4: "MC_ERR_NOCONV",
           5: "MC_ERR_UNSUPPORTED", 6: "MC_ERR_INTERNAL"}
 
-KID = {"zinv": 0, "yinv": 1, "xrho": 2, "xvpsi": 3, "yfwd": 4, "zfwd": 5, "misc": 6, "gemm": 7, "vofrho": 8}
+KID = {"zinv": 0, "yinv": 1, "xrho": 2, "xvpsi": 3, "yfwd": 4, "zfwd": 5, "misc": 6, "gemm": 7, "vofrho": 8,
+       "gemm_tn": 9}
 
 
 class McError(RuntimeError):
@@ -59,6 +60,10 @@ SIGNATURES = {
     "mc_cp_velocity": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_double]),
     "mc_posupa_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _dp,
                                 C.POINTER(C.c_int)]),
+    "mc_rortv": (C.c_int, [_vp, _dp, _dp, C.c_int, _dp, C.c_int]),
+    "mc_rortv_dev": (C.c_int, [_vp, _vp, _vp, _dp, C.c_int]),
+    "mc_rekine_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_double, _dp]),
+    "mc_cp_velupa_rortv_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double, _dp, _dp]),
     "mc_hesele": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_double]),
     "mc_csize_dev": (C.c_int, [_vp, _vp, _dp, C.c_int, _dp, _dp]),
     "mc_sd_update_dev": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double]),

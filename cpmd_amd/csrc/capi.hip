@@ -161,6 +161,20 @@ int mc_rortog(mc_ctx* ctx, const double* c0, double* cm, int ldc, double* gam, i
   MC_API_END
 }
 
+int mc_rortv(mc_ctx* ctx, const double* c0, double* cm, int ldc, double* gamy, int nstate) {
+  MC_API_BEGIN
+  check_ctx(ctx);
+  MC_CHECK(c0 && cm && nstate >= 0, MC_ERR_ARG, "rortv: bad argument");
+  if (nstate == 0) return MC_OK;
+  HostWfn h0, hm;
+  double2* d0 = h0.upload(ctx, c0, ldc, nstate);
+  double2* dm = hm.upload(ctx, cm, ldc, nstate);
+  mc::rortv_dev(ctx, d0, dm, gamy, nstate);
+  mc::dl_finish(ctx);
+  mc::wfn_download(ctx, dm, nstate, cm, ldc);
+  MC_API_END
+}
+
 int mc_set_projectors(mc_ctx* ctx, int nsp, const int* na, int nhxs, const int* ngh, const int* lang,
                       const double* hmat, const double* twnl, int ldtw) {
   MC_API_BEGIN

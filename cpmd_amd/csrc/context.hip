@@ -1061,6 +1061,75 @@ int mc_wfn_dotp(mc_ctx* ctx, const void* a_dev, const void* b_dev, int nstate, d
   MC_API_END
 }
 
+}  // extern "C"
+
+namespace mc {
+// rekine (src/rekine_utils.mod.F90:34-49, no tmass): ekinc = emass sum_i
+// dotp(cm_i, cm_i); the per-state sums are queued for download (dl_finish
+// fills out[0..n) on the host)
+static void rekine_queue(mc_ctx* ctx, const double2* cm, int n, double* out) {
+  ctx->w_red2.ensure(sizeof(double) * n);
+  hipLaunchKernelGGL(k_dotp_cols, dim3(n), dim3(256), 0, ctx->stream, cm, cm, ctx->ngw, ctx->ldg, ctx->geq0,
+                     ctx->w_red2.as<double>());
+  MC_HIP(hipGetLastError());
+  dl_add(ctx, out, ctx->w_red2.p, sizeof(double) * n);
+}
+static double rekine_sum(const std::vector<double>& d, double emass) {
+  double s = 0.0;
+  for (double v : d) s += v;
+  return s * emass;
+}
+}  // namespace mc
+
+extern "C" {
+
+int mc_rekine_dev(mc_ctx* ctx, const void* cm_dev, int nstate, double emass, double* ekinc) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && cm_dev && ekinc && nstate >= 0, MC_ERR_ARG, "rekine: bad argument");
+  *ekinc = 0.0;
+  if (nstate == 0) return MC_OK;
+  std::vector<double> d(nstate);
+  mc::rekine_queue(ctx, (const double2*)cm_dev, nstate, d.data());
+  mc::dl_finish(ctx);
+  *ekinc = mc::rekine_sum(d, emass);
+  MC_API_END
+}
+
+int mc_rortv_dev(mc_ctx* ctx, const void* c0_dev, void* cm_dev, double* gamy, int nstate) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c0_dev && cm_dev && nstate >= 0, MC_ERR_ARG, "rortv: bad argument");
+  MC_CHECK(c0_dev != cm_dev, MC_ERR_ARG, "rortv: c0 and cm must not alias");
+  mc::rortv_dev(ctx, (const double2*)c0_dev, (double2*)cm_dev, gamy, nstate);
+  mc::dl_finish(ctx);
+  MC_API_END
+}
+
+int mc_cp_velupa_rortv_dev(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, void* cm_dev, int nstate,
+                           double dtm, double emass, double* gamy, double* ekinc) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c0_dev && cm_dev && nstate >= 0, MC_ERR_ARG, "velupa/rortv: bad argument");
+  MC_CHECK(c0_dev != cm_dev && c2_dev != cm_dev, MC_ERR_ARG, "velupa/rortv: cm must not alias c0 or c2");
+  if (ekinc) *ekinc = 0.0;
+  if (nstate == 0) return MC_OK;
+  const size_t n = (size_t)ctx->ldg * nstate * 2;
+  if (c2_dev) {   // velupa(c0, cm, c2, nstate, 1): cm += dt / (2 emass) c2
+    mc::timer_begin(ctx, MC_KID_MISC);
+    hipLaunchKernelGGL(mc::k_axpy, dim3(2048), dim3(256), 0, ctx->stream, dtm, (const double*)c2_dev,
+                       (double*)cm_dev, n);
+    MC_HIP(hipGetLastError());
+    mc::timer_end(ctx, MC_KID_MISC);
+  }
+  mc::rortv_dev(ctx, (const double2*)c0_dev, (double2*)cm_dev, gamy, nstate);
+  std::vector<double> d;
+  if (ekinc) {
+    d.resize(nstate);
+    mc::rekine_queue(ctx, (const double2*)cm_dev, nstate, d.data());
+  }
+  mc::dl_finish(ctx);
+  if (ekinc) *ekinc = mc::rekine_sum(d, emass);
+  MC_API_END
+}
+
 int mc_wfn_upload(mc_ctx* ctx, const double* c_host, int ldc, int nstate, void* c_dev) {
   MC_API_BEGIN
   MC_CHECK(ctx && c_host && c_dev, MC_ERR_ARG, "null argument");

@@ -386,7 +386,7 @@ void gemm_tn(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, in
   if (proj) pa = *proj;
   const int total = nsplit * tm * tn;
   const int launch = (total + kXcd - 1) / kXcd * kXcd;
-  timer_begin(ctx, MC_KID_GEMM);
+  timer_begin(ctx, MC_KID_GEMM_TN);
   if (BT == 128)
     hipLaunchKernelGGL(k_gemm_tn<128>, dim3(launch), dim3(256), 0, ctx->stream, A, lda, B, ldb, K, m, n, kchunk,
                        ctx->w_gemm.as<double>(), symmetric ? 1 : 0, pa, tm, tn, total);
@@ -394,7 +394,7 @@ void gemm_tn(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, in
     hipLaunchKernelGGL(k_gemm_tn<64>, dim3(launch), dim3(256), 0, ctx->stream, A, lda, B, ldb, K, m, n, kchunk,
                        ctx->w_gemm.as<double>(), symmetric ? 1 : 0, pa, tm, tn, total);
   MC_HIP(hipGetLastError());
-  timer_end(ctx, MC_KID_GEMM);
+  timer_end(ctx, MC_KID_GEMM_TN);
   reduce_tn(ctx, nsplit, m, n, alpha, A, lda, B, ldb, g0corr, symmetric, out, ldo);
 }
 

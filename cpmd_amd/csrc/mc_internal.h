@@ -215,7 +215,7 @@ struct mc_ctx {
   mc::DevBuf w_f;          // occupations on device
 
   mc::DevBuf w_gemm;       // split-K partials
-  mc::DevBuf w_small[8];    // n x n matrices
+  mc::DevBuf w_small[8];    // n x n matrices (7: single overlaps, rotations and rortv's gamy)
   mc::DevBuf w_red2, w_info, w_wfn, w_f2, w_dd;
   // preconditioner of the electronic gradient (mc_hesele), device rows
   mc::DevBuf d_vpp, w_hw;
@@ -339,6 +339,8 @@ void rotate_dev(mc_ctx* ctx, double a, const double2* c1, double b, double2* c2,
 void lagrange_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, int n);
 void rortog_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n, double epsog,
                 int maxit, int* iters);
+// gamy_host (n x n, may be null) is filled by a queued download: dl_finish
+void rortv_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gamy_host, int n);
 void set_projectors(mc_ctx* ctx, int nsp, const int* na, int nhxs, const int* ngh, const int* lang,
                     const double* hmat, const double* twnl, int ldtw);
 void set_eigr(mc_ctx* ctx, const double* eigr, int lde, int nat);

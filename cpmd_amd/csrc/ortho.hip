@@ -424,6 +424,7 @@ static void inv_sqrt(mc_ctx* ctx, const double* S, int n, double* Zout, int* ite
 
 void lowdin_dev(mc_ctx* ctx, double2* c, int n, int* iters) {
   if (n <= 0) return;
+  MC_CHECK(n < kX3Min, MC_ERR_UNSUPPORTED, "lowdin: nstate beyond 16383");
   double* S = small(ctx, 0, n);
   double* A = small(ctx, 1, n);
   ovlap_dev(ctx, n, S, n, c, c);
@@ -480,8 +481,37 @@ void posupa_dev(mc_ctx* ctx, const double2* c0, const double2* c2, double2* cm, 
   rortog_core(ctx, c0, cp, gam_host, 0, n, epsog, maxit, iters, cm, dt);
 }
 
+// gamy = -sym(a): gamy(i,j) = gamy(j,i) = -(a(i,j) + a(j,i))/2 (rortv's
+// symmetrisation, src/rortv_utils.mod.F90:151-158)
+__global__ void k_neg_symm(double* __restrict__ a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i > j || j >= n) return;
+  const double v = -0.5 * (a[i + (size_t)n * j] + a[j + (size_t)n * i]);
+  a[i + (size_t)n * j] = v;
+  a[j + (size_t)n * i] = v;
+}
+
+// rortv (src/rortv_utils.mod.F90:39-166, overlap constraints, no tharm /
+// tmass): gamy = ovlap(cm, c0), gamy <- -sym(gamy), cm += c0 gamy, so that
+// <c0|cm> + <cm|c0> = 0 for orthonormal c0 (the velocity constraint of the
+// velocity Verlet CP step)
+void rortv_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gamy_host, int n) {
+  if (n <= 0) return;
+  MC_CHECK(n < kX3Min, MC_ERR_UNSUPPORTED, "rortv: nstate beyond 16383");
+  PlaneScope keep(ctx);   // c0 and cm are read-only until the rotation
+  double* G = small(ctx, 7, n);
+  ovlap_dev(ctx, n, G, n, cm, c0);
+  hipLaunchKernelGGL(k_neg_symm, dim3((n + 255) / 256, n), dim3(256), 0, ctx->stream, G, n);
+  MC_HIP(hipGetLastError());
+  rotate_dev(ctx, 1.0, c0, 1.0, cm, G, n);
+  if (gamy_host) dl_add(ctx, gamy_host, G, sizeof(double) * (size_t)n * n);
+}
+
 static void rortog_core(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n,
                         double epsog, int maxit, int* iters, double2* vel, double vdt) {
+  // the n x n products below go through gemm_tn / gemm_nn, which send
+  // products with K or M >= kX3Min to the split-bf16 kernels in mixed precision
+  MC_CHECK(n < kX3Min, MC_ERR_UNSUPPORTED, "rortog: nstate beyond 16383");
   double* rho = small(ctx, 0, n);
   double* sig = small(ctx, 1, n);
   double* del = small(ctx, 2, n);

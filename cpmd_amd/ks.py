@@ -9,7 +9,8 @@ for the Gamma, closed-shell, LDA, norm-conserving path
 E_KS = ekin + ehep + epseu + exc + esr - eself + enl (ener_com%etot).  The
 wavefunctions stay on the device; only energies, fnl and forces come back.
 KSDevice.optimize is the wavefunction optimisation at fixed ions with the
-mixed-then-FP64 precision policy.
+mixed-then-FP64 precision policy; KSDevice.cp_init / cp_step are CPMD's
+velocity Verlet Car-Parrinello step (src/mdmain_utils.mod.F90:529-966).
 """
 from __future__ import annotations
 
@@ -165,3 +166,61 @@ class KSDevice:
             ctx.odiis_release()      # 2 mdiis n ldg complex doubles of history
         return dict(etot=e["etot"] if e else float("nan"), terms=e, gemax=gemax, cnorm=cnorm, converged=converged,
                     iters_mixed=iters["mixed"], iters_fp64=iters["fp64"], history=hist)
+
+    # ------------------------------------------------------------------ CP MD
+    def cp_init(self, d_c0: ops.DeviceArray, d_cm: ops.DeviceArray, d_c2: ops.DeviceArray, f: np.ndarray,
+                tau: np.ndarray, vion: np.ndarray | None = None, emass: float = 400.0,
+                ion_mass: float | np.ndarray = 1.0) -> dict:
+        """Start of a CP run (src/mdmain_utils.mod.F90:529-643): forces at
+        the start (forcedr: E_KS, c2, F_ion), the velocity constraint rortv on
+        cm and ekinc (rekine).  Returns the state that cp_step advances:
+        tau, vion (nat, 3), fion (3, nat), etot, ekinc, ekinp, eham."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        n = len(f)
+        tau = np.array(tau, dtype=np.float64)
+        vion = np.zeros_like(tau) if vion is None else np.array(vion, dtype=np.float64)
+        r = self.energy(d_c0, f, tau=tau, gradient=d_c2, forces=True)
+        ekinc, _ = self.ctx.cp_velupa_rortv_dev(d_c0, None, d_cm, n, 0.0, emass)
+        st = dict(tau=tau, vion=vion, fion=r["fion"], etot=r["etot"], terms=r, ekinc=ekinc, emass=emass,
+                  mass=np.broadcast_to(np.asarray(ion_mass, dtype=np.float64), (tau.shape[0],))[:, None].copy())
+        self._cp_constants(st)
+        return st
+
+    @staticmethod
+    def _cp_constants(st: dict):
+        # md_driver's constants of motion: econs = etot + ekinp, eham = econs + ekinc
+        st["ekinp"] = 0.5 * float(np.sum(st["mass"] * st["vion"] * st["vion"]))
+        st["econs"] = st["etot"] + st["ekinp"]
+        st["eham"] = st["econs"] + st["ekinc"]
+
+    def cp_step(self, st: dict, bufs: dict, f: np.ndarray, dt: float, epsog: float = 1e-8, maxit: int = 60) -> dict:
+        """One Car-Parrinello step in CPMD's velocity Verlet order
+        (src/mdmain_utils.mod.F90:710-966, no thermostats or constraints):
+
+            velupi (vion += dt/2M F), velupa + posupi + phfac + posupa
+            (cm += dt/2mu c2, tau += dt vion, cp = c0 + dt cm, rortog, cm),
+            forcedr (E_KS, c2, F at the new positions),
+            velupi, velupa + rortv + rekine (mc_cp_velupa_rortv_dev).
+
+        bufs holds the device arrays 'c0', 'cm', 'cp', 'c2'; 'c0' and 'cp'
+        are swapped (CPMD's final dcopy of posupa).  st is updated in place
+        and returned; st['iters'] is the rortog iteration count."""
+        ctx = self.ctx
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        n = len(f)
+        emass = st["emass"]
+        dtm = 0.5 * dt / emass
+        st["vion"] = st["vion"] + 0.5 * dt / st["mass"] * st["fion"].T          # velupi
+        st["tau"] = st["tau"] + dt * st["vion"]                                  # posupi
+        self.set_ions(st["tau"])                                                 # phfac
+        _, it = ctx.posupa_dev(bufs["c0"], bufs["c2"], bufs["cm"], bufs["cp"], n, dtm, dt, epsog=epsog,
+                               maxit=maxit)                                      # velupa + posupa
+        bufs["c0"], bufs["cp"] = bufs["cp"], bufs["c0"]
+        r = self.energy(bufs["c0"], f, tau=st["tau"], gradient=bufs["c2"], forces=True)   # forcedr
+        st["fion"], st["etot"], st["terms"] = r["fion"], r["etot"], r
+        st["vion"] = st["vion"] + 0.5 * dt / st["mass"] * st["fion"].T          # velupi
+        st["ekinc"], _ = ctx.cp_velupa_rortv_dev(bufs["c0"], bufs["c2"], bufs["cm"], n, dtm, emass)
+        st["iters"] = it
+        self._cp_constants(st)
+        return st
+

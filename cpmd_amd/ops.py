@@ -158,6 +158,36 @@ class Context:
         """cm = (cp - c0) / dt (velocities after rortog)."""
         check(lib().mc_cp_velocity(self.h, c0.ptr, cp.ptr, cm.ptr, nstate, dt))
 
+    def rortv(self, c0: np.ndarray, cm: np.ndarray, want_gam: bool = False):
+        """rortv on host arrays: returns (cm after the velocity constraint, gamy or None)."""
+        c0 = np.asfortranarray(c0, dtype=np.complex128)
+        out = np.array(cm, dtype=np.complex128, order="F", copy=True)
+        n = c0.shape[1]
+        gam = np.zeros((n, n), order="F") if want_gam else None
+        check(lib().mc_rortv(self.h, _p(c0), _p(out), c0.shape[0], _p(gam) if want_gam else None, n))
+        return out, gam
+
+    def rortv_dev(self, c0: DeviceArray, cm: DeviceArray, nstate: int, want_gam: bool = False):
+        """cm += c0 gamy with gamy = -sym(<cm|c0>) (the CP velocity constraint); returns gamy or None."""
+        gam = np.zeros((nstate, nstate), order="F") if want_gam else None
+        check(lib().mc_rortv_dev(self.h, c0.ptr, cm.ptr, _p(gam) if want_gam else None, nstate))
+        return gam
+
+    def rekine_dev(self, cm: DeviceArray, nstate: int, emass: float) -> float:
+        """ekinc = emass sum_i dotp(cm_i, cm_i)."""
+        e = C.c_double()
+        check(lib().mc_rekine_dev(self.h, cm.ptr, nstate, emass, C.byref(e)))
+        return e.value
+
+    def cp_velupa_rortv_dev(self, c0: DeviceArray, c2: DeviceArray | None, cm: DeviceArray, nstate: int,
+                            dtm: float, emass: float, want_gam: bool = False):
+        """End of the velocity Verlet CP step: cm += dtm c2, rortv, rekine.  Returns (ekinc, gamy or None)."""
+        gam = np.zeros((nstate, nstate), order="F") if want_gam else None
+        e = C.c_double()
+        check(lib().mc_cp_velupa_rortv_dev(self.h, c0.ptr, c2.ptr if c2 is not None else None, cm.ptr, nstate, dtm,
+                                           emass, _p(gam) if want_gam else None, C.byref(e)))
+        return e.value, gam
+
     def hesele(self, prec: bool, svar2: float = 0.0, vploc: float = 0.0, hthrs: float = 0.5):
         """Set the diagonal preconditioner vpp(G) of the electronic gradient."""
         check(lib().mc_hesele(self.h, int(prec), svar2, vploc, hthrs))

@@ -50,9 +50,12 @@ enum {
   MC_KID_YFWD = 4,   /* forward y pass                             */
   MC_KID_ZFWD = 5,   /* forward z pass + Gamma decode + kinetic    */
   MC_KID_MISC = 6,   /* reductions, permutes, small kernels        */
-  MC_KID_GEMM = 7,   /* MFMA contractions (FP64 or split bf16)      */
+  MC_KID_GEMM = 7,   /* MFMA contractions (FP64 or split bf16) other
+                        than the A^T B ones, bf16 plane builds       */
   MC_KID_VOFRHO = 8, /* dense density transforms, per-G, LDA, Ewald  */
-  MC_KID_COUNT = 9
+  MC_KID_GEMM_TN = 9,/* A^T B contractions (overlaps, fnl, dfnl):
+                        the split-K MFMA kernel, without reductions  */
+  MC_KID_COUNT = 10
 };
 
 typedef struct mc_ctx mc_ctx;
@@ -158,6 +161,27 @@ int mc_posupa_dev(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, void* cm_
  * cm += c0 gamx / dt (src/posupa_utils.mod.F90:150-158) when rortog returned
  * cp = c0 + dt cm + c0 gamx. */
 int mc_cp_velocity(mc_ctx* ctx, const void* c0_dev, const void* cp_dev, void* cm_dev, int nstate, double dt);
+
+/* rortv: the velocity constraint of the velocity Verlet CP step,
+ * gamy = ovlap(cm, c0), gamy(i,j) = gamy(j,i) = -(gamy(i,j) + gamy(j,i))/2,
+ * cm += c0 gamy, so that <c0|cm> + <cm|c0> = 0 for orthonormal c0; gamy
+ * (nstate x nstate, may be NULL) receives the matrix.  Synchronous.
+ * Replaces rortv(c0,cm,c2,sc0,gamy,nstate), src/rortv_utils.mod.F90:39-166
+ * (overlap constraints; cntl%nonort, tharm, tmass and tdmal off). */
+int mc_rortv(mc_ctx* ctx, const double* c0, double* cm, int ldc, double* gamy, int nstate);
+int mc_rortv_dev(mc_ctx* ctx, const void* c0_dev, void* cm_dev, double* gamy, int nstate);
+/* rekine: ekinc = emass sum_i dotp(cm_i, cm_i), the fictitious kinetic
+ * energy of the electrons.  Synchronous.
+ * Replaces rekine(cm,nstate,ekinc), src/rekine_utils.mod.F90:34-49 (tmass off). */
+int mc_rekine_dev(mc_ctx* ctx, const void* cm_dev, int nstate, double emass, double* ekinc);
+/* The end of CPMD's velocity Verlet CP step in one call: velupa (cm += dtm
+ * c2, dtm = dt/(2 emass); skipped when c2 is NULL), rortv(c0, cm) (gamy as in
+ * mc_rortv_dev, may be NULL) and rekine (ekinc, may be NULL).  With
+ * mc_posupa_dev(dtm = dt/(2 emass)) before the force evaluation this is the
+ * step of src/mdmain_utils.mod.F90:712-966: velupa + posupa(+rortog), forces,
+ * velupa + rortv, rekine.  Synchronous. */
+int mc_cp_velupa_rortv_dev(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, void* cm_dev, int nstate,
+                           double dtm, double emass, double* gamy, double* ekinc);
 
 /* Wavefunction optimisation by preconditioned steepest descent.
  * hesele: the diagonal preconditioner vpp(G), kept in the context.  With

@@ -341,6 +341,21 @@ def rortho(c0, cm, tnon=False, epsog=1e-8, maxit=30, geq0=True):
     return out, gam, it
 
 
+def rortv(c0, cm, geq0=True):
+    """rortv, src/rortv_utils.mod.F90:149-160 (overlap constraints, no tharm /
+    tmass / tdmal): gamy = ovlap(cm, c0); gamy(i,j) = gamy(j,i) =
+    -(gamy(i,j) + gamy(j,i))/2; cm += c0 gamy.  Returns (cm, gamy)."""
+    a = ovlap(cm, c0, geq0)
+    gamy = -0.5 * (a + a.T)
+    return rotate(1.0, c0, 1.0, cm, gamy), np.asfortranarray(gamy)
+
+
+def rekine(cm, emass, geq0=True):
+    """rekine, src/rekine_utils.mod.F90:46-50 (no tmass): emass sum_i dotp(cm_i, cm_i)."""
+    cm = np.asarray(cm, dtype=np.complex128)
+    return emass * float(_dotp_cols(cm, cm, geq0).sum())
+
+
 class Projectors:
     """Arguments of the projector routines (CPMD's nlps/sgpp/cppt data)."""
 

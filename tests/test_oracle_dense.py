@@ -8,6 +8,9 @@ routines (numpy/scipy LAPACK), not by restating the oracle's loops:
 * the Lagrange projection annihilates gradients in the span of C0 with a
   symmetric coefficient matrix (and keeps the unoccupied rows);
 * rortho returns a symmetric gam and an orthonormal set (the CP constraint);
+* rortv removes exactly the symmetric part of <c0|cm> (cm = c0 B + P with P
+  orthogonal to c0 becomes c0 (B - B^T)/2 + P) and rekine is emass times the
+  real-space norm (1/N^3) sum_r psi(r)^2 of the velocities;
 * rnlsm1 equals the full-sphere projection sum_G conj(beta(G)) c(G), for an s
   and a p projector;
 * fnonloc is -1/2 the gradient of E_NL = sum_i f_i fnl^T h fnl in the Gamma
@@ -141,6 +144,33 @@ def test_rortho_constraint():
     assert np.abs(res).max() < 1e-11
     # non-convergence is reported
     assert oracle.rortho(c0, cm2, epsog=1e-30, maxit=3)[2] == -1
+
+
+def test_rortv_removes_symmetric_part():
+    cell, gs, c0 = small_system("cubic", 6, seed=4)
+    rng = np.random.default_rng(9)
+    d = (rng.standard_normal(c0.shape) + 1j * rng.standard_normal(c0.shape)) * np.exp(-gs.hg[:gs.ngw, None] / 4)
+    d[0] = d[0].real
+    p = d - c0 @ S.dotp_gamma(c0, d)                   # orthogonal to span(c0)
+    b = rng.standard_normal((6, 6))
+    cm = np.asfortranarray(c0 @ b + p)
+    out, gamy = oracle.rortv(c0, cm)
+    assert np.abs(gamy + 0.5 * (b + b.T)).max() < 1e-12
+    assert np.abs(gamy - gamy.T).max() == 0.0
+    assert np.abs(out - (c0 @ (0.5 * (b - b.T)) + p)).max() < 1e-12
+    a = S.dotp_gamma(c0, out)
+    assert np.abs(a + a.T).max() < 1e-12               # <c0|cm> + <cm|c0> = 0
+    again, g2 = oracle.rortv(c0, out)                  # a constrained velocity is left alone
+    assert np.abs(g2).max() < 1e-12 and np.abs(again - out).max() < 1e-13
+
+
+def test_rekine_is_real_space_norm():
+    cell, gs, c = small_system("ortho2", 3, seed=6)
+    cm = c * (0.3 + 0.1j)
+    cm[0] = cm[0].real
+    n = cell.nr[0] * cell.nr[1] * cell.nr[2]
+    ref = sum(float((psi_direct(cell, gs.miller, cm[:, i]).real ** 2).sum()) / n for i in range(3))
+    assert oracle.rekine(cm, 400.0) == pytest.approx(400.0 * ref, rel=1e-12)
 
 
 def _proj_case(lval):
