@@ -35,6 +35,16 @@ class mc_layout(C.Structure):
                 ("tpiba2", C.c_double), ("omega", C.c_double), ("geq0", C.c_int)]
 
 
+class mc_restart_header(C.Structure):
+    _fields_ = [("ibrav", C.c_int), ("indpg", C.c_int), ("celldm", C.c_double * 6),
+                ("nsp", C.c_int), ("nat", C.c_int), ("ecut", C.c_double), ("cdual", C.c_double),
+                ("dual", C.c_int), ("nel", C.c_int), ("nr1s", C.c_int), ("nr2s", C.c_int), ("nr3s", C.c_int),
+                ("nstate", C.c_int), ("ngwks", C.c_int), ("ngwls", C.c_int), ("nhgs", C.c_int),
+                ("nhgls", C.c_int), ("sections", C.c_int)]
+
+
+_rhp = C.POINTER(mc_restart_header)
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "mc_last_error": (C.c_char_p, []),
@@ -108,6 +118,11 @@ SIGNATURES = {
     "mc_fnonloc": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp]),
     "mc_fnonloc_dev": (C.c_int, [_vp, _vp, _dp, C.c_int, _dp]),
     "mc_forces_c2_dev": (C.c_int, [_vp, _vp, _vp, _dp, _vp, C.c_int]),
+    "mc_restart_write": (C.c_int, [C.c_char_p, _rhp, _ip, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int]),
+    "mc_restart_read_header": (C.c_int, [C.c_char_p, _rhp]),
+    "mc_restart_read": (C.c_int, [C.c_char_p, _rhp, _ip, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int]),
+    "mc_restart_write_dev": (C.c_int, [_vp, C.c_char_p, _rhp, _ip, _dp, _dp, _dp, _dp, _vp, _vp]),
+    "mc_restart_read_dev": (C.c_int, [_vp, C.c_char_p, _rhp, _ip, _dp, _dp, _dp, _dp, _vp, _vp]),
     "mc_timer_enable": (C.c_int, [_vp, C.c_int]),
     "mc_timer_reset": (C.c_int, [_vp]),
     "mc_timer_get": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_longlong)]),

@@ -423,6 +423,55 @@ int mc_vofrho_dev(mc_ctx* ctx, const void* rho_dev, void* v_dev, double* energie
  * and the eself sum of src/rinforce_utils.mod.F90:239-249. */
 int mc_ewald(mc_ctx* ctx, int iesr, double* esr, double* eself, double* fion);
 
+/* ------------------------------------------------------------------------
+ * RESTART files ("STUTTGART VERSION 3.0", Fortran sequential unformatted:
+ * 4-byte little-endian record lengths before and after each record).
+ * Written: section 1 (header), 2 (cell: ibrav, indpg; celldm), 3 (nsp;
+ * na), 4 (tau0), 5 (ionic velocities), 6 (initial coordinates), 7 (ecut,
+ * cdual, dual, nel, nr1s..nr3s), 8 (nstate, nkpts = 0, ngwks, ngwls, nhgs,
+ * nhgls), 9 (C0 in w_wfnio's layout: iwfntype 1, uncompressed), 10 (CM in
+ * wr30wfn's layout, uncompressed), 23 (occupations); sections 11-22 and
+ * 24-99 as empty markers.  Read: the same sections (C0/CM in either layout,
+ * uncompressed, Gamma only), everything else skipped.  Coefficients are in
+ * CPMD's G order (the host layout of every host-pointer entry point); atoms
+ * in isa order, positions (3, nat) in bohr.  Host-only: no GPU needed
+ * except by the *_dev variants, which download / upload through the
+ * context's layout (ngwks must equal the context's ngw).
+ * Replaces zhwwf/wv30 (src/wv30_utils.mod.F90:99-757; sections 9 and 10 at
+ * :484-535 through w_wfnio src/wfnio_utils.mod.F90:52-233 and wr30wfn
+ * src/wr30wfn_utils.mod.F90:569-693) and the matching sections of rv30
+ * (src/rv30_utils.mod.F90:286-735).
+ * ------------------------------------------------------------------------ */
+enum { MC_RST_VELP = 1, MC_RST_TAUI = 2, MC_RST_C0 = 4, MC_RST_CM = 8, MC_RST_OCC = 16 };
+typedef struct {
+  int ibrav, indpg;         /* section 2: parm%ibrav, symmi%indpg          */
+  double celldm[6];         /*            cell_com%celldm                   */
+  int nsp, nat;             /* section 3: species; nat = sum(na) (0: any)   */
+  double ecut, cdual;       /* section 7: cntr%ecut, dual00%cdual           */
+  int dual, nel;            /*            dual00%dual (logical), NINT(nel)  */
+  int nr1s, nr2s, nr3s;     /*            mesh                              */
+  int nstate, ngwks, ngwls, nhgs, nhgls;   /* section 8 (spar%...)          */
+  int sections;             /* MC_RST_* of the optional sections present    */
+} mc_restart_header;
+/* Writes path: the optional sections flagged in h->sections (their array
+ * must be non-NULL); c0 / cm are (ldc, nstate) complex in CPMD G order. */
+int mc_restart_write(const char* path, const mc_restart_header* h, const int* na, const double* tau0,
+                     const double* velp, const double* taui, const double* f, const double* c0, const double* cm,
+                     int ldc);
+/* Header and section flags of path (sizes for the buffers of mc_restart_read). */
+int mc_restart_read_header(const char* path, mc_restart_header* h);
+/* Reads path into the non-NULL buffers: na (nsp), tau0 / velp / taui (3, nat),
+ * f (nstate), c0 / cm (ldc, nstate). */
+int mc_restart_read(const char* path, mc_restart_header* h, int* na, double* tau0, double* velp, double* taui,
+                    double* f, double* c0, double* cm, int ldc);
+/* As mc_restart_write / mc_restart_read with device wavefunctions in the
+ * library's layout (synchronous). */
+int mc_restart_write_dev(mc_ctx* ctx, const char* path, const mc_restart_header* h, const int* na,
+                         const double* tau0, const double* velp, const double* taui, const double* f,
+                         const void* c0_dev, const void* cm_dev);
+int mc_restart_read_dev(mc_ctx* ctx, const char* path, mc_restart_header* h, int* na, double* tau0, double* velp,
+                        double* taui, double* f, void* c0_dev, void* cm_dev);
+
 /* Optional per-kernel timing with HIP events on the context stream. */
 int mc_timer_enable(mc_ctx* ctx, int on);
 int mc_timer_reset(mc_ctx* ctx);
