@@ -87,6 +87,7 @@ SIGNATURES = {
     "mc_rhoofr_dev": (C.c_int, [_vp, _vp, _dp, C.c_int, _vp, _dp, _dp, _dp]),
     "mc_vpsi": (C.c_int, [_vp, _dp, C.c_int, _dp, _dp, _dp, C.c_int]),
     "mc_vpsi_dev": (C.c_int, [_vp, _vp, _vp, _dp, _vp, C.c_int]),
+    "mc_vpsi_dev_noalloc": (C.c_int, [_vp, _vp, _vp, _dp, _vp, C.c_int]),
     "mc_ovlap": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, C.c_int]),
     "mc_ovlap_dev": (C.c_int, [_vp, C.c_int, _dp, _vp, _vp]),
     "mc_rgs": (C.c_int, [_vp, _dp, C.c_int, C.c_int, _dp]),

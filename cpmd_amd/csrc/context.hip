@@ -707,6 +707,7 @@ int mc_ctx_create(const mc_layout* lay, mc_ctx** out) {
     mc::upload_twiddles(ctx->d_tw1, ctx->n1);
     mc::upload_twiddles(ctx->d_tw2, ctx->n2);
     mc::upload_twiddles(ctx->d_tw3, ctx->n3);
+    ctx->scratch = mc::scratch_init(ctx->device);
   } catch (...) {
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1176,6 +1177,16 @@ int mc_vpsi_dev(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const double* f, 
   MC_CHECK(ctx && c0_dev && c2_dev && f && vpot_dev, MC_ERR_ARG, "null argument");
   MC_CHECK(nstate >= 0, MC_ERR_ARG, "nstate < 0");
   mc::vpsi_dev(ctx, (const double2*)c0_dev, (double2*)c2_dev, f, (const double*)vpot_dev, nstate);
+  MC_API_END
+}
+
+int mc_vpsi_dev_noalloc(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const double* f, const void* vpot_dev,
+                        int nstate) {
+  MC_API_BEGIN
+  MC_CHECK(ctx && c0_dev && c2_dev && f && vpot_dev, MC_ERR_ARG, "null argument");
+  MC_CHECK(nstate >= 0, MC_ERR_ARG, "nstate < 0");
+  MC_HIP(hipSetDevice(ctx->device));
+  mc::vpsi_noalloc_dev(ctx, (const double2*)c0_dev, (double2*)c2_dev, f, (const double*)vpot_dev, nstate);
   MC_API_END
 }
 

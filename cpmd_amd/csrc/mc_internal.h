@@ -245,6 +245,9 @@ struct mc_ctx {
   int batch_pairs_vpsi = 64;
   int batch_pairs_rho = 128;
 
+  // device address of this GPU's allocation-free scratch array (wfft.hip)
+  void* scratch = nullptr;
+
   mc::KernelTimer timer;
 };
 
@@ -361,4 +364,10 @@ void vpsi_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host,
               const double* vpot, int nstate, bool acc = true);
 void forces_c2_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host,
                    const double* vpot, int nstate);
+// address of the allocation-free scratch array on `device` (the current
+// device), looked up once per device with hipGetSymbolAddress
+void* scratch_init(int device);
+// vpsi (accumulating) with the workspaces and occupations in the scratch array
+void vpsi_noalloc_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, const double* vpot,
+                      int nstate);
 }  // namespace mc

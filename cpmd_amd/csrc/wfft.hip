@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -1025,23 +1027,19 @@ static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, in
   if (csumr) *csumr = cr;
 }
 
+// the pass sequence of vpsi over batches of at most P pairs, with the
+// workspaces A, B (P pairs each) and the occupations f_dev given
 template <typename T>
-static void vpsi_impl(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, const double* vpot,
-                      int nstate, bool acc) {
-  using V = v2<T>;
+static void vpsi_run(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_dev, const double* vpot,
+                     int nstate, bool acc, int P, v2<T>* A, v2<T>* B) {
   const WParams w = make_params(ctx);
   const Shapes s = shapes<T>(ctx);
   const int npairs = (nstate + 1) / 2;
-  if (npairs == 0) return;
-  const int P = std::max(1, std::min(ctx->batch_pairs_vpsi, npairs));
-  ensure_workspace<T>(ctx, P);
-  upload_f(ctx, f_host, nstate);
+  using V = v2<T>;
   const size_t shz = sizeof(V) * (size_t)ctx->n3 * (1 + 2 * s.lo);
   const size_t shy = sizeof(V) * (size_t)ctx->n2 * (1 + s.tz);
   const size_t shx = sizeof(V) * (size_t)ctx->n1 * (1 + s.ty);
   const double scale = 1.0 / ((double)ctx->n1 * ctx->n2 * ctx->n3);
-  V* A = ctx->w_a.as<V>();
-  V* B = ctx->w_b.as<V>();
   for (int pair0 = 0; pair0 < npairs; pair0 += P) {
     const int np = std::min(P, npairs - pair0);
     with_length(ctx->n3, [&](auto NN) {
@@ -1063,9 +1061,100 @@ static void vpsi_impl(mc_ctx* ctx, const double2* c0, double2* c2, const double*
     ypass<T, -1>(ctx, w, s, A, B, np, shy);
     with_length(ctx->n3, [&](auto NN) {
       LAUNCH(MC_KID_ZFWD, (k_zfwd<T, decltype(NN)::value>), dim3(s.gx, np), dim3(kThreads), shz, w, A, c0, c2,
-             ctx->w_f.as<double>(), nstate, pair0, s.lo, scale, acc ? 1 : 0);
+             f_dev, nstate, pair0, s.lo, scale, acc ? 1 : 0);
     });
   }
+}
+
+template <typename T>
+static void vpsi_impl(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, const double* vpot,
+                      int nstate, bool acc) {
+  (void)shapes<T>(ctx);   // rejects unsupported lengths before any allocation
+  const int npairs = (nstate + 1) / 2;
+  if (npairs == 0) return;
+  const int P = std::max(1, std::min(ctx->batch_pairs_vpsi, npairs));
+  ensure_workspace<T>(ctx, P);
+  upload_f(ctx, f_host, nstate);
+  vpsi_run<T>(ctx, c0, c2, ctx->w_f.as<double>(), vpot, nstate, acc, P, ctx->w_a.as<v2<T>>(), ctx->w_b.as<v2<T>>());
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free vpsi.  The occupations and the transform workspaces live in
+// `scratch`, one fixed array per GPU that the code object carries, instead of
+// context buffers that grow with hipMalloc.  In host code `scratch` names a
+// host-side stand-in: only the address that hipGetSymbolAddress returns for
+// the current device (kept per device by scratch_init) is passed to kernels
+// and copies.  Every call on one GPU shares the array: calls that overlap in
+// time on one GPU (other streams or contexts) race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[MC_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+void* scratch_init(int device) {
+  MC_CHECK(device >= 0 && device < kMaxScratchDevices, MC_ERR_UNSUPPORTED, "device ordinal beyond 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    MC_HIP(hipGetDevice(&cur));
+    MC_CHECK(cur == device, MC_ERR_INTERNAL, "scratch_init: device is not current");
+    void* p = nullptr;
+    MC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MC_CHECK(p != nullptr, MC_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+
+static size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+// the per-kernel timer creates events: it is held off for the call
+struct TimerHold {
+  mc_ctx* c;
+  bool was;
+  explicit TimerHold(mc_ctx* ctx) : c(ctx), was(ctx->timer.enabled) { c->timer.enabled = false; }
+  ~TimerHold() { c->timer.enabled = was; }
+};
+
+template <typename T>
+static void vpsi_noalloc_impl(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host,
+                              const double* vpot, int nstate) {
+  using V = v2<T>;
+  (void)shapes<T>(ctx);
+  const int npairs = (nstate + 1) / 2;
+  if (npairs == 0) return;
+  MC_CHECK(ctx->scratch != nullptr, MC_ERR_INTERNAL, "no scratch address for this device");
+  const size_t fbytes = round256(sizeof(double) * (size_t)nstate);
+  const size_t abytes = sizeof(V) * (size_t)ctx->n_slots * ctx->n3;           // sticks of one pair
+  const size_t bbytes = sizeof(V) * (size_t)ctx->n3 * ctx->ncols * ctx->n2;   // columns of one pair
+  const size_t cap = MC_NOALLOC_SCRATCH_BYTES;
+  // P pairs take fbytes + round256(P abytes) + P bbytes <= cap
+  long long P = 0;
+  if (fbytes + 256 < cap) P = (long long)((cap - fbytes - 256) / (abytes + bbytes));
+  P = std::min<long long>(P, std::min(ctx->batch_pairs_vpsi, npairs));
+  if (P < 1)
+    throw Error{MC_ERR_ARG, "mc_vpsi_dev_noalloc: the occupations and one state pair's transform workspaces need " +
+                                std::to_string(fbytes + round256(abytes) + bbytes) + " bytes, beyond the " +
+                                std::to_string(cap) + "-byte scratch"};
+  unsigned char* base = static_cast<unsigned char*>(ctx->scratch);
+  double* f_dev = reinterpret_cast<double*>(base);
+  V* A = reinterpret_cast<V*>(base + fbytes);
+  V* B = reinterpret_cast<V*>(base + fbytes + round256((size_t)P * abytes));
+  MC_CHECK(fbytes + round256((size_t)P * abytes) + (size_t)P * bbytes <= cap, MC_ERR_INTERNAL,
+           "scratch carving exceeds the array");
+  TimerHold hold(ctx);
+  MC_HIP(hipMemcpyAsync(f_dev, f_host, sizeof(double) * nstate, hipMemcpyHostToDevice, ctx->stream));
+  vpsi_run<T>(ctx, c0, c2, f_dev, vpot, nstate, true, (int)P, A, B);
+}
+
+void vpsi_noalloc_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, const double* vpot,
+                      int nstate) {
+  if (ctx->fft_fp32)
+    vpsi_noalloc_impl<float>(ctx, c0, c2, f_host, vpot, nstate);
+  else
+    vpsi_noalloc_impl<double>(ctx, c0, c2, f_host, vpot, nstate);
 }
 
 void rhoofr_dev(mc_ctx* ctx, const double2* c0, const double* f_host, int nstate, double* rho, double* ekin,

@@ -263,6 +263,26 @@ int mc_vpsi(mc_ctx* ctx, const double* c0, int ldc, double* c2, const double* f,
 int mc_vpsi_dev(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const double* f,
                 const void* vpot_dev, int nstate);
 
+/* vpsi that allocates nothing: the same computation and kernels as
+ * mc_vpsi_dev (c2 += ..., bit-identical results), but the occupations and the
+ * transform workspaces live in one fixed array of MC_NOALLOC_SCRATCH_BYTES
+ * (128 MiB) of device memory per GPU, which the library holds from its first
+ * context on that GPU until the process ends, instead of context buffers
+ * that grow with hipMalloc.  The call makes no hipMalloc, hipFree or other
+ * allocating call (the per-kernel timer, which creates events, is held off
+ * during it).  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).
+ * Pairs per launch: the batch size of mc_ctx_set_batch, or fewer, as many as
+ * fit.  MC_ERR_ARG when the occupations and one state pair's workspaces
+ * (16 B in FP64, 8 B in FP32 transforms, times sticks x nr3 + columns x nr2 x
+ * nr3) exceed the array (FP64 transforms on a 240^3 mesh with 9,281 sticks
+ * in 109 columns need 136 MB).  Every call on one GPU uses the same array:
+ * calls that overlap in time on one GPU (other streams or contexts) race. */
+#define MC_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int mc_vpsi_dev_noalloc(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const double* f,
+                        const void* vpot_dev, int nstate);
+
 /* ------------------------------------------------------------------------
  * Dense contractions (FP64 MFMA).  Wavefunctions are viewed as real
  * (2*ngw x nstate) matrices; matrices are column-major nstate x nstate on the
