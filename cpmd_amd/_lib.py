@@ -63,6 +63,9 @@ SIGNATURES = {
     "mc_memcpy_h2d": (C.c_int, [_vp, _vp, C.c_size_t]),
     "mc_memcpy_d2h": (C.c_int, [_vp, _vp, C.c_size_t]),
     "mc_memset_d": (C.c_int, [_vp, C.c_int, C.c_size_t]),
+    "mc_set_mem_limit": (C.c_int, [C.c_size_t]),
+    "mc_mem_stats": (C.c_int, [C.POINTER(C.c_size_t)] * 3),
+    "mc_mem_reset_peak": (C.c_int, []),
     "mc_wfn_zero": (C.c_int, [_vp, _vp, C.c_int]),
     "mc_wfn_axpy": (C.c_int, [_vp, C.c_double, _vp, _vp, C.c_int]),
     "mc_wfn_axpby": (C.c_int, [_vp, C.c_double, _vp, C.c_double, _vp, C.c_int]),

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 
 #include "fft_device.h"
@@ -31,11 +32,60 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 void clear_error() { g_last_error.clear(); }
 
+// ---------------------------------------------------------------------------
+// device memory accounting: every allocation of the library (context buffers
+// and mc_dev_alloc) is counted; with a limit set, an allocation that would
+// take the count beyond it fails with MC_ERR_ARG instead of being made
+// ---------------------------------------------------------------------------
+namespace {
+struct MemTrack {
+  std::mutex mu;
+  size_t cur = 0, peak = 0, limit = 0;
+  std::map<void*, size_t> user;   // mc_dev_alloc'ed blocks
+};
+MemTrack& mem() {
+  static MemTrack* t = new MemTrack();   // never destroyed: buffers may be freed at exit
+  return *t;
+}
+// count `add` bytes that replace `drop` counted bytes; throws beyond the limit
+void mem_take(size_t add, size_t drop, const char* what) {
+  auto& t = mem();
+  std::lock_guard<std::mutex> lk(t.mu);
+  const size_t after = t.cur - drop + add;
+  if (t.limit && add > drop && after > t.limit)
+    throw Error{MC_ERR_ARG, std::string(what) + ": device memory limit of " + std::to_string(t.limit) +
+                                " bytes exceeded (" + std::to_string(t.cur - drop) + " in use, " +
+                                std::to_string(add) + " requested)"};
+  t.cur = after;
+  t.peak = std::max(t.peak, t.cur);
+}
+void mem_give(size_t bytes) {
+  auto& t = mem();
+  std::lock_guard<std::mutex> lk(t.mu);
+  t.cur -= std::min(bytes, t.cur);
+}
+}  // namespace
+
+size_t mem_limit() {
+  std::lock_guard<std::mutex> lk(mem().mu);
+  return mem().limit;
+}
+size_t mem_current() {
+  std::lock_guard<std::mutex> lk(mem().mu);
+  return mem().cur;
+}
+
 void DevBuf::ensure(size_t nbytes) {
   if (nbytes <= bytes) return;
   release();
   if (nbytes == 0) return;
-  MC_HIP(hipMalloc(&p, nbytes));
+  mem_take(nbytes, 0, "device buffer");
+  const hipError_t e = hipMalloc(&p, nbytes);
+  if (e != hipSuccess) {
+    p = nullptr;
+    mem_give(nbytes);
+    MC_HIP(e);
+  }
   bytes = nbytes;
 }
 void upload_async(mc_ctx* ctx, void* dst, const void* src, size_t bytes) {
@@ -84,7 +134,10 @@ void dl_add(mc_ctx* ctx, void* dst, const void* src, size_t bytes) {
 }
 
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p) {
+    (void)hipFree(p);
+    mem_give(bytes);
+  }
   p = nullptr;
   bytes = 0;
 }
@@ -780,12 +833,55 @@ int mc_dev_alloc(size_t bytes, void** ptr) {
   MC_API_BEGIN
   MC_CHECK(ptr, MC_ERR_ARG, "null pointer");
   *ptr = nullptr;
-  if (bytes) MC_HIP(hipMalloc(ptr, bytes));
+  if (bytes) {
+    mc::mem_take(bytes, 0, "mc_dev_alloc");
+    const hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+      *ptr = nullptr;
+      mc::mem_give(bytes);
+      MC_HIP(e);
+    }
+    std::lock_guard<std::mutex> lk(mc::mem().mu);
+    mc::mem().user[*ptr] = bytes;
+  }
   MC_API_END
 }
 int mc_dev_free(void* ptr) {
   MC_API_BEGIN
-  if (ptr) MC_HIP(hipFree(ptr));
+  if (ptr) {
+    MC_HIP(hipFree(ptr));
+    size_t b = 0;
+    {
+      std::lock_guard<std::mutex> lk(mc::mem().mu);
+      auto it = mc::mem().user.find(ptr);
+      if (it != mc::mem().user.end()) {
+        b = it->second;
+        mc::mem().user.erase(it);
+      }
+    }
+    mc::mem_give(b);
+  }
+  MC_API_END
+}
+
+int mc_set_mem_limit(size_t bytes) {
+  MC_API_BEGIN
+  std::lock_guard<std::mutex> lk(mc::mem().mu);
+  mc::mem().limit = bytes;
+  MC_API_END
+}
+int mc_mem_stats(size_t* current, size_t* peak, size_t* limit) {
+  MC_API_BEGIN
+  std::lock_guard<std::mutex> lk(mc::mem().mu);
+  if (current) *current = mc::mem().cur;
+  if (peak) *peak = mc::mem().peak;
+  if (limit) *limit = mc::mem().limit;
+  MC_API_END
+}
+int mc_mem_reset_peak(void) {
+  MC_API_BEGIN
+  std::lock_guard<std::mutex> lk(mc::mem().mu);
+  mc::mem().peak = mc::mem().cur;
   MC_API_END
 }
 int mc_memcpy_h2d(void* dst, const void* src, size_t bytes) {

@@ -67,7 +67,13 @@ struct FftPlan {
 
 FftPlan make_plan(int n);  // throws on unsupported lengths
 
-// Device buffer owned by the context; grows on demand.
+// Device memory the library holds (all contexts and mc_dev_alloc blocks)
+// and the limit of mc_set_mem_limit (0: none).
+size_t mem_limit();
+size_t mem_current();
+
+// Device buffer owned by the context; grows on demand (counted against the
+// device memory limit: MC_ERR_ARG when it would be exceeded).
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;

@@ -947,6 +947,30 @@ static void ypass(mc_ctx* ctx, const WParams& w, const Shapes& s, v2<T>* A, v2<T
   });
 }
 
+// Pairs per transform batch: at most P, and under a device memory limit as
+// many as fit beside everything else the library holds (the workspaces are
+// shrunk when they hold more); MC_ERR_ARG when not even one pair fits.
+template <typename T>
+static int fit_pairs(mc_ctx* ctx, int P, size_t extra) {
+  const size_t lim = mem_limit();
+  if (!lim) return P;
+  const size_t per = sizeof(v2<T>) * ((size_t)ctx->n_slots * ctx->n3 + (size_t)ctx->n3 * ctx->ncols * ctx->n2);
+  const size_t held = ctx->w_a.bytes + ctx->w_b.bytes;
+  const size_t others = mem_current() - std::min(held, mem_current()) + extra;
+  const size_t avail = lim > others ? lim - others : 0;
+  const long long fit = (long long)(avail / per);
+  if (fit < 1)
+    throw Error{MC_ERR_ARG, "device memory limit of " + std::to_string(lim) + " bytes leaves " +
+                                std::to_string(avail) + " bytes for the transform workspaces; one state pair needs " +
+                                std::to_string(per)};
+  if (fit < P) P = (int)fit;
+  if (held > (size_t)P * per) {   // shrink to what the batch needs
+    ctx->w_a.release();
+    ctx->w_b.release();
+  }
+  return P;
+}
+
 template <typename T>
 static void ensure_workspace(mc_ctx* ctx, int P) {
   ctx->w_a.ensure(sizeof(v2<T>) * (size_t)P * ctx->n_slots * ctx->n3);
@@ -967,10 +991,14 @@ static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, in
   const WParams w = make_params(ctx);
   const Shapes s = shapes<T>(ctx);
   const int npairs = (nstate + 1) / 2;
-  const int P = std::max(1, std::min(ctx->batch_pairs_rho, npairs));
+  const int nblk_sum = 1024;
+  const size_t red_bytes =
+      sizeof(double) * ((size_t)std::max(npairs, 1) * s.gx * 4 + (size_t)npairs * 4 + nblk_sum) +
+      sizeof(double) * std::max(nstate, 1);
+  const int P = fit_pairs<T>(ctx, std::max(1, std::min(ctx->batch_pairs_rho, npairs)),
+                             std::max(red_bytes, ctx->w_red.bytes + ctx->w_f.bytes) - ctx->w_red.bytes - ctx->w_f.bytes);
   ensure_workspace<T>(ctx, P);
   upload_f(ctx, f_host, nstate);
-  const int nblk_sum = 1024;
   ctx->w_red.ensure(sizeof(double) * ((size_t)std::max(npairs, 1) * s.gx * 4 + (size_t)npairs * 4 + nblk_sum));
   double* red = ctx->w_red.as<double>();
   double* pair_sums = red + (size_t)std::max(npairs, 1) * s.gx * 4;
@@ -1072,7 +1100,9 @@ static void vpsi_impl(mc_ctx* ctx, const double2* c0, double2* c2, const double*
   (void)shapes<T>(ctx);   // rejects unsupported lengths before any allocation
   const int npairs = (nstate + 1) / 2;
   if (npairs == 0) return;
-  const int P = std::max(1, std::min(ctx->batch_pairs_vpsi, npairs));
+  const size_t f_bytes = sizeof(double) * std::max(nstate, 1);
+  const int P = fit_pairs<T>(ctx, std::max(1, std::min(ctx->batch_pairs_vpsi, npairs)),
+                             f_bytes > ctx->w_f.bytes ? f_bytes - ctx->w_f.bytes : 0);
   ensure_workspace<T>(ctx, P);
   upload_f(ctx, f_host, nstate);
   vpsi_run<T>(ctx, c0, c2, ctx->w_f.as<double>(), vpot, nstate, acc, P, ctx->w_a.as<v2<T>>(), ctx->w_b.as<v2<T>>());

@@ -69,6 +69,22 @@ class DeviceArray:
             pass
 
 
+def set_mem_limit(nbytes: int):
+    """Cap the device memory the library holds (0: no cap); see mc_set_mem_limit."""
+    check(lib().mc_set_mem_limit(int(nbytes)))
+
+
+def mem_stats() -> tuple[int, int, int]:
+    """(bytes held now, most held since the last reset, the cap)."""
+    cur, peak, lim = C.c_size_t(), C.c_size_t(), C.c_size_t()
+    check(lib().mc_mem_stats(C.byref(cur), C.byref(peak), C.byref(lim)))
+    return cur.value, peak.value, lim.value
+
+
+def mem_reset_peak():
+    check(lib().mc_mem_reset_peak())
+
+
 def make_layout(cell, gset, ngw: int | None = None):
     """mc_layout plus the arrays it points to (kept alive by the returned tuple)."""
     ngw = gset.ngw if ngw is None else ngw

@@ -125,6 +125,20 @@ int mc_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int mc_memcpy_d2h(void* dst, const void* src, size_t bytes);
 int mc_memset_d(void* dst, int value, size_t bytes);
 
+/* Device memory budget (no counterpart in the reference, whose scratch is
+ * sized by maxfft, src/fftprp_utils.mod.F90:308-328).  The library counts
+ * every device allocation it makes in this process (context buffers of all
+ * contexts and mc_dev_alloc blocks; not pinned host staging, not the fixed
+ * array of mc_vpsi_dev_noalloc).  With a limit set (bytes > 0), rhoofr and
+ * vpsi transform as many state pairs per batch as fit beside what is already
+ * held (shrinking their workspaces when needed; the per-pair results do not
+ * depend on the batch), and any other allocation that would pass the limit
+ * fails with MC_ERR_ARG instead of being made.  0 removes the limit. */
+int mc_set_mem_limit(size_t bytes);
+/* Bytes held now, the most held since the last reset, and the limit. */
+int mc_mem_stats(size_t* current, size_t* peak, size_t* limit);
+int mc_mem_reset_peak(void);
+
 /* Zero nstate device wavefunctions (stream-ordered; CPMD: zeroing(c2),
  * src/forces_driver.mod.F90:175). */
 int mc_wfn_zero(mc_ctx* ctx, void* c_dev, int nstate);

@@ -9,7 +9,12 @@
   x 3 directions with an ionic step of 1e-4 bohr.
 
 vpsi recomputes psi(r) from C for every state pair and never stores psi(r)
-for all states (the recompute-psi backward)."""
+for all states (the recompute-psi backward).  Every check runs twice: with
+no device memory limit, and with the library's device memory capped
+(mc_set_mem_limit) so that the transforms work on two state pairs per batch
+instead of all eight."""
+import gc
+
 import numpy as np
 import pytest
 
@@ -22,20 +27,35 @@ from test_oracle_energy import ks_system, tangent_directions
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def cfg3():
+@pytest.fixture(scope="module", params=["uncapped", "capped"])
+def cfg3(request):
     sysm, K, c = ks_system(12.0, 8, 40.0, 16, seed=1234)
     ks = KSDevice(sysm)
     ks.set_ions(sysm.tau)
     d_c = ks.ctx.upload(c)
-    return sysm, K, c, ks, d_c
+    n = c.shape[1]
+    d_c2, d_p = ks.ctx.alloc_wfn(n), ks.ctx.alloc_wfn(n)
+    if request.param == "capped":
+        # one full evaluation allocates every buffer; the cap then leaves the
+        # transform workspaces room for two of the eight state pairs
+        ks.energy(d_c, sysm.f, gradient=d_c2, forces=True)
+        ks.ctx.synchronize()
+        gc.collect()                       # buffers of earlier tests are returned first
+        s, cols, _ = ops.layout_stats(sysm.cell, sysm.gset)
+        n1, n2, n3 = sysm.cell.nr
+        per_pair = 16 * (s * n3 + cols * n2 * n3)
+        cur, _, _ = ops.mem_stats()
+        ops.set_mem_limit(cur - 6 * per_pair)
+        print(f"cfg3 with the device memory capped at {cur - 6 * per_pair} bytes", flush=True)
+    yield sysm, K, c, ks, d_c, d_c2, d_p
+    ops.set_mem_limit(0)
+    ks.ctx.close()
 
 
 def test_device_ks_matches_oracle(gpu, cfg3):
-    sysm, K, c, ks, d_c = cfg3
+    sysm, K, c, ks, d_c, d_c2, _ = cfg3
     f = sysm.f
     n = c.shape[1]
-    d_c2 = ks.ctx.alloc_wfn(n)
     r = ks.energy(d_c, f, gradient=d_c2, forces=True)
     o = oracle.ks_energy(K, c, f, sysm.tau, forces=True, gradient=True)
     for k in ("ekin", "ehep", "epseu", "exc", "esr", "eself", "enl", "etot"):
@@ -46,13 +66,11 @@ def test_device_ks_matches_oracle(gpu, cfg3):
 
 
 def test_h_psi_is_gradient(gpu, cfg3):
-    sysm, K, c, ks, d_c = cfg3
+    sysm, K, c, ks, d_c, d_c2, d_p = cfg3
     f = sysm.f
     n = c.shape[1]
-    d_c2 = ks.ctx.alloc_wfn(n)
     ks.energy(d_c, f, gradient=d_c2)
     c2 = ks.ctx.download(d_c2, n)
-    d_p = ks.ctx.alloc_wfn(n)
     h = 1e-4
     for d in tangent_directions(c, 8, seed=77):
         ks.ctx.upload(np.asfortranarray(c + h * d), d_p)
@@ -64,7 +82,7 @@ def test_h_psi_is_gradient(gpu, cfg3):
 
 
 def test_forces_are_position_derivatives(gpu, cfg3):
-    sysm, K, c, ks, d_c = cfg3
+    sysm, K, c, ks, d_c, _, _ = cfg3
     f = sysm.f
     tau = sysm.tau
     fion = ks.energy(d_c, f, tau=tau, forces=True)["fion"]
