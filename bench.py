@@ -89,27 +89,83 @@ def step_model_bytes(ctx_geom, nstate: int, cw: float = 16.0) -> float:
     return npairs * (vp + rh) + 2 * r * n1 * n2 * n3
 
 
-def cpu_baseline(sysm, c0, v, max_pairs: int = 1):
-    """Oracle (single-thread C restatement of CPMD's rhoofr + vpsi) on a bounded
-    sample of the same step: `max_pairs` state pairs, scaled to all pairs."""
-    import oracle
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sysm, c0, v, sample_pairs: int = 64):
+    """CPMD's CPU algorithm for the SCF-step content of the GPU measurement,
+    on the host's cores: rhoofr + vpsi over Gamma state pairs with stick-sparse
+    3D transforms (oracle/cpu_step.c: one state pair per OpenMP thread, 1D
+    transforms through the FFTW3 interface), the projector contractions
+    rnlsm1 + fnonloc and the Loewdin orthonormalisation (threaded BLAS /
+    LAPACK).  The transforms run on `sample_pairs` of the state pairs and are
+    scaled to all of them; the contractions and the orthonormalisation run in
+    full."""
+    from oracle.cpu_step import CpuStep
+    from cpmd_amd import system as S
     nstate = c0.shape[1]
     npairs = (nstate + 1) // 2
-    ns = 2 * max_pairs
-    L = oracle.Layout(sysm.cell, sysm.gset)
-    cs = np.asfortranarray(c0[:, :ns])
-    f = sysm.f[:ns]
+    ps = min(sample_pairs, npairs)
+    cs = CpuStep(sysm.cell, sysm.gset)
+    threads = cs.threads()
+    gs = sysm.gset
+    ngw = gs.ngw
+    f = sysm.f
+    na, nhxs, ngh, lang, hmat, twnl, eigr = S.projector_set(sysm, gs.miller[:ngw], gs.hg[:ngw])
+    beta = np.asfortranarray(twnl[:, 0, 0][:, None] * eigr)           # s projector: (-i)^0 twnl eigr
+    cs.rhoofr(c0[:, :2], f[:2])                                         # plans, first touch
+    t = {}
     t0 = time.perf_counter()
-    oracle.rhoofr(L, cs, f)
-    oracle.vpsi(L, cs, np.zeros_like(cs), f, v)
-    dt = time.perf_counter() - t0
+    cs.rhoofr(c0, f, pairs=ps)
+    t["rhoofr"] = (time.perf_counter() - t0) * npairs / ps
+    t0 = time.perf_counter()
+    cs.vpsi(c0, np.zeros_like(c0), f, v, pairs=ps)
+    t["vpsi"] = (time.perf_counter() - t0) * npairs / ps
+    t0 = time.perf_counter()
+    fnl = cs.rnlsm1(c0, beta)
+    dd = -(f[None, :] * hmat[0, 0, 0]) * fnl
+    cs.fnonloc(np.zeros_like(c0), beta, dd)
+    t["rnlsm1+fnonloc"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cs.lowdin(c0)
+    t["lowdin"] = time.perf_counter() - t0
+    total = sum(t.values())
     return {
-        "value": dt * 1e3 * npairs / max_pairs,
+        "value": round(total * 1e3, 1),
         "unit": "ms/step",
-        "cores": 1,
+        "cores": threads,
+        "cpu": cpu_model(),
         "kind": "port",
-        "sample": f"oracle rhoofr+vpsi on {max_pairs} of {npairs} state pairs ({dt:.1f} s), scaled x{npairs // max_pairs}",
+        "components_ms": {k: round(x * 1e3, 1) for k, x in t.items()},
+        "sample": (f"rhoofr + vpsi with stick-sparse transforms (FFTW3 interface, one state pair per thread on "
+                   f"{threads} threads) on {ps} of {npairs} state pairs scaled x{npairs / ps:g}, plus rnlsm1 + "
+                   f"fnonloc + Loewdin in full (threaded BLAS / LAPACK): the SCF-step content of CPMD's CPU path "
+                   f"(oracle/cpu_step.py)"),
     }
+
+
+def gpu_same_content_ms(ctx, ks, buf, f, nstate, reps: int = 5) -> float:
+    """The GPU's FP64 time for the content of cpu_baseline: rnlsm1, rhoofr,
+    vpsi + fnonloc, Loewdin (ms per step, median of reps)."""
+    ctx.set_precision("fp64")
+    ts = []
+    for _ in range(reps + 1):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.rnlsm_dev(buf["c0"], nstate, tfor=False, want_fnl=False)
+        ctx.rhoofr_dev(buf["c0"], f, nstate, ks.d_rho)
+        ctx.forces_c2_dev(buf["c0"], buf["c2"], f, ks.d_v, nstate)
+        ctx.lowdin_dev(buf["cp"], nstate)
+        ctx.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts[1:]) * 1e3)
 
 
 FFT_KIDS = ("zinv", "yinv", "xrho", "xvpsi", "yfwd", "zfwd")
@@ -418,10 +474,14 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        # 8 of the 256 pairs: about 12 s of single-thread work on the host
-        log("[bench] cpu baseline (oracle, 8 pair sample) ...")
-        cpu = cpu_baseline(sysm, c0, np.ascontiguousarray(ks.d_v.to_host(ks.nnr1)), max_pairs=8)
-        log(f"[bench] cpu baseline {cpu['value']:.1f} ms/step")
+        log("[bench] cpu baseline (threaded stick-sparse CPU path) ...")
+        cpu = cpu_baseline(sysm, c0, np.ascontiguousarray(ks.d_v.to_host(ks.nnr1)))
+        ctx.upload(c0, buf["c0"])
+        ctx.upload(c0, buf["cp"])
+        cpu["gpu_fp64_same_content_ms"] = round(gpu_same_content_ms(ctx, ks, buf, f, nstate), 3)
+        ctx.set_precision(modes[head])
+        log(f"[bench] cpu baseline {cpu['value']:.1f} ms/step on {cpu['cores']} threads ({cpu['cpu']}): "
+            f"{cpu['components_ms']}; the same content on the GPU in FP64 {cpu['gpu_fp64_same_content_ms']:.2f} ms")
 
     opt = None
     if args.opt_policy:
