@@ -21,7 +21,8 @@ _ip = C.POINTER(C.c_int)
 
 
 def build() -> Path:
-    subprocess.run(["make", "-s", "-C", str(HERE), "-B", "liboracle.so"], check=True)
+    """Compiles the oracle's C restatement and the threaded CPU path of the bench's baseline."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "-B", "liboracle.so", "libcpustep.so"], check=True)
     return LIB_PATH
 
 
