@@ -39,6 +39,8 @@ def hipcc() -> str:
 FORTRAN_DIR = PKG / "fortran"
 DRIVER_SRC = ROOT / "tests" / "fortran" / "shim_driver.F90"
 DRIVER = FORTRAN_DIR / "build" / "shim_driver.x"
+CP_DRIVER_SRC = ROOT / "tests" / "fortran" / "cp_driver.F90"
+CP_DRIVER = FORTRAN_DIR / "build" / "cp_driver.x"
 
 
 def amdflang() -> str:
@@ -49,18 +51,20 @@ def amdflang() -> str:
 
 
 def build_fortran(verbose: bool = False) -> Path:
-    """Compiles the ISO_C_BINDING module and the test driver that calls the
-    library with CPMD's Fortran signatures."""
+    """Compiles the ISO_C_BINDING module and the test drivers that call the
+    library with CPMD's Fortran signatures (routine parity: shim_driver; the
+    device-resident CP step: cp_driver)."""
     out = DRIVER.parent
     out.mkdir(parents=True, exist_ok=True)
-    cmd = [amdflang(), "-O2", "-module-dir", str(out), str(FORTRAN_DIR / "micpmd.F90"), str(DRIVER_SRC),
-           "-L", str(PKG), "-lmicpmd", f"-Wl,-rpath,{PKG}", "-o", str(DRIVER)]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("amdflang failed building the Fortran shim")
+    for src, exe in ((DRIVER_SRC, DRIVER), (CP_DRIVER_SRC, CP_DRIVER)):
+        cmd = [amdflang(), "-O2", "-module-dir", str(out), str(FORTRAN_DIR / "micpmd.F90"), str(src),
+               "-L", str(PKG), "-lmicpmd", f"-Wl,-rpath,{PKG}", "-o", str(exe)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("amdflang failed building the Fortran shim")
     return DRIVER
 
 

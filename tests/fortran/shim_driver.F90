@@ -32,7 +32,8 @@ PROGRAM shim_driver
   CALL micpmd_init(nr1, nr2, nr3, kr1, kr2, kr3, ngw, inyh, hg, tp(1), tp(2), .TRUE.)
 
   ALLOCATE(rhoe(nnr1, 1), psi(1), c2(ngw, nstate), smat(nstate, nstate), cl(ngw, nstate), cn(ngw, nstate))
-  CALL micpmd_rhoofr(c0, rhoe, psi, nstate, f)
+  CALL micpmd_set_occupations(f)
+  CALL micpmd_rhoofr(c0, rhoe, psi, nstate)
   c2 = (0.0_dp, 0.0_dp)
   CALL micpmd_vpsi(c0, c2, f, vpot, psi, nstate, 1, 1, .FALSE.)
   cl = cp
