@@ -364,6 +364,12 @@ def main():
             tot, cnt = ctx.timer_get(kid)
             if cnt:
                 kernels[kid] = {"total_ms_per_step": tot / args.steps, "launches": cnt, "avg_ms": tot / cnt}
+        # charge sums of the path's rhoofr on the end state: csumr before the
+        # FP32 rescale against the FP64 G-space csumg (CPMD's check is
+        # |csumr - csumg| < 1e-6, rhoofr:625-635)
+        ctx.rhoofr_dev(buf["c0"], f, nstate, ks.d_rho)
+        cg, cr_raw = ctx.rhoofr_raw_charge()
+        charge = {"csumg": cg, "csumr_before_rescale": cr_raw, "diff": cr_raw - cg, "rel": (cr_raw - cg) / cg}
         # the state after warmup + steps, evaluated in FP64
         ctx.set_precision("fp64")
         energy = None
@@ -391,7 +397,7 @@ def main():
         log(f"[bench] {path}: {args.steps} steps: {ms:.3f} ms/step, E_KS(start) {start['etot']:.10f}, "
             f"E_KS(end, fp64) {end['etot']:.10f}, |C^T C - I| {orth:.2e}, iters {iters[-args.steps:]}")
         return {"ms": ms, "kernels": kernels, "start": start, "end": end, "orth": orth,
-                "iters": iters[-args.steps:], "energy": energy}
+                "iters": iters[-args.steps:], "energy": energy, "charge": charge}
 
     paths = [p.strip() for p in args.paths.split(",") if p.strip()]
     res = {p: run_path(p) for p in paths}
@@ -572,6 +578,7 @@ def main():
         "paths": {p: {"ms_per_step": round(res[p]["ms"], 4), "E_KS_start_Ha": res[p]["start"]["etot"],
                       "E_KS_end_fp64_Ha": res[p]["end"]["etot"], "orthonormality_err": res[p]["orth"],
                       "constraint_iters": res[p]["iters"], "cp_energy": res[p]["energy"],
+                      "charge": res[p]["charge"],
                       "mfma_roofline": mfma_roofline(p, res[p]),
                       "kernels": res[p]["kernels"]} for p in paths},
         "cpu_baseline": cpu,

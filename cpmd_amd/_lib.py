@@ -88,6 +88,7 @@ SIGNATURES = {
     "mc_wfn_download": (C.c_int, [_vp, _vp, C.c_int, _dp, C.c_int]),
     "mc_rhoofr": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, _dp]),
     "mc_rhoofr_dev": (C.c_int, [_vp, _vp, _dp, C.c_int, _vp, _dp, _dp, _dp]),
+    "mc_rhoofr_raw_charge": (C.c_int, [_vp, _dp, _dp]),
     "mc_vpsi": (C.c_int, [_vp, _dp, C.c_int, _dp, _dp, _dp, C.c_int]),
     "mc_vpsi_dev": (C.c_int, [_vp, _vp, _vp, _dp, _vp, C.c_int]),
     "mc_vpsi_dev_noalloc": (C.c_int, [_vp, _vp, _vp, _dp, _vp, C.c_int]),

@@ -1276,6 +1276,14 @@ int mc_vpsi_dev(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const double* f, 
   MC_API_END
 }
 
+int mc_rhoofr_raw_charge(mc_ctx* ctx, double* csumg, double* csumr_raw) {
+  MC_API_BEGIN
+  MC_CHECK(ctx, MC_ERR_ARG, "null context");
+  if (csumg) *csumg = ctx->last_csumg;
+  if (csumr_raw) *csumr_raw = ctx->last_csumr_raw;
+  MC_API_END
+}
+
 int mc_vpsi_dev_noalloc(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const double* f, const void* vpot_dev,
                         int nstate) {
   MC_API_BEGIN

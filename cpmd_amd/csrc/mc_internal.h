@@ -251,6 +251,9 @@ struct mc_ctx {
   int batch_pairs_vpsi = 64;
   int batch_pairs_rho = 128;
 
+  // charge sums of the last rhoofr: G space and real space before any rescale
+  double last_csumg = 0.0, last_csumr_raw = 0.0;
+
   // device address of this GPU's allocation-free scratch array (wfft.hip)
   void* scratch = nullptr;
 

@@ -948,15 +948,18 @@ static void ypass(mc_ctx* ctx, const WParams& w, const Shapes& s, v2<T>* A, v2<T
 }
 
 // Pairs per transform batch: at most P, and under a device memory limit as
-// many as fit beside everything else the library holds (the workspaces are
-// shrunk when they hold more); MC_ERR_ARG when not even one pair fits.
+// many as fit beside everything else the library holds and a reserve for the
+// buffers the rest of a step allocates after the transforms (vofrho's dense
+// mesh, contraction partials: max(256 MiB, limit / 16)); the workspaces are
+// shrunk when they hold more; MC_ERR_ARG when not even one pair fits.
 template <typename T>
 static int fit_pairs(mc_ctx* ctx, int P, size_t extra) {
   const size_t lim = mem_limit();
   if (!lim) return P;
   const size_t per = sizeof(v2<T>) * ((size_t)ctx->n_slots * ctx->n3 + (size_t)ctx->n3 * ctx->ncols * ctx->n2);
   const size_t held = ctx->w_a.bytes + ctx->w_b.bytes;
-  const size_t others = mem_current() - std::min(held, mem_current()) + extra;
+  const size_t reserve = std::max((size_t)256 << 20, lim / 16);
+  const size_t others = mem_current() - std::min(held, mem_current()) + extra + reserve;
   const size_t avail = lim > others ? lim - others : 0;
   const long long fit = (long long)(avail / per);
   if (fit < 1)
@@ -1041,11 +1044,17 @@ static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, in
   }
   for (int b = 0; b < nblk_sum; ++b) sr += hs[(size_t)npairs * 4 + b];
   double cr = sr * ctx->omega / ((double)ctx->n1 * ctx->n2 * ctx->n3);
+  ctx->last_csumg = sg;
+  ctx->last_csumr_raw = cr;
   if (std::is_same<T, float>::value && cr > 0.0 && sg > 0.0) {
-    // FP32 transforms lose ~1e-7 of the charge (a systematic rounding of the
-    // norm, csumr - csumg ~ -1.6e-4 at 1024 electrons); rescale rho so that
-    // its integral is the FP64 G-space charge csumg (the invariant CPMD's
-    // csumr/csumg check enforces, rhoofr:625-635)
+    // The FP32 engine contracts norms systematically: its DFT constants and
+    // twiddles rounded to FP32 make each 192-point transform scale |x|^2 by
+    // 1 - 5.9e-8 on average (the rounded sin(2 pi/3), cos(pi/8), ... lie
+    // below their exact values), so the three inverse passes take 1.8e-7 of
+    // the charge (csumr - csumg = -1.6e-4 at 1024 electrons).  rho is
+    // rescaled so that its integral is the FP64 G-space charge csumg (the
+    // invariant of CPMD's csumr / csumg check, rhoofr:625-635); the raw sum
+    // stays available through mc_rhoofr_raw_charge.
     const double scale = sg / cr;
     LAUNCH(MC_KID_MISC, k_scale, dim3(1024), dim3(256), 0, rho, nnr1, scale);
     cr *= scale;

@@ -268,6 +268,12 @@ class Context:
                                   C.byref(cr)))
         return ek.value, cg.value, cr.value
 
+    def rhoofr_raw_charge(self) -> tuple[float, float]:
+        """(csumg, csumr before the FP32 rescale) of the last rhoofr."""
+        g, r = C.c_double(), C.c_double()
+        check(lib().mc_rhoofr_raw_charge(self.h, C.byref(g), C.byref(r)))
+        return g.value, r.value
+
     # ------------------------------------------------------------ vpsi
     def vpsi(self, c0: np.ndarray, c2: np.ndarray, f: np.ndarray, vpot: np.ndarray) -> np.ndarray:
         """c2 += vpsi force (returned; c2 is updated in place when it is a Fortran-ordered complex128)."""

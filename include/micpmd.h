@@ -131,9 +131,10 @@ int mc_memset_d(void* dst, int value, size_t bytes);
  * contexts and mc_dev_alloc blocks; not pinned host staging, not the fixed
  * array of mc_vpsi_dev_noalloc).  With a limit set (bytes > 0), rhoofr and
  * vpsi transform as many state pairs per batch as fit beside what is already
- * held (shrinking their workspaces when needed; the per-pair results do not
- * depend on the batch), and any other allocation that would pass the limit
- * fails with MC_ERR_ARG instead of being made.  0 removes the limit. */
+ * held and a reserve of max(256 MiB, limit / 16) for the buffers allocated
+ * later in a step (shrinking their workspaces when needed; the per-pair
+ * results do not depend on the batch), and any allocation that would pass
+ * the limit fails with MC_ERR_ARG instead of being made.  0 removes it. */
 int mc_set_mem_limit(size_t bytes);
 /* Bytes held now, the most held since the last reset, and the limit. */
 int mc_mem_stats(size_t* current, size_t* peak, size_t* limit);
@@ -266,6 +267,11 @@ int mc_rhoofr(mc_ctx* ctx, const double* c0, int ldc, const double* f, int nstat
               double* rhoe, double* ekin, double* csumg, double* csumr);
 int mc_rhoofr_dev(mc_ctx* ctx, const void* c0_dev, const double* f, int nstate,
                   void* rhoe_dev, double* ekin, double* csumg, double* csumr);
+/* csumg and the real-space charge of the last rhoofr before the FP32 rescale
+ * (equal to the returned csumr in FP64).  The FP32 transforms contract
+ * norms by a deterministic 1.8e-7 at 192^3 (their rounded DFT constants and
+ * twiddles), which this value shows. */
+int mc_rhoofr_raw_charge(mc_ctx* ctx, double* csumg, double* csumr_raw);
 
 /* vpsi: c2(:,i) += -(f_i/2)(tpiba2 hg c0(:,i) + 2 P_G[V psi_i]) with f=0 -> 1
  * (Gamma, closed shell, no LSE/CDFT/TDG/MGGA branches).

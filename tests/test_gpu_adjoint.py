@@ -45,8 +45,9 @@ def cfg3(request):
         n1, n2, n3 = sysm.cell.nr
         per_pair = 16 * (s * n3 + cols * n2 * n3)
         cur, _, _ = ops.mem_stats()
-        ops.set_mem_limit(cur - 6 * per_pair)
-        print(f"cfg3 with the device memory capped at {cur - 6 * per_pair} bytes", flush=True)
+        cap = cur - 6 * per_pair + (256 << 20)   # the library keeps 256 MiB for what a step allocates later
+        ops.set_mem_limit(cap)
+        print(f"cfg3 with the device memory capped at {cap} bytes", flush=True)
     yield sysm, K, c, ks, d_c, d_c2, d_p
     ops.set_mem_limit(0)
     ks.ctx.close()

@@ -262,3 +262,64 @@ def test_forces_c2_combined(gpu, nstate):
         a, b = out.to_host(nd), ref.to_host(nd)
         assert np.array_equal(a, b)
         assert np.abs(b).max() > 0.0
+
+
+def test_cfg4_full_size_mixed_against_fp64(gpu):
+    """Config 4 at full size (512 states, 192^3, 256 atoms): from the seeded
+    coefficients after 5 FP64 steepest-descent steps, E_KS and F_ion of the
+    mixed path against FP64 at the same C and R (|dE_KS| < 1e-6 Ha/atom,
+    max |dF_ion| < 1e-5 Ha/bohr), and ||C^T C - I|| < 1e-6 after one mixed
+    mc_posupa_dev (velupa + posupa + rortog, dt 1 a.u.)."""
+    from cpmd_amd.ks import KSDevice
+    cfg = S.CONFIGS["cfg4"]
+    sysm = S.make_system(cfg["a"], cfg["nat"], cfg["ecut"], nstate=cfg["nstate"], with_density=True)
+    n, nat, f = sysm.nstate, sysm.nat, sysm.f
+    ks = KSDevice(sysm)
+    ctx = ks.ctx
+    buf = {k: ctx.alloc_wfn(n) for k in ("c0", "cm", "cp", "c2")}
+    ctx.set_precision("fp64")
+    ctx.upload(S.initial_coefficients(sysm), buf["c0"])
+    ks.set_ions(sysm.tau)
+    for _ in range(5):
+        ks.energy(buf["c0"], f, gradient=buf["c2"])
+        ctx.lagrange_dev(buf["c0"], buf["c2"], f, n)
+        ctx.axpy_wfn(0.02, buf["c2"], buf["c0"], n)
+        ctx.lowdin_dev(buf["c0"], n)
+    print("cfg4: FP64 and mixed E_KS / F_ion at the same state", flush=True)
+    e64 = ks.energy(buf["c0"], f, tau=sysm.tau, forces=True)
+    ctx.set_precision("mixed")
+    emx = ks.energy(buf["c0"], f, tau=sysm.tau, gradient=buf["c2"], forces=True)
+    assert emx["etot"] != e64["etot"]
+    assert abs(emx["etot"] - e64["etot"]) / nat < 1e-6
+    assert np.abs(emx["fion"] - e64["fion"]).max() < 1e-5
+    ctx.axpby_wfn(0.0, buf["c0"], 0.0, buf["cm"], n)
+    _, it = ctx.posupa_dev(buf["c0"], buf["c2"], buf["cm"], buf["cp"], n, 0.5 / 400.0, 1.0, epsog=1e-8, maxit=60)
+    ctx.set_precision("fp64")
+    ov = ctx.ovlap_dev(buf["cp"], buf["cp"], n)
+    assert np.abs(ov - np.eye(n)).max() < 1e-6
+
+
+
+def test_fp32_charge_loss_is_the_engine_contraction(gpu):
+    """The FP32 rhoofr's real-space charge before its rescale, against the
+    first-order prediction from the FP32 engine's per-frequency norm
+    contraction (oracle/fp32_engine.py) for config 2's 128 states on 128^3:
+    the measured loss is that deterministic contraction (within 5%), and
+    after the rescale the returned csumr equals csumg."""
+    from oracle import fp32_engine as E
+    from util import config_system
+    sysm, c = config_system("cfg2")
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    ctx.set_precision("fp32")
+    rho, ek, cg, cr = ctx.rhoofr(c, sysm.f)
+    g, cr_raw = ctx.rhoofr_raw_charge()
+    assert g == cg and cr == pytest.approx(cg, rel=1e-14)
+    measured = (cr_raw - cg) / cg
+    predicted = E.predicted_charge_loss(sysm.cell, sysm.gset.miller, c, sysm.f)
+    print(f"FP32 charge loss: measured {measured:.4e}, predicted {predicted:.4e}", flush=True)
+    assert predicted < -5e-8
+    assert measured == pytest.approx(predicted, rel=0.05)
+    ctx.set_precision("fp64")
+    ctx.rhoofr(c, sysm.f)
+    g64, cr64 = ctx.rhoofr_raw_charge()
+    assert cr64 == pytest.approx(g64, rel=1e-13)          # FP64: nothing to rescale

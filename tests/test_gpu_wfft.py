@@ -116,6 +116,30 @@ def test_cfg2_parity_subset(gpu):
     _check_vpsi(ctx, L, c, f, S.real_space_potential(sysm))
 
 
+def test_cfg2_all_states(gpu):
+    """rhoofr and vpsi on all 128 states of config 2 (128^3, 68,338 plane
+    waves) against the oracle's threaded CPU path (oracle/cpu_step: the same
+    arithmetic as oracle.c, checked against it to 1e-14 in
+    tests/test_oracle_cpu_step.py), at the FP64 parity tolerances."""
+    from oracle.cpu_step import CpuStep
+    sysm, c = config_system("cfg2")
+    assert c.shape[1] == 128
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    cs = CpuStep(sysm.cell, sysm.gset)
+    f = sysm.f.copy()
+    f[-3:] = [1.0, 0.0, 0.0]                       # a partly filled and two empty states
+    rho_g, ek_g, cg_g, cr_g = ctx.rhoofr(c, f)
+    rho_o = cs.rhoofr(c, f)
+    assert np.abs(rho_g - rho_o).max() < RHO_TOL
+    assert cr_g == pytest.approx(cg_g, rel=1e-12)  # Parseval: the FP64 charge sums agree
+    v = S.real_space_potential(sysm)
+    rng = np.random.default_rng(8)
+    c2 = np.asfortranarray(rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape))
+    out_g = ctx.vpsi(c, c2.copy(order="F"), f, v)
+    out_o = cs.vpsi(c, c2, f, v)
+    assert np.abs(out_g - out_o).max() < C2_TOL * np.abs(out_o).max()
+
+
 def test_device_resident_roundtrip(gpu):
     sysm, c = config_system("cfg1")
     ctx = ops.Context(sysm.cell, sysm.gset)
