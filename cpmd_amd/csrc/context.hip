@@ -22,6 +22,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <vector>
 
 #include "fft_device.h"
 #include "mc_internal.h"
@@ -42,6 +43,7 @@ struct MemTrack {
   std::mutex mu;
   size_t cur = 0, peak = 0, limit = 0;
   std::map<void*, size_t> user;   // mc_dev_alloc'ed blocks
+  std::vector<DevBuf*> ws;        // reclaimable workspaces
 };
 MemTrack& mem() {
   static MemTrack* t = new MemTrack();   // never destroyed: buffers may be freed at exit
@@ -64,7 +66,60 @@ void mem_give(size_t bytes) {
   std::lock_guard<std::mutex> lk(t.mu);
   t.cur -= std::min(bytes, t.cur);
 }
+// frees the registered workspaces no call is using (other than `keep`);
+// returns the bytes freed.  They are detached under the lock, so a call that
+// marks them in use afterwards finds them empty and sizes them again.  hipFree
+// waits for the device, so kernels still queued on them finish first.
+size_t ws_reclaim(const DevBuf* keep) {
+  std::vector<std::pair<void*, size_t>> drop;
+  {
+    auto& t = mem();
+    std::lock_guard<std::mutex> lk(t.mu);
+    for (DevBuf* b : t.ws)
+      if (b != keep && b->in_use == 0 && b->p) {
+        drop.emplace_back(b->p, b->bytes);
+        b->p = nullptr;
+        b->bytes = 0;
+      }
+  }
+  size_t freed = 0;
+  for (const auto& d : drop) {
+    (void)hipFree(d.first);
+    mem_give(d.second);
+    freed += d.second;
+  }
+  return freed;
+}
+// mem_take with one retry after freeing the idle workspaces
+void take_or_reclaim(size_t add, const char* what, const DevBuf* self) {
+  try {
+    mem_take(add, 0, what);
+  } catch (const Error&) {
+    if (!ws_reclaim(self)) throw;
+    mem_take(add, 0, what);
+  }
+}
 }  // namespace
+
+void ws_register(DevBuf* b) {
+  std::lock_guard<std::mutex> lk(mem().mu);
+  mem().ws.push_back(b);
+}
+void ws_unregister(DevBuf* b) {
+  std::lock_guard<std::mutex> lk(mem().mu);
+  auto& v = mem().ws;
+  v.erase(std::remove(v.begin(), v.end(), b), v.end());
+}
+WsUse::WsUse(DevBuf& x, DevBuf& y) : a(x), b(y) {
+  std::lock_guard<std::mutex> lk(mem().mu);
+  ++a.in_use;
+  ++b.in_use;
+}
+WsUse::~WsUse() {
+  std::lock_guard<std::mutex> lk(mem().mu);
+  --a.in_use;
+  --b.in_use;
+}
 
 size_t mem_limit() {
   std::lock_guard<std::mutex> lk(mem().mu);
@@ -79,7 +134,7 @@ void DevBuf::ensure(size_t nbytes) {
   if (nbytes <= bytes) return;
   release();
   if (nbytes == 0) return;
-  mem_take(nbytes, 0, "device buffer");
+  take_or_reclaim(nbytes, "device buffer", this);
   const hipError_t e = hipMalloc(&p, nbytes);
   if (e != hipSuccess) {
     p = nullptr;
@@ -834,7 +889,7 @@ int mc_dev_alloc(size_t bytes, void** ptr) {
   MC_CHECK(ptr, MC_ERR_ARG, "null pointer");
   *ptr = nullptr;
   if (bytes) {
-    mc::mem_take(bytes, 0, "mc_dev_alloc");
+    mc::take_or_reclaim(bytes, "mc_dev_alloc", nullptr);
     const hipError_t e = hipMalloc(ptr, bytes);
     if (e != hipSuccess) {
       *ptr = nullptr;

@@ -77,11 +77,25 @@ size_t mem_current();
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  int in_use = 0;   // workspaces: > 0 while a call uses them (never reclaimed then)
   void ensure(size_t nbytes);
   void release();
   ~DevBuf() { release(); }
   template <class T>
   T* as() const { return static_cast<T*>(p); }
+};
+
+// Reclaimable workspaces (the transform batch buffers of every context).  An
+// allocation that would take the library beyond the device memory limit first
+// frees the registered workspaces that no call is using, then tries again;
+// the next transform call sizes them again to what is left.
+void ws_register(DevBuf* b);
+void ws_unregister(DevBuf* b);
+// Marks two workspaces in use for a scope (before they are sized).
+struct WsUse {
+  DevBuf &a, &b;
+  WsUse(DevBuf& x, DevBuf& y);
+  ~WsUse();
 };
 
 // bf16 part planes of up to two tall operands (gemm_planes.hip).  Inside a
@@ -166,6 +180,17 @@ struct DensitySet {
 
 // The context: static layout of the plane-wave set on one GPU plus workspaces.
 struct mc_ctx {
+  mc_ctx() {
+    mc::ws_register(&w_a);
+    mc::ws_register(&w_b);
+  }
+  ~mc_ctx() {
+    mc::ws_unregister(&w_a);
+    mc::ws_unregister(&w_b);
+  }
+  mc_ctx(const mc_ctx&) = delete;
+  mc_ctx& operator=(const mc_ctx&) = delete;
+
   // mesh and cell
   int n1 = 0, n2 = 0, n3 = 0;     // transform lengths
   int kr1 = 0, kr2 = 0, kr3 = 0;  // padded real-space leading dimensions

@@ -951,7 +951,9 @@ static void ypass(mc_ctx* ctx, const WParams& w, const Shapes& s, v2<T>* A, v2<T
 // many as fit beside everything else the library holds and a reserve for the
 // buffers the rest of a step allocates after the transforms (vofrho's dense
 // mesh, contraction partials: max(256 MiB, limit / 16)); the workspaces are
-// shrunk when they hold more; MC_ERR_ARG when not even one pair fits.
+// shrunk when they hold more; MC_ERR_ARG when not even one pair fits.  A later
+// allocation that still does not fit frees them while they are idle
+// (ws_reclaim), and the next transform call sizes them again.
 template <typename T>
 static int fit_pairs(mc_ctx* ctx, int P, size_t extra) {
   const size_t lim = mem_limit();
@@ -995,6 +997,7 @@ static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, in
   const Shapes s = shapes<T>(ctx);
   const int npairs = (nstate + 1) / 2;
   const int nblk_sum = 1024;
+  WsUse use(ctx->w_a, ctx->w_b);
   const size_t red_bytes =
       sizeof(double) * ((size_t)std::max(npairs, 1) * s.gx * 4 + (size_t)npairs * 4 + nblk_sum) +
       sizeof(double) * std::max(nstate, 1);
@@ -1110,6 +1113,7 @@ static void vpsi_impl(mc_ctx* ctx, const double2* c0, double2* c2, const double*
   const int npairs = (nstate + 1) / 2;
   if (npairs == 0) return;
   const size_t f_bytes = sizeof(double) * std::max(nstate, 1);
+  WsUse use(ctx->w_a, ctx->w_b);
   const int P = fit_pairs<T>(ctx, std::max(1, std::min(ctx->batch_pairs_vpsi, npairs)),
                              f_bytes > ctx->w_f.bytes ? f_bytes - ctx->w_f.bytes : 0);
   ensure_workspace<T>(ctx, P);
