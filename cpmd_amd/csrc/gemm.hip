@@ -82,11 +82,11 @@ __device__ __forceinline__ int xcd_tile(int total) {
 }
 
 // ---------------------------------------------------------------------------
-// A^T B partials.  Launch ceil(total/8)*8 workgroups, total = nsplit * tm * tn.
+// A^T B partials.  Launch ceil(total/8)*8 workgroups, total = nsplit * tile_count(upper, tm, tn).
 //   A: (K x m) column-major, lda; if proj.eigr != nullptr, A is the projector
 //      operand (K = 2*ngw rows: 2g = Re, 2g+1 = Im).  B: (K x n), ldb.
 //   P: partials [nsplit][n][m] (column-major m x n per split).
-// upper: tiles with tile_row > tile_col are skipped.
+// upper (m == n): only the tiles on or above the diagonal are launched.
 // ---------------------------------------------------------------------------
 template <int BT>
 __global__ __launch_bounds__(256) void k_gemm_tn(const double* __restrict__ A, int lda,
@@ -99,10 +99,10 @@ __global__ __launch_bounds__(256) void k_gemm_tn(const double* __restrict__ A, i
   __shared__ double Bs[kBK * LD];
   const int t = xcd_tile(total);
   if (t >= total) return;
-  const int s = t / (tm * tn);
-  const int tile = t - s * tm * tn;
-  const int ti = tile / tn, tj = tile - ti * tn;
-  if (upper && ti > tj) return;
+  const int tiles = tile_count(upper, tm, tn);
+  const int s = t / tiles;
+  int ti, tj;
+  tile_ij(t - s * tiles, upper, tm, tn, ti, tj);
   const int i0 = ti * BT, j0 = tj * BT;
   const int kbeg = s * kchunk;
   const int kend = min(K, kbeg + kchunk);
@@ -375,7 +375,8 @@ void gemm_tn(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, in
   // 128-tiles for long K, 64-tiles (more workgroups) for short K
   const int BT = K >= 16384 ? 128 : 64;
   const int tm = (m + BT - 1) / BT, tn = (n + BT - 1) / BT;
-  const int tiles = symmetric ? (tm * (tm + 1)) / 2 : tm * tn;
+  if (symmetric && m != n) throw Error{MC_ERR_INTERNAL, "symmetric A^T B with m != n"};
+  const int tiles = tile_count(symmetric, tm, tn);
   // about 1024 busy workgroups (4 per CU), K chunks of at least 512 rows
   int nsplit = std::max(1, std::min((1024 + tiles - 1) / tiles, std::max(1, K / 512)));
   int kchunk = (K + nsplit - 1) / nsplit;
@@ -384,7 +385,7 @@ void gemm_tn(mc_ctx* ctx, const double* A, int lda, const double* B, int ldb, in
   ctx->w_gemm.ensure(sizeof(double) * (size_t)nsplit * m * n);
   ProjA pa{};
   if (proj) pa = *proj;
-  const int total = nsplit * tm * tn;
+  const int total = nsplit * tiles;
   const int launch = (total + kXcd - 1) / kXcd * kXcd;
   timer_begin(ctx, MC_KID_GEMM_TN);
   if (BT == 128)

@@ -5,7 +5,7 @@
 // bits, and stored as "planes": blocks of 128 columns x 32 rows, the NP parts
 // of a block contiguous (NP x 8 KiB), blocks of one 128-column group
 // consecutive along K.  Inside a part, column c holds its 32 rows in 64 bytes
-// as four 16-byte chunks (k = 8q .. 8q+7), chunk q stored at q ^ ((c >> 2) & 3):
+// as four 16-byte chunks (k = 8q .. 8q+7), chunk q stored at q ^ ((c >> 1) & 3):
 // the MFMA operand reads (16 lanes = 16 columns, one chunk each) then hit 16
 // distinct 16-byte bank groups.
 //
@@ -278,8 +278,9 @@ __global__ __launch_bounds__(256) void k_proj_planes_stacked(ProjA pa, const dou
 
 // ---------------------------------------------------------------------------
 // A^T B partials from planes: P[s] (m x n, FP64) over the K blocks
-// [s kbper, (s+1) kbper); launch ceil(total/8)*8 workgroups of 256, total =
-// nsplit * tm * tn; upper: skip tiles below the diagonal.
+// [s kbper, (s+1) kbper); launch ceil(total/8)*8 workgroups, total = nsplit *
+// tiles, tiles = tm tn, or with upper (m == n) the tm (tm + 1) / 2 tiles on or
+// above the diagonal.
 // ---------------------------------------------------------------------------
 template <int NP>
 __global__ __launch_bounds__(64 * kTnWaves, 1) void k_tn_planes(const __bf16* __restrict__ Apl,
@@ -298,10 +299,10 @@ __global__ __launch_bounds__(64 * kTnWaves, 1) void k_tn_planes(const __bf16* __
   __shared__ __attribute__((aligned(16))) unsigned char ring[kStages * kStage];
   const int t = xcd_tile_p(total);
   if (t >= total) return;
-  const int s = t / (tm * tn);
-  const int tile = t - s * tm * tn;
-  const int ti = tile / tn, tj = tile - ti * tn;
-  if (upper && ti > tj) return;
+  const int tiles = tile_count(upper, tm, tn);
+  const int s = t / tiles;
+  int ti, tj;
+  tile_ij(t - s * tiles, upper, tm, tn, ti, tj);
   const int kb0 = s * kbper, kb1 = min(nkb, kb0 + kbper);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const unsigned char* Ab = reinterpret_cast<const unsigned char*>(Apl) + (size_t)ti * nkb * NP * kPart;
@@ -704,14 +705,15 @@ int tn_planes_partials(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np
                        bool symmetric) {
   const int nkb = (K + kPK - 1) / kPK;
   const int tm = (m + kPT - 1) / kPT, tn = (n + kPT - 1) / kPT;
-  const int tiles = symmetric ? (tm * (tm + 1)) / 2 : tm * tn;
+  if (symmetric && m != n) throw Error{MC_ERR_INTERNAL, "symmetric A^T B with m != n"};
+  const int tiles = tile_count(symmetric, tm, tn);
   // one workgroup per CU (the ring takes 96-144 KiB of LDS): at least two
   // rounds of 256 workgroups, K chunks of at most kMaxSlices slices
   int nsplit = std::max(1, std::min(nkb, std::max((512 + tiles - 1) / tiles, (nkb + kMaxSlices - 1) / kMaxSlices)));
   const int kbper = (nkb + nsplit - 1) / nsplit;
   nsplit = (nkb + kbper - 1) / kbper;
   ctx->w_gemm.ensure(sizeof(double) * (size_t)nsplit * m * n);
-  const int total = nsplit * tm * tn;
+  const int total = nsplit * tiles;
   const int launch = (total + kXcdP - 1) / kXcdP * kXcdP;
   timer_begin(ctx, MC_KID_GEMM_TN);
   if (np == 3)

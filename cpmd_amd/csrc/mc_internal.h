@@ -294,6 +294,24 @@ void timer_flush(mc_ctx* ctx);
 __global__ void k_zero(double* __restrict__ x, size_t n);
 __global__ void k_fill(double* __restrict__ x, size_t n, double v);
 
+// Output tiles of an A^T B launch: the tm x tn tiles row-major, or with upper
+// (a symmetric product, tm == tn) only the tm (tm + 1) / 2 tiles on or above
+// the diagonal, row by row (row ti holds tm - ti of them).
+__host__ __device__ inline int tile_count(bool upper, int tm, int tn) { return upper ? tm * (tm + 1) / 2 : tm * tn; }
+__device__ __forceinline__ void tile_ij(int tile, bool upper, int tm, int tn, int& ti, int& tj) {
+  if (upper) {
+    ti = 0;
+    while (tile >= tm - ti) {
+      tile -= tm - ti;
+      ++ti;
+    }
+    tj = ti + tile;
+  } else {
+    ti = tile / tn;
+    tj = tile - ti * tn;
+  }
+}
+
 // vofrho / ewald (vofrho.hip)
 void set_density_gset(mc_ctx* ctx, int nhg, const int* inyh, const double* hg, const double* gk);
 void set_local_pp(mc_ctx* ctx, int nsp, const int* na, const double* zv, const double* raggio, const double* vps,
