@@ -173,6 +173,24 @@ AMU = 1822.888486                  # electron masses per atomic mass unit
 ION_MASS = 12.0 * AMU             # the synthetic species (Z = 6)
 
 
+def pmc_copy(mib: int, repeat: int = 3) -> int:
+    """y = 1 x + 0 y over whole wavefunction columns (mc_wfn_axpby): reads and
+    writes about `mib` MiB per call, for scaling the PMC byte counters."""
+    from cpmd_amd import ops
+    from cpmd_amd import system as S
+    sysm = S.make_system(12.0, 8, 40.0, nstate=16)
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    per_state = 16 * ctx.ld
+    nstate = max(1, (mib << 20) // per_state)
+    x = ctx.alloc_wfn(nstate)
+    y = ctx.alloc_wfn(nstate)
+    for _ in range(repeat):
+        ctx.axpby_wfn(1.0, x, 0.0, y, nstate)
+    ctx.synchronize()
+    print(f"copy: {repeat} x {nstate * per_state} bytes read and written by mc::k_axpby", flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,6 +222,9 @@ def main():
                     help="optimisation time step (CPMD TIMESTEP; hthrs 2); its bound from the Hartree "
                          "curvature at the smallest G falls as 1/a: 10 for a = 24 bohr, 7 for a = 32")
     ap.add_argument("--opt-maxiter", type=int, default=4000)
+    ap.add_argument("--pmc-copy", type=int, default=0, metavar="MIB",
+                    help="only run a copy of MIB MiB (read and written) three times through mc::k_axpby: "
+                         "the known-size kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -217,6 +238,8 @@ def main():
     from cpmd_amd import system as S
     from cpmd_amd.ks import KSDevice
 
+    if args.pmc_copy:
+        return pmc_copy(args.pmc_copy)
     cfg = S.CONFIGS[args.config]
     nstate = args.nstate or cfg["nstate"]
     # CP from the optimised ground state runs at dt = 4 a.u.; from a
@@ -589,4 +612,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    sys.exit(main() or 0)
