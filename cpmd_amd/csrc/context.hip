@@ -1119,6 +1119,7 @@ int mc_odiis_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, const double* f,
       if (g1 / g2 > raim && D.gnorm[D.nowv - 1] / D.gnorm[nowm - 1] > raim) {
         D.istate = 0;
         D.gamma = 0.0;
+        *reinit = 1;   // returned to the caller, as odiis returns reinit (:584-593)
       }
     }
   }

@@ -222,11 +222,13 @@ class Context:
     def odiis_dev(self, c0: DeviceArray, c2: DeviceArray, f: np.ndarray, nstate: int, etot: float, svar2: float,
                   mdiis: int = 10, reinit: bool = False, nreset: int | None = None):
         """One DIIS step of the occupied states (CPMD's odiis); the history stays in the context.
-        nreset defaults to mdiis, CPMD's setting for wavefunction optimisation."""
+        nreset defaults to mdiis, CPMD's setting for wavefunction optimisation.  Returns True
+        when the step restarted the history for insufficient progress (odiis's reinit)."""
         f = np.ascontiguousarray(f, dtype=np.float64)
         r = C.c_int(int(reinit))
         nr = mdiis if nreset is None else nreset
         check(lib().mc_odiis_dev(self.h, c0.ptr, c2.ptr, _p(f), nstate, etot, svar2, mdiis, nr, C.byref(r)))
+        return bool(r.value)
 
     def odiis_release(self):
         """Free the ODIIS history held by the context."""

@@ -227,14 +227,15 @@ int mc_sd_update_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, int nstate, 
  * from the DIIS equations over <c2_i| 2 ff^2 vpp'^2 |c2_j> (solved as LAPACK
  * dgelss, rcond 0.2e-15), ff = 1/f with prec (f > 0.1), vpp' = vpp / (1 +
  * vpp gamma), gamma += 1/vpp(G=0) when etot rose since the last call, else
- * gamma *= 0.75; empty states: c0 += svar2 c2.  *reinit != 0 (or a new
- * nstate / mdiis) restarts the history; it is cleared on return.  With
- * nreset >= 3 the history also restarts once more than nreset steps are
- * stored and neither the largest gradient element (grimax) nor the gradient
- * norm fell below (1 - mdiis/100) of the oldest stored one (CPMD sets
- * nreset = mdiis for wavefunction optimisation,
- * src/control_test_utils.mod.F90:452-465).  The orthonormalisation (rgs)
- * follows as in updwf.
+ * gamma *= 0.75; empty states: c0 += svar2 c2.  *reinit is in/out: != 0 on
+ * entry (or a new nstate / mdiis) restarts the history; on return it is 1
+ * when this call restarted the history for insufficient progress, else 0
+ * (CPMD's updwf then counts a DIIS reset, src/updwf_utils.mod.F90:102-109).
+ * The progress test (nreset >= 3): more than nreset steps are stored and
+ * neither the largest gradient element (grimax) nor the gradient norm fell
+ * below (1 - mdiis/100) of the oldest stored one (CPMD sets nreset = mdiis
+ * for wavefunction optimisation, src/control_test_utils.mod.F90:452-465).
+ * The orthonormalisation (rgs) follows as in updwf.
  * Replaces odiis(c0,c2,vpp,nstate,pme,gde,svar2,reinit),
  * src/odiis_utils.mod.F90:492-740 (closed shell, uncompressed history). */
 int mc_odiis_dev(mc_ctx* ctx, void* c0_dev, const void* c2_dev, const double* f, int nstate, double etot,

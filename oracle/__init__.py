@@ -273,6 +273,7 @@ class ODIIS:
             self.ndiis = self.nowv = self.nsize = 0
     def step(self, c0, c2, vpp, f, etot, svar2, reinit=False, geq0=True):
         n = c0.shape[1]
+        self.reset = False
         if reinit or self.istate == 0:
             self.istate, self.eold, self.gamma = 0, 9999.0, 0.0
             self.nocc = int(np.count_nonzero(np.asarray(f) > 1e-5))
@@ -293,6 +294,7 @@ class ODIIS:
                 raim = 1.0 - self.mdiis / 100.0
                 if g1 / g2 > raim and self.gnorm[self.nowv - 1] / self.gnorm[nowm - 1] > raim:
                     self.istate, self.gamma = 0, 0.0
+                    self.reset = True          # odiis returns reinit = .TRUE. (:584-593)
         if self.istate == 0:
             self._updis(0)
         self.istate = 2

@@ -65,6 +65,35 @@ def test_rgs_not_positive_definite(gpu, cfg1_70):
     assert e.value.code == 3 and "6" in str(e.value)
 
 
+@pytest.mark.parametrize("n", [200, 300])
+def test_rgs_blocked(gpu, n):
+    """The blocked Cholesky beyond two blocks: 64-column diagonal blocks with
+    a ragged last one (200 = 3 x 64 + 8, 300 = 4 x 64 + 44), several trsm
+    workgroups and update tiles above and below the diagonal."""
+    sysm = S.make_system(12.0, 8, 40.0, nstate=n)
+    rng = np.random.default_rng(n)
+    ngw = sysm.gset.ngw
+    cp = (rng.standard_normal((ngw, n)) + 1j * rng.standard_normal((ngw, n))) * np.exp(
+        -sysm.gset.hg[:ngw, None] * sysm.cell.tpiba2 / 8)
+    cp[0] = cp[0].real
+    cp = np.asfortranarray(cp)
+    ctx = ops.Context(sysm.cell, sysm.gset)
+    g, smat = ctx.rgs(cp)
+    o, osmat = oracle.rgs(cp)
+    assert rel(smat, osmat) < 1e-12
+    assert rel(g, o) < 1e-12
+    assert np.abs(S.dotp_gamma(g, g) - np.eye(n)).max() < 1e-12
+    # a null state in the third block: the leading minor of order 151 fails,
+    # reported by the diagonal step of block 2 (info path of a later block)
+    bad = np.asfortranarray(cp.copy())
+    bad[:, 150] = 0.0
+    with pytest.raises(_lib.McError) as e:
+        ctx.rgs(bad)
+    assert e.value.code == 3 and "order 151 " in str(e.value)
+    with pytest.raises(np.linalg.LinAlgError, match="leading minor 151 "):
+        oracle.rgs(bad)
+
+
 def test_lowdin(gpu, cfg1_70):
     sysm, c, cp = cfg1_70
     ctx = ops.Context(sysm.cell, sysm.gset)

@@ -131,18 +131,21 @@ def test_odiis_parity(small):
     rng = np.random.default_rng(21)
     cur = c.copy()
     d0, d2 = ctx.alloc_wfn(n), ctx.alloc_wfn(n)
+    resets = []
     for k, e in enumerate([10.0, 9.0, 9.5, 8.0, 7.0, 6.5, 6.0]):
         c2 = np.asfortranarray(0.05 * (rng.standard_normal((ngw, n)) + 1j * rng.standard_normal((ngw, n))))
         c2[0] = c2[0].real
         ctx.upload(cur, d0)
         ctx.upload(c2, d2)
-        ctx.odiis_dev(d0, d2, f, n, e, svar2, mdiis=3, reinit=(k == 0))
+        reset = ctx.odiis_dev(d0, d2, f, n, e, svar2, mdiis=3, reinit=(k == 0))
         got = ctx.download(d0, n)
         want = D.step(cur, c2, vpp, f, e, svar2)
         err = np.abs(got - want).max() / np.abs(want).max()
         assert err < 1e-11, (k, err)
+        assert reset == D.reset, k       # the reset is reported where the oracle's fires
+        resets.append(reset)
         cur = want
-    assert D.ndiis < 7                 # the history was restarted on the way
+    assert D.ndiis < 7 and any(resets)   # the history was restarted on the way
 
 
 def test_optimize_diis_mixed_then_fp64(gpu):
