@@ -204,13 +204,15 @@ FftPlan make_plan(int n) {
   MC_CHECK(n >= 2 && n <= 2048, MC_ERR_UNSUPPORTED,
            "transform length " + std::to_string(n) + " outside [2, 2048]");
   int m = n;
-  // radix-4 first, then one radix-2, then 3s and 5s
+  // radix-4 first, then one radix-2, then 3s, 5s and 7s (the 7s: lengths of
+  // the FFTW table of src/fftchk_utils.mod.F90:73-97)
   while (m % 4 == 0) { pl.rad[pl.nf++] = 4; m /= 4; }
   while (m % 2 == 0) { pl.rad[pl.nf++] = 2; m /= 2; }
   while (m % 3 == 0) { pl.rad[pl.nf++] = 3; m /= 3; }
   while (m % 5 == 0) { pl.rad[pl.nf++] = 5; m /= 5; }
+  while (m % 7 == 0) { pl.rad[pl.nf++] = 7; m /= 7; }
   MC_CHECK(m == 1 && pl.nf <= kMaxStages, MC_ERR_UNSUPPORTED,
-           "transform length " + std::to_string(n) + " is not of the form 2^a 3^b 5^c");
+           "transform length " + std::to_string(n) + " is not of the form 2^a 3^b 5^c 7^d");
   return pl;
 }
 

@@ -13,8 +13,8 @@
 //    (stride B), multiplies by w_N^{n2 k1}, the lines are transposed in LDS and
 //    each thread does one length-B DFT in registers.  Two LDS round trips per
 //    transform.  Register DFTs of composite length recurse at compile time into
-//    radix 2/3/4/5 butterflies with exact constant twiddles (twiddle_consts.h).
-//  * fft_lines<T, DIR>: any 2^a 3^b 5^c length from a runtime plan: mixed-radix
+//    radix 2/3/4/5/7 butterflies with exact constant twiddles (twiddle_consts.h).
+//  * fft_lines<T, DIR>: any 2^a 3^b 5^c 7^d length from a runtime plan: mixed-radix
 //    Stockham stages; stage s with radix R and p = product of previous radices
 //    maps butterfly i of a line to inputs x[i + q n/R], twiddles w^{q k n/(pR)}
 //    (k = i mod p) and outputs y[(i-k) R + k + s p], in place with two barriers.
@@ -102,9 +102,43 @@ __device__ __forceinline__ void bfly5(V* v) {
   v[2] = cadd(a2, b2); v[3] = csub(a2, b2);
 }
 
+// 7-point DFT: t_k = v_k + v_{7-k}, u_k = v_k - v_{7-k} (k = 1..3);
+// y_m = v_0 + sum_k cos(2 pi k m/7) t_k + DIR i sum_k sin(2 pi k m/7) u_k and
+// y_{7-m} with the sine term negated (m = 1..3)
+template <int DIR, typename V>
+__device__ __forceinline__ void bfly7(V* v) {
+  using T = decltype(V().x);
+  V t[3], u[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    t[k] = cadd(v[k + 1], v[6 - k]);
+    u[k] = csub(v[k + 1], v[6 - k]);
+  }
+  const V x0 = v[0];
+  v[0] = cadd(x0, cadd(t[0], cadd(t[1], t[2])));
+#pragma unroll
+  for (int m = 1; m <= 3; ++m) {
+    V a = x0, b = cmk<V>((T)0, (T)0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int j = ((k + 1) * m) % 7;
+      const T c = (T)TwC<7>::c[j], s = (T)TwC<7>::s[j];
+      a = cmk<V>(a.x + c * t[k].x, a.y + c * t[k].y);
+      b = cmk<V>(b.x + s * u[k].x, b.y + s * u[k].y);
+    }
+    b = mul_di<DIR>(b);
+    v[m] = cadd(a, b);
+    v[7 - m] = csub(a, b);
+  }
+}
+
 template <int R>
 struct Split {   // R = P * Q for the register recursion
-  static constexpr int P = (R % 4 == 0 && R != 4) ? 4 : (R % 3 == 0 && R != 3) ? 3 : (R % 2 == 0 && R != 2) ? 2 : 5;
+  static constexpr int P = (R % 4 == 0 && R != 4) ? 4
+                           : (R % 3 == 0 && R != 3) ? 3
+                           : (R % 2 == 0 && R != 2) ? 2
+                           : (R % 5 == 0 && R != 5) ? 5
+                                                    : 7;
   static constexpr int Q = R / P;
 };
 
@@ -131,6 +165,8 @@ __device__ __forceinline__ void dftn(V* v) {
     bfly4<DIR>(v);
   } else if constexpr (R == 5) {
     bfly5<DIR>(v);
+  } else if constexpr (R == 7) {
+    bfly7<DIR>(v);
   } else {
     constexpr int P = Split<R>::P, Q = Split<R>::Q;
     V y[R];
@@ -167,10 +203,11 @@ MC_FAST_SHAPE(24, 6, 4)
 MC_FAST_SHAPE(54, 9, 6)
 MC_FAST_SHAPE(128, 16, 8)
 MC_FAST_SHAPE(192, 16, 12)
+MC_FAST_SHAPE(224, 16, 14)
 MC_FAST_SHAPE(240, 16, 15)
 #undef MC_FAST_SHAPE
 // the lengths with a compile-time engine (host dispatch lists the same)
-#define MC_FAST_SIZES(X) X(16) X(18) X(24) X(54) X(128) X(192) X(240)
+#define MC_FAST_SIZES(X) X(16) X(18) X(24) X(54) X(128) X(192) X(224) X(240)
 
 // In-place transform of `nlines` lines (line stride lds) with tw = LDS copy of
 // the N-point table; nlines * A <= kThreads.  Begins and ends with a barrier
@@ -265,7 +302,8 @@ __device__ void fft_lines(v2<T>* buf, int nlines, int lds, const FftPlan& pl, co
     if (R == 4) fft_stage<4, DIR>(buf, nlines, lds, pl.n, p, tw);
     else if (R == 2) fft_stage<2, DIR>(buf, nlines, lds, pl.n, p, tw);
     else if (R == 3) fft_stage<3, DIR>(buf, nlines, lds, pl.n, p, tw);
-    else fft_stage<5, DIR>(buf, nlines, lds, pl.n, p, tw);
+    else if (R == 5) fft_stage<5, DIR>(buf, nlines, lds, pl.n, p, tw);
+    else fft_stage<7, DIR>(buf, nlines, lds, pl.n, p, tw);
     p *= R;
   }
 }

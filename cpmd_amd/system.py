@@ -4,7 +4,8 @@ CPMD's order, and the seeded synthetic systems the kernels are measured on.
 This is the host-side mirror of the setup CPMD performs before its hot loop:
 
 * mesh choice      -- ``src/numpw_utils.mod.F90:188-219`` (``change_gcut``) and the
-                      default transform-length table of ``src/fftchk_utils.mod.F90:149-156``;
+                      default transform-length table of ``src/fftchk_utils.mod.F90:149-156``
+                      or that of an FFTW build (``:73-97``, radices 2, 3, 5, 7);
 * leading dims     -- ``leadim`` ``src/loadpa_utils.mod.F90:508-525``;
 * G-vector set     -- generation loop ``src/loadpa_utils.mod.F90:283-331`` (half sphere
                       i>0 | i=0,j>0 | i=j=0,k>=0; shell tie-break key
@@ -35,21 +36,43 @@ FFT_LENGTHS = (
     1080, 1152, 1200, 1280, 1296, 1350, 1440, 1458, 1500, 1536, 1600, 1620,
     1728, 1800, 1920, 1944, 2000, 2048)
 
+
+# The table of a CPMD built with FFTW (fftchk_fftw, roots 2, 3, 5, 7,
+# src/fftchk_utils.mod.F90:73-97): the 7-smooth numbers from 2 to 8192, none
+# left out, so it is generated rather than listed.
+def _smooth7(limit: int) -> tuple:
+    out = []
+    for v in range(2, limit + 1):
+        m = v
+        for r in (2, 3, 5, 7):
+            while m % r == 0:
+                m //= r
+        if m == 1:
+            out.append(v)
+    return tuple(out)
+
+
+FFT_LENGTHS_FFTW = _smooth7(8192)
+FFT_TABLES = {"default": FFT_LENGTHS, "fftw": FFT_LENGTHS_FFTW}
+
 EPSGX = 1.0e-12  # src/gvec.mod.F90:12
 
 
-def fftchk(m: int, n: int = 2) -> int:
+def fftchk(m: int, n: int = 2, table: str = "default") -> int:
     """Next acceptable transform length >= m (n=2: next even one, n=1: next one,
-    n<0: next smaller one), src/fftchk_utils.mod.F90:135-170."""
-    for i, v in enumerate(FFT_LENGTHS):
+    n<0: next smaller one) from CPMD's default table (fftchk_def,
+    src/fftchk_utils.mod.F90:135-170) or, with table="fftw", the table of an
+    FFTW build (fftchk_fftw, :59-133)."""
+    lft = FFT_TABLES[table]
+    for i, v in enumerate(lft):
         if v >= m:
             if n < 0:
-                return FFT_LENGTHS[i - 1]
+                return lft[i - 1]
             if n == 1:
                 return v
             if n == 2 and v % 2 == 0:
                 return v
-    raise ValueError(f"FFTCHK: minimal mesh size is larger than {FFT_LENGTHS[-1]} (m={m})")
+    raise ValueError(f"FFTCHK: minimal mesh size is larger than {lft[-1]} (m={m})")
 
 
 def fortran_nint(x: float) -> int:
@@ -62,10 +85,10 @@ def leadim(nr: int) -> int:
     return nr + (nr + 1) % 2
 
 
-def mesh_cubic(alat: float, ecut_ry: float, dual: float = 4.0, mesh_min: int = 0) -> int:
+def mesh_cubic(alat: float, ecut_ry: float, dual: float = 4.0, mesh_min: int = 0, table: str = "default") -> int:
     """Mesh for a simple cubic cell, src/numpw_utils.mod.F90:188-193,213-219."""
     mnr = fortran_nint(alat / math.pi * math.sqrt(dual * ecut_ry) + 0.5)
-    return fftchk(max(mnr, mesh_min), 2)
+    return fftchk(max(mnr, mesh_min), 2, table)
 
 
 @dataclass
@@ -120,15 +143,15 @@ class Cell:
         return self.gcutw * self.dual
 
 
-def cubic_cell(a: float, ecut: float, dual: float = 4.0, mesh: int = 0) -> Cell:
-    n = mesh_cubic(a, ecut, dual, mesh)
+def cubic_cell(a: float, ecut: float, dual: float = 4.0, mesh: int = 0, table: str = "default") -> Cell:
+    n = mesh_cubic(a, ecut, dual, mesh, table)
     return Cell(alat=a, avec=np.eye(3) * a, ecut=ecut, dual=dual, nr=(n, n, n))
 
 
-def orthorhombic_cell(a: float, b: float, c: float, ecut: float, dual: float = 4.0) -> Cell:
+def orthorhombic_cell(a: float, b: float, c: float, ecut: float, dual: float = 4.0, table: str = "default") -> Cell:
     """Orthorhombic cell (alat = a); mesh per axis as the general branch of
     src/numpw_utils.mod.F90:206-212."""
-    ns = tuple(fftchk(fortran_nint(x / math.pi * math.sqrt(dual * ecut) + 0.5), 2) for x in (a, b, c))
+    ns = tuple(fftchk(fortran_nint(x / math.pi * math.sqrt(dual * ecut) + 0.5), 2, table) for x in (a, b, c))
     return Cell(alat=a, avec=np.diag([a, b, c]).astype(float), ecut=ecut, dual=dual, nr=ns)
 
 
@@ -225,8 +248,11 @@ def place_atoms(a: float, nat: int, seed: int, dmin: float = 3.0) -> np.ndarray:
 
 
 def make_system(a: float, nat: int, ecut: float, nstate: int | None = None,
-                seed: int = 1234, mesh: int = 0, with_density: bool = False) -> SyntheticSystem:
-    cell = cubic_cell(a, ecut, mesh=mesh)
+                seed: int = 1234, mesh: int = 0, with_density: bool = False,
+                fft_table: str = "default") -> SyntheticSystem:
+    """fft_table: "default" (CPMD's own table) or "fftw" (the mesh a CPMD
+    built with FFTW chooses)."""
+    cell = cubic_cell(a, ecut, mesh=mesh, table=fft_table)
     gs = gvectors(cell, with_density=with_density)
     tau = place_atoms(a, nat, seed)
     if nstate is None:

@@ -2,7 +2,7 @@
 
 A restatement of the arithmetic structure of the device's compile-time
 transform engine (cpmd_amd/csrc/fft_device.h: the in-register DFTs of
-length 2..16 recursing into radix 2/3/4/5 butterflies with the constants of
+length 2..16 recursing into radix 2/3/4/5/7 butterflies with the constants of
 twiddle_consts.h, and the four-step N = A B composition with the N-point
 twiddle table) with every constant and twiddle rounded to the engine's
 arithmetic type, evaluated in FP64.  It explains the charge the FP32
@@ -21,7 +21,8 @@ import functools
 
 import numpy as np
 
-SHAPES = {16: (4, 4), 18: (6, 3), 24: (6, 4), 54: (9, 6), 128: (16, 8), 192: (16, 12), 240: (16, 15)}
+SHAPES = {16: (4, 4), 18: (6, 3), 24: (6, 4), 54: (9, 6), 128: (16, 8), 192: (16, 12), 224: (16, 14),
+          240: (16, 15)}
 
 S3 = 0.86602540378443864676
 C51, C52 = 0.30901699437494742410, -0.80901699437494742410
@@ -35,7 +36,9 @@ def _split(r: int):
         return 3, r // 3
     if r % 2 == 0 and r != 2:
         return 2, r // 2
-    return 5, r // 5
+    if r % 5 == 0 and r != 5:
+        return 5, r // 5
+    return 7, r // 7
 
 
 def _rt(x: float, t) -> float:
@@ -66,6 +69,17 @@ def dftn(r: int, d: int, v: list, t=np.float32) -> list:
         a1, a2 = v[0] + c1 * t1 + c2 * t2, v[0] + c2 * t1 + c1 * t2
         b1, b2 = _di(s1 * t3 + s2 * t4, d), _di(s2 * t3 - s1 * t4, d)
         return [v[0] + t1 + t2, a1 + b1, a2 + b2, a2 - b2, a1 - b1]
+    if r == 7:      # bfly7: cos / sin of 2 pi j / 7 rounded to t
+        cj = [_rt(np.cos(2 * np.pi * j / 7), t) for j in range(7)]
+        sj = [_rt(np.sin(2 * np.pi * j / 7), t) for j in range(7)]
+        tt = [v[k] + v[7 - k] for k in (1, 2, 3)]
+        uu = [v[k] - v[7 - k] for k in (1, 2, 3)]
+        out = [v[0] + tt[0] + tt[1] + tt[2]] + [None] * 6
+        for m in (1, 2, 3):
+            a = v[0] + sum(cj[(k * m) % 7] * tt[k - 1] for k in (1, 2, 3))
+            b = _di(sum(sj[(k * m) % 7] * uu[k - 1] for k in (1, 2, 3)), d)
+            out[m], out[7 - m] = a + b, a - b
+        return out
     p, q = _split(r)
     e = np.arange(r)
     cc, ss = np.cos(2 * np.pi * e / r), np.sin(2 * np.pi * e / r)   # W_r^e = c - i s (twiddle_consts.h)

@@ -11,7 +11,7 @@ import pytest
 from oracle import fp32_engine as E
 
 
-@pytest.mark.parametrize("n", [16, 24, 54, 128, 192])
+@pytest.mark.parametrize("n", [16, 24, 54, 128, 192, 224])
 def test_engine_restatement_is_the_dft(n):
     j = np.arange(n)
     ref = np.exp(2j * np.pi * np.outer(j, j) / n)         # inverse: e^{+i 2 pi j k / n}
@@ -30,3 +30,13 @@ def test_fp32_constants_contract_the_norm():
     for r, ref in ((16, -2.28e-8), (12, -2.09e-8), (3, -1.20e-8)):
         m = np.array(E.dftn(r, 1, list(np.eye(r, dtype=complex)))).T
         assert (np.abs(m) ** 2).sum() / r ** 2 - 1 == pytest.approx(ref, rel=0.03)
+
+
+@pytest.mark.parametrize("r", [7, 14, 21, 28, 35, 49])
+def test_register_dft_radix7(r):
+    """The register DFTs with a factor 7 (bfly7 and the recursion through it)
+    in FP64 are the DFT, both directions."""
+    j = np.arange(r)
+    for d in (1, -1):
+        m = np.array(E.dftn(r, d, list(np.eye(r, dtype=complex)), np.float64)).T
+        assert np.abs(m - np.exp(d * 2j * np.pi * np.outer(j, j) / r)).max() < 1e-13

@@ -195,9 +195,9 @@ def test_ppener_ion_part_without_electrons():
 
 
 # --------------------------------------------------------------------------- E_KS
-def ks_system(a=9.0, nat=3, ecut=25.0, nstate=6, seed=5):
+def ks_system(a=9.0, nat=3, ecut=25.0, nstate=6, seed=5, fft_table="default"):
     """The KS functional's inputs for a seeded system (also used by the GPU tests)."""
-    sysm = S.make_system(a, nat, ecut, nstate=nstate, seed=seed, with_density=True)
+    sysm = S.make_system(a, nat, ecut, nstate=nstate, seed=seed, with_density=True, fft_table=fft_table)
     gs = sysm.gset
     L = oracle.Layout(sysm.cell, gs)
     zv, rg, vps, rhops = S.local_pp_set(sysm, gs.hg)

@@ -48,6 +48,24 @@ def test_fftchk_table():
     assert S.leadim(54) == 55 and S.leadim(55) == 55
 
 
+def test_fftchk_fftw_table():
+    """The table of an FFTW build (fftchk_fftw, src/fftchk_utils.mod.F90:73-97):
+    316 entries, 2 .. 8192, exactly the 7-smooth numbers; the meshes it gives
+    for the 8-atom 12 bohr and 512-atom 36 bohr systems are 50 and 224 where the
+    default table gives 54 and 240."""
+    t = S.FFT_LENGTHS_FFTW
+    assert len(t) == 316 and t[0] == 2 and t[-1] == 8192 and list(t) == sorted(set(t))
+    for v in (7, 14, 49, 50, 224, 343, 2401, 7203, 8192):
+        assert v in t
+    for v in (11, 13, 22, 26, 33, 8191):
+        assert v not in t
+    assert S.fftchk(49, 2, "fftw") == 50 and S.fftchk(218, 2, "fftw") == 224 and S.fftchk(47, 1, "fftw") == 48
+    assert S.fftchk(100, -1, "fftw") == 98
+    assert S.mesh_cubic(12.0, 40.0, table="fftw") == 50 and S.mesh_cubic(12.0, 40.0) == 54
+    assert S.mesh_cubic(36.0, 90.0, table="fftw") == 224 and S.mesh_cubic(36.0, 90.0) == 240
+    assert set(S.FFT_LENGTHS) <= set(t)
+
+
 def test_gset_properties():
     cell = S.cubic_cell(12.0, 40.0)
     gs = S.gvectors(cell, with_density=True)
