@@ -594,6 +594,7 @@ def main():
         **compare,
         "roofline": roof,
         "fft_kernel_roofline": fft_kernel_roofline(dom_fft),
+        "fft_stages": {k: fft_kernel_roofline(k) for k in FFT_KIDS if k in kernels},
         "fft_roofline": {"model_bytes": step_bytes, "ms_per_step": round(fft_ms, 4),
                          "achieved": round(step_bytes / (fft_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(step_bytes / (fft_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},

@@ -365,9 +365,33 @@ __global__ void k_zero(double* __restrict__ x, size_t n) {
     x[i] = 0.0;
 }
 
+// y += alpha x: 16-byte accesses, four per lane in flight, when both are
+// 16-byte aligned
 __global__ void k_axpy(double alpha, const double* __restrict__ x, double* __restrict__ y, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    y[i] += alpha * x[i];
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y)) & 15) != 0) {
+    for (; i < n; i += stride) y[i] += alpha * x[i];
+    return;
+  }
+  const double2* __restrict__ x2 = reinterpret_cast<const double2*>(x);
+  double2* __restrict__ y2 = reinterpret_cast<double2*>(y);
+  const size_t n2 = n / 2;
+  for (; i + 3 * stride < n2; i += 4 * stride) {
+    double2 a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a[k] = x2[i + k * stride];
+      b[k] = y2[i + k * stride];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) y2[i + k * stride] = make_double2(b[k].x + alpha * a[k].x, b[k].y + alpha * a[k].y);
+  }
+  for (; i < n2; i += stride) {
+    const double2 a = x2[i], b = y2[i];
+    y2[i] = make_double2(b.x + alpha * a.x, b.y + alpha * a.y);
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) y[n - 1] += alpha * x[n - 1];
 }
 
 // y = alpha x + beta y (beta = 0: y is not read)
@@ -427,8 +451,23 @@ __global__ __launch_bounds__(256) void k_dotp_cols(const double2* __restrict__ a
   const int i = blockIdx.x;
   const double2* x = a + (size_t)ldg * i;
   const double2* y = b + (size_t)ldg * i;
-  double s = 0.0;
-  for (int g = threadIdx.x + (geq0 ? 1 : 0); g < ngw; g += 256) s += x[g].x * y[g].x + x[g].y * y[g].y;
+  // eight independent partial sums per lane: eight loads of each operand in
+  // flight (fixed summation order, so the result is reproducible)
+  const int g0 = geq0 ? 1 : 0;
+  double p[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int g = g0 + threadIdx.x;
+  for (; g + 7 * 256 < ngw; g += 8 * 256) {
+    double2 u[8], v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      u[k] = x[g + 256 * k];
+      v[k] = y[g + 256 * k];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) p[k] += u[k].x * v[k].x + u[k].y * v[k].y;
+  }
+  for (int k = 0; g < ngw; g += 256, ++k) p[k & 7] += x[g].x * y[g].x + x[g].y * y[g].y;
+  double s = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   __shared__ double part[4];
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
@@ -1266,15 +1305,8 @@ int mc_cp_velupa_rortv_dev(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, 
   MC_CHECK(c0_dev != cm_dev && c2_dev != cm_dev, MC_ERR_ARG, "velupa/rortv: cm must not alias c0 or c2");
   if (ekinc) *ekinc = 0.0;
   if (nstate == 0) return MC_OK;
-  const size_t n = (size_t)ctx->ldg * nstate * 2;
-  if (c2_dev) {   // velupa(c0, cm, c2, nstate, 1): cm += dt / (2 emass) c2
-    mc::timer_begin(ctx, MC_KID_MISC);
-    hipLaunchKernelGGL(mc::k_axpy, dim3(2048), dim3(256), 0, ctx->stream, dtm, (const double*)c2_dev,
-                       (double*)cm_dev, n);
-    MC_HIP(hipGetLastError());
-    mc::timer_end(ctx, MC_KID_MISC);
-  }
-  mc::rortv_dev(ctx, (const double2*)c0_dev, (double2*)cm_dev, gamy, nstate);
+  // velupa(c0, cm, c2, nstate, 1): cm += dt / (2 emass) c2, then rortv
+  mc::rortv_dev(ctx, (const double2*)c0_dev, (double2*)cm_dev, gamy, nstate, (const double2*)c2_dev, dtm);
   std::vector<double> d;
   if (ekinc) {
     d.resize(nstate);

@@ -131,10 +131,12 @@ __global__ __launch_bounds__(256) void k_split_planes(const double* __restrict__
 // column; the fp32 values pass through LDS so that lane l then holds rows
 // 8l..8l+7, the plane chunk it splits and stores.  cm is read, not written: the
 // caller's final rotation sets the velocity (cp_final - c0) / dt.
-template <int NP>
+// KICK (velupa before rortv): cp = cm + dtm c2 instead, written over cm (cp ==
+// cm: every element is read before the same thread writes it), with the
+// planes of c0 and of the kicked cm.
+template <int NP, bool KICK>
 __global__ __launch_bounds__(256) void k_posupa_planes(const double* __restrict__ c0, const double* __restrict__ c2,
-                                                       const double* __restrict__ cm, double* __restrict__ cp, int ldx,
-                                                       int K,
+                                                       const double* cm, double* cp, int ldx, int K,
                                                        int n, int ncolp, int nseg, int nkb, double dtm, double dt,
                                                        __bf16* __restrict__ pl0, __bf16* __restrict__ plp) {
   __shared__ float4 lds[4][2][128];   // [wave][c0 | cp][512 rows]
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(256) void k_posupa_planes(const double* __restrict_
   for (int u = 0; u < 4; ++u) {
     w[u].x += dtm * f[u].x;
     w[u].y += dtm * f[u].y;
-    f[u] = make_double2(x[u].x + dt * w[u].x, x[u].y + dt * w[u].y);   // f now holds cp
+    f[u] = KICK ? w[u] : make_double2(x[u].x + dt * w[u].x, x[u].y + dt * w[u].y);   // f now holds cp
     const bool in = r0 + 128 * u < K;                                  // K even: whole complex rows
     s0[lane + 64 * u] = in ? make_float2((float)x[u].x, (float)x[u].y) : make_float2(0.f, 0.f);
     sp[lane + 64 * u] = in ? make_float2((float)f[u].x, (float)f[u].y) : make_float2(0.f, 0.f);
@@ -624,8 +626,8 @@ void split_planes(mc_ctx* ctx, const double* X, int ldx, int K, int n, int np, D
   timer_end(ctx, MC_KID_GEMM);
 }
 
-void posupa_planes(mc_ctx* ctx, const double* c0, const double* c2, const double* cm, double* cp, int ldx, int K,
-                   int n, double dtm, double dt) {
+static void update_planes(mc_ctx* ctx, bool kick, const double* c0, const double* c2, const double* cm, double* cp,
+                          int ldx, int K, int n, double dtm, double dt) {
   auto& C = ctx->planes;
   MC_CHECK(C.scoped, MC_ERR_INTERNAL, "posupa_planes outside a plane scope");
   MC_CHECK(ldx % kPK == 0 && K % 2 == 0 && K <= ldx && (reinterpret_cast<uintptr_t>(c0) & 15) == 0 &&
@@ -638,12 +640,12 @@ void posupa_planes(mc_ctx* ctx, const double* c0, const double* c2, const double
   C.buf[1].ensure(planes_bytes(K, n, np));
   const dim3 grid((unsigned)(((size_t)ncolp * nseg + 3) / 4));
   timer_begin(ctx, MC_KID_MISC);
-  if (np == 3)
-    hipLaunchKernelGGL(k_posupa_planes<3>, grid, dim3(256), 0, ctx->stream, c0, c2, cm, cp, ldx, K, n, ncolp, nseg,
-                       nkb, c2 ? dtm : 0.0, dt, C.buf[0].as<__bf16>(), C.buf[1].as<__bf16>());
-  else
-    hipLaunchKernelGGL(k_posupa_planes<2>, grid, dim3(256), 0, ctx->stream, c0, c2, cm, cp, ldx, K, n, ncolp, nseg,
-                       nkb, c2 ? dtm : 0.0, dt, C.buf[0].as<__bf16>(), C.buf[1].as<__bf16>());
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, ctx->stream, c0, c2, cm, cp, ldx, K, n, ncolp, nseg, nkb,
+                       c2 ? dtm : 0.0, dt, C.buf[0].as<__bf16>(), C.buf[1].as<__bf16>());
+  };
+  if (np == 3) kick ? go(k_posupa_planes<3, true>) : go(k_posupa_planes<3, false>);
+  else kick ? go(k_posupa_planes<2, true>) : go(k_posupa_planes<2, false>);
   MC_HIP(hipGetLastError());
   timer_end(ctx, MC_KID_MISC);
   const void* src[2] = {c0, cp};
@@ -654,6 +656,15 @@ void posupa_planes(mc_ctx* ctx, const double* c0, const double* c2, const double
     C.n[i] = n;
   }
   C.next = 0;
+}
+
+void posupa_planes(mc_ctx* ctx, const double* c0, const double* c2, const double* cm, double* cp, int ldx, int K,
+                   int n, double dtm, double dt) {
+  update_planes(ctx, false, c0, c2, cm, cp, ldx, K, n, dtm, dt);
+}
+
+void kick_planes(mc_ctx* ctx, const double* c0, const double* c2, double* cm, int ldx, int K, int n, double dtm) {
+  update_planes(ctx, true, c0, c2, cm, cm, ldx, K, n, dtm, 0.0);
 }
 
 void proj_planes(mc_ctx* ctx, const ProjA& pa, int K, int m, int np, DevBuf& out) {

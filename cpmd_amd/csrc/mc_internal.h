@@ -293,6 +293,7 @@ void timer_flush(mc_ctx* ctx);
 
 __global__ void k_zero(double* __restrict__ x, size_t n);
 __global__ void k_fill(double* __restrict__ x, size_t n, double v);
+__global__ void k_axpy(double alpha, const double* __restrict__ x, double* __restrict__ y, size_t n);
 
 // Output tiles of an A^T B launch: the tm x tn tiles row-major, or with upper
 // (a symmetric product, tm == tn) only the tm (tm + 1) / 2 tiles on or above
@@ -344,6 +345,9 @@ void proj_planes(mc_ctx* ctx, const ProjA& pa, int K, int m, int np, DevBuf& out
 // planes of c0 and cp into the plane cache of the current PlaneScope
 void posupa_planes(mc_ctx* ctx, const double* c0, const double* c2, const double* cm, double* cp, int ldx, int K,
                    int n, double dtm, double dt);
+// cm += dtm c2 in place, writing the planes of c0 and of the new cm into the
+// plane cache of the current PlaneScope
+void kick_planes(mc_ctx* ctx, const double* c0, const double* c2, double* cm, int ldx, int K, int n, double dtm);
 void cp_posupa_dev(mc_ctx* ctx, const double2* c0, const double2* c2, double2* cm, double2* cp, int nstate, double dtm,
                    double dt);
 void cp_velocity_dev(mc_ctx* ctx, const double2* c0, const double2* cp, double2* cm, int nstate, double dt);
@@ -395,7 +399,9 @@ void lagrange_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_h
 void rortog_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n, double epsog,
                 int maxit, int* iters);
 // gamy_host (n x n, may be null) is filled by a queued download: dl_finish
-void rortv_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gamy_host, int n);
+// c2 != nullptr: velupa's kick cm += dtm c2 first
+void rortv_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gamy_host, int n, const double2* c2 = nullptr,
+               double dtm = 0.0);
 void set_projectors(mc_ctx* ctx, int nsp, const int* na, int nhxs, const int* ngh, const int* lang,
                     const double* hmat, const double* twnl, int ldtw);
 void set_eigr(mc_ctx* ctx, const double* eigr, int lde, int nat);

@@ -494,11 +494,26 @@ __global__ void k_neg_symm(double* __restrict__ a, int n) {
 // rortv (src/rortv_utils.mod.F90:39-166, overlap constraints, no tharm /
 // tmass): gamy = ovlap(cm, c0), gamy <- -sym(gamy), cm += c0 gamy, so that
 // <c0|cm> + <cm|c0> = 0 for orthonormal c0 (the velocity constraint of the
-// velocity Verlet CP step)
-void rortv_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gamy_host, int n) {
+// velocity Verlet CP step).  With c2, velupa's kick cm += dtm c2 comes
+// first; in mixed precision it is one pass with the bf16 split of c0 and of
+// the kicked cm for the overlap.
+void rortv_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gamy_host, int n, const double2* c2,
+               double dtm) {
   if (n <= 0) return;
   MC_CHECK(n < kX3Min, MC_ERR_UNSUPPORTED, "rortv: nstate beyond 16383");
-  PlaneScope keep(ctx);   // c0 and cm are read-only until the rotation
+  PlaneScope keep(ctx);   // c0 and cm are read-only between the kick and the rotation
+  const int K = wfn_rows(ctx);
+  if (c2) {
+    if (ctx->gemm_x3 && K >= kX3Min) {
+      kick_planes(ctx, (const double*)c0, (const double*)c2, (double*)cm, wfn_ld(ctx), K, n, dtm);
+    } else {
+      timer_begin(ctx, MC_KID_MISC);
+      hipLaunchKernelGGL(k_axpy, dim3(2048), dim3(256), 0, ctx->stream, dtm, (const double*)c2, (double*)cm,
+                         (size_t)ctx->ldg * n * 2);
+      MC_HIP(hipGetLastError());
+      timer_end(ctx, MC_KID_MISC);
+    }
+  }
   double* G = small(ctx, 7, n);
   ovlap_dev(ctx, n, G, n, cm, c0);
   hipLaunchKernelGGL(k_neg_symm, dim3((n + 255) / 256, n), dim3(256), 0, ctx->stream, G, n);
