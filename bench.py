@@ -383,7 +383,7 @@ def main():
         ctx.timer_enable(False)
         ms = (t1 - t0) * 1e3 / args.steps
         kernels = {}
-        for kid in FFT_KIDS + ("misc", "gemm", "gemm_tn", "vofrho"):
+        for kid in FFT_KIDS + ("misc", "gemm", "gemm_tn", "tn_planes", "vofrho"):
             tot, cnt = ctx.timer_get(kid)
             if cnt:
                 kernels[kid] = {"total_ms_per_step": tot / args.steps, "launches": cnt, "avg_ms": tot / cnt}
@@ -448,23 +448,33 @@ def main():
         k_gemm_tn in FP64): the algorithm's MFMA flops per step (6 bf16
         products per FP64-equivalent product for the 3-part split, symmetric
         overlaps counted once) over its time per step."""
-        k = kernels["gemm_tn"]
         split = head in ("mixed", "bf16x3")
+        k = kernels["tn_planes" if split else "gemm_tn"]
         mult = (6.0 if head == "mixed" else 3.0) if split else 1.0
         flops = mult * tn_gemm_flops
         a = flops / (k["total_ms_per_step"] * 1e-3) / 1e12
         peak = 2500.0 if split else 78.6
-        return {"bound": "mfma", "kernel": "tn_planes" if split else "gemm_tn", "achieved": round(a, 2),
-                "peak": peak, "unit": "TFLOP/s", "frac": round(a / peak, 4),
-                "flops_per_launch": flops * args.steps / k["launches"], "avg_launch_ms": k["avg_ms"],
-                "dtype": "bf16 (3-part split)" if head == "mixed" else ("bf16 (2-part split)" if split else "f64"),
-                "useful_fp64_equiv_tflops": round(tn_gemm_flops / (k["total_ms_per_step"] * 1e-3) / 1e12, 2)}
+        out = {"bound": "mfma", "kernel": "tn_planes" if split else "gemm_tn", "achieved": round(a, 2),
+               "peak": peak, "unit": "TFLOP/s", "frac": round(a / peak, 4),
+               "flops_per_launch": flops * args.steps / k["launches"], "avg_launch_ms": k["avg_ms"],
+               "launches_per_step": k["launches"] / args.steps,
+               "dtype": "bf16 (3-part split)" if head == "mixed" else ("bf16 (2-part split)" if split else "f64"),
+               "useful_fp64_equiv_tflops": round(tn_gemm_flops / (k["total_ms_per_step"] * 1e-3) / 1e12, 2)}
+        # the rate the kernel's own MFMA instruction reaches on this GPU
+        # (mc_probe_mfma: independent chains, no memory traffic), beside the
+        # published dense peak
+        probe = "bf16_16x16x32" if split else "f64_16x16x4"
+        ceil = ops.probe_mfma(probe)
+        out["issue_ceiling"] = {"instruction": "v_mfma_f32_16x16x32_bf16" if split else "v_mfma_f64_16x16x4_f64",
+                                "tflops": round(ceil, 1), "frac": round(a / ceil, 4)}
+        return out
 
     # the dominant kernel: the largest time per step among the transform
     # kernels and the A^T B contraction kernel
-    cands = [k for k in FFT_KIDS if k in kernels] + (["gemm_tn"] if "gemm_tn" in kernels else [])
+    tn_kid = "tn_planes" if head in ("mixed", "bf16x3") else "gemm_tn"
+    cands = [k for k in FFT_KIDS if k in kernels] + ([tn_kid] if tn_kid in kernels else [])
     dom = max(cands, key=lambda k: kernels[k]["total_ms_per_step"])
-    roof = tn_kernel_roofline() if dom == "gemm_tn" else fft_kernel_roofline(dom)
+    roof = tn_kernel_roofline() if dom == tn_kid else fft_kernel_roofline(dom)
     dom_fft = max((k for k in FFT_KIDS if k in kernels), key=lambda k: kernels[k]["total_ms_per_step"])
     step_bytes = step_model_bytes(geom, nstate, cw)
     fft_ms = sum(kernels[k]["total_ms_per_step"] for k in FFT_KIDS if k in kernels)
@@ -474,7 +484,8 @@ def main():
         (6 products per FP64-equivalent product for the 3-part A^T B products,
         3 for the 2-part ones) against the 2.5 PF dense bf16 peak; 'useful'
         counts each product once."""
-        ms = sum(r["kernels"].get(k, {}).get("total_ms_per_step", 0.0) for k in ("gemm", "gemm_tn")) or float("nan")
+        ms = sum(r["kernels"].get(k, {}).get("total_ms_per_step", 0.0)
+                 for k in ("gemm", "gemm_tn", "tn_planes")) or float("nan")
         useful = gemm_flops / (ms * 1e-3) / 1e12
         if path in ("mixed", "bf16x3"):
             # 3-part A^T B products issue 6 bf16 products per FP64 product, 2-part 3

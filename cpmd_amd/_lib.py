@@ -19,7 +19,7 @@ ERRORS = {1: "MC_ERR_ARG", 2: "MC_ERR_HIP", 3: "MC_ERR_NOTPOSDEF", 4: "MC_ERR_NO
           5: "MC_ERR_UNSUPPORTED", 6: "MC_ERR_INTERNAL"}
 
 KID = {"zinv": 0, "yinv": 1, "xrho": 2, "xvpsi": 3, "yfwd": 4, "zfwd": 5, "misc": 6, "gemm": 7, "vofrho": 8,
-       "gemm_tn": 9}
+       "gemm_tn": 9, "tn_planes": 10}
 
 
 class McError(RuntimeError):
@@ -66,6 +66,7 @@ SIGNATURES = {
     "mc_set_mem_limit": (C.c_int, [C.c_size_t]),
     "mc_mem_stats": (C.c_int, [C.POINTER(C.c_size_t)] * 3),
     "mc_mem_reset_peak": (C.c_int, []),
+    "mc_probe_mfma": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
     "mc_wfn_zero": (C.c_int, [_vp, _vp, C.c_int]),
     "mc_wfn_axpy": (C.c_int, [_vp, C.c_double, _vp, _vp, C.c_int]),
     "mc_wfn_axpby": (C.c_int, [_vp, C.c_double, _vp, C.c_double, _vp, C.c_int]),

@@ -15,7 +15,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libmicpmd.so"
 
 HIP_SOURCES = ["context.hip", "wfft.hip", "gemm.hip", "gemm_x3.hip", "gemm_planes.hip", "ortho.hip", "vofrho.hip",
-               "capi.hip", "restart.hip"]
+               "capi.hip", "restart.hip", "probe.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -726,7 +726,7 @@ int tn_planes_partials(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np
   ctx->w_gemm.ensure(sizeof(double) * (size_t)nsplit * m * n);
   const int total = nsplit * tiles;
   const int launch = (total + kXcdP - 1) / kXcdP * kXcdP;
-  timer_begin(ctx, MC_KID_GEMM_TN);
+  timer_begin(ctx, MC_KID_TN_PLANES);
   if (np == 3)
     hipLaunchKernelGGL(k_tn_planes<3>, dim3(launch), dim3(64 * kTnWaves), 0, ctx->stream, Apl.as<__bf16>(), Bpl.as<__bf16>(),
                        nkb, m, n, kbper, ctx->w_gemm.as<double>(), symmetric ? 1 : 0, tm, tn, total);
@@ -734,7 +734,7 @@ int tn_planes_partials(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np
     hipLaunchKernelGGL(k_tn_planes<2>, dim3(launch), dim3(64 * kTnWaves), 0, ctx->stream, Apl.as<__bf16>(), Bpl.as<__bf16>(),
                        nkb, m, n, kbper, ctx->w_gemm.as<double>(), symmetric ? 1 : 0, tm, tn, total);
   MC_HIP(hipGetLastError());
-  timer_end(ctx, MC_KID_GEMM_TN);
+  timer_end(ctx, MC_KID_TN_PLANES);
   return nsplit;
 }
 

@@ -85,6 +85,16 @@ def mem_reset_peak():
     check(lib().mc_mem_reset_peak())
 
 
+MFMA_PROBES = {"bf16_16x16x32": 0, "bf16_32x32x16": 1, "f64_16x16x4": 2, "f32_16x16x4": 3}
+
+
+def probe_mfma(kind: str) -> float:
+    """TFLOP/s one MFMA instruction reaches on this GPU (see mc_probe_mfma)."""
+    r = C.c_double()
+    check(lib().mc_probe_mfma(MFMA_PROBES[kind], C.byref(r)))
+    return r.value
+
+
 def make_layout(cell, gset, ngw: int | None = None):
     """mc_layout plus the arrays it points to (kept alive by the returned tuple)."""
     ngw = gset.ngw if ngw is None else ngw

@@ -53,9 +53,11 @@ enum {
   MC_KID_GEMM = 7,   /* MFMA contractions (FP64 or split bf16) other
                         than the A^T B ones, bf16 plane builds       */
   MC_KID_VOFRHO = 8, /* dense density transforms, per-G, LDA, Ewald  */
-  MC_KID_GEMM_TN = 9,/* A^T B contractions (overlaps, fnl, dfnl):
+  MC_KID_GEMM_TN = 9,/* FP64 A^T B contractions (overlaps, fnl, dfnl):
                         the split-K MFMA kernel, without reductions  */
-  MC_KID_COUNT = 10
+  MC_KID_TN_PLANES = 10, /* split-bf16 A^T B from operand planes
+                        (k_tn_planes), without reductions            */
+  MC_KID_COUNT = 11
 };
 
 typedef struct mc_ctx mc_ctx;
@@ -133,12 +135,23 @@ int mc_memset_d(void* dst, int value, size_t bytes);
  * vpsi transform as many state pairs per batch as fit beside what is already
  * held and a reserve of max(256 MiB, limit / 16) for the buffers allocated
  * later in a step (shrinking their workspaces when needed; the per-pair
- * results do not depend on the batch), and any allocation that would pass
- * the limit fails with MC_ERR_ARG instead of being made.  0 removes it. */
+ * results do not depend on the batch).  An allocation that would pass the
+ * limit first frees the transform workspaces no call is using (the next
+ * transform call sizes them again) and, if it still does not fit, fails with
+ * MC_ERR_ARG instead of being made.  0 removes the limit. */
 int mc_set_mem_limit(size_t bytes);
 /* Bytes held now, the most held since the last reset, and the limit. */
 int mc_mem_stats(size_t* current, size_t* peak, size_t* limit);
 int mc_mem_reset_peak(void);
+
+/* MFMA issue rate (a diagnostic with no counterpart in the reference): the
+ * TFLOP/s one matrix instruction reaches on the current GPU with independent
+ * accumulator chains, register operands and no memory traffic, best of three
+ * timings of about 5 ms.  kind 0: v_mfma_f32_16x16x32_bf16 (the split-bf16
+ * contractions), 1: v_mfma_f32_32x32x16_bf16, 2: v_mfma_f64_16x16x4_f64 (the
+ * FP64 contractions), 3: v_mfma_f32_16x16x4_f32.  The ceiling the contraction
+ * kernels are read against beside the published dense peaks. */
+int mc_probe_mfma(int kind, double* tflops);
 
 /* Zero nstate device wavefunctions (stream-ordered; CPMD: zeroing(c2),
  * src/forces_driver.mod.F90:175). */
