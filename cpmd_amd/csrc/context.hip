@@ -853,6 +853,11 @@ int mc_ctx_create(const mc_layout* lay, mc_ctx** out) {
     mc::upload_vec(ctx->d_col_slot, L.col_slot);
     mc::upload_vec(ctx->d_col_gy, L.col_gy);
     mc::upload_vec(ctx->d_col_gx, L.col_gx);
+    int mx = 0, my = 0;   // largest |m1| of a column, |m2| of a stick
+    for (int gx : L.col_gx) mx = std::max(mx, std::abs(gx <= ctx->n1 / 2 ? gx : gx - ctx->n1));
+    for (int gy : L.col_gy) my = std::max(my, std::abs(gy <= ctx->n2 / 2 ? gy : gy - ctx->n2));
+    ctx->prune_x = 4 * mx < ctx->n1;
+    ctx->prune_y = 4 * my < ctx->n2;
     mc::upload_twiddles(ctx->d_tw1, ctx->n1);
     mc::upload_twiddles(ctx->d_tw2, ctx->n2);
     mc::upload_twiddles(ctx->d_tw3, ctx->n3);

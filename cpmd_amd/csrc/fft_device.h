@@ -191,6 +191,52 @@ __device__ __forceinline__ void dftn(V* v) {
   }
 }
 
+// 16-point register DFT for the band of the wavefunction sphere (frequencies
+// |m| < N/4 of an N = 16 B transform sit at k = 0..3 and 12..15 of the
+// four-step's 16-point DFTs).  IN: the inputs 4..11 are zero and not read
+// (the first radix-4 stage reduces to a +- d, a -+ DIR i d); OUT: only the
+// outputs 0..3 and 12..15 are formed (4..11 keep what they held).  On the
+// entries it forms it does dftn<16>'s arithmetic, so the values agree with it
+// (up to the sign of zero results).
+template <int DIR, bool IN, bool OUT, typename V>
+__device__ __forceinline__ void dft16_band(V* v) {
+  V y[16];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    V u[4];
+    if constexpr (IN) {
+      const V a = v[n2], d = v[12 + n2];
+      const V md = mul_di<DIR>(d);
+      u[0] = cadd(a, d);
+      u[1] = csub(a, md);
+      u[2] = csub(a, d);
+      u[3] = cadd(a, md);
+    } else {
+#pragma unroll
+      for (int n1 = 0; n1 < 4; ++n1) u[n1] = v[4 * n1 + n2];
+      bfly4<DIR>(u);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) y[n2 * 4 + k1] = ((n2 * k1) % 16 == 0) ? u[k1] : twc<16, DIR>(u[k1], (n2 * k1) % 16);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    V u[4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) u[n2] = y[n2 * 4 + k1];
+    if constexpr (OUT) {
+      const V t0 = cadd(u[0], u[2]), t1 = csub(u[0], u[2]);
+      const V t2 = cadd(u[1], u[3]), t3 = mul_di<DIR>(csub(u[1], u[3]));
+      v[k1] = cadd(t0, t2);
+      v[k1 + 12] = csub(t1, t3);
+    } else {
+      bfly4<DIR>(u);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = u[k2];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // compile-time four-step engine
 // ---------------------------------------------------------------------------

@@ -264,6 +264,11 @@ struct mc_ctx {
   mc::Projectors proj;
 
   bool fft_fp32 = false;   // rhoofr/vpsi transforms and workspaces in FP32
+  // the wavefunction sphere lies inside |m1| < n1/4 (resp. |m2| < n2/4): the
+  // inverse x (y) passes and the forward x pass of the 16 x B engines skip
+  // the frequencies outside that band (dft16_band); true whenever the mesh
+  // is at least twice the sphere's diameter (dual >= 4)
+  bool prune_x = false, prune_y = false;
   bool gemm_x3 = false;    // tall wavefunction contractions as split-bf16 MFMA products
   int x3_parts = 3;        // bf16 parts per operand of the split A^T B products (2 or 3)
   mc::PlaneCache planes;   // bf16 planes of the wavefunction operands of the A^T B products

@@ -499,10 +499,13 @@ struct F4 {
     for (int k = 0; k < K; ++k)
       if (off[k] != kNone) st(p, off[k], r[k]);
   }
-  template <int DIR>
+  // PR (A = 16): the inputs of the band structure of dft16_band only (the
+  // inverse passes of a wavefunction sphere with |m| < N/4)
+  template <int DIR, bool PR = false>
   __device__ __forceinline__ void phase1(V (&v)[A], V* bf) const {
     if (g < B) {
-      dftn<A, DIR>(v);
+      if constexpr (PR && A == 16) dft16_band<DIR, true, false>(v);
+      else dftn<A, DIR>(v);
       const V* tp = t1 + g;
 #pragma unroll
       for (int k1 = 1; k1 < A; ++k1) v[k1] = twmul<DIR>(v[k1], tp[k1 * B]);
@@ -536,15 +539,17 @@ struct F4 {
       for (int k1 = 0; k1 < B; ++k1) ((k1 & 1) ? w1 : w0)[A * k1 * L] = u[k1];
     }
   }
-  // v[k2] = Y(g + B k2), lanes g < B
-  template <int DIR>
+  // v[k2] = Y(g + B k2), lanes g < B; PR (A = 16): only k2 = 0..3, 12..15
+  // (the outputs inside the band of a wavefunction sphere with |m| < N/4)
+  template <int DIR, bool PR = false>
   __device__ __forceinline__ void phase4(V (&v)[A], const V* bf) const {
     if (g < B) {
       const V* r0 = bf + b34.rb[0];
       const V* r1 = bf + b34.rb[1];
 #pragma unroll
       for (int m = 0; m < A; ++m) v[m] = ((m & 1) ? r1 : r0)[m * L];
-      dftn<A, DIR>(v);
+      if constexpr (PR && A == 16) dft16_band<DIR, false, true>(v);
+      else dftn<A, DIR>(v);
     }
   }
 };
@@ -553,7 +558,7 @@ struct F4 {
 // y pass, fused form: workgroup (column c, z tile, pairs p0 .. p0+ppb-1).
 // DIR=+1: sticks of column c in A -> B (dense y); DIR=-1: B -> sticks in A.
 // ---------------------------------------------------------------------------
-template <typename T, int N, int DIR>
+template <typename T, int N, int DIR, bool PR = false>
 __global__ __launch_bounds__(kThreads) F4_WAVES void k_ypass_f(WParams w, v2<T>* __restrict__ Ag, v2<T>* __restrict__ Bg,
                                                       int npairs, int ppb) {
   using V = v2<T>;
@@ -603,7 +608,7 @@ __global__ __launch_bounds__(kThreads) F4_WAVES void k_ypass_f(WParams w, v2<T>*
   for (int p = p0; p < p1; ++p) {
     if (!kPre) load(p, r);
     V* bf = par ? s.buf[1] : s.buf[0];
-    s.template phase1<DIR>(r, bf);
+    s.template phase1<DIR, PR>(r, bf);
     if (kPre && p + 1 < p1) load(p + 1, r);   // in flight through the rest of this pair
     __syncthreads();
     V u[B];
@@ -642,7 +647,7 @@ __device__ __forceinline__ void x_map(const X& s, const WParams& w, int n1) {
 }
 
 // xrho, fused form (same contract as k_xrho): workgroup (z tile, y)
-template <typename T, int N>
+template <typename T, int N, bool PR = false>
 __global__ __launch_bounds__(kThreads) F4_WAVES void k_xrho_f(WParams w, const v2<T>* __restrict__ Bg, int npairs, int pair0,
                                                      int nstate, const double* __restrict__ f,
                                                      double* __restrict__ rho) {
@@ -679,7 +684,7 @@ __global__ __launch_bounds__(kThreads) F4_WAVES void k_xrho_f(WParams w, const v
     const double c3 = f[is1] * rom, c4 = (is2 < nstate ? f[is2] : 0.0) * rom;
     if (!kPre) X::fetch(Bg + (size_t)pb * bslab, off, r);
     V* bf = par ? s.buf[1] : s.buf[0];
-    s.template phase1<1>(r, bf);
+    s.template phase1<1, PR>(r, bf);
     if (kPre && nx < npairs) X::fetch(Bg + (size_t)nx * bslab, off, r);
     __syncthreads();
     V u[B];
@@ -716,7 +721,7 @@ __global__ __launch_bounds__(kThreads) F4_WAVES void k_xrho_f(WParams w, const v
 // xvpsi, fused form (same contract as k_xvpsi): workgroup (z tile, y).  On
 // the FP32 path the next pair's loads are issued after phase kXPre.
 constexpr int kXPre = 1;
-template <typename T, int N>
+template <typename T, int N, bool PR = false>
 __global__ __launch_bounds__(kThreads) F4_WAVES void k_xvpsi_f(WParams w, v2<T>* __restrict__ Bg, int npairs,
                                                       const double* __restrict__ vpot) {
   using V = v2<T>;
@@ -751,11 +756,11 @@ __global__ __launch_bounds__(kThreads) F4_WAVES void k_xvpsi_f(WParams w, v2<T>*
     V* Bs = Bg + (size_t)pb * bslab;
     V v[A];
     if constexpr (kPre) {
-      s.template phase1<1>(r, s.buf[0]);
+      s.template phase1<1, PR>(r, s.buf[0]);
       if (kXPre == 1 && pb + 1 < npairs) X::fetch(Bs + bslab, off, r);   // in flight through phases 2-4
     } else {
       X::fetch(Bs, off, v);
-      s.template phase1<1>(v, s.buf[0]);
+      s.template phase1<1, PR>(v, s.buf[0]);
     }
     __syncthreads();
     V u[B];
@@ -766,8 +771,14 @@ __global__ __launch_bounds__(kThreads) F4_WAVES void k_xvpsi_f(WParams w, v2<T>*
     s.template phase3<-1>(u, s.buf[1]);
     if (kPre && kXPre == 3 && pb + 1 < npairs) X::fetch(Bs + bslab, off, r);   // in flight through phase 4
     __syncthreads();
-    s.template phase4<-1>(v, s.buf[1]);
-    X::store(Bs, off, v);
+    s.template phase4<-1, PR>(v, s.buf[1]);
+    if constexpr (PR && A == 16) {   // outputs 4..11 lie outside the sphere's band (no column): not formed
+#pragma unroll
+      for (int k = 0; k < A; ++k)
+        if ((k < 4 || k >= 12) && off[k] != X::kNone) X::st(Bs, off[k], v[k]);
+    } else {
+      X::store(Bs, off, v);
+    }
     if (X::NB == 1) __syncthreads();
   }
 }
@@ -937,9 +948,18 @@ static void ypass(mc_ctx* ctx, const WParams& w, const Shapes& s, v2<T>* A, v2<T
       const int zt = (ctx->n3 + L - 1) / L;
       const int runs = std::max(1, std::min(np, (2048 + ctx->ncols * zt - 1) / (ctx->ncols * zt)));
       const int ppb = (np + runs - 1) / runs;
-      f4_allow_lds(&k_ypass_f<T, N2, DIR>, f4_lds<T, N2>());
-      LAUNCH(kid, (k_ypass_f<T, N2, DIR>), dim3(ctx->ncols, zt, (np + ppb - 1) / ppb), dim3(kThreads),
-             (f4_lds<T, N2>()), w, A, B, np, ppb);
+      auto go = [&](auto PRC) {
+        constexpr bool PR = decltype(PRC)::value;
+        f4_allow_lds(&k_ypass_f<T, N2, DIR, PR>, f4_lds<T, N2>());
+        LAUNCH(kid, (k_ypass_f<T, N2, DIR, PR>), dim3(ctx->ncols, zt, (np + ppb - 1) / ppb), dim3(kThreads),
+               (f4_lds<T, N2>()), w, A, B, np, ppb);
+      };
+      if constexpr (DIR > 0 && FastShape<N2>::A == 16) {
+        if (ctx->prune_y) go(std::true_type{});
+        else go(std::false_type{});
+      } else {
+        go(std::false_type{});
+      }
     } else {
       LAUNCH(kid, (k_ypass<T, N2, DIR>), dim3(ctx->ncols, (ctx->n3 + s.tz - 1) / s.tz, np), dim3(kThreads), shy, w,
              A, B, s.tz);
@@ -1025,9 +1045,18 @@ static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, in
     with_length(ctx->n1, [&](auto NN) {
       constexpr int N1 = decltype(NN)::value;
       if constexpr (f4_ok<T, N1>()) {
-        f4_allow_lds(&k_xrho_f<T, N1>, f4_lds<T, N1>());
-        LAUNCH(MC_KID_XRHO, (k_xrho_f<T, N1>), dim3((ctx->n3 + F4Tile<N1>::L - 1) / F4Tile<N1>::L, ctx->n2),
-               dim3(kThreads), (f4_lds<T, N1>()), w, B, np, pair0, nstate, ctx->w_f.as<double>(), rho);
+        auto go = [&](auto PRC) {
+          constexpr bool PR = decltype(PRC)::value;
+          f4_allow_lds(&k_xrho_f<T, N1, PR>, f4_lds<T, N1>());
+          LAUNCH(MC_KID_XRHO, (k_xrho_f<T, N1, PR>), dim3((ctx->n3 + F4Tile<N1>::L - 1) / F4Tile<N1>::L, ctx->n2),
+                 dim3(kThreads), (f4_lds<T, N1>()), w, B, np, pair0, nstate, ctx->w_f.as<double>(), rho);
+        };
+        if constexpr (FastShape<N1>::A == 16) {
+          if (ctx->prune_x) go(std::true_type{});
+          else go(std::false_type{});
+        } else {
+          go(std::false_type{});
+        }
       } else {
         LAUNCH(MC_KID_XRHO, (k_xrho<T, N1>), dim3((ctx->n2 + s.ty - 1) / s.ty, ctx->n3), dim3(kThreads), shx, w, B,
                np, pair0, nstate, ctx->w_f.as<double>(), rho, s.ty);
@@ -1090,9 +1119,18 @@ static void vpsi_run(mc_ctx* ctx, const double2* c0, double2* c2, const double* 
     with_length(ctx->n1, [&](auto NN) {
       constexpr int N1 = decltype(NN)::value;
       if constexpr (f4_ok<T, N1>()) {
-        f4_allow_lds(&k_xvpsi_f<T, N1>, f4_lds<T, N1>());
-        LAUNCH(MC_KID_XVPSI, (k_xvpsi_f<T, N1>), dim3((ctx->n3 + F4Tile<N1>::L - 1) / F4Tile<N1>::L, ctx->n2),
-               dim3(kThreads), (f4_lds<T, N1>()), w, B, np, vpot);
+        auto go = [&](auto PRC) {
+          constexpr bool PR = decltype(PRC)::value;
+          f4_allow_lds(&k_xvpsi_f<T, N1, PR>, f4_lds<T, N1>());
+          LAUNCH(MC_KID_XVPSI, (k_xvpsi_f<T, N1, PR>), dim3((ctx->n3 + F4Tile<N1>::L - 1) / F4Tile<N1>::L, ctx->n2),
+                 dim3(kThreads), (f4_lds<T, N1>()), w, B, np, vpot);
+        };
+        if constexpr (FastShape<N1>::A == 16) {
+          if (ctx->prune_x) go(std::true_type{});
+          else go(std::false_type{});
+        } else {
+          go(std::false_type{});
+        }
       } else {
         LAUNCH(MC_KID_XVPSI, (k_xvpsi<T, N1>), dim3((ctx->n2 + s.ty - 1) / s.ty, ctx->n3), dim3(kThreads), shx, w,
                B, np, vpot, s.ty);
