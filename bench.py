@@ -3,21 +3,20 @@
 
 A step is one Car-Parrinello MD step of the 512-state workload on the 192^3
 mesh (198,204 Gamma plane waves, 1.6M density G vectors, cubic a = 32 bohr,
-256 atoms, ecut 80 Ry, dt = 1 a.u., fictitious mass 400 a.u., started from
-the seeded coefficients after 30 FP64 steepest-descent steps) with every
-input resident in HBM:
+256 atoms, ecut 80 Ry) with every input resident in HBM, in CPMD's velocity
+Verlet order (src/mdmain_utils.mod.F90:710-966):
 
-  phfac -> rnlsm1 + rnlsm2 -> rhoofr -> vofrho (+ potfor) -> Ewald
-        -> vpsi + fnonloc (c2 = -f H c) -> rnlfor
-        -> velocity Verlet of the ions and of the coefficients (velupa, posupa)
-        -> rortog (orthonormality constraint)
+  velupi, velupa + posupi + phfac + posupa (rortog),
+  forces: rnlsm1 + rnlsm2 -> rhoofr -> vofrho (+ potfor) -> Ewald
+          -> vpsi + fnonloc (c2 = -f H c) -> rnlfor,
+  velupi, velupa + rortv, rekine.
 
-(CPMD's forces driver, src/forces_driver.mod.F90:175-384, and the CP update
-of src/posupa_utils.mod.F90 / src/velupa_utils.mod.F90 / src/rortog_utils.mod.F90).
---step wfopt times a steepest-descent wavefunction-optimisation step instead
-(Lagrange projection, update, Loewdin).  Wavefunctions: seeded complex
-Gaussian ~exp(-|G|^2/8), Loewdin-orthonormalised; GTH pseudopotential with one
-s projector per atom; LDA (Pade).
+The default start is the ODIIS ground state of the seeded coefficients (30
+FP64 steepest-descent steps, then DIIS to gemax < 1e-5, untimed), dt = 4 a.u.,
+fictitious mass 400 a.u.  --step wfopt times a steepest-descent wavefunction
+optimisation step instead (Lagrange projection, update, Loewdin).
+Wavefunctions: seeded complex Gaussian ~exp(-|G|^2/8), Loewdin-orthonormalised;
+GTH pseudopotential with one s projector per atom; LDA (Pade).
 
 Two precision paths are timed from the same start, the first one being the
 headline `value`:
@@ -27,10 +26,15 @@ headline `value`:
   fp64   every operation in FP64.
 Reported errors: E_KS and F_ion of the headline path against FP64 on the same
 coefficients and positions, the same after the warmup + timed steps from the
-same start (both evaluated in FP64), and ||C^T C - I||.
+same start (both evaluated in FP64), ||C^T C - I||, and the E_ham = E_KS +
+ekinp + ekinc drift of each path.
 
-Prints one JSON line: metric, value (ms per CP step of the mixed path), roofline of the dominant
-transform kernel, MFMA and transform summaries, error fields and the CPU baseline.
+Prints one JSON line: metric, value (ms per CP step of the mixed path), the
+roofline of the step's dominant kernel (the largest time per step among the
+transform kernels and the A^T B contraction kernel, with the measured issue
+rate of its MFMA instruction), the HBM rate of every transform stage, MFMA
+summaries, the error fields and the CPU baseline.  --pmc-copy only runs the
+known-size copy that scales rocprofv3's byte counters.
 """
 from __future__ import annotations
 
