@@ -107,6 +107,30 @@ def test_cfg5_pair(gpu):
             assert np.abs(c2_g - c2_o).max() < rtol * np.abs(c2_o).max(), mode
 
 
+@pytest.mark.parametrize("nstate", [3, 4])
+def test_sphere_beyond_quarter_band(gpu, nstate):
+    """dual 3 on a 128^3 mesh: the wavefunction sphere reaches |m| = 36 > N/4,
+    so the x and y passes keep every frequency (the unpruned 16 x 8 engine);
+    rhoofr and vpsi against the oracle in FP64 and with FP32 transforms."""
+    cell = S.cubic_cell(19.5, 135.0, dual=3.0)
+    assert tuple(cell.nr) == (128, 128, 128)
+    gs = S.gvectors(cell, with_density=False)
+    assert np.abs(gs.miller[:gs.ngw, :2]).max() >= 128 // 4
+    sysm = S.SyntheticSystem(cell=cell, gset=gs, tau=np.zeros((1, 3)), nstate=nstate, f=np.full(nstate, 2.0))
+    c = S.initial_coefficients(sysm, seed=31, nstate=nstate)
+    ctx = ops.Context(cell, gs)
+    L = oracle.Layout(cell, gs)
+    f = np.linspace(2.0, 1.0, nstate)
+    v = S.real_space_potential(sysm, seed=7)
+    _check_rho(ctx, L, c, f)
+    _check_vpsi(ctx, L, c, f, v)
+    ctx.set_precision("fp32")
+    rho_o = oracle.rhoofr(L, c, f)[0]
+    assert np.abs(ctx.rhoofr(c, f)[0] - rho_o).max() < 2e-6 * np.abs(rho_o).max()
+    c2_o = oracle.vpsi(L, c, np.zeros_like(c), f, v)
+    assert np.abs(ctx.vpsi(c, np.zeros_like(c), f, v) - c2_o).max() < 2e-6 * np.abs(c2_o).max()
+
+
 def test_cfg2_parity_subset(gpu):
     sysm, c = config_system("cfg2", nstate=6)
     ctx = ops.Context(sysm.cell, sysm.gset)
