@@ -56,6 +56,7 @@ SIGNATURES = {
     "mc_ctx_set_stream": (C.c_int, [_vp, _vp]),
     "mc_ctx_synchronize": (C.c_int, [_vp]),
     "mc_ctx_set_batch": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "mc_ctx_set_column_reuse": (C.c_int, [_vp, C.c_int]),
     "mc_wfn_ld": (C.c_size_t, [_vp]),
     "mc_set_precision": (C.c_int, [_vp, C.c_int]),
     "mc_dev_alloc": (C.c_int, [C.c_size_t, C.POINTER(_vp)]),

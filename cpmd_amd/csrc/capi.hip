@@ -194,7 +194,7 @@ int mc_set_eigr(mc_ctx* ctx, const double* eigr, int lde, int nat) {
 }
 
 int mc_rnlsm1_dev(mc_ctx* ctx, const void* c0_dev, int nstate, double* fnl) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   check_ctx(ctx);
   MC_CHECK(c0_dev && nstate >= 0, MC_ERR_ARG, "rnlsm1: bad argument");
   mc::rnlsm1_dev(ctx, (const double2*)c0_dev, nstate, fnl);
@@ -220,7 +220,7 @@ int mc_set_gk(mc_ctx* ctx, const double* gk) {
 }
 
 int mc_rnlsm2_dev(mc_ctx* ctx, const void* c0_dev, int nstate, double* dfnl) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   check_ctx(ctx);
   MC_CHECK(c0_dev && nstate >= 0, MC_ERR_ARG, "rnlsm2: bad argument");
   mc::rnlsm2_dev(ctx, (const double2*)c0_dev, nstate, dfnl);
@@ -238,7 +238,7 @@ int mc_rnlsm2(mc_ctx* ctx, const double* c0, int ldc, int nstate, double* dfnl) 
 }
 
 int mc_rnlsm_dev(mc_ctx* ctx, const void* c0_dev, int nstate, int tfor, double* fnl, double* dfnl) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   check_ctx(ctx);
   MC_CHECK(c0_dev && nstate >= 0, MC_ERR_ARG, "rnlsm: bad argument");
   mc::rnlsm_dev(ctx, (const double2*)c0_dev, nstate, tfor != 0, fnl, dfnl);
@@ -260,7 +260,7 @@ int mc_posupa_dev(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, void* cm_
 }
 
 int mc_rnlfor(mc_ctx* ctx, const double* f, int nstate, const double* fnl, const double* dfnl, double* fion) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   check_ctx(ctx);
   MC_CHECK(f && fion && nstate >= 0, MC_ERR_ARG, "rnlfor: bad argument");
   mc::rnlfor_dev(ctx, f, nstate, fnl, dfnl, fion);
@@ -285,7 +285,7 @@ int mc_set_local_pp(mc_ctx* ctx, int nsp, const int* na, const double* zv, const
 }
 
 int mc_set_ions(mc_ctx* ctx, const double* tau, const double* ht) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   check_ctx(ctx);
   MC_CHECK(ht && (tau || ctx->dens.nat == 0), MC_ERR_ARG, "set_ions: null argument");
   mc::set_ions(ctx, tau, ht);
@@ -293,7 +293,7 @@ int mc_set_ions(mc_ctx* ctx, const double* tau, const double* ht) {
 }
 
 int mc_vofrho_dev(mc_ctx* ctx, const void* rho_dev, void* v_dev, double* energies, double* fion) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   check_ctx(ctx);
   MC_CHECK(rho_dev && v_dev && rho_dev != v_dev, MC_ERR_ARG, "vofrho: bad rho / v");
   mc::vofrho_dev(ctx, (const double*)rho_dev, (double*)v_dev, energies, fion);
@@ -316,14 +316,14 @@ int mc_vofrho(mc_ctx* ctx, const double* rho, double* v, double* energies, doubl
 }
 
 int mc_phfac(mc_ctx* ctx) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   check_ctx(ctx);
   mc::phfac(ctx);
   MC_API_END
 }
 
 int mc_ewald(mc_ctx* ctx, int iesr, double* esr, double* eself, double* fion) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   check_ctx(ctx);
   MC_CHECK(esr && eself, MC_ERR_ARG, "ewald: null argument");
   mc::ewald(ctx, iesr, esr, eself, fion);

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -32,6 +33,10 @@ namespace mc {
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 void clear_error() { g_last_error.clear(); }
+
+static std::atomic<unsigned long long> g_wfn_epoch{0};
+unsigned long long wfn_epoch() { return g_wfn_epoch.load(std::memory_order_acquire); }
+void wfn_epoch_bump() { g_wfn_epoch.fetch_add(1, std::memory_order_acq_rel); }
 
 // ---------------------------------------------------------------------------
 // device memory accounting: every allocation of the library (context buffers
@@ -900,7 +905,7 @@ int mc_ctx_set_stream(mc_ctx* ctx, void* hip_stream) {
 }
 
 int mc_ctx_synchronize(mc_ctx* ctx) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   MC_CHECK(ctx, MC_ERR_ARG, "null context");
   MC_HIP(hipStreamSynchronize(ctx->stream));
   MC_API_END
@@ -913,6 +918,16 @@ int mc_ctx_set_batch(mc_ctx* ctx, int pairs_rhoofr, int pairs_vpsi) {
            MC_ERR_ARG, "batch sizes must be in [1, 4096]");
   ctx->batch_pairs_rho = pairs_rhoofr;
   ctx->batch_pairs_vpsi = pairs_vpsi;
+  MC_API_END
+}
+
+int mc_ctx_set_column_reuse(mc_ctx* ctx, int on) {
+  MC_API_BEGIN
+  MC_CHECK(ctx, MC_ERR_ARG, "null context");
+  MC_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->col_reuse = on != 0;
+  ctx->keep_valid = false;
+  if (!ctx->col_reuse) ctx->w_keep.release();
   MC_API_END
 }
 
@@ -931,7 +946,7 @@ int mc_set_precision(mc_ctx* ctx, int fft_precision) {
 }
 
 int mc_dev_alloc(size_t bytes, void** ptr) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO   // a new buffer holds no wavefunctions yet (mc_dev_free counts an epoch)
   MC_CHECK(ptr, MC_ERR_ARG, "null pointer");
   *ptr = nullptr;
   if (bytes) {
@@ -972,7 +987,7 @@ int mc_set_mem_limit(size_t bytes) {
   MC_API_END
 }
 int mc_mem_stats(size_t* current, size_t* peak, size_t* limit) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   std::lock_guard<std::mutex> lk(mc::mem().mu);
   if (current) *current = mc::mem().cur;
   if (peak) *peak = mc::mem().peak;
@@ -991,7 +1006,7 @@ int mc_memcpy_h2d(void* dst, const void* src, size_t bytes) {
   MC_API_END
 }
 int mc_memcpy_d2h(void* dst, const void* src, size_t bytes) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   if (bytes) MC_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   MC_API_END
 }
@@ -1331,7 +1346,7 @@ int mc_wfn_upload(mc_ctx* ctx, const double* c_host, int ldc, int nstate, void* 
 }
 
 int mc_wfn_download(mc_ctx* ctx, const void* c_dev, int nstate, double* c_host, int ldc) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   MC_CHECK(ctx && c_host && c_dev, MC_ERR_ARG, "null argument");
   mc::wfn_download(ctx, (const double2*)c_dev, nstate, c_host, ldc);
   MC_API_END
@@ -1339,10 +1354,10 @@ int mc_wfn_download(mc_ctx* ctx, const void* c_dev, int nstate, double* c_host, 
 
 int mc_rhoofr_dev(mc_ctx* ctx, const void* c0_dev, const double* f, int nstate, void* rhoe_dev,
                   double* ekin, double* csumg, double* csumr) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   MC_CHECK(ctx && c0_dev && f && rhoe_dev, MC_ERR_ARG, "null argument");
   MC_CHECK(nstate >= 0, MC_ERR_ARG, "nstate < 0");
-  mc::rhoofr_dev(ctx, (const double2*)c0_dev, f, nstate, (double*)rhoe_dev, ekin, csumg, csumr);
+  mc::rhoofr_dev(ctx, (const double2*)c0_dev, f, nstate, (double*)rhoe_dev, ekin, csumg, csumr, true);
   MC_API_END
 }
 
@@ -1364,15 +1379,15 @@ int mc_rhoofr(mc_ctx* ctx, const double* c0, int ldc, const double* f, int nstat
 
 int mc_vpsi_dev(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const double* f, const void* vpot_dev,
                 int nstate) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   MC_CHECK(ctx && c0_dev && c2_dev && f && vpot_dev, MC_ERR_ARG, "null argument");
   MC_CHECK(nstate >= 0, MC_ERR_ARG, "nstate < 0");
-  mc::vpsi_dev(ctx, (const double2*)c0_dev, (double2*)c2_dev, f, (const double*)vpot_dev, nstate);
+  mc::vpsi_dev(ctx, (const double2*)c0_dev, (double2*)c2_dev, f, (const double*)vpot_dev, nstate, true, true);
   MC_API_END
 }
 
 int mc_rhoofr_raw_charge(mc_ctx* ctx, double* csumg, double* csumr_raw) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   MC_CHECK(ctx, MC_ERR_ARG, "null context");
   if (csumg) *csumg = ctx->last_csumg;
   if (csumr_raw) *csumr_raw = ctx->last_csumr_raw;
@@ -1391,7 +1406,7 @@ int mc_vpsi_dev_noalloc(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const dou
 
 int mc_forces_c2_dev(mc_ctx* ctx, const void* c0_dev, void* c2_dev, const double* f, const void* vpot_dev,
                      int nstate) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   MC_CHECK(ctx && c0_dev && c2_dev && f && vpot_dev, MC_ERR_ARG, "null argument");
   MC_CHECK(nstate >= 0, MC_ERR_ARG, "nstate < 0");
   mc::forces_c2_dev(ctx, (const double2*)c0_dev, (double2*)c2_dev, f, (const double*)vpot_dev, nstate);
@@ -1417,21 +1432,21 @@ int mc_vpsi(mc_ctx* ctx, const double* c0, int ldc, double* c2, const double* f,
 }
 
 int mc_timer_enable(mc_ctx* ctx, int on) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   MC_CHECK(ctx, MC_ERR_ARG, "null context");
   mc::timer_flush(ctx);
   ctx->timer.enabled = on != 0;
   MC_API_END
 }
 int mc_timer_reset(mc_ctx* ctx) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   MC_CHECK(ctx, MC_ERR_ARG, "null context");
   mc::timer_flush(ctx);
   for (int k = 0; k < MC_KID_COUNT; ++k) { ctx->timer.total_ms[k] = 0; ctx->timer.count[k] = 0; }
   MC_API_END
 }
 int mc_timer_get(mc_ctx* ctx, int kid, double* total_ms, long long* count) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   MC_CHECK(ctx && kid >= 0 && kid < MC_KID_COUNT, MC_ERR_ARG, "bad kernel id");
   mc::timer_flush(ctx);
   if (total_ms) *total_ms = ctx->timer.total_ms[kid];

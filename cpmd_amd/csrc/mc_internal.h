@@ -34,8 +34,19 @@ struct Error {
                                         std::to_string(__LINE__) + ")"};       \
   } while (0)
 
-#define MC_API_BEGIN \
-  try {              \
+// Every entry point counts one wavefunction epoch, except those that never
+// write device wavefunction memory (MC_API_BEGIN_RO); an intermediate kept
+// from one call for a later one on the same wavefunctions is valid only while
+// the count is unchanged.
+unsigned long long wfn_epoch();
+void wfn_epoch_bump();
+
+#define MC_API_BEGIN      \
+  try {                   \
+    mc::clear_error();    \
+    mc::wfn_epoch_bump();
+#define MC_API_BEGIN_RO \
+  try {                 \
     mc::clear_error();
 #define MC_API_END                                   \
   }                                                  \
@@ -183,10 +194,12 @@ struct mc_ctx {
   mc_ctx() {
     mc::ws_register(&w_a);
     mc::ws_register(&w_b);
+    mc::ws_register(&w_keep);
   }
   ~mc_ctx() {
     mc::ws_unregister(&w_a);
     mc::ws_unregister(&w_b);
+    mc::ws_unregister(&w_keep);
   }
   mc_ctx(const mc_ctx&) = delete;
   mc_ctx& operator=(const mc_ctx&) = delete;
@@ -240,6 +253,15 @@ struct mc_ctx {
   // workspaces
   mc::DevBuf w_a;          // sticks:  [pair][slot][z]       complex
   mc::DevBuf w_b;          // columns: [pair][z][col][y]     complex
+  // column reuse (mc_ctx_set_column_reuse): rhoofr_dev keeps the columns of
+  // every pair after the inverse y pass in w_keep, and the next vpsi on the
+  // same wavefunctions, precision and epoch starts from them
+  bool col_reuse = false;
+  mc::DevBuf w_keep;
+  bool keep_valid = false;
+  const void* keep_c0 = nullptr;
+  int keep_nstate = 0, keep_tsize = 0;
+  unsigned long long keep_epoch = 0;
   mc::DevBuf w_c0, w_c2;   // staging of host wavefunctions (device layout)
   mc::DevBuf w_v;          // staging of host potential / density
   mc::DevBuf w_red;        // reduction partials
@@ -421,10 +443,13 @@ void wfn_upload(mc_ctx* ctx, const double* host, int ldc, int nstate, double2* d
 void wfn_download(mc_ctx* ctx, const double2* dev, int nstate, double* host, int ldc);
 
 // driver entry points implemented in wfft.hip
+// keep / reuse: the column reuse of mc_ctx_set_column_reuse applies (the
+// device-resident entry points; host-pointer ones stage through buffers
+// that the next call overwrites)
 void rhoofr_dev(mc_ctx* ctx, const double2* c0, const double* f_host, int nstate,
-                double* rho, double* ekin, double* csumg, double* csumr);
+                double* rho, double* ekin, double* csumg, double* csumr, bool keep = false);
 void vpsi_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host,
-              const double* vpot, int nstate, bool acc = true);
+              const double* vpot, int nstate, bool acc = true, bool reuse = false);
 void forces_c2_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host,
                    const double* vpot, int nstate);
 // address of the allocation-free scratch array on `device` (the current

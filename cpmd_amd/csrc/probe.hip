@@ -93,7 +93,7 @@ double probe_once(hipStream_t st, float* out, int iters, double flops_per_iter_w
 }  // namespace mc
 
 extern "C" int mc_probe_mfma(int kind, double* tflops) {
-  MC_API_BEGIN
+  MC_API_BEGIN_RO
   MC_CHECK(tflops && kind >= 0 && kind <= 3, MC_ERR_ARG, "probe_mfma: kind must be 0..3");
   *tflops = 0.0;
   mc::DevBuf buf;   // counted against the device memory limit like every library buffer

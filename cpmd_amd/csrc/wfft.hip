@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <optional>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -997,9 +998,26 @@ static int fit_pairs(mc_ctx* ctx, int P, size_t extra) {
 }
 
 template <typename T>
-static void ensure_workspace(mc_ctx* ctx, int P) {
+static void ensure_workspace(mc_ctx* ctx, int P, bool need_b = true) {
   ctx->w_a.ensure(sizeof(v2<T>) * (size_t)P * ctx->n_slots * ctx->n3);
-  ctx->w_b.ensure(sizeof(v2<T>) * (size_t)P * ctx->n3 * ctx->ncols * ctx->n2);
+  if (need_b) ctx->w_b.ensure(sizeof(v2<T>) * (size_t)P * ctx->n3 * ctx->ncols * ctx->n2);
+}
+
+// Column reuse: the columns of all npairs pairs in w_keep, when reuse is on,
+// no device memory limit is set and the allocation succeeds (else nullptr:
+// the caller works in batches as usual)
+template <typename T>
+static v2<T>* keep_columns(mc_ctx* ctx, int npairs) {
+  if (!ctx->col_reuse || mem_limit() != 0 || npairs <= 0) return nullptr;
+  const size_t bytes = sizeof(v2<T>) * (size_t)npairs * ctx->n3 * ctx->ncols * ctx->n2;
+  try {
+    ctx->w_keep.ensure(bytes);
+  } catch (const Error&) {
+    (void)hipGetLastError();   // a failed hipMalloc leaves its error behind
+    ctx->w_keep.release();
+    return nullptr;
+  }
+  return ctx->w_keep.as<v2<T>>();
 }
 
 static void upload_f(mc_ctx* ctx, const double* f_host, int nstate) {
@@ -1010,7 +1028,7 @@ static void upload_f(mc_ctx* ctx, const double* f_host, int nstate) {
 
 template <typename T>
 static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, int nstate, double* rho,
-                        double* ekin, double* csumg, double* csumr) {
+                        double* ekin, double* csumg, double* csumr, bool keep) {
   using V = v2<T>;
   const size_t nnr1 = (size_t)ctx->kr1 * ctx->kr2 * ctx->kr3;
   const WParams w = make_params(ctx);
@@ -1018,12 +1036,19 @@ static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, in
   const int npairs = (nstate + 1) / 2;
   const int nblk_sum = 1024;
   WsUse use(ctx->w_a, ctx->w_b);
+  ctx->keep_valid = false;
+  V* K = keep ? keep_columns<T>(ctx, npairs) : nullptr;
+  std::optional<WsUse> use_keep;
+  if (K)
+    use_keep.emplace(ctx->w_keep, ctx->w_keep);
+  else if (mem_limit() != 0)
+    ctx->w_keep.release();   // stale columns: room for the batch workspaces
   const size_t red_bytes =
       sizeof(double) * ((size_t)std::max(npairs, 1) * s.gx * 4 + (size_t)npairs * 4 + nblk_sum) +
       sizeof(double) * std::max(nstate, 1);
   const int P = fit_pairs<T>(ctx, std::max(1, std::min(ctx->batch_pairs_rho, npairs)),
                              std::max(red_bytes, ctx->w_red.bytes + ctx->w_f.bytes) - ctx->w_red.bytes - ctx->w_f.bytes);
-  ensure_workspace<T>(ctx, P);
+  ensure_workspace<T>(ctx, P, K == nullptr);
   upload_f(ctx, f_host, nstate);
   ctx->w_red.ensure(sizeof(double) * ((size_t)std::max(npairs, 1) * s.gx * 4 + (size_t)npairs * 4 + nblk_sum));
   double* red = ctx->w_red.as<double>();
@@ -1034,9 +1059,10 @@ static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, in
   const size_t shy = sizeof(V) * (size_t)ctx->n2 * (1 + s.tz);
   const size_t shx = sizeof(V) * (size_t)ctx->n1 * (1 + s.ty);
   V* A = ctx->w_a.as<V>();
-  V* B = ctx->w_b.as<V>();
+  const size_t bslab = (size_t)ctx->n3 * ctx->ncols * ctx->n2;
   for (int pair0 = 0; pair0 < npairs; pair0 += P) {
     const int np = std::min(P, npairs - pair0);
+    V* B = K ? K + (size_t)pair0 * bslab : ctx->w_b.as<V>();
     with_length(ctx->n3, [&](auto NN) {
       LAUNCH(MC_KID_ZINV, (k_zinv<T, decltype(NN)::value>), dim3(s.gx, np), dim3(kThreads), shz, w, c0, nstate,
              pair0, A, s.lo, red + (size_t)pair0 * s.gx * 4);
@@ -1062,6 +1088,13 @@ static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, in
                np, pair0, nstate, ctx->w_f.as<double>(), rho, s.ty);
       }
     });
+  }
+  if (K) {   // the columns of every pair, for the next vpsi on these wavefunctions
+    ctx->keep_valid = true;
+    ctx->keep_c0 = c0;
+    ctx->keep_nstate = nstate;
+    ctx->keep_tsize = (int)sizeof(T);
+    ctx->keep_epoch = wfn_epoch();
   }
   if (npairs > 0) LAUNCH(MC_KID_MISC, k_reduce_pairs, dim3(npairs), dim3(64), 0, red, s.gx, pair_sums);
   LAUNCH(MC_KID_MISC, k_sum_partial, dim3(nblk_sum), dim3(kThreads), 0, rho, nnr1, rsum_part);
@@ -1100,7 +1133,7 @@ static void rhoofr_impl(mc_ctx* ctx, const double2* c0, const double* f_host, in
 // workspaces A, B (P pairs each) and the occupations f_dev given
 template <typename T>
 static void vpsi_run(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_dev, const double* vpot,
-                     int nstate, bool acc, int P, v2<T>* A, v2<T>* B) {
+                     int nstate, bool acc, int P, v2<T>* A, v2<T>* B, v2<T>* K = nullptr) {
   const WParams w = make_params(ctx);
   const Shapes s = shapes<T>(ctx);
   const int npairs = (nstate + 1) / 2;
@@ -1109,13 +1142,18 @@ static void vpsi_run(mc_ctx* ctx, const double2* c0, double2* c2, const double* 
   const size_t shy = sizeof(V) * (size_t)ctx->n2 * (1 + s.tz);
   const size_t shx = sizeof(V) * (size_t)ctx->n1 * (1 + s.ty);
   const double scale = 1.0 / ((double)ctx->n1 * ctx->n2 * ctx->n3);
+  const size_t bslab = (size_t)ctx->n3 * ctx->ncols * ctx->n2;
   for (int pair0 = 0; pair0 < npairs; pair0 += P) {
     const int np = std::min(P, npairs - pair0);
-    with_length(ctx->n3, [&](auto NN) {
-      LAUNCH(MC_KID_ZINV, (k_zinv<T, decltype(NN)::value>), dim3(s.gx, np), dim3(kThreads), shz, w, c0, nstate,
-             pair0, A, s.lo, (double*)nullptr);
-    });
-    ypass<T, 1>(ctx, w, s, A, B, np, shy);
+    if (K) {   // the inverse z and y passes of rhoofr, kept: start from its columns
+      B = K + (size_t)pair0 * bslab;
+    } else {
+      with_length(ctx->n3, [&](auto NN) {
+        LAUNCH(MC_KID_ZINV, (k_zinv<T, decltype(NN)::value>), dim3(s.gx, np), dim3(kThreads), shz, w, c0, nstate,
+               pair0, A, s.lo, (double*)nullptr);
+      });
+      ypass<T, 1>(ctx, w, s, A, B, np, shy);
+    }
     with_length(ctx->n1, [&](auto NN) {
       constexpr int N1 = decltype(NN)::value;
       if constexpr (f4_ok<T, N1>()) {
@@ -1146,17 +1184,31 @@ static void vpsi_run(mc_ctx* ctx, const double2* c0, double2* c2, const double* 
 
 template <typename T>
 static void vpsi_impl(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, const double* vpot,
-                      int nstate, bool acc) {
+                      int nstate, bool acc, bool reuse) {
   (void)shapes<T>(ctx);   // rejects unsupported lengths before any allocation
   const int npairs = (nstate + 1) / 2;
   if (npairs == 0) return;
   const size_t f_bytes = sizeof(double) * std::max(nstate, 1);
   WsUse use(ctx->w_a, ctx->w_b);
+  // rhoofr's columns of these wavefunctions, if nothing wrote device
+  // wavefunctions since (the epoch) and they are still held
+  v2<T>* K = nullptr;
+  if (reuse && ctx->col_reuse && ctx->keep_valid && ctx->keep_c0 == c0 && ctx->keep_nstate == nstate &&
+      ctx->keep_tsize == (int)sizeof(T) && ctx->keep_epoch == wfn_epoch() && ctx->w_keep.p &&
+      ctx->w_keep.bytes >= sizeof(v2<T>) * (size_t)npairs * ctx->n3 * ctx->ncols * ctx->n2)
+    K = ctx->w_keep.as<v2<T>>();
+  ctx->keep_valid = false;   // the x pass below overwrites them
+  std::optional<WsUse> use_keep;
+  if (K)
+    use_keep.emplace(ctx->w_keep, ctx->w_keep);
+  else if (mem_limit() != 0)
+    ctx->w_keep.release();
   const int P = fit_pairs<T>(ctx, std::max(1, std::min(ctx->batch_pairs_vpsi, npairs)),
                              f_bytes > ctx->w_f.bytes ? f_bytes - ctx->w_f.bytes : 0);
-  ensure_workspace<T>(ctx, P);
+  ensure_workspace<T>(ctx, P, K == nullptr);
   upload_f(ctx, f_host, nstate);
-  vpsi_run<T>(ctx, c0, c2, ctx->w_f.as<double>(), vpot, nstate, acc, P, ctx->w_a.as<v2<T>>(), ctx->w_b.as<v2<T>>());
+  vpsi_run<T>(ctx, c0, c2, ctx->w_f.as<double>(), vpot, nstate, acc, P, ctx->w_a.as<v2<T>>(),
+              K ? nullptr : ctx->w_b.as<v2<T>>(), K);
 }
 
 // ---------------------------------------------------------------------------
@@ -1239,19 +1291,19 @@ void vpsi_noalloc_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double*
 }
 
 void rhoofr_dev(mc_ctx* ctx, const double2* c0, const double* f_host, int nstate, double* rho, double* ekin,
-                double* csumg, double* csumr) {
+                double* csumg, double* csumr, bool keep) {
   if (ctx->fft_fp32)
-    rhoofr_impl<float>(ctx, c0, f_host, nstate, rho, ekin, csumg, csumr);
+    rhoofr_impl<float>(ctx, c0, f_host, nstate, rho, ekin, csumg, csumr, keep);
   else
-    rhoofr_impl<double>(ctx, c0, f_host, nstate, rho, ekin, csumg, csumr);
+    rhoofr_impl<double>(ctx, c0, f_host, nstate, rho, ekin, csumg, csumr, keep);
 }
 
 void vpsi_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_host, const double* vpot, int nstate,
-              bool acc) {
+              bool acc, bool reuse) {
   if (ctx->fft_fp32)
-    vpsi_impl<float>(ctx, c0, c2, f_host, vpot, nstate, acc);
+    vpsi_impl<float>(ctx, c0, c2, f_host, vpot, nstate, acc, reuse);
   else
-    vpsi_impl<double>(ctx, c0, c2, f_host, vpot, nstate, acc);
+    vpsi_impl<double>(ctx, c0, c2, f_host, vpot, nstate, acc, reuse);
 }
 
 // rows ngw .. ldg-1 of the first n states of c (ld ldg complex rows) <- 0
@@ -1271,7 +1323,7 @@ void forces_c2_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_
     LAUNCH(MC_KID_MISC, k_zero_pad_rows, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c2, ctx->ngw, ctx->ldg,
            nstate);
   }
-  vpsi_dev(ctx, c0, c2, f_host, vpot, nstate, false);
+  vpsi_dev(ctx, c0, c2, f_host, vpot, nstate, false, true);
   if (ctx->proj.ready) fnonloc_dev(ctx, c2, f_host, nstate, nullptr);   // no projectors: local potential only
 }
 

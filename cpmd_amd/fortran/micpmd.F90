@@ -37,6 +37,7 @@
 !   micpmd_set_occupations(f)        crge%f(:,1)    (rhoofr)
 !   micpmd_set_cp_parameters(epsog, maxit)  cntr%epsog, cnti%maxit (rortog)
 !   micpmd_set_precision(mode)       'FP64' (default), 'FP32' or 'MIXED'
+!   micpmd_set_column_reuse(on)      keep rhoofr's columns for the next forces_c2
 !
 ! The device-resident path (wavefunctions, rho and V kept in HBM between
 ! calls) is bound as micpmd_alloc_wfn / micpmd_alloc_real / micpmd_free /
@@ -96,6 +97,7 @@ MODULE micpmd
   PUBLIC :: micpmd_set_gk, micpmd_rnlsm2, micpmd_rnlfor
   PUBLIC :: micpmd_set_density, micpmd_set_local_pp, micpmd_set_ions, micpmd_vofrho, micpmd_ewald
   PUBLIC :: micpmd_set_occupations, micpmd_set_cp_parameters, micpmd_set_precision, micpmd_set_ortho
+  PUBLIC :: micpmd_set_column_reuse
   PUBLIC :: micpmd_rnlsm, micpmd_ortho, micpmd_rortv
   PUBLIC :: micpmd_alloc_wfn, micpmd_alloc_real, micpmd_free, micpmd_upload, micpmd_download, micpmd_zero_wfn
   PUBLIC :: micpmd_phfac, micpmd_rnlsm_dev, micpmd_rhoofr_dev, micpmd_vofrho_dev, micpmd_forces_c2_dev
@@ -288,6 +290,12 @@ MODULE micpmd
        INTEGER(c_int), VALUE :: mode
        INTEGER(c_int) :: ierr
      END FUNCTION mc_set_precision
+     FUNCTION mc_ctx_set_column_reuse(c, on) BIND(C, name='mc_ctx_set_column_reuse') RESULT(ierr)
+       IMPORT :: c_int, c_ptr
+       TYPE(c_ptr), VALUE :: c
+       INTEGER(c_int), VALUE :: on
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_ctx_set_column_reuse
      FUNCTION mc_dev_alloc(bytes, p) BIND(C, name='mc_dev_alloc') RESULT(ierr)
        IMPORT :: c_int, c_ptr, c_size_t
        INTEGER(c_size_t), VALUE :: bytes
@@ -699,6 +707,19 @@ CONTAINS
     END SELECT
     CALL check(mc_set_precision(ctx, m), 'set_precision')
   END SUBROUTINE micpmd_set_precision
+
+  ! Device-resident drivers: micpmd_rhoofr_dev keeps the y-transformed
+  ! columns of every state pair and the next micpmd_forces_c2_dev on the same
+  ! wavefunctions starts from them (bit-identical results, one inverse z and
+  ! y pass fewer per step).  The driver must not write the device
+  ! wavefunctions outside the library between the two calls.
+  SUBROUTINE micpmd_set_column_reuse(on)
+    LOGICAL, INTENT(IN) :: on
+    INTEGER(c_int) :: m
+    m = 0
+    IF (on) m = 1
+    CALL check(mc_ctx_set_column_reuse(ctx, m), 'set_column_reuse')
+  END SUBROUTINE micpmd_set_column_reuse
 
   ! 'LOWDIN' or 'GRAM-SCHMIDT' (CPMD's cntl%tlowd, ORTHOGONALIZATION LOWDIN)
   SUBROUTINE micpmd_set_ortho(method)

@@ -22,7 +22,7 @@ from . import ops
 class KSDevice:
     """One system on one GPU: layout, projectors, density set, local PP, ions."""
 
-    def __init__(self, sysm, raggio: float = 1.2, iesr: int = 2):
+    def __init__(self, sysm, raggio: float = 1.2, iesr: int = 2, column_reuse: bool = True):
         from . import system as S
         gs = sysm.gset
         if gs.nhg == gs.ngw:
@@ -40,6 +40,9 @@ class KSDevice:
         ctx.set_local_pp(na, zv, rg, vps, rhops)
         # G=0 local potential (ppener's vploc = eivps(G=0), src/ppener_utils.mod.F90:58-60)
         self.vploc = float(np.dot(np.asarray(na, dtype=np.float64), vps[:, 0])) if gs.hg[0] == 0.0 else 0.0
+        # energy() calls forces_c2_dev right after rhoofr_dev on the same
+        # coefficients: vpsi starts from rhoofr's y-transformed columns
+        ctx.set_column_reuse(column_reuse)
         self.ctx = ctx
         self.hmat = np.asarray(hmat, dtype=np.float64).reshape(nhxs, nhxs, -1, order="F")
         self.na = np.asarray(na)

@@ -142,6 +142,13 @@ class Context:
     def set_batch(self, pairs_rhoofr: int, pairs_vpsi: int):
         check(lib().mc_ctx_set_batch(self.h, pairs_rhoofr, pairs_vpsi))
 
+    def set_column_reuse(self, on: bool = True):
+        """rhoofr_dev keeps the y-transformed columns of every state pair; the
+        next vpsi_dev / forces_c2_dev on the same wavefunction tensor starts
+        from them (bit-identical).  The caller must not write that tensor
+        outside the library between the two calls."""
+        check(lib().mc_ctx_set_column_reuse(self.h, 1 if on else 0))
+
     def set_precision(self, mode: str):
         """'fp64' (default), 'fp32' (FP32 rhoofr/vpsi transforms), 'mixed' (FP32
         transforms and split-bf16 MFMA wavefunction contractions) or

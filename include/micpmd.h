@@ -97,6 +97,21 @@ int mc_ctx_synchronize(mc_ctx* ctx);
  * one pair takes ncols * n2 * n3 complex elements of the transform precision;
  * defaults 128 / 64). */
 int mc_ctx_set_batch(mc_ctx* ctx, int pairs_rhoofr, int pairs_vpsi);
+/* Column reuse (no counterpart in the reference; for drivers that keep the
+ * wavefunctions on the device, such as a CP step calling mc_rhoofr_dev and
+ * then mc_forces_c2_dev or mc_vpsi_dev on the same c0_dev).  While on,
+ * mc_rhoofr_dev keeps the y-transformed columns of every state pair
+ * (npairs * ncols * n2 * n3 complex elements of the transform precision,
+ * allocated once and held until off or mc_ctx_destroy) and the next
+ * mc_vpsi_dev / mc_forces_c2_dev with the same c0_dev, nstate and precision
+ * starts from them, skipping its inverse z and y passes; the results are
+ * bit-identical.  Every library call that writes device memory other than
+ * rho, V, c2 or the projections, and every host-pointer transform call,
+ * invalidates the kept columns.  The caller must not write c0_dev itself
+ * (a kernel of its own, a copy) between the two calls while reuse is on.
+ * Not used under a device memory limit (mc_set_memory_limit) or when the
+ * allocation fails: the calls then work in batches as before.  Default off. */
+int mc_ctx_set_column_reuse(mc_ctx* ctx, int on);
 size_t mc_wfn_ld(const mc_ctx* ctx);
 
 /* Arithmetic mode (no counterpart in the reference, which is FP64 only):

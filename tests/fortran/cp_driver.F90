@@ -52,6 +52,7 @@ PROGRAM cp_driver
   CALL micpmd_set_occupations(f)
   CALL micpmd_set_cp_parameters(1.0e-8_dp, 60)
   CALL micpmd_set_precision(TRIM(arg(1:256)))
+  CALL micpmd_set_column_reuse(.TRUE.)
 
   d_c0 = micpmd_alloc_wfn(nstate); d_cm = micpmd_alloc_wfn(nstate)
   d_cp = micpmd_alloc_wfn(nstate); d_c2 = micpmd_alloc_wfn(nstate)

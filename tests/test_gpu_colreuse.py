@@ -1,0 +1,175 @@
+"""Column reuse (mc_ctx_set_column_reuse): vpsi_dev / forces_c2_dev right after
+rhoofr_dev on the same device wavefunctions start from rhoofr's y-transformed
+columns.  The results must be bit-identical to the unshared transforms, the
+inverse z and y passes must actually be skipped (kernel launch counts), and
+every change that makes the kept columns stale must fall back to a full
+transform: a library write to the wavefunctions, another wavefunction
+buffer, another precision, a device memory limit.
+"""
+import numpy as np
+import pytest
+
+from cpmd_amd import ops
+from cpmd_amd import system as S
+from util import small_system
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(kind, nstate):
+    if kind == "mesh128":   # fused 16 x 8 engine, pruned x / y passes
+        cell = S.cubic_cell(19.5, 100.0)
+        assert tuple(cell.nr) == (128, 128, 128)
+        gs = S.gvectors(cell, with_density=False)
+        sysm = S.SyntheticSystem(cell=cell, gset=gs, tau=np.zeros((1, 3)), nstate=nstate,
+                                 f=np.full(nstate, 2.0))
+        c = S.initial_coefficients(sysm, seed=17, nstate=nstate)
+    else:
+        cell, gs, c = small_system(kind, nstate)
+        sysm = S.SyntheticSystem(cell=cell, gset=gs, tau=np.zeros((1, 3)), nstate=nstate,
+                                 f=np.full(nstate, 2.0))
+    ctx = ops.Context(cell, gs)
+    v = S.real_space_potential(sysm, seed=5)
+    f = np.linspace(2.0, 0.25, nstate)
+    f[nstate // 2] = 0.0   # an empty state: rhoofr's x pass skips it, vpsi still needs its columns
+    return ctx, c, f, v
+
+
+def _run(ctx, d_c0, f, n, d_v, c2_init):
+    """rhoofr_dev, then vpsi_dev into a copy of c2_init: (rho, ekin, c2).
+    Buffers are written before rhoofr: a library write in between (an
+    upload) makes the kept columns stale."""
+    d_rho = ops.DeviceArray(8 * ctx.nnr1)
+    d_c2 = ctx.upload(c2_init)
+    ek = ctx.rhoofr_dev(d_c0, f, n, d_rho)[0]
+    ctx.vpsi_dev(d_c0, d_c2, f, d_v, n)
+    return d_rho.to_host(ctx.nnr1), ek, ctx.download(d_c2, n)
+
+
+def _launches(ctx, name):
+    ctx.synchronize()
+    return ctx.timer_get(name)[1]
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32"])
+@pytest.mark.parametrize("kind,nstate,batch", [("cubic", 6, (3, 2)), ("ortho2", 5, (64, 64)),
+                                               ("ortho3", 7, (2, 3)), ("mesh128", 9, (4, 3))])
+def test_bit_identical(gpu, mode, kind, nstate, batch):
+    ctx, c, f, v = _setup(kind, nstate)
+    ctx.set_precision(mode)
+    ctx.set_batch(*batch)   # rhoofr and vpsi batches differ: the kept slabs are per pair
+    rng = np.random.default_rng(2)
+    c2_init = np.asfortranarray(rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape))
+    d_c0 = ctx.upload(c)
+    d_v = ops.DeviceArray(8 * ctx.nnr1).from_host(v)
+    rho0, ek0, c20 = _run(ctx, d_c0, f, nstate, d_v, c2_init)
+    ctx.set_column_reuse(True)
+    ctx.timer_enable(True)
+    rho1, ek1, c21 = _run(ctx, d_c0, f, nstate, d_v, c2_init)
+    zinv_rho = _launches(ctx, "zinv")
+    assert zinv_rho > 0
+    assert np.array_equal(rho1, rho0) and ek1 == ek0
+    assert np.array_equal(c21, c20)
+    # one more step: vpsi with the kept columns launched no inverse z pass
+    d_rho = ops.DeviceArray(8 * ctx.nnr1)
+    d_c2 = ctx.upload(c2_init)
+    ctx.timer_reset()
+    ctx.rhoofr_dev(d_c0, f, nstate, d_rho)
+    z_after_rho = _launches(ctx, "zinv")
+    y_after_rho = _launches(ctx, "yinv")
+    ctx.timer_reset()
+    ctx.vpsi_dev(d_c0, d_c2, f, d_v, nstate)
+    assert _launches(ctx, "zinv") == 0 and _launches(ctx, "yinv") == 0
+    assert z_after_rho > 0 and y_after_rho > 0
+    assert np.array_equal(ctx.download(d_c2, nstate), c20)
+    # the columns were consumed: a second vpsi transforms again
+    ctx.upload(c2_init, d_c2)
+    ctx.timer_reset()
+    ctx.vpsi_dev(d_c0, d_c2, f, d_v, nstate)
+    assert _launches(ctx, "zinv") > 0
+    assert np.array_equal(ctx.download(d_c2, nstate), c20)
+
+
+def test_stale_columns_fall_back(gpu):
+    ctx, c, f, v = _setup("mesh128", 8)
+    n = c.shape[1]
+    rng = np.random.default_rng(4)
+    c2_init = np.asfortranarray(rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape))
+    d_v = ops.DeviceArray(8 * ctx.nnr1).from_host(v)
+    d_c0 = ctx.upload(c)
+    d_x = ctx.upload(np.asfortranarray(rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape)))
+    d_rho = ops.DeviceArray(8 * ctx.nnr1)
+
+    def vpsi_fresh(d_c):
+        ctx.set_column_reuse(False)
+        d_c2 = ctx.upload(c2_init)
+        ctx.vpsi_dev(d_c, d_c2, f, d_v, n)
+        ctx.set_column_reuse(True)
+        return ctx.download(d_c2, n)
+
+    ctx.set_column_reuse(True)
+    ctx.timer_enable(True)
+    # 1. a library write to c0 between rhoofr and vpsi (c0 += 0.5 x)
+    d_c2 = ctx.upload(c2_init)
+    ctx.rhoofr_dev(d_c0, f, n, d_rho)
+    ctx.axpy_wfn(0.5, d_x, d_c0, n)
+    ctx.timer_reset()
+    ctx.vpsi_dev(d_c0, d_c2, f, d_v, n)
+    assert _launches(ctx, "zinv") > 0
+    got = ctx.download(d_c2, n)
+    assert np.array_equal(got, vpsi_fresh(d_c0))
+    # 2. vpsi on another wavefunction buffer
+    d_c2 = ctx.upload(c2_init)
+    ctx.rhoofr_dev(d_c0, f, n, d_rho)
+    ctx.timer_reset()
+    ctx.vpsi_dev(d_x, d_c2, f, d_v, n)
+    assert _launches(ctx, "zinv") > 0
+    assert np.array_equal(ctx.download(d_c2, n), vpsi_fresh(d_x))
+    # 3. another precision between the two calls
+    ctx.set_precision("fp32")
+    d_c2 = ctx.upload(c2_init)
+    ctx.rhoofr_dev(d_c0, f, n, d_rho)
+    ctx.set_precision("fp64")
+    ctx.vpsi_dev(d_c0, d_c2, f, d_v, n)
+    assert np.array_equal(ctx.download(d_c2, n), vpsi_fresh(d_c0))
+    # 4. fewer states than rhoofr kept
+    d_c2 = ctx.upload(c2_init)
+    ctx.rhoofr_dev(d_c0, f, n, d_rho)
+    ctx.timer_reset()
+    ctx.vpsi_dev(d_c0, d_c2, f[:n - 2], d_v, n - 2)
+    assert _launches(ctx, "zinv") > 0
+    ctx.set_column_reuse(False)
+    d_r = ctx.upload(c2_init)
+    ctx.vpsi_dev(d_c0, d_r, f[:n - 2], d_v, n - 2)
+    assert np.array_equal(ctx.download(d_c2, n), ctx.download(d_r, n))
+
+
+def test_memory_limit_disables_keep(gpu):
+    ctx, c, f, v = _setup("mesh128", 6)
+    n = c.shape[1]
+    d_c0 = ctx.upload(c)
+    d_v = ops.DeviceArray(8 * ctx.nnr1).from_host(v)
+    c2_init = np.zeros(c.shape, dtype=np.complex128, order="F")
+    ref = _run(ctx, d_c0, f, n, d_v, c2_init)
+    ctx.set_column_reuse(True)
+    cur0 = ops.mem_stats()[0]
+    try:
+        ops.set_mem_limit(cur0 + (4 << 30))
+        ctx.timer_enable(True)
+        d_rho = ops.DeviceArray(8 * ctx.nnr1)
+        d_c2 = ctx.upload(c2_init)
+        ctx.rhoofr_dev(d_c0, f, n, d_rho)
+        ctx.timer_reset()
+        ctx.vpsi_dev(d_c0, d_c2, f, d_v, n)
+        assert _launches(ctx, "zinv") > 0   # no kept columns under a limit
+        assert np.array_equal(ctx.download(d_c2, n), ref[2])
+    finally:
+        ops.set_mem_limit(0)
+    # without the limit the columns are kept: the library holds them
+    d_rho = ops.DeviceArray(8 * ctx.nnr1)
+    before = ops.mem_stats()[0]
+    ctx.rhoofr_dev(d_c0, f, n, d_rho)
+    held = ops.mem_stats()[0] - before
+    assert held > 0, held
+    ctx.set_column_reuse(False)
+    assert ops.mem_stats()[0] < before + held   # off releases them
