@@ -76,7 +76,8 @@ def kernel_bytes(ctx_geom, kid: str, np_pairs: float, cw: float = 16.0) -> float
     if kid == "xvpsi":
         return np_pairs * cw * 2 * col + r * nnn
     if kid == "zfwd":
-        return np_pairs * (cw * stick + c * 6 * ngw)
+        # c0 read, c2 written (mc_forces_c2_dev overwrites c2: no read of it)
+        return np_pairs * (cw * stick + c * 4 * ngw)
     raise KeyError(kid)
 
 
@@ -84,7 +85,10 @@ def step_model_bytes(ctx_geom, nstate: int, cw: float = 16.0) -> float:
     """Per-step algorithmic bytes of rhoofr + vpsi (the stick-sparse transform model with the
     transform workspaces at cw bytes per complex): per state pair
     vpsi = cw(4 S N + 4 N^2 Zw) + 16*6 NGW + 8 N^3, rhoofr = cw(2 S N + 2 N^2 Zw) + 16*2 NGW,
-    plus rho written once (8 N^3)."""
+    plus rho written once (8 N^3).  This per-pair model reads V once per pair and runs vpsi's
+    inverse passes; the kernels read V once per launch and, with column reuse, vpsi starts
+    from rhoofr's columns, so the transform roofline uses the bytes of the launches made
+    (kernel_bytes) and reports this model beside it."""
     ngw, S, Zw, n1, n2, n3 = ctx_geom
     r = 8.0
     npairs = (nstate + 1) // 2
@@ -157,8 +161,10 @@ def cpu_baseline(sysm, c0, v, sample_pairs: int = 64):
 
 def gpu_same_content_ms(ctx, ks, buf, f, nstate, reps: int = 5) -> float:
     """The GPU's FP64 time for the content of cpu_baseline: rnlsm1, rhoofr,
-    vpsi + fnonloc, Loewdin (ms per step, median of reps)."""
+    vpsi + fnonloc, Loewdin (ms per step, median of reps), with column reuse
+    off so that vpsi transforms from C as the CPU path does."""
     ctx.set_precision("fp64")
+    ctx.set_column_reuse(False)
     ts = []
     for _ in range(reps + 1):
         ctx.synchronize()
@@ -169,6 +175,7 @@ def gpu_same_content_ms(ctx, ks, buf, f, nstate, reps: int = 5) -> float:
         ctx.lowdin_dev(buf["cp"], nstate)
         ctx.synchronize()
         ts.append(time.perf_counter() - t0)
+    ctx.set_column_reuse(True)
     return float(np.median(ts[1:]) * 1e3)
 
 
@@ -372,10 +379,45 @@ def main():
             iters.append(ctx.lowdin_dev(buf["c0"], nstate))
 
         step = step_cp if args.step == "cp" else step_wfopt
-        for _ in range(args.warmup):
+        all_kids = FFT_KIDS + ("misc", "gemm", "gemm_tn", "tn_planes", "vofrho")
+
+        def read_timers(nsteps):
+            out = {}
+            for kid in all_kids:
+                tot, cnt = ctx.timer_get(kid)
+                if cnt:
+                    out[kid] = {"total_ms_per_step": tot / nsteps, "launches_per_step": cnt / nsteps,
+                                "avg_ms": tot / cnt}
+            return out
+
+        # Per-kernel breakdown: every kernel timed (two event records per
+        # launch) on the last warmup step; the timed steps then carry events
+        # around the dominant kernels only (the A^T B contraction kernel of
+        # the path and the largest transform kernel), whose launch averages
+        # the rooflines use.
+        kernels = None
+        for i in range(args.warmup):
+            last = i == args.warmup - 1
+            if last:
+                ctx.synchronize()
+                ctx.timer_reset()
+                ctx.timer_select(None)
+                ctx.timer_enable(True)
             step()
+            if last:
+                ctx.synchronize()
+                ctx.timer_enable(False)
+                kernels = read_timers(1)
         ctx.synchronize()
         log(f"[bench] {path}: {args.warmup} warmup steps done")
+        if kernels:
+            tn_kid = "tn_planes" if modes[path] in ("mixed", "mixed_bf16x3") else "gemm_tn"
+            live = {k for k in (tn_kid, max((k for k in FFT_KIDS if k in kernels), default=None,
+                                            key=lambda k: kernels[k]["total_ms_per_step"])) if k in kernels}
+            ctx.timer_select(sorted(live))
+        else:
+            live = set(all_kids)
+            ctx.timer_select(None)
         ctx.timer_reset()
         ctx.timer_enable(True)
         ctx.synchronize()
@@ -385,12 +427,14 @@ def main():
         ctx.synchronize()
         t1 = time.perf_counter()
         ctx.timer_enable(False)
+        ctx.timer_select(None)
         ms = (t1 - t0) * 1e3 / args.steps
-        kernels = {}
-        for kid in FFT_KIDS + ("misc", "gemm", "gemm_tn", "tn_planes", "vofrho"):
-            tot, cnt = ctx.timer_get(kid)
-            if cnt:
-                kernels[kid] = {"total_ms_per_step": tot / args.steps, "launches": cnt, "avg_ms": tot / cnt}
+        timed = read_timers(args.steps)
+        if kernels is None:
+            kernels = timed
+        for k in live:   # the live launch averages of the timed steps
+            if k in timed:
+                kernels[k] = dict(timed[k], source="timed steps")
         # charge sums of the path's rhoofr on the end state: csumr before the
         # FP32 rescale against the FP64 G-space csumg (CPMD's check is
         # |csumr - csumg| < 1e-6, rhoofr:625-635)
@@ -435,14 +479,22 @@ def main():
     lr = -(-npairs // min(args.batch_rho, npairs))
     lv = -(-npairs // min(args.batch_vpsi, npairs))
 
-    def fft_kernel_roofline(k):
+    def pairs_per_launch(k):
         if k == "xrho":
-            ppl = npairs / lr
-        elif k in ("xvpsi", "yfwd", "zfwd"):
-            ppl = npairs / lv
-        else:   # zinv / yinv run in both rhoofr and vpsi batches
-            ppl = 2 * npairs / (lr + lv)
-        b = kernel_bytes(geom, k, 0, cw) + (kernel_bytes(geom, k, 1, cw) - kernel_bytes(geom, k, 0, cw)) * ppl
+            return npairs / lr
+        if k in ("xvpsi", "yfwd", "zfwd"):
+            return npairs / lv
+        # zinv / yinv: rhoofr's batches, and vpsi's unless it starts from
+        # rhoofr's kept columns (column reuse: one launch per rhoofr batch)
+        per_step = kernels[k]["launches_per_step"]
+        return npairs / lr if round(per_step) == lr else 2 * npairs / (lr + lv)
+
+    def launch_bytes(k):
+        ppl = pairs_per_launch(k)
+        return kernel_bytes(geom, k, 0, cw) + (kernel_bytes(geom, k, 1, cw) - kernel_bytes(geom, k, 0, cw)) * ppl
+
+    def fft_kernel_roofline(k):
+        b = launch_bytes(k)
         a = b / (kernels[k]["avg_ms"] * 1e-3) / 1e9
         return {"bound": "hbm", "kernel": k, "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(a / HBM_PEAK_GBS, 4), "bytes_per_launch": b, "avg_launch_ms": kernels[k]["avg_ms"]}
@@ -460,8 +512,8 @@ def main():
         peak = 2500.0 if split else 78.6
         out = {"bound": "mfma", "kernel": "tn_planes" if split else "gemm_tn", "achieved": round(a, 2),
                "peak": peak, "unit": "TFLOP/s", "frac": round(a / peak, 4),
-               "flops_per_launch": flops * args.steps / k["launches"], "avg_launch_ms": k["avg_ms"],
-               "launches_per_step": k["launches"] / args.steps,
+               "flops_per_launch": flops / k["launches_per_step"], "avg_launch_ms": k["avg_ms"],
+               "launches_per_step": k["launches_per_step"],
                "dtype": "bf16 (3-part split)" if head == "mixed" else ("bf16 (2-part split)" if split else "f64"),
                "useful_fp64_equiv_tflops": round(tn_gemm_flops / (k["total_ms_per_step"] * 1e-3) / 1e12, 2)}
         # the rate the kernel's own MFMA instruction reaches on this GPU
@@ -482,6 +534,8 @@ def main():
     dom_fft = max((k for k in FFT_KIDS if k in kernels), key=lambda k: kernels[k]["total_ms_per_step"])
     step_bytes = step_model_bytes(geom, nstate, cw)
     fft_ms = sum(kernels[k]["total_ms_per_step"] for k in FFT_KIDS if k in kernels)
+    # algorithmic bytes of the transform launches the step made
+    run_bytes = sum(launch_bytes(k) * kernels[k]["launches_per_step"] for k in FFT_KIDS if k in kernels)
 
     def mfma_roofline(path, r):
         """FP64 path: FP64 MFMA flops / 78.6 TF.  Mixed paths: bf16 MFMA issue
@@ -610,9 +664,11 @@ def main():
         "roofline": roof,
         "fft_kernel_roofline": fft_kernel_roofline(dom_fft),
         "fft_stages": {k: fft_kernel_roofline(k) for k in FFT_KIDS if k in kernels},
-        "fft_roofline": {"model_bytes": step_bytes, "ms_per_step": round(fft_ms, 4),
-                         "achieved": round(step_bytes / (fft_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(step_bytes / (fft_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "fft_roofline": {"bytes_per_step": run_bytes, "ms_per_step": round(fft_ms, 4),
+                         "achieved": round(run_bytes / (fft_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(run_bytes / (fft_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "pair_model_bytes": step_bytes,
+                         "pair_model_rate": round(step_bytes / (fft_ms * 1e-3) / 1e9, 1)},
         "mfma_roofline": mfma_roofline(head, R),
         "paths": {p: {"ms_per_step": round(res[p]["ms"], 4), "E_KS_start_Ha": res[p]["start"]["etot"],
                       "E_KS_end_fp64_Ha": res[p]["end"]["etot"], "orthonormality_err": res[p]["orth"],

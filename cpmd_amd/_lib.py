@@ -132,6 +132,7 @@ SIGNATURES = {
     "mc_restart_read_dev": (C.c_int, [_vp, C.c_char_p, _rhp, _ip, _dp, _dp, _dp, _dp, _vp, _vp]),
     "mc_timer_enable": (C.c_int, [_vp, C.c_int]),
     "mc_timer_reset": (C.c_int, [_vp]),
+    "mc_timer_select": (C.c_int, [_vp, C.c_ulonglong]),
     "mc_timer_get": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_longlong)]),
 }
 

@@ -515,19 +515,30 @@ void wfn_download(mc_ctx* ctx, const double2* dev, int nstate, double* host, int
 // ---------------------------------------------------------------------------
 // kernel timer
 // ---------------------------------------------------------------------------
+static bool timed(const mc_ctx* ctx, int kid) {
+  return ctx->timer.enabled && ((ctx->timer.mask >> kid) & 1ull);
+}
+static hipEvent_t timer_event(KernelTimer& t) {
+  hipEvent_t e;
+  if (!t.pool.empty()) {
+    e = t.pool.back();
+    t.pool.pop_back();
+  } else {
+    MC_HIP(hipEventCreate(&e));
+  }
+  return e;
+}
 void timer_begin(mc_ctx* ctx, int kid) {
-  if (!ctx->timer.enabled) return;
-  hipEvent_t a, b;
-  MC_HIP(hipEventCreate(&a));
-  MC_HIP(hipEventCreate(&b));
+  if (!timed(ctx, kid)) return;
+  auto& t = ctx->timer;
+  const hipEvent_t a = timer_event(t), b = timer_event(t);
+  t.ev.push_back(a);
+  t.ev.push_back(b);
+  t.kid.push_back(kid);
   MC_HIP(hipEventRecord(a, ctx->stream));
-  ctx->timer.ev.push_back(a);
-  ctx->timer.ev.push_back(b);
-  ctx->timer.kid.push_back(kid);
 }
 void timer_end(mc_ctx* ctx, int kid) {
-  (void)kid;
-  if (!ctx->timer.enabled) return;
+  if (!timed(ctx, kid)) return;
   MC_HIP(hipEventRecord(ctx->timer.ev.back(), ctx->stream));
 }
 void timer_flush(mc_ctx* ctx) {
@@ -539,8 +550,8 @@ void timer_flush(mc_ctx* ctx) {
     MC_HIP(hipEventElapsedTime(&ms, t.ev[2 * k], t.ev[2 * k + 1]));
     t.total_ms[t.kid[k]] += ms;
     t.count[t.kid[k]] += 1;
-    (void)hipEventDestroy(t.ev[2 * k]);
-    (void)hipEventDestroy(t.ev[2 * k + 1]);
+    t.pool.push_back(t.ev[2 * k]);
+    t.pool.push_back(t.ev[2 * k + 1]);
   }
   t.ev.clear();
   t.kid.clear();
@@ -881,6 +892,7 @@ int mc_ctx_destroy(mc_ctx* ctx) {
   if (!ctx) return MC_OK;
   (void)hipStreamSynchronize(ctx->stream);
   for (auto e : ctx->timer.ev) (void)hipEventDestroy(e);
+  for (auto e : ctx->timer.pool) (void)hipEventDestroy(e);
   if (ctx->h_stage_done) (void)hipEventDestroy(ctx->h_stage_done);
   if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
   if (ctx->h_down) (void)hipHostFree(ctx->h_down);
@@ -1436,6 +1448,13 @@ int mc_timer_enable(mc_ctx* ctx, int on) {
   MC_CHECK(ctx, MC_ERR_ARG, "null context");
   mc::timer_flush(ctx);
   ctx->timer.enabled = on != 0;
+  MC_API_END
+}
+int mc_timer_select(mc_ctx* ctx, unsigned long long kid_mask) {
+  MC_API_BEGIN_RO
+  MC_CHECK(ctx, MC_ERR_ARG, "null context");
+  mc::timer_flush(ctx);
+  ctx->timer.mask = kid_mask;
   MC_API_END
 }
 int mc_timer_reset(mc_ctx* ctx) {

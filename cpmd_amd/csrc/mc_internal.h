@@ -124,8 +124,10 @@ struct PlaneCache {
 // Per-kernel timing (HIP events around launches), enabled on request.
 struct KernelTimer {
   bool enabled = false;
+  unsigned long long mask = ~0ull;   // kernel ids timed while enabled (bit kid)
   std::vector<hipEvent_t> ev;  // pairs (start, stop)
   std::vector<int> kid;        // kernel id of each pair
+  std::vector<hipEvent_t> pool;   // events of flushed pairs, reused
   double total_ms[MC_KID_COUNT] = {0};
   long long count[MC_KID_COUNT] = {0};
 };

@@ -533,6 +533,11 @@ class Context:
     def timer_reset(self):
         check(lib().mc_timer_reset(self.h))
 
+    def timer_select(self, names=None):
+        """Time only the kernels named (None: all) while the timer is enabled."""
+        mask = (1 << 64) - 1 if names is None else sum(1 << _lib.KID[n] for n in names)
+        check(lib().mc_timer_select(self.h, mask))
+
     def timer_get(self, name: str):
         ms, cnt = C.c_double(), C.c_longlong()
         check(lib().mc_timer_get(self.h, _lib.KID[name], C.byref(ms), C.byref(cnt)))

@@ -543,6 +543,10 @@ int mc_restart_read_dev(mc_ctx* ctx, const char* path, mc_restart_header* h, int
 
 /* Optional per-kernel timing with HIP events on the context stream. */
 int mc_timer_enable(mc_ctx* ctx, int on);
+/* The kernel ids timed while enabled: bit kid of kid_mask (default all).
+ * Each timed launch adds two event records to the stream; timing only the
+ * kernels of interest keeps the rest of a measured step unperturbed. */
+int mc_timer_select(mc_ctx* ctx, unsigned long long kid_mask);
 int mc_timer_reset(mc_ctx* ctx);
 /* Synchronizes and returns total milliseconds and launch count of kernel kid. */
 int mc_timer_get(mc_ctx* ctx, int kid, double* total_ms, long long* count);
