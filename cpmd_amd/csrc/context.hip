@@ -35,8 +35,14 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 void clear_error() { g_last_error.clear(); }
 
 static std::atomic<unsigned long long> g_wfn_epoch{0};
+static thread_local bool g_ro_call = false;
 unsigned long long wfn_epoch() { return g_wfn_epoch.load(std::memory_order_acquire); }
-void wfn_epoch_bump() { g_wfn_epoch.fetch_add(1, std::memory_order_acq_rel); }
+void wfn_epoch_bump() {
+  g_ro_call = false;
+  g_wfn_epoch.fetch_add(1, std::memory_order_acq_rel);
+}
+bool in_ro_call() { return g_ro_call; }
+void ro_call_begin() { g_ro_call = true; }
 
 // ---------------------------------------------------------------------------
 // device memory accounting: every allocation of the library (context buffers
@@ -939,6 +945,7 @@ int mc_ctx_set_column_reuse(mc_ctx* ctx, int on) {
   MC_HIP(hipStreamSynchronize(ctx->stream));
   ctx->col_reuse = on != 0;
   ctx->keep_valid = false;
+  ctx->planes.keep_src = nullptr;
   if (!ctx->col_reuse) ctx->w_keep.release();
   MC_API_END
 }

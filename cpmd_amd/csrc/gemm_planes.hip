@@ -133,8 +133,10 @@ __global__ __launch_bounds__(256) void k_split_planes(const double* __restrict__
 // caller's final rotation sets the velocity (cp_final - c0) / dt.
 // KICK (velupa before rortv): cp = cm + dtm c2 instead, written over cm (cp ==
 // cm: every element is read before the same thread writes it), with the
-// planes of c0 and of the kicked cm.
-template <int NP, bool KICK>
+// planes of c0 and of the kicked cm.  C0P false: the planes of c0 are already
+// held (kept from an earlier call) and are not written; with KICK, c0 is then
+// not read at all.
+template <int NP, bool KICK, bool C0P>
 __global__ __launch_bounds__(256) void k_posupa_planes(const double* __restrict__ c0, const double* __restrict__ c2,
                                                        const double* cm, double* cp, int ldx, int K,
                                                        int n, int ncolp, int nseg, int nkb, double dtm, double dt,
@@ -156,7 +158,7 @@ __global__ __launch_bounds__(256) void k_posupa_planes(const double* __restrict_
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const bool ok = any && r0 + 128 * u < ldx;
-    x[u] = ok ? *reinterpret_cast<const double2*>(c0 + o + 128 * u) : make_double2(0.0, 0.0);
+    x[u] = ok && (C0P || !KICK) ? *reinterpret_cast<const double2*>(c0 + o + 128 * u) : make_double2(0.0, 0.0);
     w[u] = ok ? *reinterpret_cast<const double2*>(cm + o + 128 * u) : make_double2(0.0, 0.0);
     f[u] = ok && c2 ? *reinterpret_cast<const double2*>(c2 + o + 128 * u) : make_double2(0.0, 0.0);
   }
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(256) void k_posupa_planes(const double* __restrict_
     w[u].y += dtm * f[u].y;
     f[u] = KICK ? w[u] : make_double2(x[u].x + dt * w[u].x, x[u].y + dt * w[u].y);   // f now holds cp
     const bool in = r0 + 128 * u < K;                                  // K even: whole complex rows
-    s0[lane + 64 * u] = in ? make_float2((float)x[u].x, (float)x[u].y) : make_float2(0.f, 0.f);
+    if (C0P) s0[lane + 64 * u] = in ? make_float2((float)x[u].x, (float)x[u].y) : make_float2(0.f, 0.f);
     sp[lane + 64 * u] = in ? make_float2((float)f[u].x, (float)f[u].y) : make_float2(0.f, 0.f);
   }
   __syncthreads();
@@ -176,10 +178,12 @@ __global__ __launch_bounds__(256) void k_posupa_planes(const double* __restrict_
     const int c = col % kPT, q = lane & 3;
     float v[8];
     bf8 p[NP];
-    const float4 a0 = lds[wv][0][2 * lane], a1 = lds[wv][0][2 * lane + 1];
-    v[0] = a0.x; v[1] = a0.y; v[2] = a0.z; v[3] = a0.w; v[4] = a1.x; v[5] = a1.y; v[6] = a1.z; v[7] = a1.w;
-    split_parts<NP>(v, p);
-    store_parts<NP>(pl0, blk, c, q, p);
+    if constexpr (C0P) {
+      const float4 a0 = lds[wv][0][2 * lane], a1 = lds[wv][0][2 * lane + 1];
+      v[0] = a0.x; v[1] = a0.y; v[2] = a0.z; v[3] = a0.w; v[4] = a1.x; v[5] = a1.y; v[6] = a1.z; v[7] = a1.w;
+      split_parts<NP>(v, p);
+      store_parts<NP>(pl0, blk, c, q, p);
+    }
     const float4 b0 = lds[wv][1][2 * lane], b1 = lds[wv][1][2 * lane + 1];
     v[0] = b0.x; v[1] = b0.y; v[2] = b0.z; v[3] = b0.w; v[4] = b1.x; v[5] = b1.y; v[6] = b1.z; v[7] = b1.w;
     split_parts<NP>(v, p);
@@ -636,7 +640,13 @@ static void update_planes(mc_ctx* ctx, bool kick, const double* c0, const double
            MC_ERR_INTERNAL, "posupa_planes: operands not 16-byte aligned");
   const int np = ctx->x3_parts == 2 ? 2 : 3;
   const int nkb = (K + kPK - 1) / kPK, ncolp = (n + kPT - 1) / kPT * kPT, nseg = (ldx + 511) / 512;
-  C.buf[0].ensure(planes_bytes(K, n, np));
+  // the planes of c0 kept from an earlier call (this call's only epoch count
+  // since; c0 is neither cp nor cm) stay in slot 0 and are not rewritten
+  const bool have0 = kept_planes(ctx, c0, K, n, np, 1) && (const void*)c0 != cp && (const void*)c0 != cm;
+  if (!have0) {
+    C.keep_src = nullptr;
+    C.buf[0].ensure(planes_bytes(K, n, np));
+  }
   C.buf[1].ensure(planes_bytes(K, n, np));
   const dim3 grid((unsigned)(((size_t)ncolp * nseg + 3) / 4));
   timer_begin(ctx, MC_KID_MISC);
@@ -644,8 +654,13 @@ static void update_planes(mc_ctx* ctx, bool kick, const double* c0, const double
     hipLaunchKernelGGL(kern, grid, dim3(256), 0, ctx->stream, c0, c2, cm, cp, ldx, K, n, ncolp, nseg, nkb,
                        c2 ? dtm : 0.0, dt, C.buf[0].as<__bf16>(), C.buf[1].as<__bf16>());
   };
-  if (np == 3) kick ? go(k_posupa_planes<3, true>) : go(k_posupa_planes<3, false>);
-  else kick ? go(k_posupa_planes<2, true>) : go(k_posupa_planes<2, false>);
+  if (np == 3) {
+    if (kick) have0 ? go(k_posupa_planes<3, true, false>) : go(k_posupa_planes<3, true, true>);
+    else have0 ? go(k_posupa_planes<3, false, false>) : go(k_posupa_planes<3, false, true>);
+  } else {
+    if (kick) have0 ? go(k_posupa_planes<2, true, false>) : go(k_posupa_planes<2, true, true>);
+    else have0 ? go(k_posupa_planes<2, false, false>) : go(k_posupa_planes<2, false, true>);
+  }
   MC_HIP(hipGetLastError());
   timer_end(ctx, MC_KID_MISC);
   const void* src[2] = {c0, cp};
@@ -665,6 +680,26 @@ void posupa_planes(mc_ctx* ctx, const double* c0, const double* c2, const double
 
 void kick_planes(mc_ctx* ctx, const double* c0, const double* c2, double* cm, int ldx, int K, int n, double dtm) {
   update_planes(ctx, true, c0, c2, cm, cm, ldx, K, n, dtm, 0.0);
+}
+
+bool kept_planes(const mc_ctx* ctx, const void* X, int K, int n, int np, int lag) {
+  const auto& C = ctx->planes;
+  return ctx->col_reuse && X && C.keep_src == X && C.keep_np == np && C.keep_K == K && C.keep_n == n &&
+         C.buf[0].p && C.buf[0].bytes >= planes_bytes(K, n, np) && C.keep_epoch + (unsigned long long)lag == wfn_epoch();
+}
+
+void keep_c0_planes(mc_ctx* ctx, const void* c0, int K, int n) {
+  auto& C = ctx->planes;
+  const int np = ctx->x3_parts == 2 ? 2 : 3;
+  if (!ctx->col_reuse || C.src[0] != c0 || C.np[0] != np || C.K[0] != K || C.n[0] != n) {
+    C.keep_src = nullptr;
+    return;
+  }
+  C.keep_src = c0;
+  C.keep_np = np;
+  C.keep_K = K;
+  C.keep_n = n;
+  C.keep_epoch = wfn_epoch();
 }
 
 void proj_planes(mc_ctx* ctx, const ProjA& pa, int K, int m, int np, DevBuf& out) {

@@ -240,15 +240,27 @@ PlaneScope::~PlaneScope() {
   c->planes.src[0] = c->planes.src[1] = nullptr;
 }
 
-// planes of the wavefunction operand X (K x n): reused inside a PlaneScope,
-// else split into a slot other than `keep` (the other operand's planes)
+// planes of the wavefunction operand X (K x n): reused inside a PlaneScope or
+// from the kept slot, else split into a slot other than `keep` (the other
+// operand's planes).  A read-only call with column reuse on splits into slot
+// 0 and keeps it for later calls on the same wavefunctions.
 static const DevBuf& planes_for(mc_ctx* ctx, const double* X, int ldx, int K, int n, int np, const DevBuf* keep) {
   auto& C = ctx->planes;
   for (int i = 0; i < 2; ++i)
     if (C.scoped && C.src[i] == X && C.np[i] == np && C.K[i] == K && C.n[i] >= n) return C.buf[i];
-  int i = (keep == &C.buf[0]) ? 1 : (keep == &C.buf[1]) ? 0 : C.next;
+  if (keep != &C.buf[0] && kept_planes(ctx, X, K, n, np, 0)) return C.buf[0];
+  const bool persist = ctx->col_reuse && in_ro_call() && keep != &C.buf[0];
+  int i = persist ? 0 : (keep == &C.buf[0]) ? 1 : (keep == &C.buf[1]) ? 0 : C.next;
   C.next = 1 - i;
+  if (i == 0) C.keep_src = nullptr;   // slot 0 is rewritten
   split_planes(ctx, X, ldx, K, n, np, C.buf[i]);
+  if (persist) {
+    C.keep_src = X;
+    C.keep_np = np;
+    C.keep_K = K;
+    C.keep_n = n;
+    C.keep_epoch = wfn_epoch();
+  }
   C.src[i] = C.scoped ? X : nullptr;
   C.np[i] = np;
   C.K[i] = K;

@@ -40,6 +40,9 @@ struct Error {
 // the count is unchanged.
 unsigned long long wfn_epoch();
 void wfn_epoch_bump();
+// true inside an entry point declared with MC_API_BEGIN_RO (this thread)
+bool in_ro_call();
+void ro_call_begin();
 
 #define MC_API_BEGIN      \
   try {                   \
@@ -47,7 +50,8 @@ void wfn_epoch_bump();
     mc::wfn_epoch_bump();
 #define MC_API_BEGIN_RO \
   try {                 \
-    mc::clear_error();
+    mc::clear_error();  \
+    mc::ro_call_begin();
 #define MC_API_END                                   \
   }                                                  \
   catch (const mc::Error& e) {                       \
@@ -119,6 +123,12 @@ struct PlaneCache {
   int np[2] = {0, 0}, K[2] = {0, 0}, n[2] = {0, 0};
   DevBuf buf[2];
   int next = 0;
+  // Kept planes (column reuse on): buf[0] holds the planes of the device
+  // wavefunctions keep_src as of wavefunction epoch keep_epoch, across calls
+  // (rnlsm's split of c0, reused by the next rortv and posupa); nullptr: none
+  const void* keep_src = nullptr;
+  int keep_np = 0, keep_K = 0, keep_n = 0;
+  unsigned long long keep_epoch = 0;
 };
 
 // Per-kernel timing (HIP events around launches), enabled on request.
@@ -377,6 +387,12 @@ void posupa_planes(mc_ctx* ctx, const double* c0, const double* c2, const double
 // cm += dtm c2 in place, writing the planes of c0 and of the new cm into the
 // plane cache of the current PlaneScope
 void kick_planes(mc_ctx* ctx, const double* c0, const double* c2, double* cm, int ldx, int K, int n, double dtm);
+// Kept planes of X in the cache's slot 0: lag 0 for a read-only call, 1 for a
+// call that counted its own epoch and has written nothing X covers yet
+bool kept_planes(const mc_ctx* ctx, const void* X, int K, int n, int np, int lag);
+// marks slot 0 (the planes of c0 the last posupa_planes / kick_planes of
+// this call left there) as kept, when column reuse is on
+void keep_c0_planes(mc_ctx* ctx, const void* c0, int K, int n);
 void cp_posupa_dev(mc_ctx* ctx, const double2* c0, const double2* c2, double2* cm, double2* cp, int nstate, double dtm,
                    double dt);
 void cp_velocity_dev(mc_ctx* ctx, const double2* c0, const double2* cp, double2* cm, int nstate, double dt);

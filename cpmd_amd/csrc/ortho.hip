@@ -473,12 +473,16 @@ void posupa_dev(mc_ctx* ctx, const double2* c0, const double2* c2, double2* cm, 
   if (n <= 0) return;
   PlaneScope keep(ctx);
   const int K = wfn_rows(ctx);
-  if (ctx->gemm_x3 && K >= kX3Min)
+  const bool planes = ctx->gemm_x3 && K >= kX3Min;
+  if (planes)
     posupa_planes(ctx, (const double*)c0, (const double*)c2, (const double*)cm, (double*)cp, wfn_ld(ctx), K, n, dtm,
                   dt);
   else
     cp_posupa_dev(ctx, c0, c2, cm, cp, n, dtm, dt);
   rortog_core(ctx, c0, cp, gam_host, 0, n, epsog, maxit, iters, cm, dt);
+  // c0 is unchanged (cp and cm were written): its planes stay kept for the
+  // next call on it
+  if (planes && c0 != cp && c0 != cm) keep_c0_planes(ctx, c0, K, n);
 }
 
 // gamy = -sym(a): gamy(i,j) = gamy(j,i) = -(a(i,j) + a(j,i))/2 (rortv's
@@ -520,6 +524,9 @@ void rortv_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gamy_host, i
   MC_HIP(hipGetLastError());
   rotate_dev(ctx, 1.0, c0, 1.0, cm, G, n);
   if (gamy_host) dl_add(ctx, gamy_host, G, sizeof(double) * (size_t)n * n);
+  // c0 is unchanged (cm was written): the planes of c0 the kick pass left in
+  // the cache stay kept for the next call on it
+  if (c2 && ctx->gemm_x3 && K >= kX3Min && c0 != cm) keep_c0_planes(ctx, c0, K, n);
 }
 
 static void rortog_core(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n,

@@ -173,3 +173,36 @@ def test_memory_limit_disables_keep(gpu):
     assert held > 0, held
     ctx.set_column_reuse(False)
     assert ops.mem_stats()[0] < before + held   # off releases them
+
+
+@pytest.mark.parametrize("mode", ["mixed", "fp64"])
+def test_cp_steps_bit_identical(gpu, mode):
+    """Three velocity Verlet CP steps of the 128-state system on the 128^3 mesh
+    with column reuse on (rhoofr's columns for vpsi; in mixed precision also
+    rnlsm's bf16 planes of c0 for the next rortv and posupa) and off: every
+    energy, force and coefficient bit for bit."""
+    from cpmd_amd.ks import KSDevice
+    from test_oracle_energy import ks_system
+    sysm, _, c = ks_system(24.0, 64, 70.0, 128, seed=3)
+    assert 2 * sysm.gset.ngw >= 16384   # the split-bf16 contraction path in mixed precision
+    out = {}
+    for reuse in (False, True):
+        ks = KSDevice(sysm, column_reuse=reuse)
+        ctx = ks.ctx
+        ctx.set_precision(mode)
+        n = c.shape[1]
+        bufs = {k: ctx.alloc_wfn(n) for k in ("c0", "cm", "cp", "c2")}
+        ctx.upload(c, bufs["c0"])
+        print(f"CP steps, {mode}, column reuse {reuse}", flush=True)
+        st = ks.cp_init(bufs["c0"], bufs["cm"], bufs["c2"], sysm.f, sysm.tau, emass=400.0, ion_mass=21874.7)
+        hist = []
+        for _ in range(3):
+            ks.cp_step(st, bufs, sysm.f, 4.0)
+            hist.append((st["etot"], st["ekinc"], st["iters"], np.array(st["fion"])))
+        out[reuse] = (hist, ctx.download(bufs["c0"], n), ctx.download(bufs["cm"], n))
+        ctx.close()
+    (h0, c00, cm0), (h1, c01, cm1) = out[False], out[True]
+    for a, b in zip(h0, h1):
+        assert a[0] == b[0] and a[1] == b[1] and a[2] == b[2]
+        assert np.array_equal(a[3], b[3])
+    assert np.array_equal(c00, c01) and np.array_equal(cm0, cm1)
