@@ -432,6 +432,7 @@ void gemm_nn(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, i
     gemm_nn_x3(ctx, A, lda, G, ldgm, M, k, n, alpha, beta, D, ldd, upper_g, epi);
     return;
   }
+  MC_CHECK(!epi || !epi->pl, MC_ERR_INTERNAL, "planes of D_new need the split-bf16 product");
   gemm_nn_f64(ctx, A, lda, G, ldgm, M, k, n, alpha, beta, D, ldd, upper_g, epi, MC_KID_GEMM);
 }
 

@@ -571,7 +571,13 @@ __global__ __launch_bounds__(256, 3) void k_nn_planes(const double* __restrict__
       for (int it = 0; it < 4; ++it) {
         const int c = 2 * (h + it) + hl, row = 2 * pr;
         const int j = j0 + wc + 16 * b + c, i = i0 + wr + row;
-        if (j >= n || i >= M) continue;
+        // FP32 copy of D_new for the planes, over this lane's own staging slot
+        // (read above by this lane only); zero outside D
+        float2* sy = reinterpret_cast<float2*>(st + c * 64 + (((row >> 2) ^ c) * 4) + (row & 3));
+        if (j >= n || i >= M) {
+          if (epi.pl) *sy = make_float2(0.f, 0.f);
+          continue;
+        }
         double* d = D + (size_t)j * ldd + i;
         const double x0 = (double)xs[it].x, x1 = (double)xs[it].y;
         if (epi.twnl) {
@@ -597,12 +603,42 @@ __global__ __launch_bounds__(256, 3) void k_nn_planes(const double* __restrict__
             v[0] = epi.vodt * (y0 - av[it].x);
             v[1] = epi.vodt * (y1 - av[it].y);
           }
+          if (epi.pl) *sy = make_float2((float)y0, (float)y1);
         }
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (epi.pl) {
+      // planes of D_new for the wave's 64 rows x 16 columns: lane (c, r) splits
+      // rows 8 r8 .. 8 r8 + 7 of column c, r8 = r and r + 4 (the chunks of
+      // split_planes: rows >= M and columns >= n zero, as staged)
+      const int c = lane & 15, j = j0 + wc + 16 * b + c;
+#pragma unroll
+      for (int rr = 0; rr < 2; ++rr) {
+        const int r8 = (lane >> 4) + 4 * rr;
+        const int kb = (i0 + wr + 8 * r8) >> 5;
+        if (kb >= epi.pl_nkb) continue;
+        const float4 lo = *reinterpret_cast<const float4*>(st + c * 64 + (((2 * r8) ^ c) * 4));
+        const float4 hi = *reinterpret_cast<const float4*>(st + c * 64 + (((2 * r8 + 1) ^ c) * 4));
+        float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        const size_t blk = (size_t)(j / kPT) * epi.pl_nkb + kb;
+        const int q = ((wr + 8 * r8) >> 3) & 3;
+        if (epi.pl_np == 3) {
+          bf8 p[3];
+          split_parts<3>(v, p);
+          store_parts<3>(epi.pl, blk, j % kPT, q, p);
+        } else {
+          bf8 p[2];
+          split_parts<2>(v, p);
+          store_parts<2>(epi.pl, blk, j % kPT, q, p);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");   // every lane's reads before the next b's staging
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
   }
 }
 
@@ -688,6 +724,16 @@ bool kept_planes(const mc_ctx* ctx, const void* X, int K, int n, int np, int lag
          C.buf[0].p && C.buf[0].bytes >= planes_bytes(K, n, np) && C.keep_epoch + (unsigned long long)lag == wfn_epoch();
 }
 
+void keep_new_planes(mc_ctx* ctx, const void* X, int K, int n) {
+  auto& C = ctx->planes;
+  C.src[0] = nullptr;   // slot 0 no longer holds the scope's first operand
+  C.keep_src = ctx->col_reuse ? X : nullptr;
+  C.keep_np = ctx->x3_parts == 2 ? 2 : 3;
+  C.keep_K = K;
+  C.keep_n = n;
+  C.keep_epoch = wfn_epoch();
+}
+
 void keep_c0_planes(mc_ctx* ctx, const void* c0, int K, int n) {
   auto& C = ctx->planes;
   const int np = ctx->x3_parts == 2 ? 2 : 3;
@@ -734,7 +780,13 @@ void proj_planes_stacked(mc_ctx* ctx, const ProjA& base, const double* gk, int v
 }
 
 void gemm_nn_planes(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n,
-                    double alpha, double beta, double* D, int ldd, bool upper_g, const FnlEpi& epi) {
+                    double alpha, double beta, double* D, int ldd, bool upper_g, const FnlEpi& epi_in) {
+  FnlEpi epi = epi_in;
+  if (epi.pl) {
+    MC_CHECK(!epi.twnl, MC_ERR_INTERNAL, "planes of D_new with the fnonloc epilogue");
+    epi.pl_nkb = (M + kPK - 1) / kPK;
+    epi.pl_np = ctx->x3_parts == 2 ? 2 : 3;
+  }
   split_planes(ctx, G, ldgm, k, n, 2, ctx->w_plg);
   const int gnkb = (k + kPK - 1) / kPK;
   const int tm = (M + kPT - 1) / kPT, tn = (n + kPT - 1) / kPT;

@@ -163,6 +163,11 @@ struct FnlEpi {
   // at the leading dimension of D (A and D of one layout); nullptr: off
   double* vcm = nullptr;
   double vodt = 0.0;
+  // plain GEMM, split-bf16 path only: also the bf16 planes of D_new (pl_np
+  // parts, the layout of split_planes for K = M rows); nullptr: off.  The
+  // host wrapper fills pl_nkb and pl_np.
+  __bf16* pl = nullptr;
+  int pl_nkb = 0, pl_np = 0;
 };
 
 // Nonlocal pseudopotential data of the current geometry.
@@ -393,6 +398,9 @@ bool kept_planes(const mc_ctx* ctx, const void* X, int K, int n, int np, int lag
 // marks slot 0 (the planes of c0 the last posupa_planes / kick_planes of
 // this call left there) as kept, when column reuse is on
 void keep_c0_planes(mc_ctx* ctx, const void* c0, int K, int n);
+// marks slot 0 as the planes of X, written there by a final rotation's
+// epilogue (FnlEpi::pl), when column reuse is on
+void keep_new_planes(mc_ctx* ctx, const void* X, int K, int n);
 void cp_posupa_dev(mc_ctx* ctx, const double2* c0, const double2* c2, double2* cm, double2* cp, int nstate, double dtm,
                    double dt);
 void cp_velocity_dev(mc_ctx* ctx, const double2* c0, const double2* cp, double2* cm, int nstate, double dt);

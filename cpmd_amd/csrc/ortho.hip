@@ -453,7 +453,8 @@ void lagrange_dev(mc_ctx* ctx, const double2* c0, double2* c2, const double* f_h
 
 // CP orthonormality constraint, rortho (src/rortog_utils.mod.F90:315-498).
 static void rortog_core(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n,
-                        double epsog, int maxit, int* iters, double2* vel = nullptr, double vdt = 0.0);
+                        double epsog, int maxit, int* iters, double2* vel = nullptr, double vdt = 0.0,
+                        __bf16* planes_out = nullptr);
 
 void rortog_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n, double epsog,
                 int maxit, int* iters) {
@@ -479,10 +480,17 @@ void posupa_dev(mc_ctx* ctx, const double2* c0, const double2* c2, double2* cm, 
                   dt);
   else
     cp_posupa_dev(ctx, c0, c2, cm, cp, n, dtm, dt);
-  rortog_core(ctx, c0, cp, gam_host, 0, n, epsog, maxit, iters, cm, dt);
-  // c0 is unchanged (cp and cm were written): its planes stay kept for the
-  // next call on it
-  if (planes && c0 != cp && c0 != cm) keep_c0_planes(ctx, c0, K, n);
+  // With column reuse the final rotation also writes the planes of the new
+  // coefficients into slot 0 (c0's planes are not needed after it), kept for
+  // rnlsm and the rest of the next step on them; otherwise c0's planes stay
+  // kept (c0 is unchanged: cp and cm were written).
+  const bool alias = c0 == cp || c0 == cm;
+  __bf16* pl = planes && ctx->col_reuse && !alias && n % 2 == 0 ? ctx->planes.buf[0].as<__bf16>() : nullptr;
+  rortog_core(ctx, c0, cp, gam_host, 0, n, epsog, maxit, iters, cm, dt, pl);
+  if (pl)
+    keep_new_planes(ctx, cp, K, n);
+  else if (planes && !alias)
+    keep_c0_planes(ctx, c0, K, n);
 }
 
 // gamy = -sym(a): gamy(i,j) = gamy(j,i) = -(a(i,j) + a(j,i))/2 (rortv's
@@ -530,7 +538,7 @@ void rortv_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gamy_host, i
 }
 
 static void rortog_core(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n,
-                        double epsog, int maxit, int* iters, double2* vel, double vdt) {
+                        double epsog, int maxit, int* iters, double2* vel, double vdt, __bf16* planes_out) {
   // the n x n products below go through gemm_tn / gemm_nn, which send
   // products with K or M >= kX3Min to the split-bf16 kernels in mixed precision
   MC_CHECK(n < kX3Min, MC_ERR_UNSUPPORTED, "rortog: nstate beyond 16383");
@@ -586,6 +594,7 @@ static void rortog_core(mc_ctx* ctx, const double2* c0, double2* cm, double* gam
     FnlEpi e{};
     e.vcm = (double*)vel;
     e.vodt = 1.0 / vdt;
+    e.pl = planes_out;
     gemm_nn(ctx, (const double*)c0, wfn_ld(ctx), gam, n, wfn_ld(ctx), n, n, 1.0, 1.0, (double*)cm, wfn_ld(ctx), false,
             &e);
   } else {
