@@ -175,15 +175,17 @@ def test_memory_limit_disables_keep(gpu):
     assert ops.mem_stats()[0] < before + held   # off releases them
 
 
-@pytest.mark.parametrize("mode", ["mixed", "fp64"])
-def test_cp_steps_bit_identical(gpu, mode):
+@pytest.mark.parametrize("mode,nstate", [("mixed", 128), ("mixed_bf16x3", 128), ("fp64", 128), ("mixed", 127)])
+def test_cp_steps_bit_identical(gpu, mode, nstate):
     """Three velocity Verlet CP steps of the 128-state system on the 128^3 mesh
     with column reuse on (rhoofr's columns for vpsi; in mixed precision also
-    rnlsm's bf16 planes of c0 for the next rortv and posupa) and off: every
-    energy, force and coefficient bit for bit."""
+    the kept bf16 planes of the coefficients: written by posupa's final
+    rotation for even state counts, split by rnlsm otherwise, and used by the
+    next rortv and posupa) and off: every energy, force and coefficient bit
+    for bit."""
     from cpmd_amd.ks import KSDevice
     from test_oracle_energy import ks_system
-    sysm, _, c = ks_system(24.0, 64, 70.0, 128, seed=3)
+    sysm, _, c = ks_system(24.0, 64, 70.0, nstate, seed=3)
     assert 2 * sysm.gset.ngw >= 16384   # the split-bf16 contraction path in mixed precision
     out = {}
     for reuse in (False, True):
@@ -193,7 +195,7 @@ def test_cp_steps_bit_identical(gpu, mode):
         n = c.shape[1]
         bufs = {k: ctx.alloc_wfn(n) for k in ("c0", "cm", "cp", "c2")}
         ctx.upload(c, bufs["c0"])
-        print(f"CP steps, {mode}, column reuse {reuse}", flush=True)
+        print(f"CP steps, {mode}, {n} states, column reuse {reuse}", flush=True)
         st = ks.cp_init(bufs["c0"], bufs["cm"], bufs["c2"], sysm.f, sysm.tau, emass=400.0, ion_mass=21874.7)
         hist = []
         for _ in range(3):
