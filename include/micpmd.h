@@ -109,9 +109,11 @@ int mc_ctx_set_batch(mc_ctx* ctx, int pairs_rhoofr, int pairs_vpsi);
  * rho, V, c2 or the projections, and every host-pointer transform call,
  * invalidates the kept columns.  The caller must not write c0_dev itself
  * (a kernel of its own, a copy) between the two calls while reuse is on.
- * In mixed precision the switch also keeps the bf16 planes of c0 that
- * mc_rnlsm_dev splits for the next mc_cp_velupa_rortv_dev (with c2) and
- * mc_posupa_dev on the same c0 (one split per CP step instead of three).
+ * In mixed precision the switch also keeps the bf16 planes of the
+ * coefficients across calls: mc_posupa_dev's final rotation writes those of
+ * cp_dev (even nstate), mc_rnlsm_dev splits c0_dev when none are held, and
+ * the next mc_rnlsm_dev, mc_cp_velupa_rortv_dev (with c2) and mc_posupa_dev
+ * on the same pointer use them (no split per CP step instead of three).
  * Not used under a device memory limit (mc_set_mem_limit) or when the
  * allocation fails: the calls then work in batches as before.  Default off. */
 int mc_ctx_set_column_reuse(mc_ctx* ctx, int on);
