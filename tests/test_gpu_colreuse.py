@@ -208,3 +208,58 @@ def test_cp_steps_bit_identical(gpu, mode, nstate):
         assert a[0] == b[0] and a[1] == b[1] and a[2] == b[2]
         assert np.array_equal(a[3], b[3])
     assert np.array_equal(c00, c01) and np.array_equal(cm0, cm1)
+
+
+def test_kept_planes_stale_cases(gpu):
+    """The kept bf16 planes of c0 (mixed precision, column reuse on) are used
+    only when nothing could have changed c0 since they were made; each case
+    below runs with reuse on and off and must agree bit for bit:
+    rnlsm, a library write to c0, then velupa + rortv; rnlsm on one buffer,
+    velupa + rortv on another; rnlsm, velupa + rortv, then posupa after a
+    library write to c0."""
+    from cpmd_amd.ks import KSDevice
+    from test_oracle_energy import ks_system
+    sysm, _, c = ks_system(24.0, 64, 70.0, 64, seed=5)
+    n = c.shape[1]
+    rng = np.random.default_rng(9)
+    # small perturbations: c0 stays orthonormal to 1e-5, so that rortog converges
+    x = np.asfortranarray(1e-8 * (rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape)))
+    x[0] = x[0].real
+    g = np.asfortranarray(1e-5 * (rng.standard_normal(c.shape) + 1j * rng.standard_normal(c.shape)))
+    g[0] = g[0].real
+
+    def run(reuse):
+        ks = KSDevice(sysm, column_reuse=reuse)
+        ctx = ks.ctx
+        ctx.set_precision("mixed")
+        ks.set_ions(sysm.tau)
+        d = {k: ctx.upload(v) for k, v in (("c0", c), ("c1", c), ("x", x), ("c2", g), ("cm", g))}
+        d["cp"] = ctx.alloc_wfn(n)
+        out = []
+        # 1. rnlsm(c0); c0 += x (library write); velupa + rortv on c0
+        ctx.rnlsm_dev(d["c0"], n, tfor=True)
+        ctx.axpy_wfn(1.0, d["x"], d["c0"], n)
+        out.append(ctx.cp_velupa_rortv_dev(d["c0"], d["c2"], d["cm"], n, 0.01, 400.0)[0])
+        out.append(ctx.download(d["cm"], n))
+        # 2. rnlsm on c1, velupa + rortv on c0
+        ctx.rnlsm_dev(d["c1"], n, tfor=True)
+        out.append(ctx.cp_velupa_rortv_dev(d["c0"], d["c2"], d["cm"], n, 0.01, 400.0)[0])
+        out.append(ctx.download(d["cm"], n))
+        # 3. rnlsm(c1), velupa + rortv on c1 (planes kept), c1 += x, posupa on c1
+        ctx.rnlsm_dev(d["c1"], n, tfor=True)
+        ctx.cp_velupa_rortv_dev(d["c1"], d["c2"], d["cm"], n, 0.01, 400.0)
+        ctx.axpy_wfn(1.0, d["x"], d["c1"], n)
+        it = ctx.posupa_dev(d["c1"], d["c2"], d["cm"], d["cp"], n, 0.01, 4.0)[1]
+        out.append(it)
+        out.append(ctx.download(d["cp"], n))
+        out.append(ctx.download(d["cm"], n))
+        ctx.close()
+        return out
+
+    print("kept planes: stale cases, reuse on and off", flush=True)
+    a, b = run(False), run(True)
+    for u, v in zip(a, b):
+        if isinstance(u, np.ndarray):
+            assert np.array_equal(u, v)
+        else:
+            assert u == v
