@@ -63,3 +63,39 @@ def test_null_arguments():
     assert L.mc_layout_stats(None, None, None, None) == 1
     assert b"null" in L.mc_last_error()
     assert L.mc_ctx_destroy(None) == 0
+
+
+# Entry points that never write device memory other than rho, V, c2 and the
+# projections: they leave the wavefunction epoch unchanged (MC_API_BEGIN_RO),
+# so the intermediates kept under column reuse survive them.  A writing entry
+# point added to this list would let a kept intermediate go stale.
+READ_ONLY = {
+    "mc_rhoofr_dev", "mc_vofrho_dev", "mc_ewald", "mc_set_ions", "mc_phfac",
+    "mc_rnlsm_dev", "mc_rnlsm1_dev", "mc_rnlsm2_dev", "mc_rnlfor",
+    "mc_forces_c2_dev", "mc_vpsi_dev",
+    "mc_ctx_synchronize", "mc_mem_stats", "mc_memcpy_d2h", "mc_wfn_download", "mc_rhoofr_raw_charge",
+    "mc_timer_enable", "mc_timer_reset", "mc_timer_get", "mc_timer_select", "mc_probe_mfma", "mc_dev_alloc",
+}
+
+
+def entry_kinds():
+    """{entry point: 'ro' | 'rw'} from the library sources."""
+    kinds = {}
+    for src in (ROOT / "cpmd_amd" / "csrc").glob("*.hip"):
+        txt = src.read_text()
+        for m in re.finditer(r"\bint\s+(mc_[a-z0-9_]+)\s*\([^)]*\)\s*\{\s*(MC_API_BEGIN(?:_RO)?)\b", txt, flags=re.S):
+            kinds[m.group(1)] = "ro" if m.group(2).endswith("_RO") else "rw"
+    return kinds
+
+
+def test_read_only_entry_points():
+    kinds = entry_kinds()
+    assert len(kinds) > 80
+    ro = {k for k, v in kinds.items() if v == "ro"}
+    assert ro == READ_ONLY, (ro - READ_ONLY, READ_ONLY - ro)
+    # every entry point that writes wavefunctions counts an epoch
+    for name in ("mc_posupa_dev", "mc_cp_velupa_rortv_dev", "mc_rortv_dev", "mc_wfn_axpy", "mc_wfn_axpby",
+                 "mc_wfn_upload", "mc_memcpy_h2d", "mc_memset_d", "mc_dev_free", "mc_rgs_dev", "mc_lowdin_dev",
+                 "mc_lagrange_dev", "mc_sd_update_dev", "mc_odiis_dev", "mc_set_precision",
+                 "mc_ctx_set_column_reuse", "mc_vpsi_dev_noalloc", "mc_restart_read_dev"):
+        assert kinds.get(name) == "rw", name
