@@ -1344,12 +1344,20 @@ int mc_cp_velupa_rortv_dev(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, 
   MC_CHECK(c0_dev != cm_dev && c2_dev != cm_dev, MC_ERR_ARG, "velupa/rortv: cm must not alias c0 or c2");
   if (ekinc) *ekinc = 0.0;
   if (nstate == 0) return MC_OK;
-  // velupa(c0, cm, c2, nstate, 1): cm += dt / (2 emass) c2, then rortv
-  mc::rortv_dev(ctx, (const double2*)c0_dev, (double2*)cm_dev, gamy, nstate, (const double2*)c2_dev, dtm);
+  // velupa(c0, cm, c2, nstate, 1): cm += dt / (2 emass) c2, then rortv; in
+  // mixed precision the rotation's epilogue gives rekine's norms
   std::vector<double> d;
   if (ekinc) {
     d.resize(nstate);
-    mc::rekine_queue(ctx, (const double2*)cm_dev, nstate, d.data());
+    ctx->w_red2.ensure(sizeof(double) * nstate);
+  }
+  const bool fused = mc::rortv_dev(ctx, (const double2*)c0_dev, (double2*)cm_dev, gamy, nstate,
+                                   (const double2*)c2_dev, dtm, ekinc ? ctx->w_red2.as<double>() : nullptr);
+  if (ekinc) {
+    if (fused)
+      mc::dl_add(ctx, d.data(), ctx->w_red2.p, sizeof(double) * nstate);
+    else
+      mc::rekine_queue(ctx, (const double2*)cm_dev, nstate, d.data());
   }
   mc::dl_finish(ctx);
   if (ekinc) *ekinc = mc::rekine_sum(d, emass);

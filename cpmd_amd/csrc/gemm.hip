@@ -424,6 +424,11 @@ void reduce_tn_rows(mc_ctx* ctx, int nsplit, int ldp, int r0, int m, int n, doub
   timer_end(ctx, MC_KID_MISC);
 }
 
+bool nn_plane_form(const mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int ldd) {
+  return ctx->gemm_x3 && M >= kX3Min && M % 2 == 0 && lda % 2 == 0 && ldd % 2 == 0 &&
+         (reinterpret_cast<uintptr_t>(A) & 15) == 0 && ldgm % 2 == 0 && (reinterpret_cast<uintptr_t>(G) & 15) == 0;
+}
+
 void gemm_nn(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n, double alpha,
              double beta, double* D, int ldd, bool upper_g, const FnlEpi* epi) {
   if (M <= 0 || n <= 0) return;
@@ -432,7 +437,8 @@ void gemm_nn(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, i
     gemm_nn_x3(ctx, A, lda, G, ldgm, M, k, n, alpha, beta, D, ldd, upper_g, epi);
     return;
   }
-  MC_CHECK(!epi || !epi->pl, MC_ERR_INTERNAL, "planes of D_new need the split-bf16 product");
+  MC_CHECK(!epi || (!epi->pl && !epi->colsq), MC_ERR_INTERNAL,
+           "planes of D_new and column sums need the split-bf16 product");
   gemm_nn_f64(ctx, A, lda, G, ldgm, M, k, n, alpha, beta, D, ldd, upper_g, epi, MC_KID_GEMM);
 }
 

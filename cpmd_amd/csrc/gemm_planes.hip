@@ -432,6 +432,10 @@ __global__ __launch_bounds__(64 * kTnWaves, 1) void k_tn_planes(const __bf16* __
 // alpha/beta, or the fnonloc twnl * ctm factors).  upper_g: G upper
 // triangular (the k loop stops after the diagonal block).
 // ---------------------------------------------------------------------------
+// PL: the epilogue also writes the planes of D_new (FnlEpi::pl); SQ: the
+// per-block column sums of |D_new|^2 (FnlEpi::colsq).  Compile-time switches,
+// so that the other products carry none of their code.
+template <bool PL, bool SQ>
 __global__ __launch_bounds__(256, 3) void k_nn_planes(const double* __restrict__ A, int lda,
                                                       const __bf16* __restrict__ Gpl, int gnkb, int M, int k, int n,
                                                       double alpha, double beta, double* __restrict__ D, int ldd,
@@ -574,8 +578,12 @@ __global__ __launch_bounds__(256, 3) void k_nn_planes(const double* __restrict__
         // FP32 copy of D_new for the planes, over this lane's own staging slot
         // (read above by this lane only); zero outside D
         float2* sy = reinterpret_cast<float2*>(st + c * 64 + (((row >> 2) ^ c) * 4) + (row & 3));
+        // |D_new|^2 of this lane's complex row, over the staging words of its
+        // column (the column's values were all read above, in one instruction)
+        double* sq = reinterpret_cast<double*>(st) + c * 32 + pr;
         if (j >= n || i >= M) {
-          if (epi.pl) *sy = make_float2(0.f, 0.f);
+          if constexpr (PL) *sy = make_float2(0.f, 0.f);
+          if constexpr (SQ) *sq = 0.0;
           continue;
         }
         double* d = D + (size_t)j * ldd + i;
@@ -603,14 +611,15 @@ __global__ __launch_bounds__(256, 3) void k_nn_planes(const double* __restrict__
             v[0] = epi.vodt * (y0 - av[it].x);
             v[1] = epi.vodt * (y1 - av[it].y);
           }
-          if (epi.pl) *sy = make_float2((float)y0, (float)y1);
+          if constexpr (PL) *sy = make_float2((float)y0, (float)y1);
+          if constexpr (SQ) *sq = (epi.sq_g0 && i == 0) ? 0.0 : y0 * y0 + y1 * y1;
         }
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (epi.pl) {
+    if constexpr (PL) {
       // planes of D_new for the wave's 64 rows x 16 columns: lane (c, r) splits
       // rows 8 r8 .. 8 r8 + 7 of column c, r8 = r and r + 4 (the chunks of
       // split_planes: rows >= M and columns >= n zero, as staged)
@@ -639,6 +648,41 @@ __global__ __launch_bounds__(256, 3) void k_nn_planes(const double* __restrict__
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    if constexpr (SQ) {
+      // lane c < 16 sums column c's 32 complex rows of this wave in order
+      if (lane < 16) {
+        const double* q = reinterpret_cast<const double*>(st) + lane * 32;
+        double t = 0.0;
+        for (int r = 0; r < 32; ++r) t += q[r];
+        const int j = j0 + wc + 16 * b + lane;
+        if (j < n) epi.colsq[(size_t)j * epi.sq_nrb + (i0 + wr) / kColsqRows] = t;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+}
+
+// out[j] = 2 sum_r P[j * nrb + r] + (g0 ? D(0, j)^2 : 0): one workgroup per
+// column, thread t summing r = t, t + 256, ... in order, then a fixed tree
+__global__ __launch_bounds__(256) void k_colsq_finish(const double* __restrict__ P, int nrb,
+                                                      const double* __restrict__ D, int ldd, int g0,
+                                                      double* __restrict__ out) {
+  const int j = blockIdx.x;
+  const double* p = P + (size_t)j * nrb;
+  double s = 0.0;
+  for (int r = threadIdx.x; r < nrb; r += 256) s += p[r];
+  __shared__ double part[256];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double d0 = g0 ? D[(size_t)j * ldd] : 0.0;
+    out[j] = 2.0 * part[0] + d0 * d0;
   }
 }
 
@@ -782,6 +826,8 @@ void proj_planes_stacked(mc_ctx* ctx, const ProjA& base, const double* gk, int v
 void gemm_nn_planes(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int k, int n,
                     double alpha, double beta, double* D, int ldd, bool upper_g, const FnlEpi& epi_in) {
   FnlEpi epi = epi_in;
+  MC_CHECK(!epi.colsq || (!epi.twnl && kPT % kColsqRows == 0), MC_ERR_INTERNAL,
+           "column sums with the fnonloc epilogue");
   if (epi.pl) {
     MC_CHECK(!epi.twnl, MC_ERR_INTERNAL, "planes of D_new with the fnonloc epilogue");
     epi.pl_nkb = (M + kPK - 1) / kPK;
@@ -790,13 +836,31 @@ void gemm_nn_planes(mc_ctx* ctx, const double* A, int lda, const double* G, int 
   split_planes(ctx, G, ldgm, k, n, 2, ctx->w_plg);
   const int gnkb = (k + kPK - 1) / kPK;
   const int tm = (M + kPT - 1) / kPT, tn = (n + kPT - 1) / kPT;
+  MC_CHECK(!epi.colsq || epi.sq_nrb >= tm * (kPT / kColsqRows), MC_ERR_INTERNAL, "column-sum blocks too few");
   const int total = tm * tn;
   const int launch = (total + kXcdP - 1) / kXcdP * kXcdP;
   timer_begin(ctx, MC_KID_GEMM);
-  hipLaunchKernelGGL(k_nn_planes, dim3(launch), dim3(256), 0, ctx->stream, A, lda, ctx->w_plg.as<__bf16>(), gnkb, M,
-                     k, n, alpha, beta, D, ldd, upper_g ? 1 : 0, epi, tn, total);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(launch), dim3(256), 0, ctx->stream, A, lda, ctx->w_plg.as<__bf16>(), gnkb, M, k, n,
+                       alpha, beta, D, ldd, upper_g ? 1 : 0, epi, tn, total);
+  };
+  MC_CHECK(!(epi.pl && epi.colsq), MC_ERR_INTERNAL, "planes and column sums in one product");
+  if (epi.pl)
+    go(k_nn_planes<true, false>);
+  else if (epi.colsq)
+    go(k_nn_planes<false, true>);
+  else
+    go(k_nn_planes<false, false>);
   MC_HIP(hipGetLastError());
   timer_end(ctx, MC_KID_GEMM);
+}
+
+void colsq_finish(mc_ctx* ctx, const double* colsq, int nrb, int n, const double* D, int ldd, int g0, double* out) {
+  if (n <= 0) return;
+  timer_begin(ctx, MC_KID_MISC);
+  hipLaunchKernelGGL(k_colsq_finish, dim3(n), dim3(256), 0, ctx->stream, colsq, nrb, D, ldd, g0, out);
+  MC_HIP(hipGetLastError());
+  timer_end(ctx, MC_KID_MISC);
 }
 
 int tn_planes_partials(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np, int K, int m, int n,

@@ -318,7 +318,7 @@ void gemm_nn_x3(mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm
     gemm_nn_planes(ctx, A, lda, G, ldgm, M, k, n, alpha, beta, D, ldd, upper_g, e);
     return;
   }
-  MC_CHECK(!e.pl, MC_ERR_INTERNAL, "planes of D_new need the plane form of the product");
+  MC_CHECK(!e.pl && !e.colsq, MC_ERR_INTERNAL, "planes of D_new and column sums need the plane form of the product");
   const int tm = (M + kT - 1) / kT, tn = (n + kT - 1) / kT;
   const int total = tm * tn;
   const int launch = (total + kXcd - 1) / kXcd * kXcd;

@@ -168,6 +168,12 @@ struct FnlEpi {
   // host wrapper fills pl_nkb and pl_np.
   __bf16* pl = nullptr;
   int pl_nkb = 0, pl_np = 0;
+  // plain GEMM, split-bf16 path only: colsq[j * sq_nrb + r] = the sum over
+  // the complex rows of 64-row block r of column j of |D_new|^2, the G = 0
+  // row left out when sq_g0 (rekine's norms, finished by colsq_finish);
+  // nullptr: off
+  double* colsq = nullptr;
+  int sq_g0 = 0, sq_nrb = 0;
 };
 
 // Nonlocal pseudopotential data of the current geometry.
@@ -287,6 +293,7 @@ struct mc_ctx {
   mc::DevBuf w_gemm;       // split-K partials
   mc::DevBuf w_small[8];    // n x n matrices (7: single overlaps, rotations and rortv's gamy)
   mc::DevBuf w_red2, w_info, w_wfn, w_f2, w_dd;
+  mc::DevBuf w_colsq;   // per-block column sums of a colsq epilogue (rekine fused into rortv)
   // preconditioner of the electronic gradient (mc_hesele), device rows
   mc::DevBuf d_vpp, w_hw;
   bool vpp_ready = false;
@@ -401,6 +408,14 @@ void keep_c0_planes(mc_ctx* ctx, const void* c0, int K, int n);
 // marks slot 0 as the planes of X, written there by a final rotation's
 // epilogue (FnlEpi::pl), when column reuse is on
 void keep_new_planes(mc_ctx* ctx, const void* X, int K, int n);
+// whether gemm_nn takes the split-bf16 plane form for these operands (the
+// form that supports FnlEpi::pl and FnlEpi::colsq)
+bool nn_plane_form(const mc_ctx* ctx, const double* A, int lda, const double* G, int ldgm, int M, int ldd);
+// out[j] = 2 sum_r colsq[j * nrb + r] + (g0 ? D(0, j)^2 : 0) (a fixed-order
+// sum): the Gamma-point norms <d_j|d_j> from the partials of a colsq epilogue
+void colsq_finish(mc_ctx* ctx, const double* colsq, int nrb, int n, const double* D, int ldd, int g0, double* out);
+// rows per colsq block (the nrb of a product with M rows: ceil(M / kColsqRows))
+constexpr int kColsqRows = 64;
 void cp_posupa_dev(mc_ctx* ctx, const double2* c0, const double2* c2, double2* cm, double2* cp, int nstate, double dtm,
                    double dt);
 void cp_velocity_dev(mc_ctx* ctx, const double2* c0, const double2* cp, double2* cm, int nstate, double dt);
@@ -453,8 +468,11 @@ void rortog_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, i
                 int maxit, int* iters);
 // gamy_host (n x n, may be null) is filled by a queued download: dl_finish
 // c2 != nullptr: velupa's kick cm += dtm c2 first
-void rortv_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gamy_host, int n, const double2* c2 = nullptr,
-               double dtm = 0.0);
+// dotp_dev (device, n doubles; may be null): <cm_i|cm_i> of the result, from
+// the rotation's epilogue when the product takes the split-bf16 plane form;
+// returns whether dotp_dev was written
+bool rortv_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gamy_host, int n, const double2* c2 = nullptr,
+               double dtm = 0.0, double* dotp_dev = nullptr);
 void set_projectors(mc_ctx* ctx, int nsp, const int* na, int nhxs, const int* ngh, const int* lang,
                     const double* hmat, const double* twnl, int ldtw);
 void set_eigr(mc_ctx* ctx, const double* eigr, int lde, int nat);

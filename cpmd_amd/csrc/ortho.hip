@@ -509,9 +509,9 @@ __global__ void k_neg_symm(double* __restrict__ a, int n) {
 // velocity Verlet CP step).  With c2, velupa's kick cm += dtm c2 comes
 // first; in mixed precision it is one pass with the bf16 split of c0 and of
 // the kicked cm for the overlap.
-void rortv_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gamy_host, int n, const double2* c2,
-               double dtm) {
-  if (n <= 0) return;
+bool rortv_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gamy_host, int n, const double2* c2,
+               double dtm, double* dotp_dev) {
+  if (n <= 0) return false;
   MC_CHECK(n < kX3Min, MC_ERR_UNSUPPORTED, "rortv: nstate beyond 16383");
   PlaneScope keep(ctx);   // c0 and cm are read-only between the kick and the rotation
   const int K = wfn_rows(ctx);
@@ -530,11 +530,27 @@ void rortv_dev(mc_ctx* ctx, const double2* c0, double2* cm, double* gamy_host, i
   ovlap_dev(ctx, n, G, n, cm, c0);
   hipLaunchKernelGGL(k_neg_symm, dim3((n + 255) / 256, n), dim3(256), 0, ctx->stream, G, n);
   MC_HIP(hipGetLastError());
-  rotate_dev(ctx, 1.0, c0, 1.0, cm, G, n);
+  const int ld = wfn_ld(ctx);
+  bool fused = false;
+  if (dotp_dev && c0 != cm && nn_plane_form(ctx, (const double*)c0, ld, G, n, ld, ld)) {
+    // cm += c0 gamy with rekine's column norms of the result in its epilogue
+    const int nrb = (ld + 127) / 128 * (128 / kColsqRows);
+    ctx->w_colsq.ensure(sizeof(double) * (size_t)nrb * n);
+    FnlEpi e{};
+    e.colsq = ctx->w_colsq.as<double>();
+    e.sq_g0 = ctx->geq0;
+    e.sq_nrb = nrb;
+    gemm_nn(ctx, (const double*)c0, ld, G, n, ld, n, n, 1.0, 1.0, (double*)cm, ld, false, &e);
+    colsq_finish(ctx, e.colsq, nrb, n, (const double*)cm, ld, ctx->geq0, dotp_dev);
+    fused = true;
+  } else {
+    rotate_dev(ctx, 1.0, c0, 1.0, cm, G, n);
+  }
   if (gamy_host) dl_add(ctx, gamy_host, G, sizeof(double) * (size_t)n * n);
   // c0 is unchanged (cm was written): the planes of c0 the kick pass left in
   // the cache stay kept for the next call on it
   if (c2 && ctx->gemm_x3 && K >= kX3Min && c0 != cm) keep_c0_planes(ctx, c0, K, n);
+  return fused;
 }
 
 static void rortog_core(mc_ctx* ctx, const double2* c0, double2* cm, double* gam_host, int tnon, int n,

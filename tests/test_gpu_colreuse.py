@@ -201,6 +201,9 @@ def test_cp_steps_bit_identical(gpu, mode, nstate):
         for _ in range(3):
             ks.cp_step(st, bufs, sysm.f, 4.0)
             hist.append((st["etot"], st["ekinc"], st["iters"], np.array(st["fion"])))
+        # rekine (from the rotation's epilogue in mixed precision) against the
+        # per-state Gamma norms of the final cm
+        assert st["ekinc"] == pytest.approx(400.0 * ctx.dotp_wfn(bufs["cm"], bufs["cm"], n).sum(), rel=1e-12)
         out[reuse] = (hist, ctx.download(bufs["c0"], n), ctx.download(bufs["cm"], n))
         ctx.close()
     (h0, c00, cm0), (h1, c01, cm1) = out[False], out[True]
