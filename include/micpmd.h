@@ -227,7 +227,9 @@ int mc_rekine_dev(mc_ctx* ctx, const void* cm_dev, int nstate, double emass, dou
  * mc_rortv_dev, may be NULL) and rekine (ekinc, may be NULL).  With
  * mc_posupa_dev(dtm = dt/(2 emass)) before the force evaluation this is the
  * step of src/mdmain_utils.mod.F90:712-966: velupa + posupa(+rortog), forces,
- * velupa + rortv, rekine.  Synchronous. */
+ * velupa + rortv, rekine.  In mixed precision rekine's norms come from the
+ * epilogue of rortv's rotation (summed in another fixed order than
+ * mc_rekine_dev's: equal to rounding).  Synchronous. */
 int mc_cp_velupa_rortv_dev(mc_ctx* ctx, const void* c0_dev, const void* c2_dev, void* cm_dev, int nstate,
                            double dtm, double emass, double* gamy, double* ekinc);
 
