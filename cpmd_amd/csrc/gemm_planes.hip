@@ -48,6 +48,7 @@ constexpr int kStages = 3;
 constexpr int kXcdP = 8;
 constexpr int kMaxSlices = 64;            // K chunk (slices of 32 rows) per workgroup: FP32 accumulation
 constexpr int kTnWaves = 8;               // waves per k_tn_planes workgroup (two per SIMD)
+constexpr int kTnWorkgroups = 512;        // k_tn_planes workgroups aimed at (two per CU)
 
 // chunk q (16 bytes) of column c of a part sits at slot swz(c, q): columns are
 // 64 bytes apart, so in a 128-byte bank row columns c and c + 2 share the
@@ -342,14 +343,20 @@ __global__ __launch_bounds__(64 * kTnWaves, 1) void k_tn_planes(const __bf16* __
   // small products first, so that none is added to a larger partial sum
   // (the MFMA aligns its products to the accumulator it is given and drops
   // what falls below its last bit, a same-signed loss on sums of squares);
-  // the slice sums go to an FP32 accumulator on the VALU.  The host keeps K
-  // chunks to at most kMaxSlices slices and reduces the partials in FP64.
+  // the slice sums go to an FP32 accumulator on the VALU, which is added to
+  // an FP64 one every kMaxSlices slices; the host reduces the workgroups'
+  // FP64 partials in a fixed order.
   f4 acc32[4][NBC];
+  double acc64[4][NBC][4];   // the chunk sums in FP64
   const f4 z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < NBC; ++b) acc32[a][b] = z;
+    for (int b = 0; b < NBC; ++b) {
+      acc32[a][b] = z;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc64[a][b][r] = 0.0;
+    }
   const int nsl = kb1 - kb0;
   if (nsl > 0) issue(kb0, 0);
   if (nsl > 1) issue(kb0 + 1, 1);
@@ -404,6 +411,16 @@ __global__ __launch_bounds__(64 * kTnWaves, 1) void k_tn_planes(const __bf16* __
 #pragma unroll
       for (int a = 0; a < 4; ++a) acc32[a][b] += u[a];
     }
+    if ((i + 1) % kMaxSlices == 0 || i + 1 == nsl) {   // a chunk of <= kMaxSlices slices ends
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < NBC; ++b) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc64[a][b][r] += (double)acc32[a][b][r];
+          acc32[a][b] = z;
+        }
+    }
   }
   double* Ps = P + (size_t)s * m * n;
   const int i0 = ti * kPT, j0 = tj * kPT;
@@ -415,7 +432,7 @@ __global__ __launch_bounds__(64 * kTnWaves, 1) void k_tn_planes(const __bf16* __
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = i0 + wr + 16 * a + 4 * lq + r;
-        if (i < m && j < n) Ps[(size_t)j * m + i] = (double)acc32[a][b][r];
+        if (i < m && j < n) Ps[(size_t)j * m + i] = acc64[a][b][r];
       }
     }
 }
@@ -869,9 +886,12 @@ int tn_planes_partials(mc_ctx* ctx, const DevBuf& Apl, const DevBuf& Bpl, int np
   const int tm = (m + kPT - 1) / kPT, tn = (n + kPT - 1) / kPT;
   if (symmetric && m != n) throw Error{MC_ERR_INTERNAL, "symmetric A^T B with m != n"};
   const int tiles = tile_count(symmetric, tm, tn);
-  // one workgroup per CU (the ring takes 96-144 KiB of LDS): at least two
-  // rounds of 256 workgroups, K chunks of at most kMaxSlices slices
-  int nsplit = std::max(1, std::min(nkb, std::max((512 + tiles - 1) / tiles, (nkb + kMaxSlices - 1) / kMaxSlices)));
+  // one workgroup per CU (the ring takes 96-144 KiB of LDS): about two rounds
+  // of 256 workgroups, each over a contiguous K range that it sums in FP32
+  // chunks of at most kMaxSlices slices added in FP64 in registers, writing
+  // one FP64 partial tile (the fewer partials, the less traffic here and in
+  // the reduction)
+  int nsplit = std::max(1, std::min(nkb, kTnWorkgroups / tiles));
   const int kbper = (nkb + nsplit - 1) / nsplit;
   nsplit = (nkb + kbper - 1) / kbper;
   ctx->w_gemm.ensure(sizeof(double) * (size_t)nsplit * m * n);
