@@ -628,7 +628,7 @@ __global__ __launch_bounds__(256, 3) void k_nn_planes(const double* __restrict__
             v[0] = epi.vodt * (y0 - av[it].x);
             v[1] = epi.vodt * (y1 - av[it].y);
           }
-          if constexpr (PL) *sy = make_float2((float)y0, (float)y1);
+          if constexpr (PL) *sy = i < epi.pl_rows ? make_float2((float)y0, (float)y1) : make_float2(0.f, 0.f);
           if constexpr (SQ) *sq = (epi.sq_g0 && i == 0) ? 0.0 : y0 * y0 + y1 * y1;
         }
       }
@@ -847,7 +847,9 @@ void gemm_nn_planes(mc_ctx* ctx, const double* A, int lda, const double* G, int 
            "column sums with the fnonloc epilogue");
   if (epi.pl) {
     MC_CHECK(!epi.twnl, MC_ERR_INTERNAL, "planes of D_new with the fnonloc epilogue");
-    epi.pl_nkb = (M + kPK - 1) / kPK;
+    MC_CHECK(epi.pl_rows > 0 && epi.pl_rows <= M && epi.pl_rows % 2 == 0, MC_ERR_INTERNAL,
+             "planes of D_new: bad row count");
+    epi.pl_nkb = (epi.pl_rows + kPK - 1) / kPK;   // the plane layout of pl_rows rows (split_planes' K)
     epi.pl_np = ctx->x3_parts == 2 ? 2 : 3;
   }
   split_planes(ctx, G, ldgm, k, n, 2, ctx->w_plg);

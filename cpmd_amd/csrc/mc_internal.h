@@ -163,11 +163,12 @@ struct FnlEpi {
   // at the leading dimension of D (A and D of one layout); nullptr: off
   double* vcm = nullptr;
   double vodt = 0.0;
-  // plain GEMM, split-bf16 path only: also the bf16 planes of D_new (pl_np
-  // parts, the layout of split_planes for K = M rows); nullptr: off.  The
-  // host wrapper fills pl_nkb and pl_np.
+  // plain GEMM, split-bf16 path only: also the bf16 planes of the first
+  // pl_rows rows of D_new (pl_np parts, the layout of split_planes for K =
+  // pl_rows, rows beyond it zero); nullptr: off.  The caller sets pl_rows, the
+  // host wrapper pl_nkb and pl_np.
   __bf16* pl = nullptr;
-  int pl_nkb = 0, pl_np = 0;
+  int pl_rows = 0, pl_nkb = 0, pl_np = 0;
   // plain GEMM, split-bf16 path only: colsq[j * sq_nrb + r] = the sum over
   // the complex rows of 64-row block r of column j of |D_new|^2, the G = 0
   // row left out when sq_g0 (rekine's norms, finished by colsq_finish);

@@ -611,6 +611,7 @@ static void rortog_core(mc_ctx* ctx, const double2* c0, double2* cm, double* gam
     e.vcm = (double*)vel;
     e.vodt = 1.0 / vdt;
     e.pl = planes_out;
+    e.pl_rows = wfn_rows(ctx);   // the K of the planes the contractions read
     gemm_nn(ctx, (const double*)c0, wfn_ld(ctx), gam, n, wfn_ld(ctx), n, n, 1.0, 1.0, (double*)cm, wfn_ld(ctx), false,
             &e);
   } else {

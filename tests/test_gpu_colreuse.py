@@ -175,17 +175,23 @@ def test_memory_limit_disables_keep(gpu):
     assert ops.mem_stats()[0] < before + held   # off releases them
 
 
-@pytest.mark.parametrize("mode,nstate", [("mixed", 128), ("mixed_bf16x3", 128), ("fp64", 128), ("mixed", 127)])
-def test_cp_steps_bit_identical(gpu, mode, nstate):
+@pytest.mark.parametrize("mode,nstate,ecut", [("mixed", 128, 70.0), ("mixed_bf16x3", 128, 70.0), ("fp64", 128, 70.0),
+                                              ("mixed", 127, 70.0), ("mixed", 128, 66.0)])
+def test_cp_steps_bit_identical(gpu, mode, nstate, ecut):
     """Three velocity Verlet CP steps of the 128-state system on the 128^3 mesh
     with column reuse on (rhoofr's columns for vpsi; in mixed precision also
     the kept bf16 planes of the coefficients: written by posupa's final
     rotation for even state counts, split by rnlsm otherwise, and used by the
     next rortv and posupa) and off: every energy, force and coefficient bit
-    for bit."""
+    for bit.  At 66 Ry the padded leading dimension exceeds the plane rows by
+    a whole 32-row block (ldg - ngw = 52 >= 16): the planes written by the
+    rotation must follow the K = 2 ngw layout of split_planes, not M = 2 ldg."""
     from cpmd_amd.ks import KSDevice
     from test_oracle_energy import ks_system
-    sysm, _, c = ks_system(24.0, 64, 70.0, nstate, seed=3)
+    sysm, _, c = ks_system(24.0, 64, ecut, nstate, seed=3)
+    if ecut == 66.0:
+        ngw = sysm.gset.ngw
+        assert (2 * ngw + 31) // 32 < 2 * ((ngw + 63) // 64 * 64) // 32
     assert 2 * sysm.gset.ngw >= 16384   # the split-bf16 contraction path in mixed precision
     out = {}
     for reuse in (False, True):
