@@ -1,8 +1,8 @@
 """Writes ks_128states_128cubed.npz: the oracle's E_KS terms, ionic forces and
 fingerprints of the gradient c2 = -f H c for the seeded 128-state system on the
 128^3 mesh (a = 24 bohr, 64 atoms, 70 Ry; the same inputs the GPU test builds),
-so that the GPU test checks the whole step at that size without the oracle's
-~90 s on the GPU box.  Fingerprints of c2: per state the Gamma-metric norm
+so that the GPU test checks the whole step at that size without spending the
+oracle's ~90 s of host time.  Fingerprints of c2: per state the Gamma-metric norm
 <c2_i|c2_i> and the overlap <r_i|c2_i> with a seeded random vector r_i.
 
 Run (about 90 s): python tests/golden/make_ks_128states.py"""
