@@ -252,7 +252,17 @@ __global__ __launch_bounds__(256) void k_proj_planes_stacked(ProjA pa, const dou
     const int g = (k0 >> 1) + u;
     er[u] = ei[u] = tw[u] = 0.0;
     if (col < m && 2 * g + 1 < K) {
-      const double2 e = pa.eigr[(size_t)col * pa.ldg + g];
+      double2 e;
+      if (pa.ph1) {   // eigr from the phase tables, as k_eigr forms it
+        const int isa = pa.isa0 + col;
+        const double2 a = pa.ph1[(size_t)isa * (pa.n1 + 1) + pa.gmw[3 * g] + pa.n1 / 2];
+        const double2 b = pa.ph2[(size_t)isa * (pa.n2 + 1) + pa.gmw[3 * g + 1] + pa.n2 / 2];
+        const double2 c = pa.ph3[(size_t)isa * (pa.n3 + 1) + pa.gmw[3 * g + 2] + pa.n3 / 2];
+        const double2 ab = make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+        e = make_double2(ab.x * c.x - ab.y * c.y, ab.x * c.y + ab.y * c.x);
+      } else {
+        e = pa.eigr[(size_t)col * pa.ldg + g];
+      }
       er[u] = e.x;
       ei[u] = e.y;
       tw[u] = pa.twnl[g] * ((pa.geq0 && g == 0) ? 0.5 : 1.0);

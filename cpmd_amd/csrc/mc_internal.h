@@ -151,6 +151,12 @@ struct ProjA {
   int lmod4;             // l mod 4 of (-i)^l
   int geq0;
   const double* gkk = nullptr;   // (ldg) one Cartesian component of G (rnlsm2), or nullptr
+  // the per-axis phase tables eigr was built from (set when they still match
+  // it): the projector plane builds form eigr(g, isa0 + col) from them instead
+  // of reading the (ldg, nat) table; nullptr: read eigr
+  const int* gmw = nullptr;      // int[3 ngw] centred Miller indices of the rows
+  const double2 *ph1 = nullptr, *ph2 = nullptr, *ph3 = nullptr;
+  int n1 = 0, n2 = 0, n3 = 0, isa0 = 0;
 };
 
 // Epilogue of fnonloc (src/fnonloc_utils.mod.F90:272-300):
@@ -181,6 +187,7 @@ struct FnlEpi {
 struct Projectors {
   bool ready = false;
   int nsp = 0, nhxs = 0, nat = 0, eigr_nat = -1, fnl_nstate = -1;
+  int eigr_serial = -1;   // the phase tables' version d_eigr was built from
   std::vector<int> na, ngh, lang;
   std::vector<double> h_h;
   DevBuf d_twnl;   // (ldg, nhxs, nsp)
@@ -196,6 +203,7 @@ struct Projectors {
 // Density G set, local pseudopotential and ions of vofrho / ewald (vofrho.hip).
 struct DensitySet {
   bool ready = false, pp_ready = false, ions_ready = false;
+  int ions_serial = 0;   // counts mc_set_ions calls (the phase tables' version)
   int nhg = 0, nsp = 0, nat = 0;
   std::vector<int> na;
   std::vector<double> zv, raggio, tau;

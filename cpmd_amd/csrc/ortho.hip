@@ -666,6 +666,7 @@ void set_eigr(mc_ctx* ctx, const double* eigr, int lde, int nat) {
   auto& P = ctx->proj;
   P.d_eigr.ensure(sizeof(double2) * (size_t)ctx->ldg * std::max(nat, 1));
   P.eigr_nat = nat;
+  P.eigr_serial = -1;   // set from the host: not the phase tables' product
   if (!nat) return;
   DevBuf stage;
   stage.ensure(sizeof(double2) * (size_t)lde * nat);
@@ -685,6 +686,7 @@ void set_eigr(mc_ctx* ctx, const double* eigr, int lde, int nat) {
 static void rnlsm_groups(mc_ctx* ctx, const double2* c0, int n, bool do_fnl, bool do_dfnl, double* fnl_host,
                          double* dfnl_host) {
   auto& P = ctx->proj;
+  const auto& D = ctx->dens;
   MC_CHECK(P.ready, MC_ERR_ARG, "rnlsm: projectors not set (mc_set_projectors)");
   MC_CHECK(P.eigr_nat == P.nat, MC_ERR_ARG, "rnlsm: structure factors not set for all atoms (mc_set_eigr)");
   MC_CHECK(!do_dfnl || P.gk_ready, MC_ERR_ARG, "rnlsm2: G vectors not set (mc_set_gk)");
@@ -716,6 +718,17 @@ static void rnlsm_groups(mc_ctx* ctx, const double2* c0, int n, bool do_fnl, boo
       pa.ldg = ctx->ldg;
       pa.lmod4 = ((P.lang[iv + (size_t)P.nhxs * is] % 4) + 4) % 4;
       pa.geq0 = ctx->geq0;
+      if (x3 && D.ions_ready && P.eigr_serial == D.ions_serial && ctx->d_gmw.p) {
+        // the plane build forms eigr from the phase tables it was made from
+        pa.gmw = ctx->d_gmw.as<int>();
+        pa.ph1 = D.d_ph1.as<double2>();
+        pa.ph2 = D.d_ph2.as<double2>();
+        pa.ph3 = D.d_ph3.as<double2>();
+        pa.n1 = ctx->n1;
+        pa.n2 = ctx->n2;
+        pa.n3 = ctx->n3;
+        pa.isa0 = isa0;
+      }
       double* fout = P.d_fnl.as<double>() + isa0 + (size_t)P.nat * iv;
       double* dout[3];
       for (int k = 0; k < 3; ++k)

@@ -603,6 +603,7 @@ void set_ions(mc_ctx* ctx, const double* tau, const double* ht) {
     MC_HIP(hipGetLastError());
   }
   D.ions_ready = true;
+  ++D.ions_serial;
 }
 
 static GParams make_gparams(mc_ctx* ctx) {
@@ -833,6 +834,7 @@ void phfac(mc_ctx* ctx) {
   MC_CHECK(P.ready && P.nat == D.nat, MC_ERR_ARG, "phfac: projector atoms differ from the ions of mc_set_ions");
   P.d_eigr.ensure(sizeof(double2) * (size_t)ctx->ldg * std::max(D.nat, 1));
   P.eigr_nat = D.nat;
+  P.eigr_serial = D.ions_serial;
   if (D.nat == 0) return;
   LAUNCH(MC_KID_VOFRHO, k_eigr, dim3((ctx->ldg + 255) / 256, D.nat), dim3(256), 0, ctx->d_gmw.as<int>(), ctx->ngw,
          ctx->ldg, ctx->n1, ctx->n2, ctx->n3, D.d_ph1.as<double2>(), D.d_ph2.as<double2>(), D.d_ph3.as<double2>(),
