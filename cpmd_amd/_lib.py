@@ -115,6 +115,7 @@ SIGNATURES = {
     "mc_rnlsm2_dev": (C.c_int, [_vp, _vp, C.c_int, _dp]),
     "mc_rnlsm_dev": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _dp, _dp]),
     "mc_rnlfor": (C.c_int, [_vp, _dp, C.c_int, _dp, _dp, _dp]),
+    "mc_rnlrh": (C.c_int, [_vp, _dp, C.c_int, _dp]),
     "mc_set_density_gset": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp]),
     "mc_set_local_pp": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _dp, _dp]),
     "mc_set_ions": (C.c_int, [_vp, _dp, _dp]),

@@ -267,6 +267,14 @@ int mc_rnlfor(mc_ctx* ctx, const double* f, int nstate, const double* fnl, const
   MC_API_END
 }
 
+int mc_rnlrh(mc_ctx* ctx, const double* f, int nstate, double* enl) {
+  MC_API_BEGIN_RO
+  check_ctx(ctx);
+  MC_CHECK(f && enl && nstate >= 0, MC_ERR_ARG, "rnlrh: bad argument");
+  mc::rnlrh_dev(ctx, f, nstate, enl);
+  MC_API_END
+}
+
 int mc_set_density_gset(mc_ctx* ctx, int nhg, const int* inyh, const double* hg, const double* gk) {
   MC_API_BEGIN
   check_ctx(ctx);

@@ -491,6 +491,7 @@ void rnlsm2_dev(mc_ctx* ctx, const double2* c0, int n, double* dfnl_host);
 void rnlsm_dev(mc_ctx* ctx, const double2* c0, int n, bool tfor, double* fnl_host, double* dfnl_host);
 void rnlfor_dev(mc_ctx* ctx, const double* f_host, int n, const double* fnl_host, const double* dfnl_host,
                 double* fion_host);
+void rnlrh_dev(mc_ctx* ctx, const double* f_host, int n, double* enl);
 void fnonloc_dev(mc_ctx* ctx, double2* c2, const double* f_host, int n, const double* fnl_host);
 void wfn_upload(mc_ctx* ctx, const double* host, int ldc, int nstate, double2* dev);
 void wfn_download(mc_ctx* ctx, const double2* dev, int nstate, double* host, int ldc);

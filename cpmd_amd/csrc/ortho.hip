@@ -314,6 +314,40 @@ __global__ __launch_bounds__(256) void k_rnlfor(const double* __restrict__ fnl, 
   if (threadIdx.x == 0) out[k + 3 * (size_t)isa] = -red[0];
 }
 
+// E_NL of atom isa = sum_i f_i sum_{iv,jv} h(iv,jv) fnl(isa,iv,i) fnl(isa,jv,i)
+// (rnlrh / rnlrhg, src/rnlrh_utils.mod.F90:43-200 and :271-343 with imagp = 1:
+// the diagonal wsg terms and the symmetric hlsg pairs, iv != jv counted twice,
+// are the full double sum over a symmetric h); one workgroup per atom, a fixed
+// reduction tree
+__global__ __launch_bounds__(256) void k_rnlrh(const double* __restrict__ fnl, const double* __restrict__ f, int n,
+                                               int nat, int nhxs, const int* __restrict__ sp_of,
+                                               const int* __restrict__ ngh, const double* __restrict__ h,
+                                               double* __restrict__ out) {
+  __shared__ double red[256];
+  const int isa = blockIdx.x;
+  const int is = sp_of[isa];
+  const int nh = ngh[is];
+  const double* hs = h + (size_t)nhxs * nhxs * is;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double* x = fnl + isa + (size_t)nat * nhxs * i;
+    double s = 0.0;
+    for (int iv = 0; iv < nh; ++iv) {
+      double t = 0.0;
+      for (int jv = 0; jv < nh; ++jv) t += hs[iv + nhxs * jv] * x[(size_t)nat * jv];
+      s += x[(size_t)nat * iv] * t;
+    }
+    acc += f[i] * s;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[isa] = red[0];
+}
+
 // ---------------------------------------------------------------------------
 // drivers (device-resident wavefunctions)
 // ---------------------------------------------------------------------------
@@ -829,6 +863,35 @@ void rnlfor_dev(mc_ctx* ctx, const double* f_host, int n, const double* fnl_host
   dl_add(ctx, out.data(), base + off_out, sizeof(double) * out.size());
   dl_finish(ctx);
   for (size_t e = 0; e < out.size(); ++e) fion_host[e] += out[e];
+}
+
+void rnlrh_dev(mc_ctx* ctx, const double* f_host, int n, double* enl) {
+  auto& P = ctx->proj;
+  MC_CHECK(P.ready, MC_ERR_ARG, "rnlrh: projectors not set (mc_set_projectors)");
+  *enl = 0.0;
+  if (n <= 0 || P.nat == 0) return;
+  MC_CHECK(P.fnl_nstate == n, MC_ERR_ARG, "rnlrh: no fnl for this number of states (call rnlsm1 first)");
+  std::vector<int> sp_of(P.nat);
+  for (int is = 0, isa = 0; is < P.nsp; ++is)
+    for (int ia = 0; ia < P.na[is]; ++ia) sp_of[isa++] = is;
+  // small staging area: sp_of (nat ints), ngh (nsp ints), f (n), out (nat)
+  const size_t off_ngh = sizeof(int) * P.nat, off_f = (off_ngh + sizeof(int) * P.nsp + 15) / 16 * 16;
+  const size_t off_out = off_f + sizeof(double) * n, total = off_out + sizeof(double) * P.nat;
+  ctx->w_f.ensure(total);
+  char* base = (char*)ctx->w_f.p;
+  MC_HIP(hipMemcpyAsync(base, sp_of.data(), sizeof(int) * P.nat, hipMemcpyHostToDevice, ctx->stream));
+  MC_HIP(hipMemcpyAsync(base + off_ngh, P.ngh.data(), sizeof(int) * P.nsp, hipMemcpyHostToDevice, ctx->stream));
+  MC_HIP(hipMemcpyAsync(base + off_f, f_host, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_rnlrh, dim3(P.nat), dim3(256), 0, ctx->stream, P.d_fnl.as<double>(),
+                     (const double*)(base + off_f), n, P.nat, P.nhxs, (const int*)base, (const int*)(base + off_ngh),
+                     P.d_h.as<double>(), (double*)(base + off_out));
+  MC_HIP(hipGetLastError());
+  std::vector<double> out((size_t)P.nat);
+  dl_add(ctx, out.data(), base + off_out, sizeof(double) * out.size());
+  dl_finish(ctx);
+  double e = 0.0;
+  for (double v : out) e += v;   // atoms in isa order
+  *enl = e;
 }
 
 void fnonloc_dev(mc_ctx* ctx, double2* c2, const double* f_host, int n, const double* fnl_host) {

@@ -45,7 +45,7 @@
 ! micpmd_rhoofr_dev, micpmd_vofrho_dev, micpmd_forces_c2_dev,
 ! micpmd_rnlfor_dev, micpmd_posupa_dev, micpmd_velupa_rortv_dev,
 ! micpmd_lagrange_dev, micpmd_rgs_dev, micpmd_hesele, micpmd_csize_dev,
-! micpmd_odiis_dev and micpmd_enl.
+! micpmd_odiis_dev, micpmd_enl and micpmd_rnlrh_dev.
 !
 ! Every non-zero status of the C ABI is turned into a call of the
 ! procedure registered with micpmd_set_stop (CPMD: stopgm(procedureN,
@@ -102,7 +102,7 @@ MODULE micpmd
   PUBLIC :: micpmd_alloc_wfn, micpmd_alloc_real, micpmd_free, micpmd_upload, micpmd_download, micpmd_zero_wfn
   PUBLIC :: micpmd_phfac, micpmd_rnlsm_dev, micpmd_rhoofr_dev, micpmd_vofrho_dev, micpmd_forces_c2_dev
   PUBLIC :: micpmd_rnlfor_dev, micpmd_posupa_dev, micpmd_velupa_rortv_dev, micpmd_lagrange_dev, micpmd_rgs_dev
-  PUBLIC :: micpmd_hesele, micpmd_csize_dev, micpmd_odiis_dev, micpmd_enl
+  PUBLIC :: micpmd_hesele, micpmd_csize_dev, micpmd_odiis_dev, micpmd_enl, micpmd_rnlrh_dev
 
   INTERFACE
      FUNCTION mc_last_error() BIND(C, name='mc_last_error') RESULT(p)
@@ -238,6 +238,14 @@ MODULE micpmd
        REAL(c_double), INTENT(INOUT) :: fion(*)
        INTEGER(c_int) :: ierr
      END FUNCTION mc_rnlfor
+     FUNCTION mc_rnlrh(c, f, nstate, enl) BIND(C, name='mc_rnlrh') RESULT(ierr)
+       IMPORT :: c_int, c_ptr, c_double
+       TYPE(c_ptr), VALUE :: c
+       REAL(c_double), INTENT(IN) :: f(*)
+       INTEGER(c_int), VALUE :: nstate
+       REAL(c_double), INTENT(OUT) :: enl
+       INTEGER(c_int) :: ierr
+     END FUNCTION mc_rnlrh
      FUNCTION mc_set_density_gset(c, nhg, inyh, hg, gk) BIND(C, name='mc_set_density_gset') RESULT(ierr)
        IMPORT :: c_int, c_ptr, c_double
        TYPE(c_ptr), VALUE :: c
@@ -967,5 +975,14 @@ CONTAINS
        isa0 = isa0 + na_s(is)
     ENDDO
   END FUNCTION micpmd_enl
+
+  ! rnlrh(enl,nstate,nkpoint), src/rnlrh_utils.mod.F90:43: E_NL from the device
+  ! fnl of the last micpmd_rnlsm_dev, summed on the device
+  FUNCTION micpmd_rnlrh_dev(nstate) RESULT(e)
+    INTEGER, INTENT(IN) :: nstate
+    REAL(real_8) :: e
+    CALL need_f('rnlrh', nstate)
+    CALL check(mc_rnlrh(ctx, f_s, INT(nstate, c_int), e), 'rnlrh')
+  END FUNCTION micpmd_rnlrh_dev
 
 END MODULE micpmd

@@ -58,21 +58,6 @@ class KSDevice:
         self.ctx.phfac()
         self._tau = np.array(tau, dtype=np.float64)
 
-    def _enl(self, fnl: np.ndarray, f: np.ndarray) -> float:
-        """sum_i f_i sum_isa sum_{iv,jv} h(iv,jv) fnl(isa,iv,i) fnl(isa,jv,i)."""
-        e = 0.0
-        isa0 = 0
-        for is_, na in enumerate(self.na):
-            h = self.hmat[:, :, is_]
-            blk = fnl[isa0:isa0 + na]                                   # (na, nhxs, n)
-            for j in range(blk.shape[1]):
-                for k in range(blk.shape[1]):
-                    if h[j, k] != 0.0:
-                        # per-state column dots, then the occupation-weighted sum
-                        e += float(h[j, k] * (np.einsum("ai,ai->i", blk[:, j, :], blk[:, k, :]) @ f))
-            isa0 += na
-        return e
-
     def energy(self, d_c0: ops.DeviceArray, f: np.ndarray, tau: np.ndarray | None = None,
                gradient: ops.DeviceArray | None = None, forces: bool = False) -> dict:
         """E_KS terms of the device coefficients d_c0; with gradient=d_c2,
@@ -82,16 +67,17 @@ class KSDevice:
         n = len(f)
         if tau is not None and (self._tau is None or not np.array_equal(tau, self._tau)):
             self.set_ions(tau)
-        fnl, _ = ctx.rnlsm_dev(d_c0, n, tfor=forces, want_fnl=True)
+        ctx.rnlsm_dev(d_c0, n, tfor=forces)                 # fnl (and dfnl) stay on the device
         ekin, _, _ = ctx.rhoofr_dev(d_c0, f, n, self.d_rho)
         en, floc = ctx.vofrho_dev(self.d_rho, self.d_v, forces=forces)
         esr, eself, fesr = ctx.ewald(self.iesr, forces=forces)
-        out = dict(ekin=ekin, ehep=en["ehep"], epseu=en["epseu"], exc=en["exc"], esr=esr, eself=eself,
-                   enl=self._enl(fnl, f))
-        out["etot"] = (out["ekin"] + out["ehep"] + out["epseu"] + out["exc"] + out["esr"] - out["eself"]
-                       + out["enl"])
         if gradient is not None:
             ctx.forces_c2_dev(d_c0, gradient, f, self.d_v, n)
+        # E_NL after the gradient's launches: its read-back waits while they run
+        out = dict(ekin=ekin, ehep=en["ehep"], epseu=en["epseu"], exc=en["exc"], esr=esr, eself=eself,
+                   enl=ctx.rnlrh(f))
+        out["etot"] = (out["ekin"] + out["ehep"] + out["epseu"] + out["exc"] + out["esr"] - out["eself"]
+                       + out["enl"])
         if forces:
             out["fion"] = ctx.rnlfor(f, fion=floc + fesr)
         return out

@@ -454,6 +454,13 @@ class Context:
                               _p(b) if b is not None else None, _p(out)))
         return out
 
+    def rnlrh(self, f: np.ndarray) -> float:
+        """E_NL from the device fnl of the last rnlsm1 / rnlsm for len(f) states."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        e = C.c_double(0.0)
+        check(lib().mc_rnlrh(self.h, _p(f), len(f), C.byref(e)))
+        return e.value
+
     # ------------------------------------------------------------ vofrho / ewald
     VOFRHO_KEYS = ("ehep", "ehii", "ehee", "epseu", "vploc", "exc", "vxc")
 

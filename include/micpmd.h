@@ -460,6 +460,14 @@ int mc_rnlsm_dev(mc_ctx* ctx, const void* c0_dev, int nstate, int tfor, double* 
  * (Gamma, wk = 1). */
 int mc_rnlfor(mc_ctx* ctx, const double* f, int nstate, const double* fnl, const double* dfnl, double* fion);
 
+/* rnlrh: the nonlocal energy E_NL = sum_i f_i sum_isa sum_{iv,jv}
+ * h(iv,jv,is) fnl(isa,iv,i) fnl(isa,jv,i) from the device fnl of the last
+ * mc_rnlsm1 / mc_rnlsm (no host copy of fnl needed); summed per atom on the
+ * device, over atoms in isa order on the host.
+ * Replaces rnlrh(enl,nstate,nkpoint), src/rnlrh_utils.mod.F90:43-200, and
+ * rnlrhg, :271-343 (Gamma, imagp = 1, wk = 1). */
+int mc_rnlrh(mc_ctx* ctx, const double* f, int nstate, double* enl);
+
 /* ------------------------------------------------------------------------
  * Density -> potential (vofrhoa + vofrhob, closed shell, LDA) and Ewald
  * ------------------------------------------------------------------------ */

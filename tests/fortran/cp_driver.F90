@@ -101,7 +101,8 @@ CONTAINS
     CALL micpmd_vofrho_dev(d_rho, d_v, ener, floc)
     fesr = 0.0_dp
     CALL micpmd_ewald(iesr, esr, eself, fesr)
-    enl = micpmd_enl(nstate)
+    enl = micpmd_rnlrh_dev(nstate)
+    IF (ABS(enl - micpmd_enl(nstate)) > 1.0e-10_dp * MAX(1.0_dp, ABS(enl))) ERROR STOP 'rnlrh: device E_NL differs'
     etot = micpmd_ekin + ener(1) + ener(4) + ener(6) + esr - eself + enl
     CALL micpmd_forces_c2_dev(d_c0, d_c2, d_v, nstate)
     fion = floc + fesr

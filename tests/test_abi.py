@@ -71,7 +71,7 @@ def test_null_arguments():
 # point added to this list would let a kept intermediate go stale.
 READ_ONLY = {
     "mc_rhoofr_dev", "mc_vofrho_dev", "mc_ewald", "mc_set_ions", "mc_phfac",
-    "mc_rnlsm_dev", "mc_rnlsm1_dev", "mc_rnlsm2_dev", "mc_rnlfor",
+    "mc_rnlsm_dev", "mc_rnlsm1_dev", "mc_rnlsm2_dev", "mc_rnlfor", "mc_rnlrh",
     "mc_forces_c2_dev", "mc_vpsi_dev",
     "mc_ctx_synchronize", "mc_mem_stats", "mc_memcpy_d2h", "mc_wfn_download", "mc_rhoofr_raw_charge",
     "mc_timer_enable", "mc_timer_reset", "mc_timer_get", "mc_timer_select", "mc_probe_mfma", "mc_dev_alloc",

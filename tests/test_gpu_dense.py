@@ -244,6 +244,10 @@ def test_rnlsm2_rnlfor(gpu, kind):
     base = np.ones((3, OP.nat))
     got = ctx.rnlfor(f, fion=base)                                   # device copies, accumulates
     assert rel(got - base, ofion) < 1e-12
+    oenl = oracle.enl(OP, ofnl, f)                                   # E_NL from the device fnl
+    assert abs(ctx.rnlrh(f) - oenl) < 1e-12 * abs(oenl)
+    with pytest.raises(_lib.McError):
+        ctx.rnlrh(f[:-1])                                            # no fnl for that many states
 
 
 def test_cp_posupa_velocity(gpu):
