@@ -945,7 +945,7 @@ static Shapes shapes(mc_ctx* ctx) {
 }
 
 // y pass of np pairs: the fused kernel (workgroups over columns, z tiles and
-// runs of pairs, >= ~2048 workgroups) for compile-time lengths, else k_ypass
+// runs of pairs, >= ~65536 workgroups) for compile-time lengths, else k_ypass
 template <typename T, int DIR>
 static void ypass(mc_ctx* ctx, const WParams& w, const Shapes& s, v2<T>* A, v2<T>* B, int np, size_t shy) {
   const int kid = DIR > 0 ? MC_KID_YINV : MC_KID_YFWD;
@@ -954,7 +954,10 @@ static void ypass(mc_ctx* ctx, const WParams& w, const Shapes& s, v2<T>* A, v2<T
     if constexpr (f4_ok<T, N2>()) {
       constexpr int L = F4Tile<N2>::L;
       const int zt = (ctx->n3 + L - 1) / L;
-      const int runs = std::max(1, std::min(np, (2048 + ctx->ncols * zt - 1) / (ctx->ncols * zt)));
+      // runs of pairs per (column, z tile): aimed at >= 65536 workgroups, i.e. a
+      // few pairs per workgroup (at 192^3: 1.78 + 1.85 ms of y passes per CP
+      // step against 1.97 + 2.06 ms aimed at 2048, 2.11 + 1.99 ms at one pair)
+      const int runs = std::max(1, std::min(np, (65536 + ctx->ncols * zt - 1) / (ctx->ncols * zt)));
       const int ppb = (np + runs - 1) / runs;
       auto go = [&](auto PRC) {
         constexpr bool PR = decltype(PRC)::value;
